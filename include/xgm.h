@@ -1,0 +1,114 @@
+/*
+ * xgm.h - pattern-counting C ABI of libxgm (HIP / gfx950).
+ *
+ * Contract kept from the reference driver: a CSR graph and a pattern go in,
+ * one uint64 count comes out.  The reference binds the same contract by
+ * link-time solver substitution -- TCSolver(Graph&, uint64_t&, int, int)
+ * (codes/GraphMiner/src/triangle/main.cc:5), CliqueSolver(Graph&, int k,
+ * uint64_t&, int, int) (codes/GraphMiner/src/clique/main.cc:6),
+ * SglSolver(Graph&, Pattern&, uint64_t&, int, int)
+ * (codes/GraphMiner/src/sgl/main.cc:7) -- and, in xgminer, by the GLUMIN
+ * solver methods CliqueSolver_on_G2Miner / CliqueSolver_LUT_on_G2Miner /
+ * PatternSolver_LUT_on_G2Miner(Graph_V2&) (xgminer/include/glumin.h:82-86)
+ * that leave the count in `std::vector<uint64_t> total`
+ * (xgminer/include/glumin.h:108).
+ *
+ * Every entry point returns an xgm_status (0 = XGM_OK); nothing exits or
+ * throws across the ABI (the reference's CUDA_SAFE_CALL exits instead,
+ * xgminer/include/cuda_utils/gpu_utils.h:4-12).  Ownership: the caller owns
+ * host CSR arrays; an xgm_ctx owns every device buffer, stream and event it
+ * creates and frees them in xgm_ctx_destroy().  A context is not thread-safe;
+ * use one per host thread.
+ */
+#ifndef XGM_H
+#define XGM_H
+
+#include <stdint.h>
+#include "xgm_graph.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Patterns.  Mapping from the reference's ids:
+ *   XGM_TRIANGLE     <- GraphMiner tc (triangle/omp_base.cc:6-26); xgminer
+ *                       XGMinerPatternType::TRIANGLE (common.h:187)
+ *   XGM_CLIQUE4      <- GLUMIN --patternID 20 (local id 4, glumin.h:36-39),
+ *                       GraphMiner clique k=4 (clique/cpu_kernels/automine_omp.h:67-83)
+ *   XGM_CLIQUE5      <- GLUMIN --patternID 21 (local id 5),
+ *                       GraphMiner clique k=5 (automine_omp.h:138-157)
+ *   XGM_DIAMOND_EDGE <- GraphMiner sgl "diamond" (sgl/cpu_kernels/diamond.h:1-14)
+ *   XGM_CYCLE4_EDGE  <- GraphMiner sgl "rectangle" (sgl/cpu_kernels/rectangle.h:1-11)
+ * TRIANGLE / CLIQUE4 / CLIQUE5 count on a degree-oriented DAG
+ * (XGM_CSR_DAG); DIAMOND_EDGE / CYCLE4_EDGE on the undirected graph
+ * (XGM_CSR_SYMMETRIC).  Counts are unique embeddings (GraphMiner semantics). */
+enum xgm_pattern {
+  XGM_TRIANGLE = 1,
+  XGM_CLIQUE4 = 2,
+  XGM_CLIQUE5 = 3,
+  XGM_DIAMOND_EDGE = 4,
+  XGM_CYCLE4_EDGE = 5
+};
+
+/* Intersection / counting strategies (xgm_opts.strategy).  All produce
+ * identical counts; they differ only in speed. */
+enum xgm_strategy {
+  XGM_STRAT_AUTO = 0,   /* library heuristic (degree thresholds) */
+  XGM_STRAT_HASH = 1,   /* vertex task, LDS hash of the staged list, flattened probes */
+  XGM_STRAT_BSEARCH = 2,/* vertex task, binary search in the staged sorted list */
+  XGM_STRAT_LUT = 3,    /* per-task bitmap LUT in LDS (cliques) */
+  XGM_STRAT_LIST = 4    /* edge task, list-based nested intersections (cliques) */
+};
+
+typedef struct xgm_opts {
+  int strategy;          /* xgm_strategy; 0 = auto */
+  int lut_max_degree;    /* LUT path for tasks with out-degree <= this; 0 = 1024 */
+  uint32_t flags;        /* reserved, must be 0 */
+  int reserved;
+} xgm_opts;
+
+typedef struct xgm_ctx xgm_ctx;
+typedef struct xgm_dgraph xgm_dgraph;
+
+/* Create a context on HIP device `device` (must be gfx950). */
+int xgm_ctx_create(int device, xgm_ctx **out);
+void xgm_ctx_destroy(xgm_ctx *ctx);
+
+/* Last error message recorded by ctx (static until the next call). */
+const char *xgm_ctx_last_error(const xgm_ctx *ctx);
+
+/* Copy a CSR into HBM and build its task lists (degree bins).  The CSR must
+ * satisfy xgm_csr_validate() for its flags.  Replaces GraphGPU's H2D copy
+ * (xgminer/include/graph_v2_gpu.h:81-134) and the host task prep of
+ * glumin_GM_solver.cu:326-335. */
+int xgm_graph_upload(xgm_ctx *ctx, const xgm_csr *g, xgm_dgraph **out);
+void xgm_graph_free(xgm_dgraph *g);
+
+/* Count `pattern` on a device graph; *out receives the uint64 count.
+ * opts may be NULL (auto).  Synchronous. */
+int xgm_count(xgm_ctx *ctx, const xgm_dgraph *g, int pattern, const xgm_opts *opts,
+              uint64_t *out);
+
+/* One-shot convenience: upload, count, free. */
+int xgm_count_csr(xgm_ctx *ctx, const xgm_csr *g, int pattern, const xgm_opts *opts,
+                  uint64_t *out);
+
+/* Timing of the most recent xgm_count on ctx, from HIP events recorded on
+ * the stream the kernels run on. */
+#define XGM_MAX_PROFILE_KERNELS 16
+typedef struct xgm_kernel_time {
+  char name[40];
+  float ms;              /* elapsed time of this launch */
+  int64_t tasks;         /* tasks the launch processed */
+} xgm_kernel_time;
+typedef struct xgm_profile {
+  float total_ms;        /* first launch start to last launch end */
+  int n_kernels;
+  xgm_kernel_time k[XGM_MAX_PROFILE_KERNELS];
+} xgm_profile;
+int xgm_last_profile(const xgm_ctx *ctx, xgm_profile *out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* XGM_H */

@@ -1,0 +1,300 @@
+// ORACLE - test infrastructure only.  A CPU restatement of the reference's
+// GraphMiner OpenMP path, used by tests/, by the smoke check and by the
+// benchmark's cpu_baseline leg as the checker.  Nothing in the product
+// library links or calls this file.
+//
+// Every function follows the reference file:line named beside it, with the
+// same set semantics (sorted int32 lists, merge-based counting) and the
+// same `schedule(dynamic,1)` vertex loop.  Each pattern also totals the
+// algorithmic bytes of its set-operation stream:
+//   B = sum over set ops of 4*(|A|+|B|)  +  40 per task
+// (a task is one DAG or undirected edge; 8 B src/dst + 32 B of rowptr).
+// For bounded operations (x < up) only the elements below the bound count.
+//
+// Pinning (see tests/test_oracle.py): brute force vs std::set_* on random
+// lists; reference known answers on citeseer (1166 / 255 / 3730 / 6059);
+// closed forms on complete and complete-bipartite graphs; identities
+// between patterns; and the reference's own GraphMiner binaries built from
+// source into oracle/_ref/ by oracle/Makefile.
+#include <omp.h>
+#include <stdint.h>
+
+#include <algorithm>
+#include <vector>
+
+typedef int32_t vid;
+typedef int64_t eid;
+typedef unsigned long long u64;
+
+namespace {
+
+// GraphMiner VertexSet::get_intersect_num (include/VertexSet.h:65-76).
+inline u64 isect_num(const vid *a, eid na, const vid *b, eid nb) {
+  eid i = 0, j = 0;
+  u64 c = 0;
+  while (i < na && j < nb) {
+    vid x = a[i], y = b[j];
+    if (x <= y) ++i;
+    if (y <= x) ++j;
+    if (x == y) ++c;
+  }
+  return c;
+}
+
+// VertexSet::intersect_ns (include/VertexSet.h:110-122): count x < up.
+inline u64 isect_num_bound(const vid *a, eid na, const vid *b, eid nb, vid up) {
+  eid i = 0, j = 0;
+  u64 c = 0;
+  while (i < na && j < nb) {
+    vid x = a[i], y = b[j];
+    if (x >= up || y >= up) break;
+    if (x <= y) ++i;
+    if (y <= x) ++j;
+    if (x == y) ++c;
+  }
+  return c;
+}
+
+// VertexSet::operator& (include/VertexSet.h:53-64): materialise A ∩ B.
+inline eid isect_set(const vid *a, eid na, const vid *b, eid nb, vid *out) {
+  eid i = 0, j = 0, k = 0;
+  while (i < na && j < nb) {
+    vid x = a[i], y = b[j];
+    if (x <= y) ++i;
+    if (y <= x) ++j;
+    if (x == y) out[k++] = x;
+  }
+  return k;
+}
+
+// number of entries of sorted a[0..n) below up
+inline eid below(const vid *a, eid n, vid up) { return std::lower_bound(a, a + n, up) - a; }
+
+struct G {
+  eid n;
+  const eid *rp;
+  const vid *ci;
+  const vid *N(eid v) const { return ci + rp[v]; }
+  eid d(eid v) const { return rp[v + 1] - rp[v]; }
+};
+
+eid max_degree(const G &g) {
+  eid m = 0;
+  for (eid v = 0; v < g.n; ++v) m = std::max(m, g.d(v));
+  return m;
+}
+
+void set_threads(int threads) {
+  if (threads > 0) omp_set_num_threads(threads);
+}
+
+}  // namespace
+
+extern "C" {
+
+// ---- set-operation primitives (pinned against std::set_* by brute force) ----
+
+u64 orc_intersect_num(const vid *a, eid na, const vid *b, eid nb) { return isect_num(a, na, b, nb); }
+
+u64 orc_intersect_num_bound(const vid *a, eid na, const vid *b, eid nb, vid up) {
+  return isect_num_bound(a, na, b, nb, up);
+}
+
+eid orc_intersect_set(const vid *a, eid na, const vid *b, eid nb, vid *out) {
+  return isect_set(a, na, b, nb, out);
+}
+
+// GraphMiner difference: |{x in A : x < up, x not in B, x != vid_b}|, the
+// vid(B) exclusion of VertexSet::difference_ns (src/common/VertexSet.cc:69-88).
+u64 orc_difference_num(const vid *a, eid na, const vid *b, eid nb, vid up, vid vid_b) {
+  eid i = 0, j = 0;
+  u64 c = 0;
+  while (i < na && j < nb) {
+    vid x = a[i], y = b[j];
+    if (x >= up || y >= up) break;
+    if (x <= y) ++i;
+    if (y <= x) ++j;
+    if (x < y && x != vid_b) ++c;
+  }
+  while (i < na) {
+    vid x = a[i];
+    if (x >= up) break;
+    ++i;
+    if (x != vid_b) ++c;
+  }
+  return c;
+}
+
+// GLUMIN GPU difference: |{x in A : x < up, x not in B}| with no owner
+// exclusion (xgminer/include/glumin/set_difference.cuh:20-35, 110-129).
+u64 orc_difference_num_noexcl(const vid *a, eid na, const vid *b, eid nb, vid up) {
+  u64 c = 0;
+  eid j = 0;
+  for (eid i = 0; i < na && a[i] < up; ++i) {
+    while (j < nb && b[j] < a[i]) ++j;
+    if (!(j < nb && b[j] == a[i])) ++c;
+  }
+  return c;
+}
+
+// ---- orientation: Graph::orientation (src/common/graph.cc:233-279) ----
+// out_rp has n+1 entries; out_ci has room for nnz/2 entries.  Returns the
+// number of DAG edges.
+eid orc_orient(eid n, const eid *rp, const vid *ci, eid *out_rp, vid *out_ci) {
+  std::vector<eid> deg(n), nd(n, 0);
+  for (eid v = 0; v < n; ++v) deg[v] = rp[v + 1] - rp[v];
+  auto keep = [&](eid s, vid t) { return deg[t] > deg[s] || (deg[t] == deg[s] && t > s); };
+  for (eid s = 0; s < n; ++s)
+    for (eid k = rp[s]; k < rp[s + 1]; ++k) nd[s] += keep(s, ci[k]);
+  out_rp[0] = 0;
+  for (eid s = 0; s < n; ++s) out_rp[s + 1] = out_rp[s] + nd[s];
+  for (eid s = 0; s < n; ++s) {
+    eid o = out_rp[s];
+    for (eid k = rp[s]; k < rp[s + 1]; ++k)
+      if (keep(s, ci[k])) out_ci[o++] = ci[k];
+  }
+  return out_rp[n];
+}
+
+// ---- patterns.  Vertex loop over [v_begin, v_end) (the whole graph when
+// v_end < 0); `threads` <= 0 keeps the OpenMP default; *bytes (optional)
+// receives the algorithmic byte total. ----
+
+// TCSolver (src/triangle/omp_base.cc:15-21) on a DAG.
+u64 orc_tc(eid n, const eid *rp, const vid *ci, eid v_begin, eid v_end, int threads, u64 *bytes) {
+  G g{n, rp, ci};
+  if (v_end < 0 || v_end > n) v_end = n;
+  set_threads(threads);
+  u64 count = 0, B = 0;
+#pragma omp parallel for schedule(dynamic, 1) reduction(+ : count, B)
+  for (eid u = v_begin; u < v_end; ++u) {
+    const vid *yu = g.N(u);
+    const eid du = g.d(u);
+    for (eid k = 0; k < du; ++k) {
+      vid v = yu[k];
+      count += isect_num(yu, du, g.N(v), g.d(v));
+      B += 40 + 4 * (u64)(du + g.d(v));
+    }
+  }
+  if (bytes) *bytes = B;
+  return count;
+}
+
+// automine_4clique (src/clique/cpu_kernels/automine_omp.h:67-83) on a DAG.
+u64 orc_clique4(eid n, const eid *rp, const vid *ci, eid v_begin, eid v_end, int threads, u64 *bytes) {
+  G g{n, rp, ci};
+  if (v_end < 0 || v_end > n) v_end = n;
+  set_threads(threads);
+  const eid md = max_degree(g);
+  u64 count = 0, B = 0;
+#pragma omp parallel reduction(+ : count, B)
+  {
+    std::vector<vid> y01(md + 1);
+#pragma omp for schedule(dynamic, 1)
+    for (eid v0 = v_begin; v0 < v_end; ++v0) {
+      const vid *y0 = g.N(v0);
+      const eid d0 = g.d(v0);
+      for (eid k = 0; k < d0; ++k) {
+        vid v1 = y0[k];
+        eid s = isect_set(y0, d0, g.N(v1), g.d(v1), y01.data());
+        B += 40 + 4 * (u64)(d0 + g.d(v1));
+        for (eid i = 0; i < s; ++i) {
+          vid v2 = y01[i];
+          count += isect_num(y01.data(), s, g.N(v2), g.d(v2));
+          B += 4 * (u64)(s + g.d(v2));
+        }
+      }
+    }
+  }
+  if (bytes) *bytes = B;
+  return count;
+}
+
+// automine_5clique (src/clique/cpu_kernels/automine_omp.h:138-157) on a DAG.
+u64 orc_clique5(eid n, const eid *rp, const vid *ci, eid v_begin, eid v_end, int threads, u64 *bytes) {
+  G g{n, rp, ci};
+  if (v_end < 0 || v_end > n) v_end = n;
+  set_threads(threads);
+  const eid md = max_degree(g);
+  u64 count = 0, B = 0;
+#pragma omp parallel reduction(+ : count, B)
+  {
+    std::vector<vid> y12(md + 1), y123(md + 1);
+#pragma omp for schedule(dynamic, 1)
+    for (eid v1 = v_begin; v1 < v_end; ++v1) {
+      const vid *y1 = g.N(v1);
+      const eid d1 = g.d(v1);
+      for (eid k = 0; k < d1; ++k) {
+        vid v2 = y1[k];
+        eid s2 = isect_set(y1, d1, g.N(v2), g.d(v2), y12.data());
+        B += 40 + 4 * (u64)(d1 + g.d(v2));
+        for (eid i = 0; i < s2; ++i) {
+          vid v3 = y12[i];
+          eid s3 = isect_set(y12.data(), s2, g.N(v3), g.d(v3), y123.data());
+          B += 4 * (u64)(s2 + g.d(v3));
+          for (eid j = 0; j < s3; ++j) {
+            vid v4 = y123[j];
+            count += isect_num(y123.data(), s3, g.N(v4), g.d(v4));
+            B += 4 * (u64)(s3 + g.d(v4));
+          }
+        }
+      }
+    }
+  }
+  if (bytes) *bytes = B;
+  return count;
+}
+
+// Edge-induced diamond (src/sgl/cpu_kernels/diamond.h:1-14) on the
+// undirected graph: sum over v1 < v0 of C(|N(v0) ∩ N(v1)|, 2).
+u64 orc_diamond(eid n, const eid *rp, const vid *ci, eid v_begin, eid v_end, int threads, u64 *bytes) {
+  G g{n, rp, ci};
+  if (v_end < 0 || v_end > n) v_end = n;
+  set_threads(threads);
+  u64 count = 0, B = 0;
+#pragma omp parallel for schedule(dynamic, 1) reduction(+ : count, B)
+  for (eid v0 = v_begin; v0 < v_end; ++v0) {
+    const vid *y0 = g.N(v0);
+    const eid d0 = g.d(v0);
+    for (eid k = 0; k < d0; ++k) {
+      vid v1 = y0[k];
+      if (v1 >= v0) break;
+      u64 c = isect_num(y0, d0, g.N(v1), g.d(v1));
+      count += c * (c - (c > 0)) / 2;  // C(c,2); c*(c-1)/2 with c=0 -> 0
+      B += 40 + 4 * (u64)(d0 + g.d(v1));
+    }
+  }
+  if (bytes) *bytes = B;
+  return count;
+}
+
+// Edge-induced 4-cycle (src/sgl/cpu_kernels/rectangle.h:1-11) on the
+// undirected graph: for v1 < v0 in N(v0), v2 < v1 in N(v0), count
+// |N(v1) ∩ N(v2) ∩ [0, v0)|.
+u64 orc_cycle4(eid n, const eid *rp, const vid *ci, eid v_begin, eid v_end, int threads, u64 *bytes) {
+  G g{n, rp, ci};
+  if (v_end < 0 || v_end > n) v_end = n;
+  set_threads(threads);
+  u64 count = 0, B = 0;
+#pragma omp parallel for schedule(dynamic, 1) reduction(+ : count, B)
+  for (eid v0 = v_begin; v0 < v_end; ++v0) {
+    const vid *y0 = g.N(v0);
+    const eid d0 = g.d(v0);
+    for (eid k1 = 0; k1 < d0; ++k1) {
+      vid v1 = y0[k1];
+      if (v1 >= v0) break;
+      const eid b1 = below(g.N(v1), g.d(v1), (vid)v0);
+      B += 40;
+      for (eid k2 = 0; k2 < d0; ++k2) {
+        vid v2 = y0[k2];
+        if (v2 >= v1) break;
+        count += isect_num_bound(g.N(v1), g.d(v1), g.N(v2), g.d(v2), (vid)v0);
+        B += 4 * (u64)(b1 + below(g.N(v2), g.d(v2), (vid)v0));
+      }
+    }
+  }
+  if (bytes) *bytes = B;
+  return count;
+}
+
+}  // extern "C"

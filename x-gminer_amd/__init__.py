@@ -1,0 +1,304 @@
+"""xgminer_amd - Python host binding of libxgm (gfx950 graph pattern counting).
+
+The product is the C ABI in ``include/xgm.h`` / ``include/xgm_graph.h``,
+implemented by ``libxgm.so`` (built in-tree by ``make -C x-gminer_amd``).
+This module binds it with ctypes and mirrors the reference's solver entry
+points so callers read like GraphMiner's drivers:
+
+* ``TCSolver(g)``             <- codes/GraphMiner/src/triangle/main.cc:5
+* ``CliqueSolver(g, k)``      <- codes/GraphMiner/src/clique/main.cc:6
+* ``SglSolver(g, pattern)``   <- codes/GraphMiner/src/sgl/main.cc:7
+  (pattern names "diamond" / "rectangle" as in codes/GraphMiner/include/pattern.hh:62-63)
+
+Errors raise ``XGMError`` carrying the C status code; there is no CPU
+fallback: if ``libxgm.so`` is missing the import-time load raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libxgm.so")
+
+# status codes (include/xgm_graph.h)
+XGM_OK, XGM_EINVAL, XGM_ENOMEM, XGM_EUNSUPPORTED, XGM_EHIP, XGM_ENODEV, XGM_EIO = range(7)
+# csr flags
+CSR_SORTED, CSR_SYMMETRIC, CSR_DAG = 0x1, 0x2, 0x4
+# patterns (include/xgm.h)
+TRIANGLE, CLIQUE4, CLIQUE5, DIAMOND_EDGE, CYCLE4_EDGE = 1, 2, 3, 4, 5
+PATTERN_NAMES = {TRIANGLE: "triangle", CLIQUE4: "clique4", CLIQUE5: "clique5",
+                 DIAMOND_EDGE: "diamond", CYCLE4_EDGE: "rectangle"}
+# strategies
+STRAT_AUTO, STRAT_HASH, STRAT_BSEARCH, STRAT_LUT, STRAT_LIST = 0, 1, 2, 3, 4
+
+
+class XGMError(RuntimeError):
+    def __init__(self, status: int, msg: str = ""):
+        self.status = status
+        super().__init__(f"xgm status {status}: {msg}")
+
+
+class _Csr(C.Structure):
+    _fields_ = [("n", C.c_int64), ("nnz", C.c_int64),
+                ("rowptr", C.POINTER(C.c_int64)), ("colidx", C.POINTER(C.c_int32)),
+                ("flags", C.c_uint32)]
+
+
+class _Stats(C.Structure):
+    _fields_ = [("n", C.c_int64), ("nnz", C.c_int64), ("max_degree", C.c_int64),
+                ("sum_deg_sq", C.c_double), ("sum_in_out", C.c_double)]
+
+
+class _Opts(C.Structure):
+    _fields_ = [("strategy", C.c_int), ("lut_max_degree", C.c_int),
+                ("flags", C.c_uint32), ("reserved", C.c_int)]
+
+
+class _KTime(C.Structure):
+    _fields_ = [("name", C.c_char * 40), ("ms", C.c_float), ("tasks", C.c_int64)]
+
+
+class _Profile(C.Structure):
+    _fields_ = [("total_ms", C.c_float), ("n_kernels", C.c_int), ("k", _KTime * 16)]
+
+
+def _load():
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} not built: run `make -C x-gminer_amd` "
+                          "(or __graft_entry__.build())")
+    lib = C.CDLL(LIB_PATH)
+    P, I64, I32, U64, VP = C.POINTER, C.c_int64, C.c_int32, C.c_uint64, C.c_void_p
+    sigs = {
+        "xgm_strerror": (C.c_char_p, [C.c_int]),
+        "xgm_hgraph_view": (C.c_int, [VP, P(_Csr)]),
+        "xgm_hgraph_free": (None, [VP]),
+        "xgm_gen_rmat": (C.c_int, [C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, U64, P(VP)]),
+        "xgm_gen_chung_lu": (C.c_int, [I64, C.c_double, C.c_double, C.c_double, U64, P(VP)]),
+        "xgm_gen_erdos_renyi": (C.c_int, [I64, C.c_double, U64, P(VP)]),
+        "xgm_build_undirected": (C.c_int, [I64, I64, P(I32), P(I32), P(VP)]),
+        "xgm_orient_dag": (C.c_int, [P(_Csr), P(VP)]),
+        "xgm_load_csr": (C.c_int, [C.c_char_p, C.c_uint32, P(VP)]),
+        "xgm_save_csr": (C.c_int, [C.c_char_p, P(_Csr)]),
+        "xgm_csr_validate": (C.c_int, [P(_Csr)]),
+        "xgm_csr_hash": (U64, [P(_Csr)]),
+        "xgm_csr_get_stats": (C.c_int, [P(_Csr), P(_Stats)]),
+        "xgm_ctx_create": (C.c_int, [C.c_int, P(VP)]),
+        "xgm_ctx_destroy": (None, [VP]),
+        "xgm_ctx_last_error": (C.c_char_p, [VP]),
+        "xgm_graph_upload": (C.c_int, [VP, P(_Csr), P(VP)]),
+        "xgm_graph_free": (None, [VP]),
+        "xgm_count": (C.c_int, [VP, VP, C.c_int, P(_Opts), P(U64)]),
+        "xgm_count_csr": (C.c_int, [VP, P(_Csr), C.c_int, P(_Opts), P(U64)]),
+        "xgm_last_profile": (C.c_int, [VP, P(_Profile)]),
+    }
+    for name, (res, args) in sigs.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    return lib
+
+
+lib = _load()
+
+
+def _check(rc: int, ctx=None):
+    if rc != XGM_OK:
+        msg = lib.xgm_strerror(rc).decode()
+        if ctx is not None:
+            msg += " (" + lib.xgm_ctx_last_error(ctx).decode() + ")"
+        raise XGMError(rc, msg)
+
+
+# --------------------------------------------------------------------------
+# host graphs
+
+class Graph:
+    """A CSR graph: int64 rowptr[n+1], int32 colidx[nnz], flags.
+
+    Either owned by the library (generators, loader, orientation) or a view
+    of caller-owned numpy arrays (``Graph.from_arrays``)."""
+
+    def __init__(self, handle=None, rowptr=None, colidx=None, flags=0):
+        self._h = handle
+        if handle is not None:
+            v = _Csr()
+            _check(lib.xgm_hgraph_view(handle, C.byref(v)))
+            self.n, self.nnz, self.flags = v.n, v.nnz, v.flags
+            self.rowptr = np.ctypeslib.as_array(v.rowptr, shape=(v.n + 1,))
+            self.colidx = (np.ctypeslib.as_array(v.colidx, shape=(v.nnz,)) if v.nnz
+                           else np.zeros(0, np.int32))
+        else:
+            self.rowptr = np.ascontiguousarray(rowptr, dtype=np.int64)
+            self.colidx = np.ascontiguousarray(colidx, dtype=np.int32)
+            self.n = len(self.rowptr) - 1
+            self.nnz = len(self.colidx)
+            self.flags = flags
+        self._csr = _Csr(self.n, self.nnz,
+                         self.rowptr.ctypes.data_as(C.POINTER(C.c_int64)),
+                         self.colidx.ctypes.data_as(C.POINTER(C.c_int32)), self.flags)
+
+    def __del__(self):
+        if getattr(self, "_h", None) is not None:
+            lib.xgm_hgraph_free(self._h)
+            self._h = None
+
+    @classmethod
+    def from_arrays(cls, rowptr, colidx, flags):
+        return cls(rowptr=rowptr, colidx=colidx, flags=flags)
+
+    @property
+    def csr(self):
+        return C.byref(self._csr)
+
+    def degree(self):
+        return np.diff(self.rowptr)
+
+    def orient(self) -> "Graph":
+        """Degree-oriented DAG (GraphMiner Graph::orientation semantics)."""
+        h = C.c_void_p()
+        _check(lib.xgm_orient_dag(self.csr, C.byref(h)))
+        return Graph(h)
+
+    def validate(self) -> bool:
+        return lib.xgm_csr_validate(self.csr) == XGM_OK
+
+    def hash(self) -> int:
+        return int(lib.xgm_csr_hash(self.csr))
+
+    def stats(self) -> dict:
+        s = _Stats()
+        _check(lib.xgm_csr_get_stats(self.csr, C.byref(s)))
+        return dict(n=s.n, nnz=s.nnz, max_degree=s.max_degree,
+                    sum_deg_sq=s.sum_deg_sq, sum_in_out=s.sum_in_out)
+
+    def save(self, prefix: str):
+        _check(lib.xgm_save_csr(prefix.encode(), self.csr))
+
+    @property
+    def num_edges(self) -> int:
+        """Undirected edges (= DAG edges)."""
+        return self.nnz if self.flags & CSR_DAG else self.nnz // 2
+
+
+def _gen(fn, *args) -> Graph:
+    h = C.c_void_p()
+    _check(fn(*args, C.byref(h)))
+    return Graph(h)
+
+
+def gen_rmat(scale, edge_factor=16, a=0.57, b=0.19, c=0.19, seed=1) -> Graph:
+    return _gen(lib.xgm_gen_rmat, scale, edge_factor, a, b, c, seed)
+
+
+def gen_chung_lu(n, avg_degree, beta, w_max, seed) -> Graph:
+    return _gen(lib.xgm_gen_chung_lu, n, avg_degree, beta, w_max, seed)
+
+
+def gen_erdos_renyi(n, avg_degree, seed) -> Graph:
+    return _gen(lib.xgm_gen_erdos_renyi, n, avg_degree, seed)
+
+
+def from_edges(n, src, dst) -> Graph:
+    src = np.ascontiguousarray(src, dtype=np.int32)
+    dst = np.ascontiguousarray(dst, dtype=np.int32)
+    return _gen(lib.xgm_build_undirected, n, len(src),
+                src.ctypes.data_as(C.POINTER(C.c_int32)), dst.ctypes.data_as(C.POINTER(C.c_int32)))
+
+
+def load_csr(prefix: str, flags=CSR_SORTED | CSR_SYMMETRIC) -> Graph:
+    return _gen(lib.xgm_load_csr, prefix.encode(), flags)
+
+
+# --------------------------------------------------------------------------
+# device
+
+@dataclass
+class KernelTime:
+    name: str
+    ms: float
+    tasks: int
+
+
+class Context:
+    """One HIP device context (stream, scratch, events).  Not thread-safe."""
+
+    def __init__(self, device: int = 0):
+        h = C.c_void_p()
+        _check(lib.xgm_ctx_create(device, C.byref(h)))
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None) is not None:
+            lib.xgm_ctx_destroy(self._h)
+            self._h = None
+
+    __del__ = close
+
+    def upload(self, g: Graph) -> "DeviceGraph":
+        return DeviceGraph(self, g)
+
+    def count(self, g, pattern: int, strategy: int = STRAT_AUTO, lut_max_degree: int = 0) -> int:
+        opts = _Opts(strategy, lut_max_degree, 0, 0)
+        out = C.c_uint64()
+        if isinstance(g, DeviceGraph):
+            _check(lib.xgm_count(self._h, g._h, pattern, C.byref(opts), C.byref(out)), self._h)
+        else:
+            _check(lib.xgm_count_csr(self._h, g.csr, pattern, C.byref(opts), C.byref(out)), self._h)
+        return int(out.value)
+
+    def last_profile(self):
+        p = _Profile()
+        _check(lib.xgm_last_profile(self._h, C.byref(p)))
+        ks = [KernelTime(p.k[i].name.decode(), p.k[i].ms, p.k[i].tasks) for i in range(p.n_kernels)]
+        return p.total_ms, ks
+
+
+class DeviceGraph:
+    def __init__(self, ctx: Context, g: Graph):
+        h = C.c_void_p()
+        _check(lib.xgm_graph_upload(ctx._h, g.csr, C.byref(h)), ctx._h)
+        self._h, self.ctx, self.host = h, ctx, g
+
+    def close(self):
+        if getattr(self, "_h", None) is not None:
+            lib.xgm_graph_free(self._h)
+            self._h = None
+
+    __del__ = close
+
+
+# --------------------------------------------------------------------------
+# reference-shaped solver entry points (GraphMiner main.cc contracts)
+
+_default_ctx = None
+
+
+def _ctx():
+    global _default_ctx
+    if _default_ctx is None:
+        _default_ctx = Context(0)
+    return _default_ctx
+
+
+def TCSolver(g: Graph) -> int:
+    """Triangles of a DAG-oriented graph (GraphMiner: Graph g(path, USE_DAG))."""
+    return _ctx().count(g, TRIANGLE)
+
+
+def CliqueSolver(g: Graph, k: int) -> int:
+    if k == 3:
+        return _ctx().count(g, TRIANGLE)
+    if k not in (4, 5):
+        raise XGMError(XGM_EUNSUPPORTED, f"k={k} not supported")
+    return _ctx().count(g, CLIQUE4 if k == 4 else CLIQUE5)
+
+
+def SglSolver(g: Graph, pattern: str) -> int:
+    pid = {"diamond": DIAMOND_EDGE, "rectangle": CYCLE4_EDGE}.get(pattern)
+    if pid is None:
+        raise XGMError(XGM_EUNSUPPORTED, f"pattern {pattern!r} not supported")
+    return _ctx().count(g, pid)

@@ -1,0 +1,118 @@
+// Wave64 building blocks shared by every gfx950 kernel in libxgm:
+// wave-scope synchronisation, reductions and scans over 64 lanes, and the
+// two exact membership structures used on staged neighbour lists (an
+// open-addressing hash and a branch-free binary search).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+typedef unsigned long long u64;
+
+#define XGM_WAVE 64
+
+// Order LDS traffic between lanes of one wave (no s_barrier needed).
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ int lane_id() { return threadIdx.x & 63; }
+
+__device__ __forceinline__ u64 lanemask_lt(int lane) { return (1ull << lane) - 1ull; }
+__device__ __forceinline__ u64 lanemask_le(int lane) { return lane == 63 ? ~0ull : ((2ull << lane) - 1ull); }
+
+__device__ __forceinline__ u64 wave_sum_u64(u64 v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+__device__ __forceinline__ int wave_sum_i32(int v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+__device__ __forceinline__ u64 wave_or_u64(u64 v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v |= __shfl_xor(v, o, 64);
+  return v;
+}
+
+// Inclusive prefix sum over the 64 lanes.
+__device__ __forceinline__ int wave_incl_scan(int v, int lane) {
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    int t = __shfl_up(v, o, 64);
+    if (lane >= o) v += t;
+  }
+  return v;
+}
+
+// First index in a[0..n) with a[idx] > key (a non-decreasing).  All lanes
+// usually pass the same (a, n, key) so the loads broadcast.
+template <class P>
+__device__ __forceinline__ int upper_bound_dev(P a, int n, int key) {
+  int lo = 0, hi = n;
+  while (lo < hi) {
+    int mid = (lo + hi) >> 1;
+    if (a[mid] <= key) lo = mid + 1; else hi = mid;
+  }
+  return lo;
+}
+
+// First index in a[0..n) with a[idx] >= key.
+template <class P>
+__device__ __forceinline__ int lower_bound_dev(P a, int n, int key) {
+  int lo = 0, hi = n;
+  while (lo < hi) {
+    int mid = (lo + hi) >> 1;
+    if (a[mid] < key) lo = mid + 1; else hi = mid;
+  }
+  return lo;
+}
+
+// Membership in a strictly increasing list; fixed trip count per list size.
+template <class P>
+__device__ __forceinline__ bool sorted_contains(P a, int n, int key) {
+  if (n <= 0) return false;
+  int pos = 0;
+  int step = 1 << (31 - __clz(n));  // largest power of two <= n
+  for (; step > 0; step >>= 1) {
+    int nxt = pos + step;
+    if (nxt <= n && a[nxt - 1] < key) pos = nxt;
+  }
+  return pos < n && a[pos] == key;
+}
+
+// Open-addressing hash over int32 keys >= 0; empty slot = -1; table size
+// 2^bits with bits chosen so the load factor stays <= 1/2.
+__device__ __forceinline__ uint32_t hash_slot(int key, int bits) {
+  return ((uint32_t)key * 0x9E3779B1u) >> (32 - bits);
+}
+
+__device__ __forceinline__ void hash_insert(int *tab, int bits, int key) {
+  const uint32_t mask = (1u << bits) - 1u;
+  uint32_t h = hash_slot(key, bits);
+  while (atomicCAS(&tab[h], -1, key) != -1) h = (h + 1) & mask;
+}
+
+__device__ __forceinline__ bool hash_contains(const int *tab, int bits, int key) {
+  const uint32_t mask = (1u << bits) - 1u;
+  uint32_t h = hash_slot(key, bits);
+  for (;;) {
+    int k = tab[h];
+    if (k == key) return true;
+    if (k < 0) return false;
+    h = (h + 1) & mask;
+  }
+}
+
+// Table bits for n keys: smallest b >= 6 with 2^b >= 2n.
+__device__ __forceinline__ int hash_bits_for(int n) {
+  int b = 6;
+  while ((1 << b) < 2 * n) ++b;
+  return b;
+}

@@ -1,0 +1,53 @@
+// Internal interface between the C-ABI layer (xgm_api.hip) and the kernel
+// translation units.  Not part of the public ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <functional>
+#include <string>
+#include <vector>
+
+typedef unsigned long long u64;
+
+namespace xgm {
+
+// Device-resident CSR (HBM): int64 rowptr[n+1], int32 colidx[nnz].
+struct DevCSR {
+  const int64_t *rowptr = nullptr;
+  const int32_t *colidx = nullptr;
+  int64_t n = 0, nnz = 0;
+};
+
+// A list of vertex tasks (int32 ids) resident in HBM, ordered by
+// decreasing estimated work so dynamic claiming approximates LPT.
+struct TaskList {
+  const int32_t *tasks = nullptr;
+  int64_t count = 0;
+};
+
+// Everything a launcher needs from its context.
+struct Launch {
+  hipStream_t stream = nullptr;
+  int num_cu = 256;
+  u64 *d_out = nullptr;   // accumulator slot the launch adds into (pre-zeroed)
+  int *d_queue = nullptr; // work counter the launch claims from (pre-zeroed)
+  // Bracket one kernel launch with events: begin(name, tasks) / end().
+  std::function<void(const char *, int64_t)> begin;
+  std::function<void()> end;
+};
+
+// Degree tiers of vertex tasks for the DAG patterns (out-degree d+).
+enum TcTier { TIER_W64 = 0, TIER_W256 = 1, TIER_B2048 = 2, TIER_BIG = 3, NUM_TIERS = 4 };
+constexpr int kTierMaxDeg[NUM_TIERS] = {64, 256, 2048, 1 << 30};
+
+// Triangle counting over a degree-oriented DAG.  strategy: 1 = LDS hash,
+// 2 = LDS binary search.  Adds the count into l.d_out.
+hipError_t launch_tc(const DevCSR &g, const TaskList tiers[NUM_TIERS], int strategy, Launch &l,
+                     int *d_queues /* NUM_TIERS counters */);
+
+// Small utility kernel: zero `n` 64-bit words (graph-capture safe, unlike memset nodes).
+hipError_t launch_zero(void *ptr, size_t n_u64, hipStream_t s);
+
+}  // namespace xgm
