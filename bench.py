@@ -1,0 +1,174 @@
+"""Benchmark: |E|/s per pattern on R-MAT scale 22 (edge factor 16,
+(a,b,c,d) = (0.57,0.19,0.19,0.05), seed 2, degree-oriented int32 CSR) on one
+MI355X.  A step is one full count of the pattern over the resident graph.
+
+Prints one JSON line (rank 0): metric/value/unit, ms_per_step, the roofline
+of the dominant kernel (algorithmic bytes / HIP-event duration vs 8 TB/s),
+and the reference CPU path timed on this host's cores (cpu_baseline).
+Progress goes to stderr."""
+import argparse
+import json
+import os
+import subprocess
+import sys
+import tempfile
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "tests"))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+PATTERN_IDS = {"triangle": 1, "clique4": 2, "clique5": 3}
+# tier bounds of the triangle kernels (out-degree), mirrored from kernels.h
+TC_TIERS = [("tc_wave_64", 2, 64), ("tc_wave_256", 65, 256), ("tc_block_2048", 257, 2048),
+            ("tc_block_big", 2049, 1 << 40)]
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def tc_alg_bytes_per_tier(d):
+    """Algorithmic bytes of triangle counting per vertex task u:
+    sum over v in N+(u) of 4*(d+(u) + d+(v)) + 40 (8 B src/dst + 32 B rowptr)."""
+    dout = np.diff(d.rowptr).astype(np.int64)
+    nz = dout > 0
+    L = np.zeros(d.n, np.int64)
+    if d.nnz:
+        sums = np.add.reduceat(dout[d.colidx], d.rowptr[:-1][nz])
+        L[nz] = sums
+    per_u = 4 * (dout * dout + L) + 40 * dout
+    out = {}
+    for name, lo, hi in TC_TIERS:
+        m = (dout >= lo) & (dout <= hi)
+        out[name] = int(per_u[m].sum())
+    return out, int(per_u.sum())
+
+
+def cpu_baseline(g, golden_count, threads):
+    import oracle
+    if not oracle.ref_available("triangle"):
+        return None
+    log(f"cpu baseline: reference GraphMiner tc_omp_base, {threads} threads")
+    c, t = oracle.ref_count_graph("triangle", g, threads=threads, timeout=600)
+    return dict(value=g.nnz // 2 / t, unit="|E|/s", cores=threads, kind="reference",
+                sample="full R-MAT-22 seed 2 triangle count (counting loop only)",
+                seconds=t, count=c, count_matches_gpu=(c == golden_count))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--pattern", default="triangle", choices=list(PATTERN_IDS))
+    ap.add_argument("--scale", type=int, default=22)
+    ap.add_argument("--seed", type=int, default=2)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1",
+                          "n_gpus": args.gpus}))
+        sys.exit(2)
+
+    from conftest import load_xgm
+    xgm = load_xgm()
+    t0 = time.time()
+    g = xgm.gen_rmat(args.scale, 16, 0.57, 0.19, 0.19, args.seed)
+    d = g.orient()
+    m = d.nnz
+    log(f"graph R-MAT-{args.scale} seed {args.seed}: n={g.n} m={m} hash={g.hash():016x} "
+        f"({time.time()-t0:.1f}s)")
+    gold = json.load(open(os.path.join(ROOT, "tests", "golden", "counts.json")))
+    gname = f"rmat{args.scale}_s{args.seed}"
+    golden = None
+    if gname in gold and gold[gname]["hash"] == f"{g.hash():016x}":
+        golden = gold[gname]["counts"].get(args.pattern)
+
+    try:
+        import torch
+        sync = torch.cuda.synchronize if torch.cuda.is_available() else (lambda: None)
+    except Exception:  # torch is plumbing only
+        sync = lambda: None
+
+    ctx = xgm.Context(0)
+    t0 = time.time()
+    dg = ctx.upload(d)
+    log(f"upload {time.time()-t0:.2f}s")
+    pid = PATTERN_IDS[args.pattern]
+    count = None
+    for i in range(args.warmup):
+        count = ctx.count(dg, pid)
+        log(f"warmup {i}: count={count}")
+    ktimes = {}
+    totals = []
+    sync()
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        c = ctx.count(dg, pid)
+        total, ks = ctx.last_profile()
+        totals.append(total)
+        for k in ks:
+            ktimes.setdefault(k.name, []).append(k.ms)
+        if count is not None and c != count:
+            raise SystemExit(f"non-deterministic count: {c} vs {count}")
+        count = c
+    sync()
+    wall = time.perf_counter() - t0
+    ms_per_step = wall * 1e3 / args.steps
+    log(f"count={count} golden={golden} ms/step={ms_per_step:.3f} kernel total ms={np.mean(totals):.3f}")
+    if golden is not None and count != golden:
+        raise SystemExit(f"COUNT MISMATCH vs golden: {count} != {golden}")
+
+    roof = None
+    extra = {}
+    if args.pattern == "triangle":
+        per_tier, btot = tc_alg_bytes_per_tier(d)
+        avg = {k: float(np.mean(v)) for k, v in ktimes.items()}
+        dom = max(avg, key=avg.get)
+        ach = per_tier[dom] / (avg[dom] * 1e-3) / 1e9
+        roof = dict(bound="hbm", achieved=round(ach, 1), peak=HBM_PEAK_GBS, unit="GB/s",
+                    frac=round(ach / HBM_PEAK_GBS, 4), traffic=None, kernel=dom,
+                    alg_bytes_per_launch=per_tier[dom], avg_launch_ms=round(avg[dom], 4))
+        pat_ach = btot / (float(np.mean(totals)) * 1e-3) / 1e9
+        extra["pattern_roofline"] = dict(alg_bytes=btot, kernel_ms=round(float(np.mean(totals)), 4),
+                                         achieved=round(pat_ach, 1), frac=round(pat_ach / HBM_PEAK_GBS, 4))
+        extra["kernels_ms"] = {k: round(v, 4) for k, v in avg.items()}
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(g, count, args.cpu_threads) if args.pattern == "triangle" else None
+
+    out = {
+        "metric": f"|E|/s on R-MAT-{args.scale} ({args.pattern})",
+        "value": m / (ms_per_step * 1e-3),
+        "unit": "|E|/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int32",
+        "data": "synthetic",
+        "config": {"workload": f"{args.pattern} count on R-MAT scale {args.scale}, edge factor 16, "
+                               f"(0.57,0.19,0.19,0.05), seed {args.seed}, degree-oriented DAG",
+                   "n": g.n, "dag_edges": m, "graph_hash": f"{g.hash():016x}", "parallelism": "single GPU"},
+        "count": count,
+        "count_matches_golden": (golden == count) if golden is not None else None,
+        "matches_per_s": count / (ms_per_step * 1e-3),
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    out.update(extra)
+    print(json.dumps(out))
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,101 @@
+"""Host graph substrate of libxgm (no GPU): generators, cleaning,
+orientation, on-disk format, hashing."""
+import json
+import os
+
+import numpy as np
+
+from conftest import CITESEER, GOLDEN, random_graph
+
+
+def _check_clean(g, xgm):
+    assert g.flags & xgm.CSR_SORTED and g.flags & xgm.CSR_SYMMETRIC
+    assert g.validate()
+    for u in range(min(g.n, 2000)):
+        lst = g.colidx[g.rowptr[u]:g.rowptr[u + 1]]
+        assert np.all(np.diff(lst) > 0)
+        assert not np.any(lst == u)
+
+
+def test_rmat_deterministic_and_clean(xgm):
+    a = xgm.gen_rmat(12, 16, 0.57, 0.19, 0.19, 3)
+    b = xgm.gen_rmat(12, 16, 0.57, 0.19, 0.19, 3)
+    c = xgm.gen_rmat(12, 16, 0.57, 0.19, 0.19, 4)
+    assert a.hash() == b.hash() != c.hash()
+    _check_clean(a, xgm)
+    gold = json.load(open(os.path.join(GOLDEN, "counts.json")))
+    assert f"{a.hash():016x}" == gold["rmat12_s3"]["hash"]
+
+
+def test_rmat_thread_count_independent(xgm, monkeypatch):
+    # the library reads XGM_NUM_THREADS at call time
+    monkeypatch.setenv("XGM_NUM_THREADS", "1")
+    h1 = xgm.gen_rmat(11, 16, 0.57, 0.19, 0.19, 9).hash()
+    monkeypatch.setenv("XGM_NUM_THREADS", "7")
+    h7 = xgm.gen_rmat(11, 16, 0.57, 0.19, 0.19, 9).hash()
+    assert h1 == h7
+
+
+def test_rmat_skew(xgm):
+    g = xgm.gen_rmat(14, 16, 0.57, 0.19, 0.19, 1)
+    deg = g.degree()
+    # vertex 0 receives quadrant a at every level: the heaviest vertex
+    assert deg[0] == deg.max()
+    assert g.nnz <= 2 * 16 * (1 << 14)
+
+
+def test_chung_lu_and_er(xgm):
+    g = xgm.gen_chung_lu(50000, 20.0, 2.1, 2000.0, 3)
+    _check_clean(g, xgm)
+    deg = g.degree()
+    assert 15 < deg.mean() <= 20.0 + 1e-9
+    assert deg.max() <= 3 * 2000
+    assert deg[:10].mean() > 10 * deg[-1000:].mean()  # weights decrease with id
+    e = xgm.gen_erdos_renyi(10000, 8.0, 5)
+    _check_clean(e, xgm)
+    assert abs(e.degree().mean() - 8.0) < 0.2
+
+
+def test_build_undirected_dedups(xgm):
+    g = xgm.from_edges(5, [0, 1, 0, 2, 3, 3], [1, 0, 1, 2, 4, 4])
+    assert g.nnz == 4  # {0-1, 3-4}, both directions; self-loop 2-2 dropped
+    assert list(g.colidx) == [1, 0, 4, 3]
+
+
+def test_orientation_invariants(xgm):
+    g = random_graph(xgm, 1000, 20000, seed=8)
+    d = g.orient()
+    assert d.nnz == g.nnz // 2
+    assert d.flags & xgm.CSR_DAG
+    deg = g.degree()
+    src = np.repeat(np.arange(d.n), d.degree())
+    dst = d.colidx
+    assert np.all((deg[dst] > deg[src]) | ((deg[dst] == deg[src]) & (dst > src)))
+    # every undirected edge exactly once
+    und = set(zip(np.minimum(src, dst).tolist(), np.maximum(src, dst).tolist()))
+    assert len(und) == d.nnz
+
+
+def test_io_roundtrip(xgm, tmp_path):
+    g = random_graph(xgm, 700, 5000, seed=4)
+    p = str(tmp_path / "graph")
+    g.save(p)
+    h = xgm.load_csr(p)
+    assert h.hash() == g.hash()
+    meta = open(p + ".meta.txt").read().split()
+    assert int(meta[0]) == g.n and int(meta[1]) == g.nnz and meta[2:6] == ["4", "8", "1", "2"]
+
+
+def test_load_reference_fixture(xgm):
+    g = xgm.load_csr(CITESEER)
+    assert g.n == 3312 and g.nnz == 9072 and int(g.degree().max()) == 99
+    assert g.validate()
+
+
+def test_validate_rejects(xgm):
+    bad = xgm.Graph.from_arrays(np.array([0, 2, 3], np.int64), np.array([1, 1, 0], np.int32),
+                                xgm.CSR_SORTED | xgm.CSR_SYMMETRIC)
+    assert not bad.validate()  # duplicate in list 0
+    asym = xgm.Graph.from_arrays(np.array([0, 1, 1], np.int64), np.array([1], np.int32),
+                                 xgm.CSR_SORTED | xgm.CSR_SYMMETRIC)
+    assert not asym.validate()

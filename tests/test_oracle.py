@@ -1,0 +1,192 @@
+"""Pins the CPU oracle (oracle/oracle.cpp) before it is trusted as the checker.
+
+* set primitives: brute force with numpy set routines on random sorted lists
+  and the edge cases the reference's GPU code special-cases (empty operands,
+  bounds at or below the minimum, disjoint and identical lists);
+* orientation: the reference rule (codes/GraphMiner/src/common/graph.cc:246-247)
+  checked edge by edge, and equality with the library's orientation;
+* patterns: reference known answers on citeseer
+  (codes/GraphMiner/src/triangle/README.md:53, src/motif/README.md:57,
+  src/sgl/README.md:53), closed forms on K_n and K_{a,b}, brute force over
+  vertex subsets on tiny graphs, the identities between patterns, and the
+  counts the reference's own binaries wrote into tests/golden/counts.json.
+"""
+import itertools
+import json
+import math
+import os
+
+import numpy as np
+import pytest
+
+from conftest import CITESEER, GOLDEN, complete_bipartite, complete_graph, random_graph
+
+# ------------------------------------------------------------------ set ops
+
+
+def _rand_sorted(rng, n, hi):
+    return np.unique(rng.integers(0, hi, n)).astype(np.int32)
+
+
+def test_intersect_bruteforce(orc):
+    rng = np.random.default_rng(0)
+    for _ in range(400):
+        a = _rand_sorted(rng, rng.integers(0, 60), 100)
+        b = _rand_sorted(rng, rng.integers(0, 60), 100)
+        ref = np.intersect1d(a, b)
+        assert orc.intersect_num(a, b) == len(ref)
+        assert np.array_equal(orc.intersect_set(a, b), ref)
+        up = int(rng.integers(-5, 110))
+        assert orc.intersect_num_bound(a, b, up) == int((ref < up).sum())
+
+
+def test_intersect_edge_cases(orc):
+    e = np.zeros(0, np.int32)
+    a = np.array([1, 5, 9], np.int32)
+    assert orc.intersect_num(e, a) == 0 and orc.intersect_num(a, e) == 0
+    assert orc.intersect_num(a, a) == 3
+    assert orc.intersect_num(a, np.array([2, 6], np.int32)) == 0
+    assert orc.intersect_num_bound(a, a, 1) == 0   # bound at the minimum
+    assert orc.intersect_num_bound(a, a, 0) == 0   # bound below the minimum
+    assert orc.intersect_num_bound(a, a, 10) == 3
+
+
+def test_difference_semantics(orc):
+    """GraphMiner excludes the owner vid(B) (src/common/VertexSet.cc:69-88);
+    GLUMIN's GPU difference does not (set_difference.cuh:20-35)."""
+    rng = np.random.default_rng(1)
+    for _ in range(400):
+        a = _rand_sorted(rng, rng.integers(0, 50), 80)
+        b = _rand_sorted(rng, rng.integers(0, 50), 80)
+        up = int(rng.integers(0, 90))
+        vid = int(rng.integers(0, 80))
+        diff = np.setdiff1d(a, b)
+        assert orc.difference_num_noexcl(a, b, up) == int((diff < up).sum())
+        # the GraphMiner merge stops at the first element of B >= up, which
+        # does not change the set below up
+        assert orc.difference_num(a, b, up, vid) == int(((diff < up) & (diff != vid)).sum())
+    e = np.zeros(0, np.int32)
+    assert orc.difference_num(np.array([3, 4], np.int32), e, 100, 4) == 1
+    assert orc.difference_num_noexcl(e, np.array([1], np.int32), 100) == 0
+
+
+# -------------------------------------------------------------- orientation
+
+
+def test_orientation_rule(xgm, orc):
+    g = random_graph(xgm, 300, 3000, seed=3)
+    deg = g.degree()
+    rp, ci = orc.orient(g.rowptr, g.colidx)
+    assert rp[-1] == g.nnz // 2
+    for u in range(g.n):
+        out = ci[rp[u]:rp[u + 1]]
+        assert np.all(np.diff(out) > 0)
+        nbr = g.colidx[g.rowptr[u]:g.rowptr[u + 1]]
+        keep = [v for v in nbr if deg[v] > deg[u] or (deg[v] == deg[u] and v > u)]
+        assert list(out) == keep
+    d = g.orient()
+    assert np.array_equal(d.rowptr, rp) and np.array_equal(d.colidx, ci)
+
+
+# ------------------------------------------------------------------ patterns
+
+
+def _counts(orc, g):
+    d = g.orient()
+    out = {}
+    for p in ("triangle", "clique4", "clique5"):
+        out[p] = orc.count(p, d.rowptr, d.colidx)[0]
+    for p in ("diamond", "rectangle"):
+        out[p] = orc.count(p, g.rowptr, g.colidx)[0]
+    return out
+
+
+def test_citeseer_known_answers(xgm, orc):
+    g = xgm.load_csr(CITESEER)
+    c = _counts(orc, g)
+    assert c["triangle"] == 1166      # triangle/README.md:53
+    assert c["clique4"] == 255        # motif/README.md:57
+    assert c["diamond"] == 3730       # sgl/README.md:53
+    assert c["rectangle"] == 6059     # sgl/README.md:53
+    assert c["clique5"] == 46         # networkx brute force (no reference file publishes it)
+    # identities from motif/README.md:53-57: edge-induced = induced + multiples of K4
+    assert c["diamond"] == 2200 + 6 * c["clique4"]
+    assert c["rectangle"] == 3094 + 2200 + 3 * c["clique4"]
+
+
+@pytest.mark.parametrize("n", [4, 5, 7, 12])
+def test_complete_graph_closed_forms(xgm, orc, n):
+    c = _counts(orc, complete_graph(xgm, n))
+    assert c["triangle"] == math.comb(n, 3)
+    assert c["clique4"] == math.comb(n, 4)
+    assert c["clique5"] == math.comb(n, 5)
+    assert c["diamond"] == 6 * math.comb(n, 4)
+    assert c["rectangle"] == 3 * math.comb(n, 4)
+
+
+@pytest.mark.parametrize("a,b", [(2, 2), (3, 5), (6, 4)])
+def test_bipartite_closed_forms(xgm, orc, a, b):
+    c = _counts(orc, complete_bipartite(xgm, a, b))
+    assert c["triangle"] == 0 and c["clique4"] == 0 and c["diamond"] == 0
+    assert c["rectangle"] == math.comb(a, 2) * math.comb(b, 2)
+
+
+def _brute(g):
+    n = g.n
+    adj = [set(g.colidx[g.rowptr[u]:g.rowptr[u + 1]].tolist()) for u in range(n)]
+    E = lambda x, y: y in adj[x]
+    tri = sum(1 for a, b, c in itertools.combinations(range(n), 3) if E(a, b) and E(a, c) and E(b, c))
+    k4 = k5 = dia = cyc = 0
+    for q in itertools.combinations(range(n), 4):
+        es = [E(x, y) for x, y in itertools.combinations(q, 2)]
+        ne = sum(es)
+        k4 += ne == 6
+        # edge-induced copies inside a 4-set: diamonds = #(5-edge subgraphs
+        # with the missing edge's endpoints...) counted directly
+        a, b, c, d = q
+        # 4-cycles on this vertex set: three possible cyclic orders
+        for x, y, z, w in ((a, b, c, d), (a, b, d, c), (a, c, b, d)):
+            cyc += E(x, y) and E(y, z) and E(z, w) and E(w, x)
+        # diamonds: choose the missing (non-required) pair; remaining 5 pairs must be edges
+        pairs = list(itertools.combinations(q, 2))
+        for skip in range(6):
+            dia += all(E(*pairs[i]) for i in range(6) if i != skip)
+    for q in itertools.combinations(range(n), 5):
+        k5 += all(E(x, y) for x, y in itertools.combinations(q, 2))
+    return dict(triangle=tri, clique4=k4, clique5=k5, diamond=dia, rectangle=cyc)
+
+
+@pytest.mark.parametrize("seed", [11, 12, 13])
+def test_bruteforce_tiny(xgm, orc, seed):
+    g = random_graph(xgm, 16, 60, seed)
+    assert _counts(orc, g) == _brute(g)
+
+
+def test_golden_reference_counts(xgm, orc):
+    """Counts written by the reference binaries (oracle/_ref) for seeded graphs."""
+    from golden.make_golden import SMALL, make_graph
+    gold = json.load(open(os.path.join(GOLDEN, "counts.json")))
+    for name in ("rmat10_s7", "rmat12_s3", "er2000_d16_s5", "chunglu20k_s9"):
+        kind, kw, _ = SMALL[name]
+        g = make_graph(xgm, kind, kw)
+        assert f"{g.hash():016x}" == gold[name]["hash"], name
+        assert _counts(orc, g) == gold[name]["counts"], name
+
+
+def test_algorithmic_bytes_tc_closed_form(xgm, orc):
+    """B_TC = 4*sum_v d+(v)*(d+(v)+d-(v)) + 40*m on the DAG."""
+    g = random_graph(xgm, 500, 5000, seed=5)
+    d = g.orient()
+    _, B = orc.count("triangle", d.rowptr, d.colidx)
+    dout = d.degree().astype(np.int64)
+    din = np.bincount(d.colidx, minlength=d.n).astype(np.int64)
+    assert B == int(4 * (dout * dout).sum() + 4 * (din * dout).sum() + 40 * d.nnz)
+
+
+@pytest.mark.skipif(not os.path.exists(os.path.join(os.path.dirname(__file__), "..", "oracle", "_ref",
+                                                     "tc_omp_base")), reason="oracle/_ref not built")
+def test_restatement_matches_reference_binaries(xgm, orc):
+    g = random_graph(xgm, 400, 6000, seed=21)
+    mine = _counts(orc, g)
+    for p in mine:
+        assert orc.ref_count_graph(p, g)[0] == mine[p], p
