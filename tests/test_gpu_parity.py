@@ -90,3 +90,73 @@ def test_profile_reports_kernels(xgm, ctx):
     ctx.count(d, xgm.TRIANGLE)
     total, ks = ctx.last_profile()
     assert ks and all(k.ms >= 0 for k in ks) and total >= 0
+
+
+# ---------------------------------------------------------------- k-cliques
+
+CLQ = {"clique4": (2, 4), "clique5": (3, 5)}
+CLQ_STRATS = {"auto": 0, "lut": 3, "list": 4}
+
+
+@pytest.mark.parametrize("n", [4, 6, 40, 100, 300, 600])
+def test_clique_complete_lut_tiers(xgm, ctx, n):
+    # out-degrees n-1 exercise the 64 / 256 / 512 / 1024-row LUT tiers
+    _say("clique complete", n)
+    d = complete_graph(xgm, n).orient()
+    dg = ctx.upload(d)
+    assert ctx.count(dg, 2) == math.comb(n, 4)
+    assert ctx.count(dg, 3) == math.comb(n, 5)
+    dg.close()
+
+
+def test_clique4_list_above_lut_threshold(xgm, ctx):
+    # d+ up to 1039 > 1024: those vertices take the list path under auto
+    _say("clique4 complete 1040")
+    d = complete_graph(xgm, 1040).orient()
+    assert ctx.count(d, 2) == math.comb(1040, 4)
+
+
+@pytest.mark.parametrize("strat", list(CLQ_STRATS))
+@pytest.mark.parametrize("seed", [4, 5])
+def test_clique_random_vs_oracle(xgm, orc, ctx, strat, seed):
+    _say("clique random", strat, seed)
+    g = random_graph(xgm, 2000, 40000, seed).orient()
+    dg = ctx.upload(g)
+    for name, (pid, _) in CLQ.items():
+        want = orc.count(name, g.rowptr, g.colidx)[0]
+        assert ctx.count(dg, pid, CLQ_STRATS[strat]) == want, name
+    dg.close()
+
+
+def test_clique_lut_threshold_option(xgm, ctx):
+    _say("clique lut_max_degree sweep")
+    g = _gold()["rmat12_s3"]
+    from golden.make_golden import SMALL, make_graph
+    d = make_graph(xgm, SMALL["rmat12_s3"][0], SMALL["rmat12_s3"][1]).orient()
+    dg = ctx.upload(d)
+    for lm in (16, 64, 200, 1024):
+        assert ctx.count(dg, 2, 0, lm) == g["counts"]["clique4"]
+        assert ctx.count(dg, 3, 0, lm) == g["counts"]["clique5"]
+    dg.close()
+
+
+@pytest.mark.parametrize("name", ["rmat10_s7", "rmat12_s3", "er2000_d16_s5", "chunglu20k_s9", "rmat16_s1"])
+def test_clique_golden(xgm, ctx, name):
+    from golden.make_golden import SMALL, make_graph
+    _say("clique golden", name)
+    gold = _gold()[name]
+    d = make_graph(xgm, SMALL[name][0], SMALL[name][1]).orient()
+    dg = ctx.upload(d)
+    for pname, (pid, _) in CLQ.items():
+        assert ctx.count(dg, pid) == gold["counts"][pname], pname
+    if name != "rmat16_s1":
+        for pname, (pid, _) in CLQ.items():
+            assert ctx.count(dg, pid, 4) == gold["counts"][pname], pname + " list"
+    dg.close()
+
+
+def test_clique_citeseer(xgm, ctx):
+    _say("clique citeseer")
+    d = xgm.load_csr(CITESEER).orient()
+    assert xgm.CliqueSolver(d, 4) == 255
+    assert xgm.CliqueSolver(d, 5) == 46

@@ -47,6 +47,19 @@ constexpr int kTierMaxDeg[NUM_TIERS] = {64, 256, 2048, 1 << 30};
 hipError_t launch_tc(const DevCSR &g, const TaskList tiers[NUM_TIERS], int strategy, Launch &l,
                      int *d_queues /* NUM_TIERS counters */);
 
+// Vertex tasks of a k-clique count: LUT tiers by d+(v0) (<=64, <=256,
+// <=512, <=lut_max) and the list-based remainder.
+struct CliqueTasks {
+  TaskList lut[4];
+  TaskList list;
+};
+
+// k-clique counting (k = 4 or 5) over a DAG.  scratch holds
+// clique_list_grid() * 2 * md ints (md = max out-degree).  Adds into l.d_out.
+hipError_t launch_clique(const DevCSR &g, const CliqueTasks &ct, int k, int *scratch, int md, Launch &l,
+                         int *d_queues /* 5 counters */);
+int clique_list_grid(int num_cu, int64_t tasks);
+
 // Small utility kernel: zero `n` 64-bit words (graph-capture safe, unlike memset nodes).
 hipError_t launch_zero(void *ptr, size_t n_u64, hipStream_t s);
 

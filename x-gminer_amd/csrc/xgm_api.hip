@@ -7,6 +7,7 @@
 #include <cstring>
 #include <memory>
 #include <new>
+#include <map>
 #include <numeric>
 #include <string>
 #include <vector>
@@ -51,9 +52,20 @@ struct xgm_dgraph {
   uint32_t flags = 0;
   int64_t *d_rowptr = nullptr;
   int32_t *d_colidx = nullptr;
-  // DAG vertex-task tiers by out-degree (see TcTier)
-  int32_t *d_tasks = nullptr;
-  TaskList tiers[NUM_TIERS];
+  int max_out = 0;
+  // DAG vertex tasks (d+ >= 2) by decreasing work estimate, and d+ per vertex
+  std::vector<int32_t> order;
+  std::vector<int32_t> dplus;
+  // device task lists, cached per key (pattern family / strategy / threshold)
+  std::map<std::string, std::pair<int32_t *, std::vector<int64_t>>> task_cache;
+  int *d_scratch = nullptr;
+  size_t scratch_ints = 0;
+  ~xgm_dgraph() {
+    for (auto &kv : task_cache) (void)hipFree(kv.second.first);
+    (void)hipFree(d_rowptr);
+    (void)hipFree(d_colidx);
+    (void)hipFree(d_scratch);
+  }
 };
 
 namespace {
@@ -86,15 +98,17 @@ Launch make_launch(xgm_ctx *ctx, int slot) {
   return l;
 }
 
-// Vertex tasks of a DAG, split into out-degree tiers, each sorted by
-// decreasing work estimate sum_{v in N+(u)} d+(v) (ties by id).
-void build_dag_tiers(const xgm_csr &g, std::vector<int32_t> &all, int64_t counts[NUM_TIERS]) {
+// DAG vertex tasks with d+ >= 2, ordered by decreasing work estimate
+// sum_{v in N+(u)} d+(v) (ties by id): dynamic claiming then approximates
+// longest-processing-time-first inside every tier.
+void build_dag_order(const xgm_csr &g, std::vector<int32_t> &order, std::vector<int32_t> &dplus, int &max_out) {
   const int64_t n = g.n;
   std::vector<int64_t> work(n, 0);
-  std::vector<int8_t> tier(n, -1);
+  dplus.assign(n, 0);
   parallel_for(n, 4096, [&](int64_t b, int64_t e) {
     for (int64_t u = b; u < e; ++u) {
       int64_t du = g.rowptr[u + 1] - g.rowptr[u];
+      dplus[u] = (int32_t)du;
       if (du < 2) continue;
       int64_t w = 0;
       for (int64_t k = g.rowptr[u]; k < g.rowptr[u + 1]; ++k) {
@@ -102,21 +116,68 @@ void build_dag_tiers(const xgm_csr &g, std::vector<int32_t> &all, int64_t counts
         w += g.rowptr[v + 1] - g.rowptr[v];
       }
       work[u] = w;
-      int t = 0;
-      while (t < NUM_TIERS - 1 && du > kTierMaxDeg[t]) ++t;
-      tier[u] = (int8_t)t;
     }
   });
-  std::vector<std::vector<int32_t>> lists(NUM_TIERS);
-  for (int64_t u = 0; u < n; ++u)
-    if (tier[u] >= 0) lists[tier[u]].push_back((int32_t)u);
-  all.clear();
-  for (int t = 0; t < NUM_TIERS; ++t) {
-    auto &L = lists[t];
-    std::stable_sort(L.begin(), L.end(), [&](int32_t a, int32_t b) { return work[a] > work[b]; });
-    counts[t] = (int64_t)L.size();
-    all.insert(all.end(), L.begin(), L.end());
+  order.clear();
+  max_out = 0;
+  for (int64_t u = 0; u < n; ++u) {
+    max_out = std::max(max_out, dplus[u]);
+    if (dplus[u] >= 2) order.push_back((int32_t)u);
   }
+  std::stable_sort(order.begin(), order.end(), [&](int32_t a, int32_t b) { return work[a] > work[b]; });
+}
+
+// Split `order` into consecutive degree ranges (lo_i, hi_i] (inclusive hi)
+// for vertices with d+ >= min_deg; cached on the device graph under `key`.
+int tasks_by_ranges(xgm_ctx *ctx, xgm_dgraph *g, const std::string &key, int min_deg,
+                    const std::vector<int> &hi, std::vector<TaskList> &out) {
+  auto it = g->task_cache.find(key);
+  if (it == g->task_cache.end()) {
+    std::vector<std::vector<int32_t>> parts(hi.size());
+    for (int32_t u : g->order) {
+      int d = g->dplus[u];
+      if (d < min_deg) continue;
+      size_t t = 0;
+      while (t + 1 < hi.size() && d > hi[t]) ++t;
+      parts[t].push_back(u);
+    }
+    std::vector<int32_t> all;
+    std::vector<int64_t> counts;
+    for (auto &p : parts) {
+      counts.push_back((int64_t)p.size());
+      all.insert(all.end(), p.begin(), p.end());
+    }
+    int32_t *d = nullptr;
+    HIP_TRY(ctx, hipMalloc(&d, std::max<size_t>(1, all.size()) * sizeof(int32_t)));
+    if (!all.empty()) {
+      hipError_t e = hipMemcpy(d, all.data(), all.size() * sizeof(int32_t), hipMemcpyHostToDevice);
+      if (e != hipSuccess) {
+        (void)hipFree(d);
+        return ctx->hip_fail(e, "H2D tasks");
+      }
+    }
+    it = g->task_cache.emplace(key, std::make_pair(d, counts)).first;
+  }
+  out.clear();
+  int64_t off = 0;
+  for (int64_t c : it->second.second) {
+    TaskList t;
+    t.tasks = it->second.first + off;
+    t.count = c;
+    out.push_back(t);
+    off += c;
+  }
+  return XGM_OK;
+}
+
+int ensure_scratch(xgm_ctx *ctx, xgm_dgraph *g, size_t ints) {
+  if (g->scratch_ints >= ints) return XGM_OK;
+  (void)hipFree(g->d_scratch);
+  g->d_scratch = nullptr;
+  g->scratch_ints = 0;
+  HIP_TRY(ctx, hipMalloc(&g->d_scratch, std::max<size_t>(1, ints) * sizeof(int)));
+  g->scratch_ints = ints;
+  return XGM_OK;
 }
 
 int check_device(xgm_ctx *ctx) {
@@ -179,12 +240,7 @@ int xgm_graph_upload(xgm_ctx *ctx, const xgm_csr *g, xgm_dgraph **out) {
   d->n = g->n;
   d->nnz = g->nnz;
   d->flags = g->flags;
-  auto cleanup = [&](int rc) {
-    hipFree(d->d_rowptr);
-    hipFree(d->d_colidx);
-    hipFree(d->d_tasks);
-    return rc;
-  };
+  auto cleanup = [&](int rc) { return rc; };  // ~xgm_dgraph frees the buffers
   hipError_t e;
   if ((e = hipMalloc(&d->d_rowptr, (g->n + 1) * sizeof(int64_t))) != hipSuccess)
     return cleanup(ctx->hip_fail(e, "hipMalloc(rowptr)"));
@@ -196,38 +252,16 @@ int xgm_graph_upload(xgm_ctx *ctx, const xgm_csr *g, xgm_dgraph **out) {
   if (g->nnz && (e = hipMemcpyAsync(d->d_colidx, g->colidx, g->nnz * sizeof(int32_t),
                                     hipMemcpyHostToDevice, ctx->stream)) != hipSuccess)
     return cleanup(ctx->hip_fail(e, "H2D colidx"));
-  if (g->flags & XGM_CSR_DAG) {
-    std::vector<int32_t> all;
-    int64_t counts[NUM_TIERS];
-    build_dag_tiers(*g, all, counts);
-    if ((e = hipMalloc(&d->d_tasks, std::max<size_t>(1, all.size()) * sizeof(int32_t))) != hipSuccess)
-      return cleanup(ctx->hip_fail(e, "hipMalloc(tasks)"));
-    if (!all.empty() && (e = hipMemcpyAsync(d->d_tasks, all.data(), all.size() * sizeof(int32_t),
-                                            hipMemcpyHostToDevice, ctx->stream)) != hipSuccess)
-      return cleanup(ctx->hip_fail(e, "H2D tasks"));
-    int64_t off = 0;
-    for (int t = 0; t < NUM_TIERS; ++t) {
-      d->tiers[t].tasks = d->d_tasks + off;
-      d->tiers[t].count = counts[t];
-      off += counts[t];
-    }
-    if ((e = hipStreamSynchronize(ctx->stream)) != hipSuccess)
-      return cleanup(ctx->hip_fail(e, "upload sync"));
-  } else {
-    if ((e = hipStreamSynchronize(ctx->stream)) != hipSuccess)
-      return cleanup(ctx->hip_fail(e, "upload sync"));
-  }
+  if (g->flags & XGM_CSR_DAG) build_dag_order(*g, d->order, d->dplus, d->max_out);
+  if ((e = hipStreamSynchronize(ctx->stream)) != hipSuccess) return cleanup(ctx->hip_fail(e, "upload sync"));
   *out = d.release();
   return XGM_OK;
 }
 
 void xgm_graph_free(xgm_dgraph *g) {
   if (!g) return;
-  hipSetDevice(g->ctx->device);
-  hipStreamSynchronize(g->ctx->stream);
-  hipFree(g->d_rowptr);
-  hipFree(g->d_colidx);
-  hipFree(g->d_tasks);
+  (void)hipSetDevice(g->ctx->device);
+  (void)hipStreamSynchronize(g->ctx->stream);
   delete g;
 }
 
@@ -247,13 +281,44 @@ int xgm_count(xgm_ctx *ctx, const xgm_dgraph *g, int pattern, const xgm_opts *op
   HIP_TRY(ctx, launch_zero(ctx->d_queues, kScratchQueues / 2, ctx->stream));
   Launch l = make_launch(ctx, 0);
   hipError_t e = hipSuccess;
+  xgm_dgraph *gm = const_cast<xgm_dgraph *>(g);
+  std::vector<TaskList> tl;
+  int rc = XGM_OK;
   switch (pattern) {
     case XGM_TRIANGLE: {
       if (!(g->flags & XGM_CSR_DAG)) return ctx->fail(XGM_EINVAL, "TRIANGLE needs a DAG-oriented CSR");
-      int s = (strategy == XGM_STRAT_BSEARCH) ? 2 : 1;
       if (strategy != XGM_STRAT_AUTO && strategy != XGM_STRAT_HASH && strategy != XGM_STRAT_BSEARCH)
         return ctx->fail(XGM_EUNSUPPORTED, "strategy not available for TRIANGLE");
-      e = launch_tc(dg, g->tiers, s, l, ctx->d_queues);
+      int s = (strategy == XGM_STRAT_BSEARCH) ? 2 : 1;
+      std::vector<int> hi(kTierMaxDeg, kTierMaxDeg + NUM_TIERS);
+      if ((rc = tasks_by_ranges(ctx, gm, "tc", 2, hi, tl))) return rc;
+      TaskList tiers[NUM_TIERS];
+      for (int t = 0; t < NUM_TIERS; ++t) tiers[t] = tl[t];
+      e = launch_tc(dg, tiers, s, l, ctx->d_queues);
+      break;
+    }
+    case XGM_CLIQUE4:
+    case XGM_CLIQUE5: {
+      if (!(g->flags & XGM_CSR_DAG)) return ctx->fail(XGM_EINVAL, "CLIQUE needs a DAG-oriented CSR");
+      if (strategy != XGM_STRAT_AUTO && strategy != XGM_STRAT_LUT && strategy != XGM_STRAT_LIST)
+        return ctx->fail(XGM_EUNSUPPORTED, "strategy not available for CLIQUE");
+      const int k = (pattern == XGM_CLIQUE4) ? 4 : 5;
+      int lut_max = (opts && opts->lut_max_degree > 0) ? std::min(opts->lut_max_degree, 1024) : 1024;
+      if (strategy == XGM_STRAT_LIST) lut_max = 0;
+      // ranges: (.,64] (64,256] (256,512] (512,lut_max] (lut_max, inf) -> list
+      std::vector<int> hi;
+      for (int b : {64, 256, 512, 1024}) hi.push_back(std::min(b, lut_max));
+      hi.push_back(INT32_MAX);
+      std::string key = "clique" + std::to_string(k) + "_" + std::to_string(lut_max);
+      if ((rc = tasks_by_ranges(ctx, gm, key, k - 1, hi, tl))) return rc;
+      CliqueTasks ct;
+      for (int t = 0; t < 4; ++t) ct.lut[t] = tl[t];
+      ct.list = tl[4];
+      if (ct.list.count) {
+        size_t need = (size_t)clique_list_grid(ctx->num_cu, ct.list.count) * 2 * std::max(1, g->max_out);
+        if ((rc = ensure_scratch(ctx, gm, need))) return rc;
+      }
+      e = launch_clique(dg, ct, k, gm->d_scratch, std::max(1, g->max_out), l, ctx->d_queues);
       break;
     }
     default:
