@@ -160,3 +160,78 @@ def test_clique_citeseer(xgm, ctx):
     d = xgm.load_csr(CITESEER).orient()
     assert xgm.CliqueSolver(d, 4) == 255
     assert xgm.CliqueSolver(d, 5) == 46
+
+
+# ------------------------------------------------- edge-induced diamond / 4-cycle
+
+SGL = {"diamond": 4, "rectangle": 5}
+
+
+@pytest.mark.parametrize("n", [4, 5, 30, 70, 1100])
+def test_sgl_complete(xgm, ctx, n):
+    # degrees n-1 cover the 64 / 1024 / 8192 diamond tiers
+    _say("sgl complete", n)
+    g = complete_graph(xgm, n)
+    dg = ctx.upload(g)
+    assert ctx.count(dg, 4) == 6 * math.comb(n, 4)
+    assert ctx.count(dg, 5) == 3 * math.comb(n, 4)
+    if n <= 70:
+        assert ctx.count(dg, 5, 4) == 3 * math.comb(n, 4)  # reference loop order (PAIR)
+    dg.close()
+
+
+@pytest.mark.parametrize("a,b", [(2, 2), (3, 7), (60, 200)])  # 60x200: dense-counter tier
+def test_sgl_bipartite(xgm, ctx, a, b):
+    from conftest import complete_bipartite
+    _say("sgl bipartite", a, b)
+    g = complete_bipartite(xgm, a, b)
+    assert ctx.count(g, 4) == 0
+    assert ctx.count(g, 5) == math.comb(a, 2) * math.comb(b, 2)
+    assert ctx.count(g, 5, 4) == math.comb(a, 2) * math.comb(b, 2)
+
+
+def test_sgl_hub_star_plus_clique(xgm, ctx, orc):
+    # one vertex of degree > 8192 (diamond tier with the set in HBM)
+    import numpy as np
+    _say("sgl hub")
+    n = 9000
+    src = np.concatenate([np.zeros(n - 1, np.int64), np.arange(1, 200)])
+    dst = np.concatenate([np.arange(1, n), np.arange(2, 201)])
+    g = xgm.from_edges(n, src, dst)
+    for name, pid in SGL.items():
+        want = orc.count(name, g.rowptr, g.colidx)[0]
+        assert ctx.count(g, pid) == want, name
+
+
+@pytest.mark.parametrize("seed", [6, 7])
+def test_sgl_random_vs_oracle(xgm, orc, ctx, seed):
+    _say("sgl random", seed)
+    g = random_graph(xgm, 3000, 40000, seed)
+    dg = ctx.upload(g)
+    for name, pid in SGL.items():
+        want = orc.count(name, g.rowptr, g.colidx)[0]
+        assert ctx.count(dg, pid) == want, name
+        assert ctx.count(dg, pid) == want, name + " (repeat: dense counters left zero)"
+    assert ctx.count(dg, 5, 4) == orc.count("rectangle", g.rowptr, g.colidx)[0]
+    dg.close()
+
+
+@pytest.mark.parametrize("name", ["rmat10_s7", "rmat12_s3", "er2000_d16_s5", "chunglu20k_s9", "rmat16_s1"])
+def test_sgl_golden(xgm, ctx, name):
+    from golden.make_golden import SMALL, make_graph
+    _say("sgl golden", name)
+    gold = _gold()[name]
+    g = make_graph(xgm, SMALL[name][0], SMALL[name][1])
+    dg = ctx.upload(g)
+    for pname, pid in SGL.items():
+        assert ctx.count(dg, pid) == gold["counts"][pname], pname
+    if name in ("rmat10_s7", "er2000_d16_s5", "chunglu20k_s9"):
+        assert ctx.count(dg, 5, 4) == gold["counts"]["rectangle"], "pair"
+    dg.close()
+
+
+def test_sgl_citeseer(xgm, ctx):
+    _say("sgl citeseer")
+    g = xgm.load_csr(CITESEER)
+    assert xgm.SglSolver(g, "diamond") == 3730
+    assert xgm.SglSolver(g, "rectangle") == 6059

@@ -1,0 +1,330 @@
+// Edge-induced diamond and 4-cycle counting on the undirected graph, gfx950.
+//
+// Diamond -- reference: sum over edges v1 < v0 of C(|N(v0) ∩ N(v1)|, 2)
+// (codes/GraphMiner/src/sgl/cpu_kernels/diamond.h:1-14; GPU
+// codes/GraphMiner/src/sgl/gpu_kernels/diamond_count.cuh:1-20).  Here every
+// undirected edge is owned by its endpoint t that is larger in (degree, id)
+// order; N(t) is staged once (LDS hash, or HBM binary search for huge
+// lists) and the lists of all partners p < t are probed against it, with a
+// per-partner hit count reduced per window and turned into C(c, 2).
+//
+// 4-cycle -- reference: for v1 < v0 and v2 < v1 in N(v0), count
+// |N(v1) ∩ N(v2) ∩ [0, v0)| (codes/GraphMiner/src/sgl/cpu_kernels/rectangle.h:1-11;
+// GPU rectangle_nested.cuh:1-24).  Two strategies with identical counts:
+//  * PAIR: that bounded-intersection loop itself, block per v0.
+//  * WEDGE (default): every 4-cycle has one highest vertex x in any total
+//    order and sum_x sum_{w < x} C(c_x(w), 2) counts each once, where
+//    c_x(w) = |{v in N(x): v < x, w in N(v)}|.  On a copy of the graph
+//    relabelled by (degree, id) rank, the lists N(v) ∩ [0, x) are prefixes,
+//    and c_x(w) is aggregated in an LDS hash (or per-block HBM counters for
+//    the few hub tasks); each increment from c to c+1 adds c to the count.
+#include <climits>
+
+#include "device_common.h"
+#include "kernels.h"
+#include "segments.h"
+
+namespace xgm {
+namespace {
+
+// ------------------------------------------------------------------ diamond
+template <int HCAP, int PCAP>
+struct DiaStage {
+  int key[2 * HCAP];
+  Segs<PCAP> seg;
+  int cnt[PCAP];
+  int ns, L, task;
+};
+
+template <int HCAP, int PCAP, int NT, bool LDS_SET>
+__global__ __launch_bounds__(NT) void k_diamond(const int64_t *__restrict__ rowptr,
+                                                const int32_t *__restrict__ colidx,
+                                                const int32_t *__restrict__ tasks, int ntasks,
+                                                int *__restrict__ queue, u64 *__restrict__ out) {
+  __shared__ DiaStage<HCAP, PCAP> S;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  u64 acc = 0;
+  for (;;) {
+    if (tid == 0) S.task = atomicAdd(queue, 1);
+    __syncthreads();
+    const int t = S.task;
+    if (t >= ntasks) break;
+    const int v = tasks[t];
+    const int64_t bt = rowptr[v];
+    const int dt = (int)(rowptr[v + 1] - bt);
+    const int32_t *nt = colidx + bt;
+    int hb = 0;
+    if (LDS_SET) {
+      hb = hash_bits_for(dt);
+      for (int q = tid; q < (1 << hb); q += NT) S.key[q] = -1;
+      __syncthreads();
+      for (int q = tid; q < dt; q += NT) hash_insert(S.key, hb, nt[q]);
+    }
+    // partner p owned by v iff (deg p, p) < (deg v, v)
+    auto keep = [dt, v](int p, int lp) { return lp < dt || (lp == dt && p < v); };
+    for (int k0 = 0; k0 < dt; k0 += PCAP) {
+      const int k1 = min(dt, k0 + PCAP);
+      __syncthreads();
+      if (wave == 0) {
+        int ns, L;
+        wave_segments(S.seg, rowptr, colidx, nt, k0, k1, -1, keep, lane, ns, L);
+        if (lane == 0) { S.ns = ns; S.L = L; }
+      }
+      __syncthreads();
+      const int ns = S.ns, L = S.L;
+      for (int q = tid; q < ns; q += NT) S.cnt[q] = 0;
+      __syncthreads();
+      for (int base = wave * 64; base < L; base += NT) {
+        WindowLane r = window_lane(S.seg, colidx, ns, L, base, lane);
+        bool hit = false;
+        if (r.valid) hit = LDS_SET ? hash_contains(S.key, hb, r.w) : sorted_contains(nt, dt, r.w);
+        const u64 hits = __ballot(hit);
+        if (r.valid) {
+          const int h = segment_run_hits(hits, r.heads, base + lane + 1 < L, lane);
+          if (h) atomicAdd(&S.cnt[r.j], h);
+        }
+      }
+      __syncthreads();
+      for (int q = tid; q < ns; q += NT) {
+        const u64 c = (u64)S.cnt[q];
+        acc += c * (c - 1) / 2;  // c = 0 gives 0 in modular arithmetic
+      }
+    }
+    __syncthreads();
+  }
+  acc = wave_sum_u64(acc);
+  if (lane == 0 && acc) atomicAdd(out, acc);
+}
+
+// ------------------------------------------------------------ 4-cycle WEDGE
+template <int HSLOTS, int PCAP>
+struct WedgeStage {
+  int key[HSLOTS];
+  int val[HSLOTS];
+  Segs<PCAP> seg;
+  int ns, L, task;
+};
+
+// insert-or-increment; returns the count before this increment
+__device__ __forceinline__ int hash_bump(int *key, int *val, int bits, int w) {
+  const uint32_t mask = (1u << bits) - 1u;
+  uint32_t h = hash_slot(w, bits);
+  for (;;) {
+    int k = key[h];
+    if (k == -1) {
+      int prev = atomicCAS(&key[h], -1, w);
+      if (prev == -1 || prev == w) return atomicAdd(&val[h], 1);
+      k = prev;
+    }
+    if (k == w) return atomicAdd(&val[h], 1);
+    h = (h + 1) & mask;
+  }
+}
+
+// Block per task x of the relabelled graph.  DENSE: counters in
+// dense + blockIdx.x * n (HBM), reset after the task; else LDS hash sized
+// for wedge work <= HSLOTS / 2.
+template <int HSLOTS, int PCAP, int NT, bool DENSE>
+__global__ __launch_bounds__(NT) void k_cycle4_wedge(const int64_t *__restrict__ rowptr,
+                                                     const int32_t *__restrict__ colidx,
+                                                     const int32_t *__restrict__ tasks, int ntasks,
+                                                     int *__restrict__ dense, int64_t n,
+                                                     int *__restrict__ queue, u64 *__restrict__ out) {
+  __shared__ WedgeStage<DENSE ? 1 : HSLOTS, PCAP> S;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  int *cnt = DENSE ? dense + (size_t)blockIdx.x * (size_t)n : nullptr;
+  u64 acc = 0;
+  for (;;) {
+    if (tid == 0) S.task = atomicAdd(queue, 1);
+    __syncthreads();
+    const int t = S.task;
+    if (t >= ntasks) break;
+    const int x = tasks[t];
+    const int64_t bx = rowptr[x];
+    const int32_t *nx = colidx + bx;
+    const int lo = lower_bound_dev(nx, (int)(rowptr[x + 1] - bx), x);  // N(x) ∩ [0, x)
+    int hb = 0;
+    if (!DENSE) {
+      hb = 1;
+      while ((1 << hb) < HSLOTS) ++hb;
+      for (int q = tid; q < HSLOTS; q += NT) { S.key[q] = -1; S.val[q] = 0; }
+    }
+    for (int k0 = 0; k0 < lo; k0 += PCAP) {
+      const int k1 = min(lo, k0 + PCAP);
+      __syncthreads();
+      if (wave == 0) {
+        int ns, L;
+        wave_segments(S.seg, rowptr, colidx, nx, k0, k1, x, KeepAll(), lane, ns, L);
+        if (lane == 0) { S.ns = ns; S.L = L; }
+      }
+      __syncthreads();
+      const int ns = S.ns, L = S.L;
+      for (int base = wave * 64; base < L; base += NT) {
+        WindowLane r = window_lane(S.seg, colidx, ns, L, base, lane);
+        if (r.valid) {
+          const int old = DENSE ? atomicAdd(&cnt[r.w], 1) : hash_bump(S.key, S.val, hb, r.w);
+          acc += (u64)old;
+        }
+      }
+      (void)ns;
+    }
+    if (DENSE) {
+      // reset the counters this task touched
+      __threadfence_block();
+      for (int k0 = 0; k0 < lo; k0 += PCAP) {
+        const int k1 = min(lo, k0 + PCAP);
+        __syncthreads();
+        if (wave == 0) {
+          int ns, L;
+          wave_segments(S.seg, rowptr, colidx, nx, k0, k1, x, KeepAll(), lane, ns, L);
+          if (lane == 0) { S.ns = ns; S.L = L; }
+        }
+        __syncthreads();
+        const int ns = S.ns, L = S.L;
+        for (int base = wave * 64; base < L; base += NT) {
+          WindowLane r = window_lane(S.seg, colidx, ns, L, base, lane);
+          if (r.valid) cnt[r.w] = 0;
+        }
+      }
+      __threadfence_block();
+    }
+    __syncthreads();
+  }
+  acc = wave_sum_u64(acc);
+  if (lane == 0 && acc) atomicAdd(out, acc);
+}
+
+// ------------------------------------------------------------- 4-cycle PAIR
+constexpr int PAIR_HCAP = 2048;
+struct PairStage {
+  int key[2 * PAIR_HCAP];
+  Segs<1024> seg;
+  int ns, L, task;
+};
+
+__global__ __launch_bounds__(256) void k_cycle4_pair(const int64_t *__restrict__ rowptr,
+                                                     const int32_t *__restrict__ colidx,
+                                                     const int32_t *__restrict__ tasks, int ntasks,
+                                                     int *__restrict__ queue, u64 *__restrict__ out) {
+  __shared__ PairStage S;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  u64 acc = 0;
+  for (;;) {
+    if (tid == 0) S.task = atomicAdd(queue, 1);
+    __syncthreads();
+    const int t = S.task;
+    if (t >= ntasks) break;
+    const int v0 = tasks[t];
+    const int64_t b0 = rowptr[v0];
+    const int32_t *n0 = colidx + b0;
+    const int lo = lower_bound_dev(n0, (int)(rowptr[v0 + 1] - b0), v0);  // v1, v2 < v0
+    for (int a = 1; a < lo; ++a) {
+      const int v1 = n0[a];
+      const int64_t b1 = rowptr[v1];
+      const int32_t *n1 = colidx + b1;
+      const int s1 = lower_bound_dev(n1, (int)(rowptr[v1 + 1] - b1), v0);  // N(v1) ∩ [0, v0)
+      if (s1 == 0) continue;
+      const bool lds = s1 <= PAIR_HCAP;
+      int hb = 0;
+      __syncthreads();
+      if (lds) {
+        hb = hash_bits_for(s1);
+        for (int q = tid; q < (1 << hb); q += 256) S.key[q] = -1;
+        __syncthreads();
+        for (int q = tid; q < s1; q += 256) hash_insert(S.key, hb, n1[q]);
+      }
+      // v2 = n0[0..a) (all < v1), lists cut at v0
+      for (int k0 = 0; k0 < a; k0 += 1024) {
+        const int k1 = min(a, k0 + 1024);
+        __syncthreads();
+        if (wave == 0) {
+          int ns, L;
+          wave_segments(S.seg, rowptr, colidx, n0, k0, k1, v0, KeepAll(), lane, ns, L);
+          if (lane == 0) { S.ns = ns; S.L = L; }
+        }
+        __syncthreads();
+        const int ns = S.ns, L = S.L;
+        for (int base = wave * 64; base < L; base += 256) {
+          WindowLane r = window_lane(S.seg, colidx, ns, L, base, lane);
+          if (r.valid) acc += lds ? hash_contains(S.key, hb, r.w) : sorted_contains(n1, s1, r.w);
+        }
+      }
+    }
+    __syncthreads();
+  }
+  acc = wave_sum_u64(acc);
+  if (lane == 0 && acc) atomicAdd(out, acc);
+}
+
+template <class K>
+int grid_of(K kernel, int nt, int num_cu, int64_t tasks, int cap_per_cu = 1 << 20) {
+  int per_cu = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, nt, 0) != hipSuccess || per_cu < 1)
+    per_cu = 1;
+  per_cu = std::min(per_cu, cap_per_cu);
+  return (int)std::max<int64_t>(1, std::min<int64_t>(tasks, (int64_t)per_cu * num_cu));
+}
+
+template <int HCAP, int PCAP, int NT, bool LDS_SET>
+hipError_t run_diamond(const char *name, const DevCSR &g, const TaskList &tl, int *queue, Launch &l) {
+  if (!tl.count) return hipSuccess;
+  auto k = k_diamond<HCAP, PCAP, NT, LDS_SET>;
+  int grid = grid_of(k, NT, l.num_cu, tl.count);
+  l.begin(name, tl.count);
+  k<<<grid, NT, 0, l.stream>>>(g.rowptr, g.colidx, tl.tasks, (int)tl.count, queue, l.d_out);
+  l.end();
+  return hipGetLastError();
+}
+
+template <int HSLOTS, int PCAP, int NT>
+hipError_t run_wedge_lds(const char *name, const DevCSR &g, const TaskList &tl, int *queue, Launch &l) {
+  if (!tl.count) return hipSuccess;
+  auto k = k_cycle4_wedge<HSLOTS, PCAP, NT, false>;
+  int grid = grid_of(k, NT, l.num_cu, tl.count);
+  l.begin(name, tl.count);
+  k<<<grid, NT, 0, l.stream>>>(g.rowptr, g.colidx, tl.tasks, (int)tl.count, nullptr, g.n, queue, l.d_out);
+  l.end();
+  return hipGetLastError();
+}
+
+}  // namespace
+
+hipError_t launch_diamond(const DevCSR &g, const TaskList tl[4], Launch &l, int *d_queues) {
+  hipError_t e;
+  if ((e = run_diamond<1, 1024, 256, false>("diamond_hbm_set", g, tl[3], d_queues + 3, l))) return e;
+  if ((e = run_diamond<8192, 1024, 256, true>("diamond_block_8192", g, tl[2], d_queues + 2, l))) return e;
+  if ((e = run_diamond<1024, 1024, 256, true>("diamond_block_1024", g, tl[1], d_queues + 1, l))) return e;
+  if ((e = run_diamond<64, 64, 64, true>("diamond_wave_64", g, tl[0], d_queues + 0, l))) return e;
+  return hipSuccess;
+}
+
+int cycle4_dense_grid(int num_cu, int64_t tasks) {
+  return grid_of(k_cycle4_wedge<1, 1024, 256, true>, 256, num_cu, tasks, 1);
+}
+
+hipError_t launch_cycle4_wedge(const DevCSR &g, const TaskList tl[3], int *dense, Launch &l, int *d_queues) {
+  hipError_t e;
+  if (tl[2].count) {
+    int grid = cycle4_dense_grid(l.num_cu, tl[2].count);
+    l.begin("cycle4_wedge_hbm", tl[2].count);
+    k_cycle4_wedge<1, 1024, 256, true><<<grid, 256, 0, l.stream>>>(g.rowptr, g.colidx, tl[2].tasks,
+                                                                    (int)tl[2].count, dense, g.n, d_queues + 2,
+                                                                    l.d_out);
+    l.end();
+    if ((e = hipGetLastError())) return e;
+  }
+  if ((e = run_wedge_lds<8192, 1024, 256>("cycle4_wedge_8192", g, tl[1], d_queues + 1, l))) return e;
+  if ((e = run_wedge_lds<1024, 256, 64>("cycle4_wedge_1024", g, tl[0], d_queues + 0, l))) return e;
+  return hipSuccess;
+}
+
+hipError_t launch_cycle4_pair(const DevCSR &g, const TaskList &tl, Launch &l, int *d_queues) {
+  if (!tl.count) return hipSuccess;
+  int grid = grid_of(k_cycle4_pair, 256, l.num_cu, tl.count);
+  l.begin("cycle4_pair", tl.count);
+  k_cycle4_pair<<<grid, 256, 0, l.stream>>>(g.rowptr, g.colidx, tl.tasks, (int)tl.count, d_queues, l.d_out);
+  l.end();
+  return hipGetLastError();
+}
+
+}  // namespace xgm

@@ -60,6 +60,20 @@ hipError_t launch_clique(const DevCSR &g, const CliqueTasks &ct, int k, int *scr
                          int *d_queues /* 5 counters */);
 int clique_list_grid(int num_cu, int64_t tasks);
 
+// Edge-induced diamond over an undirected CSR; tasks by degree tier
+// (<=64, <=1024, <=8192, larger).
+hipError_t launch_diamond(const DevCSR &g, const TaskList tl[4], Launch &l, int *d_queues);
+
+// Edge-induced 4-cycle, wedge aggregation over the (degree, id)-relabelled
+// CSR; tasks by wedge work (<=512, <=4096, larger).  dense holds
+// cycle4_dense_grid() * n ints, all zero on entry (left zero on exit).
+hipError_t launch_cycle4_wedge(const DevCSR &g, const TaskList tl[3], int *dense, Launch &l, int *d_queues);
+int cycle4_dense_grid(int num_cu, int64_t tasks);
+
+// Edge-induced 4-cycle, bounded pairwise intersections in the reference's
+// loop order (original ids).
+hipError_t launch_cycle4_pair(const DevCSR &g, const TaskList &tl, Launch &l, int *d_queues);
+
 // Small utility kernel: zero `n` 64-bit words (graph-capture safe, unlike memset nodes).
 hipError_t launch_zero(void *ptr, size_t n_u64, hipStream_t s);
 

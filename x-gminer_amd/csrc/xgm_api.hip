@@ -60,7 +60,20 @@ struct xgm_dgraph {
   std::map<std::string, std::pair<int32_t *, std::vector<int64_t>>> task_cache;
   int *d_scratch = nullptr;
   size_t scratch_ints = 0;
+  // undirected graphs: degree per vertex; (degree, id)-relabelled copy with
+  // per-vertex wedge work, built on first 4-cycle count
+  std::vector<int64_t> work;
+  bool relabelled = false;
+  int *d_dense = nullptr;          // per-block 4-cycle counters, zero between counts
+  size_t dense_ints = 0;
+  int64_t *d_rl_rowptr = nullptr;
+  int32_t *d_rl_colidx = nullptr;
+  std::vector<int32_t> rl_order;   // relabelled task ids by decreasing wedge work
+  std::vector<int64_t> rl_work;
   ~xgm_dgraph() {
+    (void)hipFree(d_dense);
+    (void)hipFree(d_rl_rowptr);
+    (void)hipFree(d_rl_colidx);
     for (auto &kv : task_cache) (void)hipFree(kv.second.first);
     (void)hipFree(d_rowptr);
     (void)hipFree(d_colidx);
@@ -180,6 +193,118 @@ int ensure_scratch(xgm_ctx *ctx, xgm_dgraph *g, size_t ints) {
   return XGM_OK;
 }
 
+// Undirected task order: vertices with degree >= 3 by decreasing owned
+// probe work sum_{p in N(t), (deg p, p) < (deg t, t)} deg(p).
+void build_undirected_order(const xgm_csr &g, std::vector<int32_t> &order, std::vector<int32_t> &deg,
+                            std::vector<int64_t> &work) {
+  const int64_t n = g.n;
+  deg.assign(n, 0);
+  work.assign(n, 0);
+  for (int64_t u = 0; u < n; ++u) deg[u] = (int32_t)(g.rowptr[u + 1] - g.rowptr[u]);
+  parallel_for(n, 4096, [&](int64_t b, int64_t e) {
+    for (int64_t t = b; t < e; ++t) {
+      int64_t w = 0;
+      for (int64_t k = g.rowptr[t]; k < g.rowptr[t + 1]; ++k) {
+        int32_t p = g.colidx[k];
+        if (deg[p] < deg[t] || (deg[p] == deg[t] && p < t)) w += deg[p];
+      }
+      work[t] = w;
+    }
+  });
+  order.clear();
+  for (int64_t u = 0; u < n; ++u)
+    if (deg[u] >= 3) order.push_back((int32_t)u);
+  std::stable_sort(order.begin(), order.end(), [&](int32_t a, int32_t b) { return work[a] > work[b]; });
+}
+
+// Relabel vertices by ascending (degree, id): the returned CSR lists are
+// sorted in the new ids; wedge work of x = sum over lower neighbours v < x of
+// |N(v) ∩ [0, x)|.  Counts of isomorphism-invariant patterns are unchanged.
+void relabel_by_degree(const xgm_csr &g, std::vector<int64_t> &rp, std::vector<int32_t> &ci,
+                       std::vector<int32_t> &order, std::vector<int64_t> &wwork) {
+  const int64_t n = g.n;
+  std::vector<int32_t> byrank(n), rank(n);
+  std::iota(byrank.begin(), byrank.end(), 0);
+  auto deg = [&](int32_t v) { return g.rowptr[v + 1] - g.rowptr[v]; };
+  std::stable_sort(byrank.begin(), byrank.end(), [&](int32_t a, int32_t b) { return deg(a) < deg(b); });
+  for (int64_t r = 0; r < n; ++r) rank[byrank[r]] = (int32_t)r;
+  std::vector<int64_t> cnt(n);
+  for (int64_t r = 0; r < n; ++r) cnt[r] = deg(byrank[r]);
+  exclusive_scan(cnt, rp);
+  ci.resize(g.nnz);
+  parallel_for(n, 2048, [&](int64_t b, int64_t e) {
+    for (int64_t r = b; r < e; ++r) {
+      int32_t v = byrank[r];
+      int32_t *dst = ci.data() + rp[r];
+      int64_t k0 = g.rowptr[v], k1 = g.rowptr[v + 1];
+      for (int64_t k = k0; k < k1; ++k) dst[k - k0] = rank[g.colidx[k]];
+      std::sort(dst, dst + (k1 - k0));
+    }
+  });
+  wwork.assign(n, 0);
+  parallel_for(n, 2048, [&](int64_t b, int64_t e) {
+    for (int64_t x = b; x < e; ++x) {
+      const int32_t *nx = ci.data() + rp[x];
+      int64_t lo = std::lower_bound(nx, nx + (rp[x + 1] - rp[x]), (int32_t)x) - nx;
+      if (lo < 2) continue;
+      int64_t w = 0;
+      for (int64_t k = 0; k < lo; ++k) {
+        int32_t v = nx[k];
+        const int32_t *nv = ci.data() + rp[v];
+        w += std::lower_bound(nv, nv + (rp[v + 1] - rp[v]), (int32_t)x) - nv;
+      }
+      wwork[x] = w;
+    }
+  });
+  order.clear();
+  for (int64_t x = 0; x < n; ++x)
+    if (wwork[x] > 0) order.push_back((int32_t)x);
+  std::stable_sort(order.begin(), order.end(), [&](int32_t a, int32_t b) { return wwork[a] > wwork[b]; });
+}
+
+// Upload a task list split by thresholds on `key_of(u)` (inclusive upper
+// bounds, last = everything else), cached under `key`.
+template <class KeyOf>
+int tasks_split(xgm_ctx *ctx, xgm_dgraph *g, const std::string &key, const std::vector<int32_t> &order,
+                KeyOf key_of, const std::vector<int64_t> &hi, std::vector<TaskList> &out) {
+  auto it = g->task_cache.find(key);
+  if (it == g->task_cache.end()) {
+    std::vector<std::vector<int32_t>> parts(hi.size());
+    for (int32_t u : order) {
+      int64_t kv = key_of(u);
+      size_t t = 0;
+      while (t + 1 < hi.size() && kv > hi[t]) ++t;
+      parts[t].push_back(u);
+    }
+    std::vector<int32_t> all;
+    std::vector<int64_t> counts;
+    for (auto &p : parts) {
+      counts.push_back((int64_t)p.size());
+      all.insert(all.end(), p.begin(), p.end());
+    }
+    int32_t *d = nullptr;
+    HIP_TRY(ctx, hipMalloc(&d, std::max<size_t>(1, all.size()) * sizeof(int32_t)));
+    if (!all.empty()) {
+      hipError_t e = hipMemcpy(d, all.data(), all.size() * sizeof(int32_t), hipMemcpyHostToDevice);
+      if (e != hipSuccess) {
+        (void)hipFree(d);
+        return ctx->hip_fail(e, "H2D tasks");
+      }
+    }
+    it = g->task_cache.emplace(key, std::make_pair(d, counts)).first;
+  }
+  out.clear();
+  int64_t off = 0;
+  for (int64_t c : it->second.second) {
+    TaskList t;
+    t.tasks = it->second.first + off;
+    t.count = c;
+    out.push_back(t);
+    off += c;
+  }
+  return XGM_OK;
+}
+
 int check_device(xgm_ctx *ctx) {
   hipDeviceProp_t p;
   HIP_TRY(ctx, hipGetDeviceProperties(&p, ctx->device));
@@ -253,6 +378,7 @@ int xgm_graph_upload(xgm_ctx *ctx, const xgm_csr *g, xgm_dgraph **out) {
                                     hipMemcpyHostToDevice, ctx->stream)) != hipSuccess)
     return cleanup(ctx->hip_fail(e, "H2D colidx"));
   if (g->flags & XGM_CSR_DAG) build_dag_order(*g, d->order, d->dplus, d->max_out);
+  else if (g->flags & XGM_CSR_SYMMETRIC) build_undirected_order(*g, d->order, d->dplus, d->work);
   if ((e = hipStreamSynchronize(ctx->stream)) != hipSuccess) return cleanup(ctx->hip_fail(e, "upload sync"));
   *out = d.release();
   return XGM_OK;
@@ -319,6 +445,76 @@ int xgm_count(xgm_ctx *ctx, const xgm_dgraph *g, int pattern, const xgm_opts *op
         if ((rc = ensure_scratch(ctx, gm, need))) return rc;
       }
       e = launch_clique(dg, ct, k, gm->d_scratch, std::max(1, g->max_out), l, ctx->d_queues);
+      break;
+    }
+    case XGM_DIAMOND_EDGE: {
+      if (!(g->flags & XGM_CSR_SYMMETRIC)) return ctx->fail(XGM_EINVAL, "DIAMOND_EDGE needs an undirected CSR");
+      if (strategy != XGM_STRAT_AUTO && strategy != XGM_STRAT_HASH)
+        return ctx->fail(XGM_EUNSUPPORTED, "strategy not available for DIAMOND_EDGE");
+      const auto &deg = gm->dplus;
+      if ((rc = tasks_split(ctx, gm, "diamond", gm->order, [&](int32_t u) { return (int64_t)deg[u]; },
+                            {64, 1024, 8192, INT64_MAX}, tl)))
+        return rc;
+      TaskList t4[4] = {tl[0], tl[1], tl[2], tl[3]};
+      e = launch_diamond(dg, t4, l, ctx->d_queues);
+      break;
+    }
+    case XGM_CYCLE4_EDGE: {
+      if (!(g->flags & XGM_CSR_SYMMETRIC)) return ctx->fail(XGM_EINVAL, "CYCLE4_EDGE needs an undirected CSR");
+      if (strategy == XGM_STRAT_LIST) {
+        // reference loop order: tasks v0 with >= 2 lower neighbours (original ids)
+        std::vector<int32_t> ord;
+        if (gm->task_cache.find("cycle4_pair") == gm->task_cache.end()) {
+          std::vector<int64_t> rp(g->n + 1);
+          std::vector<int32_t> ci(g->nnz);
+          HIP_TRY(ctx, hipMemcpy(rp.data(), g->d_rowptr, rp.size() * 8, hipMemcpyDeviceToHost));
+          HIP_TRY(ctx, hipMemcpy(ci.data(), g->d_colidx, ci.size() * 4, hipMemcpyDeviceToHost));
+          for (int64_t v = 0; v < g->n; ++v) {
+            int64_t lo = std::lower_bound(ci.data() + rp[v], ci.data() + rp[v + 1], (int32_t)v) - (ci.data() + rp[v]);
+            if (lo >= 2) ord.push_back((int32_t)v);
+          }
+        }
+        if ((rc = tasks_split(ctx, gm, "cycle4_pair", ord, [](int32_t) { return (int64_t)0; }, {INT64_MAX}, tl)))
+          return rc;
+        e = launch_cycle4_pair(dg, tl[0], l, ctx->d_queues);
+        break;
+      }
+      if (strategy != XGM_STRAT_AUTO && strategy != XGM_STRAT_HASH)
+        return ctx->fail(XGM_EUNSUPPORTED, "strategy not available for CYCLE4_EDGE");
+      if (!gm->relabelled) {
+        std::vector<int64_t> rp(g->n + 1), rrp;
+        std::vector<int32_t> ci(g->nnz), rci;
+        HIP_TRY(ctx, hipMemcpy(rp.data(), g->d_rowptr, rp.size() * 8, hipMemcpyDeviceToHost));
+        HIP_TRY(ctx, hipMemcpy(ci.data(), g->d_colidx, ci.size() * 4, hipMemcpyDeviceToHost));
+        xgm_csr hv{g->n, g->nnz, rp.data(), ci.data(), g->flags};
+        relabel_by_degree(hv, rrp, rci, gm->rl_order, gm->rl_work);
+        HIP_TRY(ctx, hipMalloc(&gm->d_rl_rowptr, rrp.size() * 8));
+        HIP_TRY(ctx, hipMalloc(&gm->d_rl_colidx, std::max<size_t>(1, rci.size()) * 4));
+        HIP_TRY(ctx, hipMemcpy(gm->d_rl_rowptr, rrp.data(), rrp.size() * 8, hipMemcpyHostToDevice));
+        if (!rci.empty()) HIP_TRY(ctx, hipMemcpy(gm->d_rl_colidx, rci.data(), rci.size() * 4, hipMemcpyHostToDevice));
+        gm->relabelled = true;
+      }
+      const auto &ww = gm->rl_work;
+      if ((rc = tasks_split(ctx, gm, "cycle4_wedge", gm->rl_order, [&](int32_t u) { return ww[u]; },
+                            {512, 4096, INT64_MAX}, tl)))
+        return rc;
+      TaskList t3[3] = {tl[0], tl[1], tl[2]};
+      if (t3[2].count) {
+        size_t need = (size_t)cycle4_dense_grid(ctx->num_cu, t3[2].count) * (size_t)g->n;
+        need = (need + 1) & ~(size_t)1;  // whole 64-bit words for launch_zero
+        if (gm->dense_ints < need) {
+          (void)hipFree(gm->d_dense);
+          gm->d_dense = nullptr;
+          gm->dense_ints = 0;
+          HIP_TRY(ctx, hipMalloc(&gm->d_dense, need * sizeof(int)));
+          HIP_TRY(ctx, launch_zero(gm->d_dense, need / 2, ctx->stream));
+          gm->dense_ints = need;
+        }
+      }
+      DevCSR rg = dg;
+      rg.rowptr = gm->d_rl_rowptr;
+      rg.colidx = gm->d_rl_colidx;
+      e = launch_cycle4_wedge(rg, t3, gm->d_dense, l, ctx->d_queues);
       break;
     }
     default:
