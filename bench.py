@@ -21,7 +21,15 @@ sys.path.insert(0, os.path.join(ROOT, "tests"))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-PATTERN_IDS = {"triangle": 1, "clique4": 2, "clique5": 3}
+PATTERN_IDS = {"triangle": 1, "clique4": 2, "clique5": 3, "diamond": 4, "rectangle": 5}
+# workload name -> (generator, params, golden key, graph the patterns run on)
+WORKLOADS = {
+    "rmat22": ("rmat", dict(scale=22, seed=2), "rmat22_s2", "dag"),
+    "rmat16": ("rmat", dict(scale=16, seed=1), "rmat16_s1", "dag"),
+    "rmat23": ("rmat", dict(scale=23, seed=4), "rmat23_s4", "dag"),
+    "chunglu": ("cl", dict(n=4000000, avg_degree=20.0, beta=2.1, w_max=50000.0, seed=3), "chunglu4m_s3",
+                "undirected"),
+}
 # tier bounds of the triangle kernels (out-degree), mirrored from kernels.h
 TC_TIERS = [("tc_wave_64", 2, 64), ("tc_wave_256", 65, 256), ("tc_block_2048", 257, 2048),
             ("tc_block_big", 2049, 1 << 40)]
@@ -65,8 +73,8 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--pattern", default="triangle", choices=list(PATTERN_IDS))
-    ap.add_argument("--scale", type=int, default=22)
-    ap.add_argument("--seed", type=int, default=2)
+    ap.add_argument("--workload", default="rmat22", choices=list(WORKLOADS))
+    ap.add_argument("--strategy", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
     args = ap.parse_args()
@@ -78,13 +86,19 @@ def main():
     from conftest import load_xgm
     xgm = load_xgm()
     t0 = time.time()
-    g = xgm.gen_rmat(args.scale, 16, 0.57, 0.19, 0.19, args.seed)
-    d = g.orient()
-    m = d.nnz
-    log(f"graph R-MAT-{args.scale} seed {args.seed}: n={g.n} m={m} hash={g.hash():016x} "
-        f"({time.time()-t0:.1f}s)")
+    kind, params, gname, on = WORKLOADS[args.workload]
+    if kind == "rmat":
+        g = xgm.gen_rmat(params["scale"], 16, 0.57, 0.19, 0.19, params["seed"])
+        desc = (f"R-MAT scale {params['scale']}, edge factor 16, (0.57,0.19,0.19,0.05), "
+                f"seed {params['seed']}")
+    else:
+        g = xgm.gen_chung_lu(params["n"], params["avg_degree"], params["beta"], params["w_max"], params["seed"])
+        desc = (f"Chung-Lu n={params['n']}, avg degree {params['avg_degree']}, beta {params['beta']}, "
+                f"w_max {params['w_max']:.0f}, seed {params['seed']}")
+    d = g.orient() if on == "dag" else g
+    m = g.nnz // 2
+    log(f"graph {args.workload}: n={g.n} m={m} hash={g.hash():016x} ({time.time()-t0:.1f}s)")
     gold = json.load(open(os.path.join(ROOT, "tests", "golden", "counts.json")))
-    gname = f"rmat{args.scale}_s{args.seed}"
     golden = None
     if gname in gold and gold[gname]["hash"] == f"{g.hash():016x}":
         golden = gold[gname]["counts"].get(args.pattern)
@@ -98,18 +112,19 @@ def main():
     ctx = xgm.Context(0)
     t0 = time.time()
     dg = ctx.upload(d)
-    log(f"upload {time.time()-t0:.2f}s")
+    upload_s = time.time() - t0
+    log(f"upload {upload_s:.2f}s")
     pid = PATTERN_IDS[args.pattern]
     count = None
     for i in range(args.warmup):
-        count = ctx.count(dg, pid)
+        count = ctx.count(dg, pid, args.strategy)
         log(f"warmup {i}: count={count}")
     ktimes = {}
     totals = []
     sync()
     t0 = time.perf_counter()
     for i in range(args.steps):
-        c = ctx.count(dg, pid)
+        c = ctx.count(dg, pid, args.strategy)
         total, ks = ctx.last_profile()
         totals.append(total)
         for k in ks:
@@ -125,10 +140,11 @@ def main():
         raise SystemExit(f"COUNT MISMATCH vs golden: {count} != {golden}")
 
     roof = None
-    extra = {}
+    extra = {"upload_s": round(upload_s, 3)}
+    avg = {k: float(np.mean(v)) for k, v in ktimes.items()}
+    extra["kernels_ms"] = {k: round(v, 4) for k, v in avg.items()}
     if args.pattern == "triangle":
         per_tier, btot = tc_alg_bytes_per_tier(d)
-        avg = {k: float(np.mean(v)) for k, v in ktimes.items()}
         dom = max(avg, key=avg.get)
         ach = per_tier[dom] / (avg[dom] * 1e-3) / 1e9
         roof = dict(bound="hbm", achieved=round(ach, 1), peak=HBM_PEAK_GBS, unit="GB/s",
@@ -137,14 +153,14 @@ def main():
         pat_ach = btot / (float(np.mean(totals)) * 1e-3) / 1e9
         extra["pattern_roofline"] = dict(alg_bytes=btot, kernel_ms=round(float(np.mean(totals)), 4),
                                          achieved=round(pat_ach, 1), frac=round(pat_ach / HBM_PEAK_GBS, 4))
-        extra["kernels_ms"] = {k: round(v, 4) for k, v in avg.items()}
 
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(g, count, args.cpu_threads) if args.pattern == "triangle" else None
+        cpu = (cpu_baseline(g, count, args.cpu_threads)
+               if args.pattern == "triangle" and args.workload == "rmat22" else None)
 
     out = {
-        "metric": f"|E|/s on R-MAT-{args.scale} ({args.pattern})",
+        "metric": f"|E|/s on {args.workload} ({args.pattern})",
         "value": m / (ms_per_step * 1e-3),
         "unit": "|E|/s",
         "n_gpus": 1,
@@ -156,9 +172,10 @@ def main():
         "vs_baseline": None,
         "dtype": "int32",
         "data": "synthetic",
-        "config": {"workload": f"{args.pattern} count on R-MAT scale {args.scale}, edge factor 16, "
-                               f"(0.57,0.19,0.19,0.05), seed {args.seed}, degree-oriented DAG",
-                   "n": g.n, "dag_edges": m, "graph_hash": f"{g.hash():016x}", "parallelism": "single GPU"},
+        "config": {"workload": f"{args.pattern} count on {desc}"
+                               + (", degree-oriented DAG" if on == "dag" else ", undirected"),
+                   "n": g.n, "undirected_edges": m, "graph_hash": f"{g.hash():016x}",
+                   "strategy": args.strategy, "parallelism": "single GPU"},
         "count": count,
         "count_matches_golden": (golden == count) if golden is not None else None,
         "matches_per_s": count / (ms_per_step * 1e-3),

@@ -78,7 +78,9 @@ void xgm_ctx_destroy(xgm_ctx *ctx);
 const char *xgm_ctx_last_error(const xgm_ctx *ctx);
 
 /* Copy a CSR into HBM and build its task lists (degree bins).  The CSR must
- * satisfy xgm_csr_validate() for its flags.  Replaces GraphGPU's H2D copy
+ * satisfy xgm_csr_validate() for its flags; nnz must be below 2^31.  A device
+ * graph is used only with the context that uploaded it; xgm_graph_free() may
+ * run before or after xgm_ctx_destroy().  Replaces GraphGPU's H2D copy
  * (xgminer/include/graph_v2_gpu.h:81-134) and the host task prep of
  * glumin_GM_solver.cu:326-335. */
 int xgm_graph_upload(xgm_ctx *ctx, const xgm_csr *g, xgm_dgraph **out);

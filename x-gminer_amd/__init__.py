@@ -17,6 +17,7 @@ from __future__ import annotations
 
 import ctypes as C
 import os
+import weakref
 from dataclasses import dataclass
 
 import numpy as np
@@ -230,9 +231,12 @@ class Context:
         h = C.c_void_p()
         _check(lib.xgm_ctx_create(device, C.byref(h)))
         self._h = h
+        self._graphs = weakref.WeakSet()
 
     def close(self):
         if getattr(self, "_h", None) is not None:
+            for g in list(self._graphs):
+                g.close()
             lib.xgm_ctx_destroy(self._h)
             self._h = None
 
@@ -262,6 +266,7 @@ class DeviceGraph:
         h = C.c_void_p()
         _check(lib.xgm_graph_upload(ctx._h, g.csr, C.byref(h)), ctx._h)
         self._h, self.ctx, self.host = h, ctx, g
+        ctx._graphs.add(self)
 
     def close(self):
         if getattr(self, "_h", None) is not None:

@@ -25,59 +25,10 @@
 
 #include "device_common.h"
 #include "kernels.h"
+#include "segments.h"
 
 namespace xgm {
 namespace {
-
-// ------------------------------------------------------------- segments
-// Flattened view of the lists N+(verts[k]) for k in a piece [k0, k1):
-// off = rowptr of each non-empty list, pre = inclusive prefix of lengths,
-// idx = position k of the owning vertex.
-template <int PCAP>
-struct SegTab {
-  int64_t off[PCAP];
-  int pre[PCAP];
-  int idx[PCAP];
-};
-
-template <int PCAP>
-__device__ __forceinline__ void wave_build_segments(SegTab<PCAP> &t, const int64_t *__restrict__ rowptr,
-                                                    const int *verts, int k0, int k1, int lane, int &ns_out,
-                                                    int &L_out) {
-  int ns = 0, carry = 0;
-  for (int k = k0; k < k1; k += 64) {
-    const int kk = k + lane;
-    int len = 0;
-    int64_t vb = 0;
-    if (kk < k1) {
-      const int v = verts[kk];
-      vb = rowptr[v];
-      len = (int)(rowptr[v + 1] - vb);
-    }
-    const int incl = wave_incl_scan(len, lane) + carry;
-    const u64 m = __ballot(len > 0);
-    if (len > 0) {
-      const int pos = ns + __popcll(m & lanemask_lt(lane));
-      t.off[pos] = vb;
-      t.pre[pos] = incl;
-      t.idx[pos] = kk;
-    }
-    ns += __popcll(m);
-    carry = __shfl(incl, 63, 64);
-  }
-  ns_out = ns;
-  L_out = carry;
-}
-
-// lane -> (segment j, element index) for window [base, base+64)
-template <int PCAP>
-__device__ __forceinline__ int window_segment(const SegTab<PCAP> &t, int ns, int base, int lane) {
-  const int jlo = upper_bound_dev(t.pre, ns, base);
-  const int e = (jlo + lane < ns) ? t.pre[jlo + lane] : INT_MAX;
-  const int p = e - base;
-  const u64 ends = wave_or_u64(p < 64 ? (1ull << p) : 0ull);
-  return jlo + __popcll(ends & lanemask_le(lane));
-}
 
 // ------------------------------------------------------------- LUT engine
 // Count cliques of `depth` vertices inside the sub-DAG encoded by rows
@@ -125,7 +76,7 @@ template <int SMAX, int W>
 struct LutStage {
   int x[SMAX];
   u64 rows[SMAX * W];
-  SegTab<SMAX> seg;
+  Segs<SMAX> seg;
   int ns, L, task;
 };
 
@@ -155,24 +106,20 @@ __global__ __launch_bounds__(NT) void k_clique_vlut(const int64_t *__restrict__ 
       __syncthreads();
       if (wave == 0) {
         int ns, L;
-        wave_build_segments(S.seg, rowptr, S.x, 0, s, lane, ns, L);
+        wave_segments(S.seg, rowptr, colidx, S.x, 0, s, -1, KeepAll(), lane, ns, L);
         if (lane == 0) { S.ns = ns; S.L = L; }
       }
       __syncthreads();
       const int ns = S.ns, L = S.L;
-      for (int base = wave * 64; base < L; base += NT) {
-        const int j = window_segment(S.seg, ns, base, lane);
-        const int i = base + lane;
-        if (i < L) {
-          const int start = j > 0 ? S.seg.pre[j - 1] : 0;
-          const int w = colidx[S.seg.off[j] + (i - start)];
-          const int pos = lower_bound_dev(S.x, s, w);
-          if (pos < s && S.x[pos] == w) {
-            const int row = S.seg.idx[j];
-            atomicOr(&S.rows[row * W + (pos >> 6)], 1ull << (pos & 63));
-          }
+      int lo, hi;
+      wave_share(L, wave, NW, lo, hi);
+      wave_walk<4>(S.seg, colidx, ns, lo, hi, lane, [&](const WindowLane &r) {
+        if (r.valid) {
+          const int pos = lower_bound_dev(S.x, s, r.w);
+          if (pos < s && S.x[pos] == r.w)
+            atomicOr(&S.rows[S.seg.idx[r.j] * W + (pos >> 6)], 1ull << (pos & 63));
         }
-      }
+      });
       __syncthreads();
       acc += lut_count<W>(S.rows, s, weff, depth, tid, NT);
     }
@@ -191,7 +138,7 @@ struct ListStage {
   int h0[2 * LIST_LCAP];  // hash of N+(v0) when s <= LIST_LCAP
   int w1[LIST_LCAP];
   int w2[LIST_LCAP];
-  SegTab<LIST_PCAP> seg;
+  Segs<LIST_PCAP> seg;
   int cnt[LIST_NT / 64];
   int ns, L, task;
 };
@@ -244,19 +191,16 @@ __device__ u64 block_sum_intersections(const int64_t *__restrict__ rowptr, const
     __syncthreads();
     if (wave == 0) {
       int ns, L;
-      wave_build_segments(S.seg, rowptr, verts, k0, k1, lane, ns, L);
+      wave_segments(S.seg, rowptr, colidx, verts, k0, k1, -1, KeepAll(), lane, ns, L);
       if (lane == 0) { S.ns = ns; S.L = L; }
     }
     __syncthreads();
     const int ns = S.ns, L = S.L;
-    for (int base = wave * 64; base < L; base += LIST_NT) {
-      const int j = window_segment(S.seg, ns, base, lane);
-      const int i = base + lane;
-      if (i < L) {
-        const int start = j > 0 ? S.seg.pre[j - 1] : 0;
-        acc += m.contains(colidx[S.seg.off[j] + (i - start)]);
-      }
-    }
+    int lo, hi;
+    wave_share(L, wave, LIST_NT / 64, lo, hi);
+    wave_walk<4>(S.seg, colidx, ns, lo, hi, lane, [&](const WindowLane &r) {
+      if (r.valid) acc += m.contains(r.w);
+    });
   }
   __syncthreads();
   return acc;

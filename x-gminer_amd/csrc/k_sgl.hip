@@ -74,16 +74,15 @@ __global__ __launch_bounds__(NT) void k_diamond(const int64_t *__restrict__ rowp
       const int ns = S.ns, L = S.L;
       for (int q = tid; q < ns; q += NT) S.cnt[q] = 0;
       __syncthreads();
-      for (int base = wave * 64; base < L; base += NT) {
-        WindowLane r = window_lane(S.seg, colidx, ns, L, base, lane);
+      int lo, hi;
+      wave_share(L, wave, NT / 64, lo, hi);
+      wave_walk<4>(S.seg, colidx, ns, lo, hi, lane, [&](const WindowLane &r) {
         bool hit = false;
         if (r.valid) hit = LDS_SET ? hash_contains(S.key, hb, r.w) : sorted_contains(nt, dt, r.w);
         const u64 hits = __ballot(hit);
-        if (r.valid) {
-          const int h = segment_run_hits(hits, r.heads, base + lane + 1 < L, lane);
-          if (h) atomicAdd(&S.cnt[r.j], h);
-        }
-      }
+        const int h = segment_run_hits(hits, r, lane);
+        if (h) atomicAdd(&S.cnt[r.j], h);
+      });
       __syncthreads();
       for (int q = tid; q < ns; q += NT) {
         const u64 c = (u64)S.cnt[q];
@@ -142,15 +141,15 @@ __global__ __launch_bounds__(NT) void k_cycle4_wedge(const int64_t *__restrict__
     const int x = tasks[t];
     const int64_t bx = rowptr[x];
     const int32_t *nx = colidx + bx;
-    const int lo = lower_bound_dev(nx, (int)(rowptr[x + 1] - bx), x);  // N(x) ∩ [0, x)
+    const int nlow = lower_bound_dev(nx, (int)(rowptr[x + 1] - bx), x);  // |N(x) ∩ [0, x)|
     int hb = 0;
     if (!DENSE) {
       hb = 1;
       while ((1 << hb) < HSLOTS) ++hb;
       for (int q = tid; q < HSLOTS; q += NT) { S.key[q] = -1; S.val[q] = 0; }
     }
-    for (int k0 = 0; k0 < lo; k0 += PCAP) {
-      const int k1 = min(lo, k0 + PCAP);
+    for (int k0 = 0; k0 < nlow; k0 += PCAP) {
+      const int k1 = min(nlow, k0 + PCAP);
       __syncthreads();
       if (wave == 0) {
         int ns, L;
@@ -159,20 +158,21 @@ __global__ __launch_bounds__(NT) void k_cycle4_wedge(const int64_t *__restrict__
       }
       __syncthreads();
       const int ns = S.ns, L = S.L;
-      for (int base = wave * 64; base < L; base += NT) {
-        WindowLane r = window_lane(S.seg, colidx, ns, L, base, lane);
+      int lo, hi;
+      wave_share(L, wave, NT / 64, lo, hi);
+      wave_walk<4>(S.seg, colidx, ns, lo, hi, lane, [&](const WindowLane &r) {
         if (r.valid) {
           const int old = DENSE ? atomicAdd(&cnt[r.w], 1) : hash_bump(S.key, S.val, hb, r.w);
           acc += (u64)old;
         }
-      }
+      });
       (void)ns;
     }
     if (DENSE) {
       // reset the counters this task touched
       __threadfence_block();
-      for (int k0 = 0; k0 < lo; k0 += PCAP) {
-        const int k1 = min(lo, k0 + PCAP);
+      for (int k0 = 0; k0 < nlow; k0 += PCAP) {
+        const int k1 = min(nlow, k0 + PCAP);
         __syncthreads();
         if (wave == 0) {
           int ns, L;
@@ -181,10 +181,11 @@ __global__ __launch_bounds__(NT) void k_cycle4_wedge(const int64_t *__restrict__
         }
         __syncthreads();
         const int ns = S.ns, L = S.L;
-        for (int base = wave * 64; base < L; base += NT) {
-          WindowLane r = window_lane(S.seg, colidx, ns, L, base, lane);
+        int lo, hi;
+        wave_share(L, wave, NT / 64, lo, hi);
+        wave_walk<4>(S.seg, colidx, ns, lo, hi, lane, [&](const WindowLane &r) {
           if (r.valid) cnt[r.w] = 0;
-        }
+        });
       }
       __threadfence_block();
     }
@@ -217,8 +218,8 @@ __global__ __launch_bounds__(256) void k_cycle4_pair(const int64_t *__restrict__
     const int v0 = tasks[t];
     const int64_t b0 = rowptr[v0];
     const int32_t *n0 = colidx + b0;
-    const int lo = lower_bound_dev(n0, (int)(rowptr[v0 + 1] - b0), v0);  // v1, v2 < v0
-    for (int a = 1; a < lo; ++a) {
+    const int nlow = lower_bound_dev(n0, (int)(rowptr[v0 + 1] - b0), v0);  // v1, v2 < v0
+    for (int a = 1; a < nlow; ++a) {
       const int v1 = n0[a];
       const int64_t b1 = rowptr[v1];
       const int32_t *n1 = colidx + b1;
@@ -244,10 +245,11 @@ __global__ __launch_bounds__(256) void k_cycle4_pair(const int64_t *__restrict__
         }
         __syncthreads();
         const int ns = S.ns, L = S.L;
-        for (int base = wave * 64; base < L; base += 256) {
-          WindowLane r = window_lane(S.seg, colidx, ns, L, base, lane);
+        int lo, hi;
+        wave_share(L, wave, 4, lo, hi);
+        wave_walk<4>(S.seg, colidx, ns, lo, hi, lane, [&](const WindowLane &r) {
           if (r.valid) acc += lds ? hash_contains(S.key, hb, r.w) : sorted_contains(n1, s1, r.w);
-        }
+        });
       }
     }
     __syncthreads();

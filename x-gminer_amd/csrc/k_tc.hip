@@ -17,79 +17,29 @@
 
 #include "device_common.h"
 #include "kernels.h"
+#include "segments.h"
 
 namespace xgm {
 namespace {
 
 constexpr int STRAT_HASH = 1;
 constexpr int STRAT_BSEARCH = 2;
+constexpr int UNROLL = 4;  // windows mapped before their loads are issued
 
-// LDS state of one staged task.
+// LDS state of one staged task: N+(u) as a hash (or sorted copy) and the
+// flattened segment table of the lists N+(v), v in N+(u).
 template <int CAP, int STRAT>
 struct Stage {
   static constexpr int KEYS = (STRAT == STRAT_HASH) ? 2 * CAP : CAP;
-  int key[KEYS];       // hash table or sorted copy of N+(u)
-  int64_t off[CAP];    // rowptr[v] of each non-empty segment
-  int pre[CAP];        // inclusive prefix of segment lengths
+  int key[KEYS];
+  Segs<CAP> seg;
 };
 
-// Probe one 64-wide window [base, base+64) of the flattened segments.
-// Returns 1 in lanes whose element is a member of the staged set.
-template <int CAP, int STRAT>
-__device__ __forceinline__ int probe_window(const Stage<CAP, STRAT> &s, int ns, int L, int base,
-                                            int lane, const int32_t *__restrict__ colidx,
-                                            const int32_t *__restrict__ nu_global, int du,
-                                            int hbits, bool lds_member) {
-  // first segment whose end exceeds base (uniform across the wave)
-  const int jlo = upper_bound_dev(s.pre, ns, base);
-  const int e = (jlo + lane < ns) ? s.pre[jlo + lane] : INT_MAX;
-  const int p = e - base;  // >= 1
-  const u64 bit = (p < 64) ? (1ull << p) : 0ull;
-  const u64 ends = wave_or_u64(bit);
-  const int j = jlo + __popcll(ends & lanemask_le(lane));
-  const int i = base + lane;
-  int hit = 0;
-  if (i < L) {
-    const int start = (j > 0) ? s.pre[j - 1] : 0;
-    const int w = colidx[s.off[j] + (i - start)];
-    if (lds_member) {
-      if (STRAT == STRAT_HASH) hit = hash_contains(s.key, hbits, w);
-      else hit = sorted_contains(s.key, du, w);
-    } else {
-      hit = sorted_contains(nu_global, du, w);
-    }
-  }
-  return hit;
-}
-
-// Build the compacted segment table for v in N+(u)[k0, k1) with one wave.
-// Returns (ns, L) through references.  Caller synchronises afterwards.
-template <int CAP, int STRAT>
-__device__ __forceinline__ void build_segments(Stage<CAP, STRAT> &s, const int64_t *__restrict__ rowptr,
-                                               const int32_t *__restrict__ nbr, int k0, int k1,
-                                               int lane, int &ns_out, int &L_out) {
-  int ns = 0, carry = 0;
-  for (int k = k0; k < k1; k += 64) {
-    const int kk = k + lane;
-    int len = 0;
-    int64_t vb = 0;
-    if (kk < k1) {
-      const int v = nbr[kk];
-      vb = rowptr[v];
-      len = (int)(rowptr[v + 1] - vb);
-    }
-    const int incl = wave_incl_scan(len, lane) + carry;
-    const u64 m = __ballot(len > 0);
-    if (len > 0) {
-      const int pos = ns + __popcll(m & lanemask_lt(lane));
-      s.off[pos] = vb;
-      s.pre[pos] = incl;
-    }
-    ns += __popcll(m);
-    carry = __shfl(incl, 63, 64);
-  }
-  ns_out = ns;
-  L_out = carry;
+template <int STRAT>
+__device__ __forceinline__ int member(const int *key, int hbits, const int32_t *nu_global, int du, bool lds, int w) {
+  if (!lds) return sorted_contains(nu_global, du, w);
+  if (STRAT == STRAT_HASH) return hash_contains(key, hbits, w);
+  return sorted_contains(key, du, w);
 }
 
 // ---------------------------------------------------------------------------
@@ -124,11 +74,12 @@ __global__ __launch_bounds__(256) void k_tc_wave(const int64_t *__restrict__ row
         for (int k = lane; k < du; k += 64) s.key[k] = nu[k];
       }
       int ns, L;
-      build_segments(s, rowptr, nu, 0, du, lane, ns, L);
+      wave_segments(s.seg, rowptr, colidx, nu, 0, du, -1, KeepAll(), lane, ns, L);
       wave_sync();
       int cnt = 0;
-      for (int base = 0; base < L; base += 64)
-        cnt += probe_window(s, ns, L, base, lane, colidx, nu, du, hbits, true);
+      wave_walk<UNROLL>(s.seg, colidx, ns, 0, L, lane, [&](const WindowLane &r) {
+        if (r.valid) cnt += member<STRAT>(s.key, hbits, nu, du, true, r.w);
+      });
       acc += (u64)cnt;
       wave_sync();  // stage is rewritten by the next task
     }
@@ -138,17 +89,19 @@ __global__ __launch_bounds__(256) void k_tc_wave(const int64_t *__restrict__ row
 }
 
 // ---------------------------------------------------------------------------
-// Block-per-task kernel: 4 waves share one Stage.  If d+(u) > CAP the
-// membership test falls back to binary search of N+(u) in HBM and the
-// segments are processed in pieces of CAP.
-template <int CAP, int STRAT>
-__global__ __launch_bounds__(256) void k_tc_block(const int64_t *__restrict__ rowptr,
-                                                  const int32_t *__restrict__ colidx,
-                                                  const int32_t *__restrict__ tasks, int ntasks,
-                                                  int *__restrict__ queue, u64 *__restrict__ out) {
+// Block-per-task kernel: NT/64 waves share one Stage, each walking its own
+// contiguous share of the flattened segments.  If d+(u) > CAP the
+// membership test binary-searches N+(u) in HBM and the segments are
+// processed in pieces of CAP.
+template <int CAP, int STRAT, int NT>
+__global__ __launch_bounds__(NT) void k_tc_block(const int64_t *__restrict__ rowptr,
+                                                 const int32_t *__restrict__ colidx,
+                                                 const int32_t *__restrict__ tasks, int ntasks,
+                                                 int *__restrict__ queue, u64 *__restrict__ out) {
   __shared__ Stage<CAP, STRAT> s;
   __shared__ int sh_task, sh_ns, sh_L;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  constexpr int NW = NT / 64;
   u64 acc = 0;
   for (;;) {
     if (threadIdx.x == 0) sh_task = atomicAdd(queue, 1);
@@ -164,11 +117,11 @@ __global__ __launch_bounds__(256) void k_tc_block(const int64_t *__restrict__ ro
     if (lds_member) {
       if (STRAT == STRAT_HASH) {
         hbits = hash_bits_for(du);
-        for (int k = threadIdx.x; k < (1 << hbits); k += 256) s.key[k] = -1;
+        for (int k = threadIdx.x; k < (1 << hbits); k += NT) s.key[k] = -1;
         __syncthreads();
-        for (int k = threadIdx.x; k < du; k += 256) hash_insert(s.key, hbits, nu[k]);
+        for (int k = threadIdx.x; k < du; k += NT) hash_insert(s.key, hbits, nu[k]);
       } else {
-        for (int k = threadIdx.x; k < du; k += 256) s.key[k] = nu[k];
+        for (int k = threadIdx.x; k < du; k += NT) s.key[k] = nu[k];
       }
     }
     int cnt = 0;
@@ -177,13 +130,15 @@ __global__ __launch_bounds__(256) void k_tc_block(const int64_t *__restrict__ ro
       __syncthreads();  // previous piece fully consumed
       if (wave == 0) {
         int ns, L;
-        build_segments(s, rowptr, nu, k0, k1, lane, ns, L);
+        wave_segments(s.seg, rowptr, colidx, nu, k0, k1, -1, KeepAll(), lane, ns, L);
         if (lane == 0) { sh_ns = ns; sh_L = L; }
       }
       __syncthreads();
-      const int ns = sh_ns, L = sh_L;
-      for (int base = wave * 64; base < L; base += 256)
-        cnt += probe_window(s, ns, L, base, lane, colidx, nu, du, hbits, lds_member);
+      int lo, hi;
+      wave_share(sh_L, wave, NW, lo, hi);
+      wave_walk<UNROLL>(s.seg, colidx, sh_ns, lo, hi, lane, [&](const WindowLane &r) {
+        if (r.valid) cnt += member<STRAT>(s.key, hbits, nu, du, lds_member, r.w);
+      });
     }
     acc += (u64)cnt;
     __syncthreads();  // sh_task and stage reused by the next claim
@@ -198,9 +153,9 @@ __global__ void k_zero_u64(u64 *p, size_t n) {
 }
 
 template <class K>
-int persistent_grid(K kernel, int num_cu, int64_t tasks, int tasks_per_block) {
+int persistent_grid(K kernel, int num_cu, int64_t tasks, int tasks_per_block, int nt = 256) {
   int per_cu = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, 256, 0) != hipSuccess || per_cu < 1)
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, nt, 0) != hipSuccess || per_cu < 1)
     per_cu = 1;
   int64_t want = (tasks + tasks_per_block - 1) / tasks_per_block;
   int64_t cap = (int64_t)per_cu * num_cu;
@@ -212,19 +167,19 @@ hipError_t launch_tc_strat(const DevCSR &g, const TaskList tiers[NUM_TIERS], Lau
   hipError_t e = hipSuccess;
   // heavy tiers first so their long tasks overlap with nothing idle at the end
   if (tiers[TIER_BIG].count) {
-    auto k = k_tc_block<2048, STRAT>;
-    int grid = persistent_grid(k, l.num_cu, tiers[TIER_BIG].count, 1);
+    auto k = k_tc_block<2048, STRAT, 512>;
+    int grid = persistent_grid(k, l.num_cu, tiers[TIER_BIG].count, 1, 512);
     l.begin("tc_block_big", tiers[TIER_BIG].count);
-    k<<<grid, 256, 0, l.stream>>>(g.rowptr, g.colidx, tiers[TIER_BIG].tasks, (int)tiers[TIER_BIG].count,
+    k<<<grid, 512, 0, l.stream>>>(g.rowptr, g.colidx, tiers[TIER_BIG].tasks, (int)tiers[TIER_BIG].count,
                                   d_queues + TIER_BIG, l.d_out);
     l.end();
     if ((e = hipGetLastError()) != hipSuccess) return e;
   }
   if (tiers[TIER_B2048].count) {
-    auto k = k_tc_block<2048, STRAT>;
-    int grid = persistent_grid(k, l.num_cu, tiers[TIER_B2048].count, 1);
+    auto k = k_tc_block<2048, STRAT, 512>;
+    int grid = persistent_grid(k, l.num_cu, tiers[TIER_B2048].count, 1, 512);
     l.begin("tc_block_2048", tiers[TIER_B2048].count);
-    k<<<grid, 256, 0, l.stream>>>(g.rowptr, g.colidx, tiers[TIER_B2048].tasks,
+    k<<<grid, 512, 0, l.stream>>>(g.rowptr, g.colidx, tiers[TIER_B2048].tasks,
                                   (int)tiers[TIER_B2048].count, d_queues + TIER_B2048, l.d_out);
     l.end();
     if ((e = hipGetLastError()) != hipSuccess) return e;

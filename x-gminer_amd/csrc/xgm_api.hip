@@ -48,6 +48,7 @@ struct xgm_ctx {
 
 struct xgm_dgraph {
   xgm_ctx *ctx = nullptr;
+  int device = 0;
   int64_t n = 0, nnz = 0;
   uint32_t flags = 0;
   int64_t *d_rowptr = nullptr;
@@ -362,6 +363,7 @@ int xgm_graph_upload(xgm_ctx *ctx, const xgm_csr *g, xgm_dgraph **out) {
   std::unique_ptr<xgm_dgraph> d(new (std::nothrow) xgm_dgraph);
   if (!d) return XGM_ENOMEM;
   d->ctx = ctx;
+  d->device = ctx->device;
   d->n = g->n;
   d->nnz = g->nnz;
   d->flags = g->flags;
@@ -386,8 +388,9 @@ int xgm_graph_upload(xgm_ctx *ctx, const xgm_csr *g, xgm_dgraph **out) {
 
 void xgm_graph_free(xgm_dgraph *g) {
   if (!g) return;
-  (void)hipSetDevice(g->ctx->device);
-  (void)hipStreamSynchronize(g->ctx->stream);
+  // hipFree waits for outstanding work on the device; the context is not
+  // touched, so a graph may be freed after its context.
+  (void)hipSetDevice(g->device);
   delete g;
 }
 
