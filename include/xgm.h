@@ -56,8 +56,10 @@ enum xgm_strategy {
   XGM_STRAT_AUTO = 0,   /* library heuristic (degree thresholds) */
   XGM_STRAT_HASH = 1,   /* vertex task, LDS hash of the staged list, flattened probes */
   XGM_STRAT_BSEARCH = 2,/* vertex task, binary search in the staged sorted list */
-  XGM_STRAT_LUT = 3,    /* per-task bitmap LUT in LDS (cliques) */
-  XGM_STRAT_LIST = 4    /* edge task, list-based nested intersections (cliques) */
+  XGM_STRAT_LUT = 3,    /* cliques: vertex tasks with a bitmap LUT in LDS, edge LUT above lut_max */
+  XGM_STRAT_LIST = 4,   /* cliques: edge tasks, nested list intersections (G2Miner plain);
+                           4-cycle: bounded pairwise intersections in the reference loop order */
+  XGM_STRAT_ELUT = 5    /* cliques: edge tasks, bitmap LUT over N+(v0) ∩ N+(v1) */
 };
 
 typedef struct xgm_opts {

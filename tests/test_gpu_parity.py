@@ -95,7 +95,7 @@ def test_profile_reports_kernels(xgm, ctx):
 # ---------------------------------------------------------------- k-cliques
 
 CLQ = {"clique4": (2, 4), "clique5": (3, 5)}
-CLQ_STRATS = {"auto": 0, "lut": 3, "list": 4}
+CLQ_STRATS = {"auto": 0, "lut": 3, "list": 4, "elut": 5}
 
 
 @pytest.mark.parametrize("n", [4, 6, 40, 100, 300, 600])

@@ -72,92 +72,31 @@ __device__ __forceinline__ u64 lut_count(const u64 *rows, int s, int weff, int d
   return acc;
 }
 
-template <int SMAX, int W>
-struct LutStage {
-  int x[SMAX];
-  u64 rows[SMAX * W];
-  Segs<SMAX> seg;
-  int ns, L, task;
-};
-
-// Vertex-task LUT kernel: one block (NT threads) per vertex v0 with
-// min_s <= d+(v0) <= SMAX.  depth = k - 1 (K4 -> triangles, K5 -> 4-cliques).
+// Stage shared by the clique kernels: a sorted vertex set x (N+(v0) for a
+// vertex task, W1 for an edge task), a second set x2, the LUT rows over x,
+// the segment table and block-reduction slots.
 template <int SMAX, int W, int NT>
-__global__ __launch_bounds__(NT) void k_clique_vlut(const int64_t *__restrict__ rowptr,
-                                                    const int32_t *__restrict__ colidx,
-                                                    const int32_t *__restrict__ tasks, int ntasks, int min_s,
-                                                    int depth, int *__restrict__ queue, u64 *__restrict__ out) {
-  __shared__ LutStage<SMAX, W> S;
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  constexpr int NW = NT / 64;
-  u64 acc = 0;
-  for (;;) {
-    if (tid == 0) S.task = atomicAdd(queue, 1);
-    __syncthreads();
-    const int t = S.task;
-    if (t >= ntasks) break;
-    const int v0 = tasks[t];
-    const int64_t b = rowptr[v0];
-    const int s = (int)(rowptr[v0 + 1] - b);
-    if (s >= min_s) {
-      const int weff = (s + 63) >> 6;
-      for (int k = tid; k < s; k += NT) S.x[k] = colidx[b + k];
-      for (int k = tid; k < s * W; k += NT) S.rows[k] = 0ull;
-      __syncthreads();
-      if (wave == 0) {
-        int ns, L;
-        wave_segments(S.seg, rowptr, colidx, S.x, 0, s, -1, KeepAll(), lane, ns, L);
-        if (lane == 0) { S.ns = ns; S.L = L; }
-      }
-      __syncthreads();
-      const int ns = S.ns, L = S.L;
-      int lo, hi;
-      wave_share(L, wave, NW, lo, hi);
-      wave_walk<4>(S.seg, colidx, ns, lo, hi, lane, [&](const WindowLane &r) {
-        if (r.valid) {
-          const int pos = lower_bound_dev(S.x, s, r.w);
-          if (pos < s && S.x[pos] == r.w)
-            atomicOr(&S.rows[S.seg.idx[r.j] * W + (pos >> 6)], 1ull << (pos & 63));
-        }
-      });
-      __syncthreads();
-      acc += lut_count<W>(S.rows, s, weff, depth, tid, NT);
-    }
-    __syncthreads();
-  }
-  acc = wave_sum_u64(acc);
-  if (lane == 0 && acc) atomicAdd(out, acc);
-}
-
-// ------------------------------------------------------------- LIST engine
-constexpr int LIST_NT = 256;
-constexpr int LIST_LCAP = 2048;  // sets up to this size live in LDS
-constexpr int LIST_PCAP = 1024;  // segment piece
-
-struct ListStage {
-  int h0[2 * LIST_LCAP];  // hash of N+(v0) when s <= LIST_LCAP
-  int w1[LIST_LCAP];
-  int w2[LIST_LCAP];
-  Segs<LIST_PCAP> seg;
-  int cnt[LIST_NT / 64];
-  int ns, L, task;
+struct CStage {
+  int x[SMAX];
+  int x2[SMAX];
+  u64 rows[W ? SMAX * W : 1];
+  Segs<SMAX> seg;
+  int cnt[NT / 64];
+  int ns, L, item;
 };
 
 struct Member {
   const int *sorted;  // generic pointer (LDS or HBM), strictly increasing
   int n;
-  const int *hash;    // LDS hash or null
-  int hbits;
-  __device__ __forceinline__ bool contains(int w) const {
-    return hash ? hash_contains(hash, hbits, w) : sorted_contains(sorted, n, w);
-  }
+  __device__ __forceinline__ bool contains(int w) const { return sorted_contains(sorted, n, w); }
 };
 
 // out[] <- a[0..na) ∩ m, in order; returns the size.  All threads call.
-__device__ int block_intersect_ordered(const int32_t *a, int na, const Member &m, int *out, ListStage &S) {
+template <int NT, class St>
+__device__ int block_intersect_ordered(const int32_t *a, int na, const Member &m, int *out, St &S) {
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   int total = 0;
-  for (int base = 0; base < na; base += LIST_NT) {
+  for (int base = 0; base < na; base += NT) {
     const int i = base + tid;
     int w = 0;
     bool hit = false;
@@ -170,7 +109,7 @@ __device__ int block_intersect_ordered(const int32_t *a, int na, const Member &m
     __syncthreads();
     int off = total, all = 0;
 #pragma unroll
-    for (int q = 0; q < LIST_NT / 64; ++q) {
+    for (int q = 0; q < NT / 64; ++q) {
       if (q < wave) off += S.cnt[q];
       all += S.cnt[q];
     }
@@ -182,12 +121,13 @@ __device__ int block_intersect_ordered(const int32_t *a, int na, const Member &m
 }
 
 // sum over v in verts[0..nv) of |N+(v) ∩ m| with flattened probes.
+template <int NT, int PCAP, class St>
 __device__ u64 block_sum_intersections(const int64_t *__restrict__ rowptr, const int32_t *__restrict__ colidx,
-                                       const int *verts, int nv, const Member &m, ListStage &S) {
+                                       const int *verts, int nv, const Member &m, St &S) {
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   u64 acc = 0;
-  for (int k0 = 0; k0 < nv; k0 += LIST_PCAP) {
-    const int k1 = min(nv, k0 + LIST_PCAP);
+  for (int k0 = 0; k0 < nv; k0 += PCAP) {
+    const int k1 = min(nv, k0 + PCAP);
     __syncthreads();
     if (wave == 0) {
       int ns, L;
@@ -195,10 +135,9 @@ __device__ u64 block_sum_intersections(const int64_t *__restrict__ rowptr, const
       if (lane == 0) { S.ns = ns; S.L = L; }
     }
     __syncthreads();
-    const int ns = S.ns, L = S.L;
     int lo, hi;
-    wave_share(L, wave, LIST_NT / 64, lo, hi);
-    wave_walk<4>(S.seg, colidx, ns, lo, hi, lane, [&](const WindowLane &r) {
+    wave_share(S.L, wave, NT / 64, lo, hi);
+    wave_walk<4>(S.seg, colidx, S.ns, lo, hi, lane, [&](const WindowLane &r) {
       if (r.valid) acc += m.contains(r.w);
     });
   }
@@ -206,66 +145,119 @@ __device__ u64 block_sum_intersections(const int64_t *__restrict__ rowptr, const
   return acc;
 }
 
-// LIST kernel: one block per vertex v0 (any degree).  k = 4 or 5.
-// scratch: per block 2*md ints in HBM for sets larger than LIST_LCAP.
-__global__ __launch_bounds__(LIST_NT) void k_clique_list(const int64_t *__restrict__ rowptr,
-                                                         const int32_t *__restrict__ colidx,
-                                                         const int32_t *__restrict__ tasks, int ntasks, int k,
-                                                         int *__restrict__ scratch, int md,
-                                                         int *__restrict__ queue, u64 *__restrict__ out) {
-  __shared__ ListStage S;
-  const int tid = threadIdx.x, lane = tid & 63;
+// Build the LUT of the induced sub-DAG on S.x[0..s) (s <= SMAX) and count
+// its cliques of `depth` vertices.  All threads call; returns this thread's
+// partial count.
+template <int SMAX, int W, int NT>
+__device__ u64 block_lut_count(const int64_t *__restrict__ rowptr, const int32_t *__restrict__ colidx,
+                               CStage<SMAX, W, NT> &S, int s, int depth) {
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int weff = (s + 63) >> 6;
+  for (int q = tid; q < s * W; q += NT) S.rows[q] = 0ull;
+  __syncthreads();
+  if (wave == 0) {
+    int ns, L;
+    wave_segments(S.seg, rowptr, colidx, S.x, 0, s, -1, KeepAll(), lane, ns, L);
+    if (lane == 0) { S.ns = ns; S.L = L; }
+  }
+  __syncthreads();
+  int lo, hi;
+  wave_share(S.L, wave, NT / 64, lo, hi);
+  wave_walk<4>(S.seg, colidx, S.ns, lo, hi, lane, [&](const WindowLane &r) {
+    if (r.valid) {
+      const int pos = lower_bound_dev(S.x, s, r.w);
+      if (pos < s && S.x[pos] == r.w) atomicOr(&S.rows[S.seg.idx[r.j] * W + (pos >> 6)], 1ull << (pos & 63));
+    }
+  });
+  __syncthreads();
+  const u64 c = lut_count<W>(S.rows, s, weff, depth, tid, NT);
+  __syncthreads();
+  return c;
+}
+
+// Vertex-task LUT kernel: one block (NT threads) per vertex v0 with
+// min_s <= d+(v0) <= SMAX.  depth = k - 1 (K4 -> triangles, K5 -> 4-cliques).
+template <int SMAX, int W, int NT>
+__global__ __launch_bounds__(NT) void k_clique_vlut(const int64_t *__restrict__ rowptr,
+                                                    const int32_t *__restrict__ colidx,
+                                                    const int32_t *__restrict__ tasks, int ntasks, int min_s,
+                                                    int depth, int *__restrict__ queue, u64 *__restrict__ out) {
+  __shared__ CStage<SMAX, W, NT> S;
+  const int tid = threadIdx.x;
+  u64 acc = 0;
+  for (;;) {
+    if (tid == 0) S.item = atomicAdd(queue, 1);
+    __syncthreads();
+    const int t = S.item;
+    if (t >= ntasks) break;
+    const int v0 = tasks[t];
+    const int64_t b = rowptr[v0];
+    const int s = (int)(rowptr[v0 + 1] - b);
+    if (s >= min_s) {
+      for (int q = tid; q < s; q += NT) S.x[q] = colidx[b + q];
+      acc += block_lut_count<SMAX, W, NT>(rowptr, colidx, S, s, depth);
+    }
+    __syncthreads();
+  }
+  acc = wave_sum_u64(acc);
+  if ((tid & 63) == 0 && acc) atomicAdd(out, acc);
+}
+
+// Edge-task kernel (G²Miner edge parallelism): one block per DAG edge
+// (v0, v1).  W1 = N+(v0) ∩ N+(v1) is materialised in order; with LUT it is
+// encoded as a bitmap sub-DAG and K4 adds its edges, K5 its triangles
+// (GLUMIN's edge-subgraph scheme); without LUT (or |W1| > SMAX) the nested
+// list intersections of automine_4clique / automine_5clique run instead.
+// scratch: per block 2*md ints in HBM for sets larger than SMAX.
+template <int SMAX, int W, int NT>
+__global__ __launch_bounds__(NT) void k_clique_edge(const int64_t *__restrict__ rowptr,
+                                                    const int32_t *__restrict__ colidx,
+                                                    const int2 *__restrict__ items, int nitems, int k,
+                                                    int *__restrict__ scratch, int md,
+                                                    int *__restrict__ queue, u64 *__restrict__ out) {
+  __shared__ CStage<SMAX, W, NT> S;
+  const int tid = threadIdx.x;
   int *g_w1 = scratch + (size_t)blockIdx.x * 2 * md;
   int *g_w2 = g_w1 + md;
   u64 acc = 0;
   for (;;) {
-    if (tid == 0) S.task = atomicAdd(queue, 1);
+    if (tid == 0) S.item = atomicAdd(queue, 1);
     __syncthreads();
-    const int t = S.task;
-    if (t >= ntasks) break;
-    const int v0 = tasks[t];
-    const int64_t b0 = rowptr[v0];
-    const int s = (int)(rowptr[v0 + 1] - b0);
-    if (s >= k - 1) {
-      Member m0{colidx + b0, s, nullptr, 0};
-      if (s <= LIST_LCAP) {
-        const int hb = hash_bits_for(s);
-        for (int q = tid; q < (1 << hb); q += LIST_NT) S.h0[q] = -1;
-        __syncthreads();
-        for (int q = tid; q < s; q += LIST_NT) hash_insert(S.h0, hb, colidx[b0 + q]);
-        __syncthreads();
-        m0.hash = S.h0;
-        m0.hbits = hb;
-      }
-      for (int a = 0; a < s; ++a) {
-        const int v1 = colidx[b0 + a];
-        const int64_t b1 = rowptr[v1];
-        const int d1 = (int)(rowptr[v1 + 1] - b1);
-        if (d1 < k - 2) continue;
-        int *w1 = (min(s, d1) <= LIST_LCAP) ? S.w1 : g_w1;
-        const int n1 = block_intersect_ordered(colidx + b1, d1, m0, w1, S);
-        if (n1 < k - 2) continue;
-        Member m1{w1, n1, nullptr, 0};
-        if (k == 4) {
-          acc += block_sum_intersections(rowptr, colidx, w1, n1, m1, S);
+    const int t = S.item;
+    if (t >= nitems) break;
+    const int2 it = items[t];
+    const int64_t b0 = rowptr[it.x], b1 = rowptr[it.y];
+    const int d0 = (int)(rowptr[it.x + 1] - b0), d1 = (int)(rowptr[it.y + 1] - b1);
+    if (d0 >= k - 2 && d1 >= k - 2) {
+      int *w1 = (min(d0, d1) <= SMAX) ? S.x : g_w1;
+      const int n1 = block_intersect_ordered<NT>(colidx + b1, d1, Member{colidx + b0, d0}, w1, S);
+      if (n1 >= k - 2) {
+        bool done = false;
+        if constexpr (W > 0) {
+          if (w1 == S.x) {
+            acc += block_lut_count<SMAX, W, NT>(rowptr, colidx, S, n1, k - 2);
+            done = true;
+          }
+        }
+        if (done) {
+        } else if (k == 4) {
+          acc += block_sum_intersections<NT, SMAX>(rowptr, colidx, w1, n1, Member{w1, n1}, S);
         } else {
           for (int c = 0; c < n1; ++c) {
             const int w = w1[c];
             const int64_t bw = rowptr[w];
             const int dw = (int)(rowptr[w + 1] - bw);
             if (dw < 2) continue;
-            int *w2 = (min(n1, dw) <= LIST_LCAP) ? S.w2 : g_w2;
-            const int n2 = block_intersect_ordered(colidx + bw, dw, m1, w2, S);
+            int *w2 = (min(n1, dw) <= SMAX) ? S.x2 : g_w2;
+            const int n2 = block_intersect_ordered<NT>(colidx + bw, dw, Member{w1, n1}, w2, S);
             if (n2 < 2) continue;
-            Member m2{w2, n2, nullptr, 0};
-            acc += block_sum_intersections(rowptr, colidx, w2, n2, m2, S);
+            acc += block_sum_intersections<NT, SMAX>(rowptr, colidx, w2, n2, Member{w2, n2}, S);
           }
         }
       }
     }
     __syncthreads();
   }
-  (void)lane;
   acc = wave_sum_u64(acc);
   if ((tid & 63) == 0 && acc) atomicAdd(out, acc);
 }
@@ -289,34 +281,53 @@ hipError_t run_vlut(const char *name, const DevCSR &g, const TaskList &tl, int k
   return hipGetLastError();
 }
 
+template <int SMAX, int W, int NT>
+hipError_t run_edge(const char *name, const DevCSR &g, const EdgeItems &ei, int k, int *scratch, int md,
+                    int *queue, Launch &l) {
+  if (!ei.count) return hipSuccess;
+  auto kern = k_clique_edge<SMAX, W, NT>;
+  int grid = grid_for(kern, NT, 0, l.num_cu, ei.count);
+  l.begin(name, ei.count);
+  kern<<<grid, NT, 0, l.stream>>>(g.rowptr, g.colidx, (const int2 *)ei.items, (int)ei.count, k, scratch, md, queue,
+                                  l.d_out);
+  l.end();
+  return hipGetLastError();
+}
+
 }  // namespace
 
-int clique_list_grid(int num_cu, int64_t tasks) {
-  return grid_for(k_clique_list, LIST_NT, 0, num_cu, tasks);
+int clique_edge_grid(int num_cu) {
+  int a = grid_for(k_clique_edge<128, 2, 64>, 64, 0, num_cu, 1 << 30);
+  int b = grid_for(k_clique_edge<1024, 16, 256>, 256, 0, num_cu, 1 << 30);
+  int c = grid_for(k_clique_edge<128, 0, 64>, 64, 0, num_cu, 1 << 30);
+  int d = grid_for(k_clique_edge<1024, 0, 256>, 256, 0, num_cu, 1 << 30);
+  return std::max(std::max(a, b), std::max(c, d));
 }
 
 hipError_t launch_clique(const DevCSR &g, const CliqueTasks &ct, int k, int *scratch, int md, Launch &l,
                          int *d_queues) {
   hipError_t e;
-  // list path first: it holds the heaviest single tasks
-  if (ct.list.count) {
-    int grid = clique_list_grid(l.num_cu, ct.list.count);
-    l.begin(k == 4 ? "k4_list" : "k5_list", ct.list.count);
-    k_clique_list<<<grid, LIST_NT, 0, l.stream>>>(g.rowptr, g.colidx, ct.list.tasks, (int)ct.list.count, k,
-                                                  scratch, md, d_queues + 0, l.d_out);
-    l.end();
-    if ((e = hipGetLastError()) != hipSuccess) return e;
-  }
   const bool k4 = (k == 4);
-  if ((e = run_vlut<1024, 16, 1024>(k4 ? "k4_lut_1024" : "k5_lut_1024", g, ct.lut[3], k, d_queues + 1, l)) !=
-      hipSuccess)
-    return e;
-  if ((e = run_vlut<512, 8, 256>(k4 ? "k4_lut_512" : "k5_lut_512", g, ct.lut[2], k, d_queues + 2, l)) != hipSuccess)
-    return e;
-  if ((e = run_vlut<256, 4, 256>(k4 ? "k4_lut_256" : "k5_lut_256", g, ct.lut[1], k, d_queues + 3, l)) != hipSuccess)
-    return e;
-  if ((e = run_vlut<64, 1, 64>(k4 ? "k4_lut_64" : "k5_lut_64", g, ct.lut[0], k, d_queues + 4, l)) != hipSuccess)
-    return e;
+  // edge items first: they carry the hub vertices
+  if (ct.edge_lut) {
+    if ((e = run_edge<1024, 16, 256>(k4 ? "k4_edge_lut_1024" : "k5_edge_lut_1024", g, ct.edge[1], k, scratch, md,
+                                     d_queues + 5, l)))
+      return e;
+    if ((e = run_edge<128, 2, 64>(k4 ? "k4_edge_lut_128" : "k5_edge_lut_128", g, ct.edge[0], k, scratch, md,
+                                  d_queues + 6, l)))
+      return e;
+  } else {
+    if ((e = run_edge<1024, 0, 256>(k4 ? "k4_edge_list_1024" : "k5_edge_list_1024", g, ct.edge[1], k, scratch, md,
+                                    d_queues + 5, l)))
+      return e;
+    if ((e = run_edge<128, 0, 64>(k4 ? "k4_edge_list_128" : "k5_edge_list_128", g, ct.edge[0], k, scratch, md,
+                                  d_queues + 6, l)))
+      return e;
+  }
+  if ((e = run_vlut<1024, 16, 1024>(k4 ? "k4_lut_1024" : "k5_lut_1024", g, ct.lut[3], k, d_queues + 1, l))) return e;
+  if ((e = run_vlut<512, 8, 256>(k4 ? "k4_lut_512" : "k5_lut_512", g, ct.lut[2], k, d_queues + 2, l))) return e;
+  if ((e = run_vlut<256, 4, 256>(k4 ? "k4_lut_256" : "k5_lut_256", g, ct.lut[1], k, d_queues + 3, l))) return e;
+  if ((e = run_vlut<64, 1, 64>(k4 ? "k4_lut_64" : "k5_lut_64", g, ct.lut[0], k, d_queues + 4, l))) return e;
   return hipSuccess;
 }
 

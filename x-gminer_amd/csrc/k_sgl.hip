@@ -36,10 +36,12 @@ struct DiaStage {
   int ns, L, task;
 };
 
+// Items (t, k0, k1, -): owner t and the slice [k0, k1) of N(t) whose
+// partners this item probes (hub owners are split over many items).
 template <int HCAP, int PCAP, int NT, bool LDS_SET>
 __global__ __launch_bounds__(NT) void k_diamond(const int64_t *__restrict__ rowptr,
                                                 const int32_t *__restrict__ colidx,
-                                                const int32_t *__restrict__ tasks, int ntasks,
+                                                const int4 *__restrict__ items, int ntasks,
                                                 int *__restrict__ queue, u64 *__restrict__ out) {
   __shared__ DiaStage<HCAP, PCAP> S;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -49,7 +51,8 @@ __global__ __launch_bounds__(NT) void k_diamond(const int64_t *__restrict__ rowp
     __syncthreads();
     const int t = S.task;
     if (t >= ntasks) break;
-    const int v = tasks[t];
+    const int4 it = items[t];
+    const int v = it.x;
     const int64_t bt = rowptr[v];
     const int dt = (int)(rowptr[v + 1] - bt);
     const int32_t *nt = colidx + bt;
@@ -62,8 +65,8 @@ __global__ __launch_bounds__(NT) void k_diamond(const int64_t *__restrict__ rowp
     }
     // partner p owned by v iff (deg p, p) < (deg v, v)
     auto keep = [dt, v](int p, int lp) { return lp < dt || (lp == dt && p < v); };
-    for (int k0 = 0; k0 < dt; k0 += PCAP) {
-      const int k1 = min(dt, k0 + PCAP);
+    for (int k0 = it.y; k0 < it.z; k0 += PCAP) {
+      const int k1 = min(it.z, k0 + PCAP);
       __syncthreads();
       if (wave == 0) {
         int ns, L;
@@ -195,6 +198,51 @@ __global__ __launch_bounds__(NT) void k_cycle4_wedge(const int64_t *__restrict__
   if (lane == 0 && acc) atomicAdd(out, acc);
 }
 
+// Hub tasks of the wedge strategy, split into items (x, k0, k1, slot): the
+// slice [k0, k1) of x's lower neighbours.  All items of one hub add into
+// the same HBM counter array hub + slot * n with device-scope atomics, so
+// the returned previous values still sum to sum_w C(c_x(w), 2).  The
+// counter region is zeroed before the launch.
+template <int PCAP, int NT>
+__global__ __launch_bounds__(NT) void k_cycle4_hub(const int64_t *__restrict__ rowptr,
+                                                   const int32_t *__restrict__ colidx,
+                                                   const int4 *__restrict__ items, int nitems,
+                                                   int *__restrict__ hub, int64_t n,
+                                                   int *__restrict__ queue, u64 *__restrict__ out) {
+  __shared__ Segs<PCAP> seg;
+  __shared__ int sh_item, sh_ns, sh_L;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  u64 acc = 0;
+  for (;;) {
+    if (tid == 0) sh_item = atomicAdd(queue, 1);
+    __syncthreads();
+    const int t = sh_item;
+    if (t >= nitems) break;
+    const int4 it = items[t];
+    const int x = it.x;
+    const int32_t *nx = colidx + rowptr[x];
+    int *cnt = hub + (size_t)it.w * (size_t)n;
+    for (int k0 = it.y; k0 < it.z; k0 += PCAP) {
+      const int k1 = min(it.z, k0 + PCAP);
+      __syncthreads();
+      if (wave == 0) {
+        int ns, L;
+        wave_segments(seg, rowptr, colidx, nx, k0, k1, x, KeepAll(), lane, ns, L);
+        if (lane == 0) { sh_ns = ns; sh_L = L; }
+      }
+      __syncthreads();
+      int lo, hi;
+      wave_share(sh_L, wave, NT / 64, lo, hi);
+      wave_walk<4>(seg, colidx, sh_ns, lo, hi, lane, [&](const WindowLane &r) {
+        if (r.valid) acc += (u64)atomicAdd(&cnt[r.w], 1);
+      });
+    }
+    __syncthreads();
+  }
+  acc = wave_sum_u64(acc);
+  if (lane == 0 && acc) atomicAdd(out, acc);
+}
+
 // ------------------------------------------------------------- 4-cycle PAIR
 constexpr int PAIR_HCAP = 2048;
 struct PairStage {
@@ -273,7 +321,7 @@ hipError_t run_diamond(const char *name, const DevCSR &g, const TaskList &tl, in
   auto k = k_diamond<HCAP, PCAP, NT, LDS_SET>;
   int grid = grid_of(k, NT, l.num_cu, tl.count);
   l.begin(name, tl.count);
-  k<<<grid, NT, 0, l.stream>>>(g.rowptr, g.colidx, tl.tasks, (int)tl.count, queue, l.d_out);
+  k<<<grid, NT, 0, l.stream>>>(g.rowptr, g.colidx, (const int4 *)tl.tasks, (int)tl.count, queue, l.d_out);
   l.end();
   return hipGetLastError();
 }
@@ -291,7 +339,7 @@ hipError_t run_wedge_lds(const char *name, const DevCSR &g, const TaskList &tl, 
 
 }  // namespace
 
-hipError_t launch_diamond(const DevCSR &g, const TaskList tl[4], Launch &l, int *d_queues) {
+hipError_t launch_diamond(const DevCSR &g, const TaskList tl[4] /* int4 items */, Launch &l, int *d_queues) {
   hipError_t e;
   if ((e = run_diamond<1, 1024, 256, false>("diamond_hbm_set", g, tl[3], d_queues + 3, l))) return e;
   if ((e = run_diamond<8192, 1024, 256, true>("diamond_block_8192", g, tl[2], d_queues + 2, l))) return e;
@@ -304,8 +352,18 @@ int cycle4_dense_grid(int num_cu, int64_t tasks) {
   return grid_of(k_cycle4_wedge<1, 1024, 256, true>, 256, num_cu, tasks, 1);
 }
 
-hipError_t launch_cycle4_wedge(const DevCSR &g, const TaskList tl[3], int *dense, Launch &l, int *d_queues) {
+hipError_t launch_cycle4_wedge(const DevCSR &g, const TaskList tl[3], int *dense, const TaskList &hub_items,
+                               int *hub, Launch &l, int *d_queues) {
   hipError_t e;
+  if (hub_items.count) {
+    auto k = k_cycle4_hub<1024, 256>;
+    int grid = grid_of(k, 256, l.num_cu, hub_items.count);
+    l.begin("cycle4_wedge_hub", hub_items.count);
+    k<<<grid, 256, 0, l.stream>>>(g.rowptr, g.colidx, (const int4 *)hub_items.tasks, (int)hub_items.count, hub, g.n,
+                                  d_queues + 3, l.d_out);
+    l.end();
+    if ((e = hipGetLastError())) return e;
+  }
   if (tl[2].count) {
     int grid = cycle4_dense_grid(l.num_cu, tl[2].count);
     l.begin("cycle4_wedge_hbm", tl[2].count);

@@ -47,27 +47,39 @@ constexpr int kTierMaxDeg[NUM_TIERS] = {64, 256, 2048, 1 << 30};
 hipError_t launch_tc(const DevCSR &g, const TaskList tiers[NUM_TIERS], int strategy, Launch &l,
                      int *d_queues /* NUM_TIERS counters */);
 
-// Vertex tasks of a k-clique count: LUT tiers by d+(v0) (<=64, <=256,
-// <=512, <=lut_max) and the list-based remainder.
+// Edge tasks (v0, v1) of a k-clique count, as int32 pairs in HBM.
+struct EdgeItems {
+  const int32_t *items = nullptr;  // 2 * count ints
+  int64_t count = 0;
+};
+
+// Tasks of a k-clique count: vertex LUT tiers by d+(v0) (<=64, <=256, <=512,
+// <=lut_max) and edge tasks split by min(d+(v0), d+(v1)) (<=128, larger),
+// run with the LUT (edge_lut) or with nested list intersections.
 struct CliqueTasks {
   TaskList lut[4];
-  TaskList list;
+  EdgeItems edge[2];
+  bool edge_lut = true;
 };
 
 // k-clique counting (k = 4 or 5) over a DAG.  scratch holds
-// clique_list_grid() * 2 * md ints (md = max out-degree).  Adds into l.d_out.
+// clique_edge_grid() * 2 * md ints (md = max out-degree).  Adds into l.d_out.
 hipError_t launch_clique(const DevCSR &g, const CliqueTasks &ct, int k, int *scratch, int md, Launch &l,
-                         int *d_queues /* 5 counters */);
-int clique_list_grid(int num_cu, int64_t tasks);
+                         int *d_queues /* 7 counters */);
+int clique_edge_grid(int num_cu);
 
-// Edge-induced diamond over an undirected CSR; tasks by degree tier
-// (<=64, <=1024, <=8192, larger).
+// Edge-induced diamond over an undirected CSR; items are int4 (owner t,
+// k0, k1, 0) (TaskList.tasks points at 4*count ints, count = items) in
+// owner-degree tiers (<=64, <=1024, <=8192, larger).
 hipError_t launch_diamond(const DevCSR &g, const TaskList tl[4], Launch &l, int *d_queues);
 
 // Edge-induced 4-cycle, wedge aggregation over the (degree, id)-relabelled
-// CSR; tasks by wedge work (<=512, <=4096, larger).  dense holds
+// CSR; vertex tasks by wedge work (<=512, <=4096, larger); dense holds
 // cycle4_dense_grid() * n ints, all zero on entry (left zero on exit).
-hipError_t launch_cycle4_wedge(const DevCSR &g, const TaskList tl[3], int *dense, Launch &l, int *d_queues);
+// hub_items are int4 (x, k0, k1, slot) slices of hub tasks adding into
+// hub + slot * n, zeroed by the caller before the launch.
+hipError_t launch_cycle4_wedge(const DevCSR &g, const TaskList tl[3], int *dense, const TaskList &hub_items,
+                               int *hub, Launch &l, int *d_queues);
 int cycle4_dense_grid(int num_cu, int64_t tasks);
 
 // Edge-induced 4-cycle, bounded pairwise intersections in the reference's

@@ -54,6 +54,8 @@ struct xgm_dgraph {
   int64_t *d_rowptr = nullptr;
   int32_t *d_colidx = nullptr;
   int max_out = 0;
+  std::vector<int64_t> h_rowptr;  // host copy of the CSR (task preparation)
+  std::vector<int32_t> h_colidx;
   // DAG vertex tasks (d+ >= 2) by decreasing work estimate, and d+ per vertex
   std::vector<int32_t> order;
   std::vector<int32_t> dplus;
@@ -71,8 +73,13 @@ struct xgm_dgraph {
   int32_t *d_rl_colidx = nullptr;
   std::vector<int32_t> rl_order;   // relabelled task ids by decreasing wedge work
   std::vector<int64_t> rl_work;
+  std::vector<int64_t> rl_rowptr;  // host copy of the relabelled CSR
+  std::vector<int32_t> rl_colidx;
+  int *d_hub = nullptr;            // per-hub 4-cycle counters (slots * n ints)
+  size_t hub_ints = 0;
   ~xgm_dgraph() {
     (void)hipFree(d_dense);
+    (void)hipFree(d_hub);
     (void)hipFree(d_rl_rowptr);
     (void)hipFree(d_rl_colidx);
     for (auto &kv : task_cache) (void)hipFree(kv.second.first);
@@ -306,6 +313,80 @@ int tasks_split(xgm_ctx *ctx, xgm_dgraph *g, const std::string &key, const std::
   return XGM_OK;
 }
 
+// Edge items (v0, v1) for every DAG edge out of the vertices v0 in `verts`
+// with d+(v1) >= min_d1, split by min(d+(v0), d+(v1)) <= split; cached.
+int edge_items(xgm_ctx *ctx, xgm_dgraph *g, const std::string &key, const std::vector<int32_t> &verts, int min_d1,
+               int split, EdgeItems out[2]) {
+  auto it = g->task_cache.find(key);
+  if (it == g->task_cache.end()) {
+    std::vector<int32_t> small, big;
+    for (int32_t v0 : verts) {
+      for (int64_t e = g->h_rowptr[v0]; e < g->h_rowptr[v0 + 1]; ++e) {
+        int32_t v1 = g->h_colidx[e];
+        if (g->dplus[v1] < min_d1) continue;
+        auto &dst = std::min(g->dplus[v0], g->dplus[v1]) <= split ? small : big;
+        dst.push_back(v0);
+        dst.push_back(v1);
+      }
+    }
+    std::vector<int32_t> all(big);
+    all.insert(all.end(), small.begin(), small.end());
+    int32_t *d = nullptr;
+    HIP_TRY(ctx, hipMalloc(&d, std::max<size_t>(2, all.size()) * sizeof(int32_t)));
+    if (!all.empty()) {
+      hipError_t e = hipMemcpy(d, all.data(), all.size() * sizeof(int32_t), hipMemcpyHostToDevice);
+      if (e != hipSuccess) {
+        (void)hipFree(d);
+        return ctx->hip_fail(e, "H2D edge items");
+      }
+    }
+    std::vector<int64_t> counts{(int64_t)small.size() / 2, (int64_t)big.size() / 2};
+    it = g->task_cache.emplace(key, std::make_pair(d, counts)).first;
+  }
+  out[1].items = it->second.first;
+  out[1].count = it->second.second[1];
+  out[0].items = it->second.first + 2 * out[1].count;
+  out[0].count = it->second.second[0];
+  return XGM_OK;
+}
+
+// Upload parts (each `stride` ints per item) as one cached device array;
+// returns one TaskList per part (count = items).
+int cache_parts(xgm_ctx *ctx, xgm_dgraph *g, const std::string &key, const std::vector<std::vector<int32_t>> &parts,
+                int stride, std::vector<TaskList> &out) {
+  auto it = g->task_cache.find(key);
+  if (it == g->task_cache.end()) {
+    std::vector<int32_t> all;
+    std::vector<int64_t> counts;
+    for (auto &p : parts) {
+      counts.push_back((int64_t)p.size() / stride);
+      all.insert(all.end(), p.begin(), p.end());
+    }
+    int32_t *d = nullptr;
+    HIP_TRY(ctx, hipMalloc(&d, std::max<size_t>(4, all.size()) * sizeof(int32_t)));
+    if (!all.empty()) {
+      hipError_t e = hipMemcpy(d, all.data(), all.size() * sizeof(int32_t), hipMemcpyHostToDevice);
+      if (e != hipSuccess) {
+        (void)hipFree(d);
+        return ctx->hip_fail(e, "H2D items");
+      }
+    }
+    it = g->task_cache.emplace(key, std::make_pair(d, counts)).first;
+  }
+  out.clear();
+  int64_t off = 0;
+  for (int64_t c : it->second.second) {
+    TaskList t;
+    t.tasks = it->second.first + off;
+    t.count = c;
+    out.push_back(t);
+    off += c * stride;
+  }
+  return XGM_OK;
+}
+
+bool cached(const xgm_dgraph *g, const std::string &key) { return g->task_cache.count(key) != 0; }
+
 int check_device(xgm_ctx *ctx) {
   hipDeviceProp_t p;
   HIP_TRY(ctx, hipGetDeviceProperties(&p, ctx->device));
@@ -379,6 +460,12 @@ int xgm_graph_upload(xgm_ctx *ctx, const xgm_csr *g, xgm_dgraph **out) {
   if (g->nnz && (e = hipMemcpyAsync(d->d_colidx, g->colidx, g->nnz * sizeof(int32_t),
                                     hipMemcpyHostToDevice, ctx->stream)) != hipSuccess)
     return cleanup(ctx->hip_fail(e, "H2D colidx"));
+  try {
+    d->h_rowptr.assign(g->rowptr, g->rowptr + g->n + 1);
+    d->h_colidx.assign(g->colidx, g->colidx + g->nnz);
+  } catch (const std::bad_alloc &) {
+    return XGM_ENOMEM;
+  }
   if (g->flags & XGM_CSR_DAG) build_dag_order(*g, d->order, d->dplus, d->max_out);
   else if (g->flags & XGM_CSR_SYMMETRIC) build_undirected_order(*g, d->order, d->dplus, d->work);
   if ((e = hipStreamSynchronize(ctx->stream)) != hipSuccess) return cleanup(ctx->hip_fail(e, "upload sync"));
@@ -429,12 +516,13 @@ int xgm_count(xgm_ctx *ctx, const xgm_dgraph *g, int pattern, const xgm_opts *op
     case XGM_CLIQUE4:
     case XGM_CLIQUE5: {
       if (!(g->flags & XGM_CSR_DAG)) return ctx->fail(XGM_EINVAL, "CLIQUE needs a DAG-oriented CSR");
-      if (strategy != XGM_STRAT_AUTO && strategy != XGM_STRAT_LUT && strategy != XGM_STRAT_LIST)
+      if (strategy != XGM_STRAT_AUTO && strategy != XGM_STRAT_LUT && strategy != XGM_STRAT_LIST &&
+          strategy != XGM_STRAT_ELUT)
         return ctx->fail(XGM_EUNSUPPORTED, "strategy not available for CLIQUE");
       const int k = (pattern == XGM_CLIQUE4) ? 4 : 5;
       int lut_max = (opts && opts->lut_max_degree > 0) ? std::min(opts->lut_max_degree, 1024) : 1024;
-      if (strategy == XGM_STRAT_LIST) lut_max = 0;
-      // ranges: (.,64] (64,256] (256,512] (512,lut_max] (lut_max, inf) -> list
+      if (strategy == XGM_STRAT_LIST || strategy == XGM_STRAT_ELUT) lut_max = 0;
+      // vertex LUT ranges: (.,64] (64,256] (256,512] (512,lut_max]; above -> edge items
       std::vector<int> hi;
       for (int b : {64, 256, 512, 1024}) hi.push_back(std::min(b, lut_max));
       hi.push_back(INT32_MAX);
@@ -442,9 +530,13 @@ int xgm_count(xgm_ctx *ctx, const xgm_dgraph *g, int pattern, const xgm_opts *op
       if ((rc = tasks_by_ranges(ctx, gm, key, k - 1, hi, tl))) return rc;
       CliqueTasks ct;
       for (int t = 0; t < 4; ++t) ct.lut[t] = tl[t];
-      ct.list = tl[4];
-      if (ct.list.count) {
-        size_t need = (size_t)clique_list_grid(ctx->num_cu, ct.list.count) * 2 * std::max(1, g->max_out);
+      ct.edge_lut = (strategy != XGM_STRAT_LIST);
+      std::vector<int32_t> hubs;
+      for (int32_t u : gm->order)
+        if (gm->dplus[u] > lut_max && gm->dplus[u] >= k - 1) hubs.push_back(u);
+      if ((rc = edge_items(ctx, gm, key + "_edges", hubs, k - 2, 128, ct.edge))) return rc;
+      if (ct.edge[0].count + ct.edge[1].count) {
+        size_t need = (size_t)clique_edge_grid(ctx->num_cu) * 2 * std::max(1, g->max_out);
         if ((rc = ensure_scratch(ctx, gm, need))) return rc;
       }
       e = launch_clique(dg, ct, k, gm->d_scratch, std::max(1, g->max_out), l, ctx->d_queues);
@@ -454,10 +546,36 @@ int xgm_count(xgm_ctx *ctx, const xgm_dgraph *g, int pattern, const xgm_opts *op
       if (!(g->flags & XGM_CSR_SYMMETRIC)) return ctx->fail(XGM_EINVAL, "DIAMOND_EDGE needs an undirected CSR");
       if (strategy != XGM_STRAT_AUTO && strategy != XGM_STRAT_HASH)
         return ctx->fail(XGM_EUNSUPPORTED, "strategy not available for DIAMOND_EDGE");
-      const auto &deg = gm->dplus;
-      if ((rc = tasks_split(ctx, gm, "diamond", gm->order, [&](int32_t u) { return (int64_t)deg[u]; },
-                            {64, 1024, 8192, INT64_MAX}, tl)))
-        return rc;
+      std::vector<std::vector<int32_t>> parts(4);
+      if (!cached(gm, "diamond")) {
+        // owners in degree tiers; owners above 1024 split into slices of
+        // ~64K probed elements so hubs spread over the whole GPU
+        const auto &deg = gm->dplus;
+        const int64_t kChunk = 1 << 16;
+        for (int32_t t : gm->order) {
+          const int d = deg[t];
+          const int tier = d <= 64 ? 0 : d <= 1024 ? 1 : d <= 8192 ? 2 : 3;
+          auto &P = parts[tier];
+          if (tier <= 1) {
+            P.insert(P.end(), {t, 0, d, 0});
+            continue;
+          }
+          const int32_t *nt = gm->h_colidx.data() + gm->h_rowptr[t];
+          int64_t acc = 0;
+          int k0 = 0;
+          for (int k = 0; k < d; ++k) {
+            const int32_t p = nt[k];
+            if (deg[p] < d || (deg[p] == d && p < t)) acc += deg[p];
+            if (acc >= kChunk || k + 1 - k0 >= 4096) {
+              P.insert(P.end(), {t, k0, k + 1, 0});
+              k0 = k + 1;
+              acc = 0;
+            }
+          }
+          if (k0 < d) P.insert(P.end(), {t, k0, d, 0});
+        }
+      }
+      if ((rc = cache_parts(ctx, gm, "diamond", parts, 4, tl))) return rc;
       TaskList t4[4] = {tl[0], tl[1], tl[2], tl[3]};
       e = launch_diamond(dg, t4, l, ctx->d_queues);
       break;
@@ -467,11 +585,9 @@ int xgm_count(xgm_ctx *ctx, const xgm_dgraph *g, int pattern, const xgm_opts *op
       if (strategy == XGM_STRAT_LIST) {
         // reference loop order: tasks v0 with >= 2 lower neighbours (original ids)
         std::vector<int32_t> ord;
-        if (gm->task_cache.find("cycle4_pair") == gm->task_cache.end()) {
-          std::vector<int64_t> rp(g->n + 1);
-          std::vector<int32_t> ci(g->nnz);
-          HIP_TRY(ctx, hipMemcpy(rp.data(), g->d_rowptr, rp.size() * 8, hipMemcpyDeviceToHost));
-          HIP_TRY(ctx, hipMemcpy(ci.data(), g->d_colidx, ci.size() * 4, hipMemcpyDeviceToHost));
+        if (!cached(gm, "cycle4_pair")) {
+          const auto &rp = gm->h_rowptr;
+          const auto &ci = gm->h_colidx;
           for (int64_t v = 0; v < g->n; ++v) {
             int64_t lo = std::lower_bound(ci.data() + rp[v], ci.data() + rp[v + 1], (int32_t)v) - (ci.data() + rp[v]);
             if (lo >= 2) ord.push_back((int32_t)v);
@@ -485,23 +601,67 @@ int xgm_count(xgm_ctx *ctx, const xgm_dgraph *g, int pattern, const xgm_opts *op
       if (strategy != XGM_STRAT_AUTO && strategy != XGM_STRAT_HASH)
         return ctx->fail(XGM_EUNSUPPORTED, "strategy not available for CYCLE4_EDGE");
       if (!gm->relabelled) {
-        std::vector<int64_t> rp(g->n + 1), rrp;
-        std::vector<int32_t> ci(g->nnz), rci;
-        HIP_TRY(ctx, hipMemcpy(rp.data(), g->d_rowptr, rp.size() * 8, hipMemcpyDeviceToHost));
-        HIP_TRY(ctx, hipMemcpy(ci.data(), g->d_colidx, ci.size() * 4, hipMemcpyDeviceToHost));
-        xgm_csr hv{g->n, g->nnz, rp.data(), ci.data(), g->flags};
-        relabel_by_degree(hv, rrp, rci, gm->rl_order, gm->rl_work);
-        HIP_TRY(ctx, hipMalloc(&gm->d_rl_rowptr, rrp.size() * 8));
-        HIP_TRY(ctx, hipMalloc(&gm->d_rl_colidx, std::max<size_t>(1, rci.size()) * 4));
-        HIP_TRY(ctx, hipMemcpy(gm->d_rl_rowptr, rrp.data(), rrp.size() * 8, hipMemcpyHostToDevice));
-        if (!rci.empty()) HIP_TRY(ctx, hipMemcpy(gm->d_rl_colidx, rci.data(), rci.size() * 4, hipMemcpyHostToDevice));
+        xgm_csr hv{g->n, g->nnz, gm->h_rowptr.data(), gm->h_colidx.data(), g->flags};
+        relabel_by_degree(hv, gm->rl_rowptr, gm->rl_colidx, gm->rl_order, gm->rl_work);
+        HIP_TRY(ctx, hipMalloc(&gm->d_rl_rowptr, gm->rl_rowptr.size() * 8));
+        HIP_TRY(ctx, hipMalloc(&gm->d_rl_colidx, std::max<size_t>(1, gm->rl_colidx.size()) * 4));
+        HIP_TRY(ctx, hipMemcpy(gm->d_rl_rowptr, gm->rl_rowptr.data(), gm->rl_rowptr.size() * 8, hipMemcpyHostToDevice));
+        if (!gm->rl_colidx.empty())
+          HIP_TRY(ctx, hipMemcpy(gm->d_rl_colidx, gm->rl_colidx.data(), gm->rl_colidx.size() * 4, hipMemcpyHostToDevice));
         gm->relabelled = true;
       }
-      const auto &ww = gm->rl_work;
-      if ((rc = tasks_split(ctx, gm, "cycle4_wedge", gm->rl_order, [&](int32_t u) { return ww[u]; },
-                            {512, 4096, INT64_MAX}, tl)))
-        return rc;
+      // vertex tasks by wedge work: <=512, <=4096, <=2^18 (per-block HBM
+      // counters); above that (at most kMaxHubs tasks) hubs split into
+      // slices of ~2^16 wedges sharing one counter array per hub
+      const int64_t kHubWork = 1 << 18, kSlice = 1 << 16;
+      const int kMaxHubs = 512;
+      std::vector<std::vector<int32_t>> parts(4), hubp(1);
+      int nhubs = 0;
+      if (!cached(gm, "cycle4_wedge")) {
+        const auto &rp = gm->rl_rowptr;
+        const auto &ci = gm->rl_colidx;
+        for (int32_t x : gm->rl_order) {
+          const int64_t w = gm->rl_work[x];
+          if (w > kHubWork && nhubs < kMaxHubs) {
+            const int32_t *nx = ci.data() + rp[x];
+            const int nlow = (int)(std::lower_bound(nx, nx + (rp[x + 1] - rp[x]), x) - nx);
+            int64_t acc = 0;
+            int k0 = 0;
+            for (int k = 0; k < nlow; ++k) {
+              const int32_t v = nx[k];
+              const int32_t *nv = ci.data() + rp[v];
+              acc += std::lower_bound(nv, nv + (rp[v + 1] - rp[v]), x) - nv;
+              if (acc >= kSlice || k + 1 - k0 >= 4096) {
+                hubp[0].insert(hubp[0].end(), {x, k0, k + 1, nhubs});
+                k0 = k + 1;
+                acc = 0;
+              }
+            }
+            if (k0 < nlow) hubp[0].insert(hubp[0].end(), {x, k0, nlow, nhubs});
+            ++nhubs;
+            continue;
+          }
+          parts[w <= 512 ? 0 : w <= 4096 ? 1 : 2].push_back(x);
+        }
+        parts[3].push_back(nhubs);  // remembered in the cache
+      }
+      std::vector<TaskList> hl;
+      if ((rc = cache_parts(ctx, gm, "cycle4_wedge", parts, 1, tl))) return rc;
+      if ((rc = cache_parts(ctx, gm, "cycle4_hubs", hubp, 4, hl))) return rc;
       TaskList t3[3] = {tl[0], tl[1], tl[2]};
+      if (hl[0].count) {
+        int32_t nh = 0;
+        HIP_TRY(ctx, hipMemcpy(&nh, tl[3].tasks, sizeof(int32_t), hipMemcpyDeviceToHost));
+        size_t need = ((size_t)nh * (size_t)g->n + 1) & ~(size_t)1;
+        if (gm->hub_ints < need) {
+          (void)hipFree(gm->d_hub);
+          gm->d_hub = nullptr;
+          gm->hub_ints = 0;
+          HIP_TRY(ctx, hipMalloc(&gm->d_hub, need * sizeof(int)));
+          gm->hub_ints = need;
+        }
+        HIP_TRY(ctx, launch_zero(gm->d_hub, need / 2, ctx->stream));
+      }
       if (t3[2].count) {
         size_t need = (size_t)cycle4_dense_grid(ctx->num_cu, t3[2].count) * (size_t)g->n;
         need = (need + 1) & ~(size_t)1;  // whole 64-bit words for launch_zero
@@ -517,7 +677,7 @@ int xgm_count(xgm_ctx *ctx, const xgm_dgraph *g, int pattern, const xgm_opts *op
       DevCSR rg = dg;
       rg.rowptr = gm->d_rl_rowptr;
       rg.colidx = gm->d_rl_colidx;
-      e = launch_cycle4_wedge(rg, t3, gm->d_dense, l, ctx->d_queues);
+      e = launch_cycle4_wedge(rg, t3, gm->d_dense, hl[0], gm->d_hub, l, ctx->d_queues);
       break;
     }
     default:
