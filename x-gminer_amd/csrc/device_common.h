@@ -110,9 +110,10 @@ __device__ __forceinline__ bool hash_contains(const int *tab, int bits, int key)
   }
 }
 
-// Table bits for n keys: smallest b >= 6 with 2^b >= 2n.
-__device__ __forceinline__ int hash_bits_for(int n) {
+// Table bits for n keys: smallest b >= 6 with 2^b >= n << shift (load
+// factor <= 2^-shift; misses probe ~1.4 slots at 1/4, ~2.5 at 1/2).
+__device__ __forceinline__ int hash_bits_for(int n, int shift = 1) {
   int b = 6;
-  while ((1 << b) < 2 * n) ++b;
+  while ((1 << b) < (n << shift)) ++b;
   return b;
 }

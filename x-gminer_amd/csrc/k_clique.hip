@@ -82,6 +82,7 @@ struct CStage {
   u64 rows[W ? SMAX * W : 1];
   Segs<SMAX> seg;
   int cnt[NT / 64];
+  int flagrow[NT / 64][64];
   int ns, L, item;
 };
 
@@ -123,7 +124,7 @@ __device__ int block_intersect_ordered(const int32_t *a, int na, const Member &m
 // sum over v in verts[0..nv) of |N+(v) ∩ m| with flattened probes.
 template <int NT, int PCAP, class St>
 __device__ u64 block_sum_intersections(const int64_t *__restrict__ rowptr, const int32_t *__restrict__ colidx,
-                                       const int *verts, int nv, const Member &m, St &S) {
+                                       const int *verts, int nv, const Member &m, St &S, WaveFlags &fl) {
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   u64 acc = 0;
   for (int k0 = 0; k0 < nv; k0 += PCAP) {
@@ -137,7 +138,7 @@ __device__ u64 block_sum_intersections(const int64_t *__restrict__ rowptr, const
     __syncthreads();
     int lo, hi;
     wave_share(S.L, wave, NT / 64, lo, hi);
-    wave_walk<4>(S.seg, colidx, S.ns, lo, hi, lane, [&](const WindowLane &r) {
+    wave_walk<4>(S.seg, colidx, S.ns, lo, hi, lane, fl, [&](const WindowLane &r) {
       if (r.valid) acc += m.contains(r.w);
     });
   }
@@ -150,7 +151,7 @@ __device__ u64 block_sum_intersections(const int64_t *__restrict__ rowptr, const
 // partial count.
 template <int SMAX, int W, int NT>
 __device__ u64 block_lut_count(const int64_t *__restrict__ rowptr, const int32_t *__restrict__ colidx,
-                               CStage<SMAX, W, NT> &S, int s, int depth) {
+                               CStage<SMAX, W, NT> &S, int s, int depth, WaveFlags &fl) {
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int weff = (s + 63) >> 6;
   for (int q = tid; q < s * W; q += NT) S.rows[q] = 0ull;
@@ -163,7 +164,7 @@ __device__ u64 block_lut_count(const int64_t *__restrict__ rowptr, const int32_t
   __syncthreads();
   int lo, hi;
   wave_share(S.L, wave, NT / 64, lo, hi);
-  wave_walk<4>(S.seg, colidx, S.ns, lo, hi, lane, [&](const WindowLane &r) {
+  wave_walk<4>(S.seg, colidx, S.ns, lo, hi, lane, fl, [&](const WindowLane &r) {
     if (r.valid) {
       const int pos = lower_bound_dev(S.x, s, r.w);
       if (pos < s && S.x[pos] == r.w) atomicOr(&S.rows[S.seg.idx[r.j] * W + (pos >> 6)], 1ull << (pos & 63));
@@ -184,6 +185,8 @@ __global__ __launch_bounds__(NT) void k_clique_vlut(const int64_t *__restrict__ 
                                                     int depth, int *__restrict__ queue, u64 *__restrict__ out) {
   __shared__ CStage<SMAX, W, NT> S;
   const int tid = threadIdx.x;
+  WaveFlags fl;
+  fl.init(S.flagrow[tid >> 6], tid & 63);
   u64 acc = 0;
   for (;;) {
     if (tid == 0) S.item = atomicAdd(queue, 1);
@@ -195,7 +198,7 @@ __global__ __launch_bounds__(NT) void k_clique_vlut(const int64_t *__restrict__ 
     const int s = (int)(rowptr[v0 + 1] - b);
     if (s >= min_s) {
       for (int q = tid; q < s; q += NT) S.x[q] = colidx[b + q];
-      acc += block_lut_count<SMAX, W, NT>(rowptr, colidx, S, s, depth);
+      acc += block_lut_count<SMAX, W, NT>(rowptr, colidx, S, s, depth, fl);
     }
     __syncthreads();
   }
@@ -217,6 +220,8 @@ __global__ __launch_bounds__(NT) void k_clique_edge(const int64_t *__restrict__ 
                                                     int *__restrict__ queue, u64 *__restrict__ out) {
   __shared__ CStage<SMAX, W, NT> S;
   const int tid = threadIdx.x;
+  WaveFlags fl;
+  fl.init(S.flagrow[tid >> 6], tid & 63);
   int *g_w1 = scratch + (size_t)blockIdx.x * 2 * md;
   int *g_w2 = g_w1 + md;
   u64 acc = 0;
@@ -235,13 +240,13 @@ __global__ __launch_bounds__(NT) void k_clique_edge(const int64_t *__restrict__ 
         bool done = false;
         if constexpr (W > 0) {
           if (w1 == S.x) {
-            acc += block_lut_count<SMAX, W, NT>(rowptr, colidx, S, n1, k - 2);
+            acc += block_lut_count<SMAX, W, NT>(rowptr, colidx, S, n1, k - 2, fl);
             done = true;
           }
         }
         if (done) {
         } else if (k == 4) {
-          acc += block_sum_intersections<NT, SMAX>(rowptr, colidx, w1, n1, Member{w1, n1}, S);
+          acc += block_sum_intersections<NT, SMAX>(rowptr, colidx, w1, n1, Member{w1, n1}, S, fl);
         } else {
           for (int c = 0; c < n1; ++c) {
             const int w = w1[c];
@@ -251,7 +256,7 @@ __global__ __launch_bounds__(NT) void k_clique_edge(const int64_t *__restrict__ 
             int *w2 = (min(n1, dw) <= SMAX) ? S.x2 : g_w2;
             const int n2 = block_intersect_ordered<NT>(colidx + bw, dw, Member{w1, n1}, w2, S);
             if (n2 < 2) continue;
-            acc += block_sum_intersections<NT, SMAX>(rowptr, colidx, w2, n2, Member{w2, n2}, S);
+            acc += block_sum_intersections<NT, SMAX>(rowptr, colidx, w2, n2, Member{w2, n2}, S, fl);
           }
         }
       }

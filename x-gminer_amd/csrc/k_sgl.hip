@@ -31,7 +31,7 @@ namespace {
 template <int HCAP, int PCAP>
 struct DiaStage {
   int key[2 * HCAP];
-  Segs<PCAP> seg;
+  Segs<PCAP, false> seg;
   int cnt[PCAP];
   int ns, L, task;
 };
@@ -44,7 +44,10 @@ __global__ __launch_bounds__(NT) void k_diamond(const int64_t *__restrict__ rowp
                                                 const int4 *__restrict__ items, int ntasks,
                                                 int *__restrict__ queue, u64 *__restrict__ out) {
   __shared__ DiaStage<HCAP, PCAP> S;
+  __shared__ int flagrow[NT / 64][64];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  WaveFlags fl;
+  fl.init(flagrow[wave], lane);
   u64 acc = 0;
   for (;;) {
     if (tid == 0) S.task = atomicAdd(queue, 1);
@@ -79,7 +82,7 @@ __global__ __launch_bounds__(NT) void k_diamond(const int64_t *__restrict__ rowp
       __syncthreads();
       int lo, hi;
       wave_share(L, wave, NT / 64, lo, hi);
-      wave_walk<4>(S.seg, colidx, ns, lo, hi, lane, [&](const WindowLane &r) {
+      wave_walk<4>(S.seg, colidx, ns, lo, hi, lane, fl, [&](const WindowLane &r) {
         bool hit = false;
         if (r.valid) hit = LDS_SET ? hash_contains(S.key, hb, r.w) : sorted_contains(nt, dt, r.w);
         const u64 hits = __ballot(hit);
@@ -103,7 +106,7 @@ template <int HSLOTS, int PCAP>
 struct WedgeStage {
   int key[HSLOTS];
   int val[HSLOTS];
-  Segs<PCAP> seg;
+  Segs<PCAP, false> seg;
   int ns, L, task;
 };
 
@@ -133,7 +136,10 @@ __global__ __launch_bounds__(NT) void k_cycle4_wedge(const int64_t *__restrict__
                                                      int *__restrict__ dense, int64_t n,
                                                      int *__restrict__ queue, u64 *__restrict__ out) {
   __shared__ WedgeStage<DENSE ? 1 : HSLOTS, PCAP> S;
+  __shared__ int flagrow[NT / 64][64];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  WaveFlags fl;
+  fl.init(flagrow[wave], lane);
   int *cnt = DENSE ? dense + (size_t)blockIdx.x * (size_t)n : nullptr;
   u64 acc = 0;
   for (;;) {
@@ -163,7 +169,7 @@ __global__ __launch_bounds__(NT) void k_cycle4_wedge(const int64_t *__restrict__
       const int ns = S.ns, L = S.L;
       int lo, hi;
       wave_share(L, wave, NT / 64, lo, hi);
-      wave_walk<4>(S.seg, colidx, ns, lo, hi, lane, [&](const WindowLane &r) {
+      wave_walk<4>(S.seg, colidx, ns, lo, hi, lane, fl, [&](const WindowLane &r) {
         if (r.valid) {
           const int old = DENSE ? atomicAdd(&cnt[r.w], 1) : hash_bump(S.key, S.val, hb, r.w);
           acc += (u64)old;
@@ -186,7 +192,7 @@ __global__ __launch_bounds__(NT) void k_cycle4_wedge(const int64_t *__restrict__
         const int ns = S.ns, L = S.L;
         int lo, hi;
         wave_share(L, wave, NT / 64, lo, hi);
-        wave_walk<4>(S.seg, colidx, ns, lo, hi, lane, [&](const WindowLane &r) {
+        wave_walk<4>(S.seg, colidx, ns, lo, hi, lane, fl, [&](const WindowLane &r) {
           if (r.valid) cnt[r.w] = 0;
         });
       }
@@ -209,9 +215,12 @@ __global__ __launch_bounds__(NT) void k_cycle4_hub(const int64_t *__restrict__ r
                                                    const int4 *__restrict__ items, int nitems,
                                                    int *__restrict__ hub, int64_t n,
                                                    int *__restrict__ queue, u64 *__restrict__ out) {
-  __shared__ Segs<PCAP> seg;
+  __shared__ Segs<PCAP, false> seg;
+  __shared__ int flagrow[NT / 64][64];
   __shared__ int sh_item, sh_ns, sh_L;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  WaveFlags fl;
+  fl.init(flagrow[wave], lane);
   u64 acc = 0;
   for (;;) {
     if (tid == 0) sh_item = atomicAdd(queue, 1);
@@ -233,7 +242,7 @@ __global__ __launch_bounds__(NT) void k_cycle4_hub(const int64_t *__restrict__ r
       __syncthreads();
       int lo, hi;
       wave_share(sh_L, wave, NT / 64, lo, hi);
-      wave_walk<4>(seg, colidx, sh_ns, lo, hi, lane, [&](const WindowLane &r) {
+      wave_walk<4>(seg, colidx, sh_ns, lo, hi, lane, fl, [&](const WindowLane &r) {
         if (r.valid) acc += (u64)atomicAdd(&cnt[r.w], 1);
       });
     }
@@ -247,7 +256,7 @@ __global__ __launch_bounds__(NT) void k_cycle4_hub(const int64_t *__restrict__ r
 constexpr int PAIR_HCAP = 2048;
 struct PairStage {
   int key[2 * PAIR_HCAP];
-  Segs<1024> seg;
+  Segs<1024, false> seg;
   int ns, L, task;
 };
 
@@ -256,7 +265,10 @@ __global__ __launch_bounds__(256) void k_cycle4_pair(const int64_t *__restrict__
                                                      const int32_t *__restrict__ tasks, int ntasks,
                                                      int *__restrict__ queue, u64 *__restrict__ out) {
   __shared__ PairStage S;
+  __shared__ int flagrow[4][64];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  WaveFlags fl;
+  fl.init(flagrow[wave], lane);
   u64 acc = 0;
   for (;;) {
     if (tid == 0) S.task = atomicAdd(queue, 1);
@@ -295,7 +307,7 @@ __global__ __launch_bounds__(256) void k_cycle4_pair(const int64_t *__restrict__
         const int ns = S.ns, L = S.L;
         int lo, hi;
         wave_share(L, wave, 4, lo, hi);
-        wave_walk<4>(S.seg, colidx, ns, lo, hi, lane, [&](const WindowLane &r) {
+        wave_walk<4>(S.seg, colidx, ns, lo, hi, lane, fl, [&](const WindowLane &r) {
           if (r.valid) acc += lds ? hash_contains(S.key, hb, r.w) : sorted_contains(n1, s1, r.w);
         });
       }

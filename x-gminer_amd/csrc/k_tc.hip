@@ -26,13 +26,13 @@ constexpr int STRAT_HASH = 1;
 constexpr int STRAT_BSEARCH = 2;
 constexpr int UNROLL = 4;  // windows mapped before their loads are issued
 
-// LDS state of one staged task: N+(u) as a hash (or sorted copy) and the
-// flattened segment table of the lists N+(v), v in N+(u).
-template <int CAP, int STRAT>
+// LDS state of one staged task: N+(u) as a hash (load <= 1/4) or sorted
+// copy, and the flattened segment table of the lists N+(v), v in N+(u).
+template <int HCAP, int PCAP, int STRAT, int HSHIFT>
 struct Stage {
-  static constexpr int KEYS = (STRAT == STRAT_HASH) ? 2 * CAP : CAP;
+  static constexpr int KEYS = (STRAT == STRAT_HASH) ? (HCAP << HSHIFT) : HCAP;
   int key[KEYS];
-  Segs<CAP> seg;
+  Segs<PCAP, false> seg;
 };
 
 template <int STRAT>
@@ -44,14 +44,17 @@ __device__ __forceinline__ int member(const int *key, int hbits, const int32_t *
 
 // ---------------------------------------------------------------------------
 // Wave-per-task kernel: 4 independent waves per block, each owning a Stage.
-template <int CAP, int STRAT, int CLAIM>
+template <int CAP, int STRAT, int CLAIM, int HSHIFT>
 __global__ __launch_bounds__(256) void k_tc_wave(const int64_t *__restrict__ rowptr,
                                                  const int32_t *__restrict__ colidx,
                                                  const int32_t *__restrict__ tasks, int ntasks,
                                                  int *__restrict__ queue, u64 *__restrict__ out) {
-  __shared__ Stage<CAP, STRAT> stage[4];
-  const int lane = threadIdx.x & 63;
-  Stage<CAP, STRAT> &s = stage[threadIdx.x >> 6];
+  __shared__ Stage<CAP, CAP, STRAT, HSHIFT> stage[4];
+  __shared__ int flagrow[4][64];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  Stage<CAP, CAP, STRAT, HSHIFT> &s = stage[wave];
+  WaveFlags fl;
+  fl.init(flagrow[wave], lane);
   u64 acc = 0;
   for (;;) {
     int t0 = 0;
@@ -66,7 +69,7 @@ __global__ __launch_bounds__(256) void k_tc_wave(const int64_t *__restrict__ row
       const int32_t *nu = colidx + b;
       int hbits = 0;
       if (STRAT == STRAT_HASH) {
-        hbits = hash_bits_for(du);
+        hbits = hash_bits_for(du, HSHIFT);
         for (int k = lane; k < (1 << hbits); k += 64) s.key[k] = -1;
         wave_sync();
         for (int k = lane; k < du; k += 64) hash_insert(s.key, hbits, nu[k]);
@@ -77,7 +80,7 @@ __global__ __launch_bounds__(256) void k_tc_wave(const int64_t *__restrict__ row
       wave_segments(s.seg, rowptr, colidx, nu, 0, du, -1, KeepAll(), lane, ns, L);
       wave_sync();
       int cnt = 0;
-      wave_walk<UNROLL>(s.seg, colidx, ns, 0, L, lane, [&](const WindowLane &r) {
+      wave_walk<UNROLL>(s.seg, colidx, ns, 0, L, lane, fl, [&](const WindowLane &r) {
         if (r.valid) cnt += member<STRAT>(s.key, hbits, nu, du, true, r.w);
       });
       acc += (u64)cnt;
@@ -89,19 +92,23 @@ __global__ __launch_bounds__(256) void k_tc_wave(const int64_t *__restrict__ row
 }
 
 // ---------------------------------------------------------------------------
-// Block-per-task kernel: NT/64 waves share one Stage, each walking its own
-// contiguous share of the flattened segments.  If d+(u) > CAP the
+// Block-per-task kernel: NT/64 waves share one Stage, build the segment
+// table together and walk contiguous shares of it.  If d+(u) > CAP the
 // membership test binary-searches N+(u) in HBM and the segments are
 // processed in pieces of CAP.
-template <int CAP, int STRAT, int NT>
+template <int CAP, int PCAP, int STRAT, int NT, int HSHIFT>
 __global__ __launch_bounds__(NT) void k_tc_block(const int64_t *__restrict__ rowptr,
                                                  const int32_t *__restrict__ colidx,
                                                  const int32_t *__restrict__ tasks, int ntasks,
                                                  int *__restrict__ queue, u64 *__restrict__ out) {
-  __shared__ Stage<CAP, STRAT> s;
+  __shared__ Stage<CAP, PCAP, STRAT, HSHIFT> s;
+  __shared__ int tmp[PCAP];
+  __shared__ int flagrow[NT / 64][64];
   __shared__ int sh_task, sh_ns, sh_L;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   constexpr int NW = NT / 64;
+  WaveFlags fl;
+  fl.init(flagrow[wave], lane);
   u64 acc = 0;
   for (;;) {
     if (threadIdx.x == 0) sh_task = atomicAdd(queue, 1);
@@ -116,7 +123,7 @@ __global__ __launch_bounds__(NT) void k_tc_block(const int64_t *__restrict__ row
     int hbits = 0;
     if (lds_member) {
       if (STRAT == STRAT_HASH) {
-        hbits = hash_bits_for(du);
+        hbits = hash_bits_for(du, HSHIFT);
         for (int k = threadIdx.x; k < (1 << hbits); k += NT) s.key[k] = -1;
         __syncthreads();
         for (int k = threadIdx.x; k < du; k += NT) hash_insert(s.key, hbits, nu[k]);
@@ -125,18 +132,13 @@ __global__ __launch_bounds__(NT) void k_tc_block(const int64_t *__restrict__ row
       }
     }
     int cnt = 0;
-    for (int k0 = 0; k0 < du; k0 += CAP) {
-      const int k1 = min(du, k0 + CAP);
+    for (int k0 = 0; k0 < du; k0 += PCAP) {
+      const int k1 = min(du, k0 + PCAP);
       __syncthreads();  // previous piece fully consumed
-      if (wave == 0) {
-        int ns, L;
-        wave_segments(s.seg, rowptr, colidx, nu, k0, k1, -1, KeepAll(), lane, ns, L);
-        if (lane == 0) { sh_ns = ns; sh_L = L; }
-      }
-      __syncthreads();
+      block_segments<NT>(s.seg, tmp, rowptr, colidx, nu, k0, k1, -1, KeepAll(), &sh_ns, &sh_L);
       int lo, hi;
       wave_share(sh_L, wave, NW, lo, hi);
-      wave_walk<UNROLL>(s.seg, colidx, sh_ns, lo, hi, lane, [&](const WindowLane &r) {
+      wave_walk<UNROLL>(s.seg, colidx, sh_ns, lo, hi, lane, fl, [&](const WindowLane &r) {
         if (r.valid) cnt += member<STRAT>(s.key, hbits, nu, du, lds_member, r.w);
       });
     }
@@ -167,7 +169,7 @@ hipError_t launch_tc_strat(const DevCSR &g, const TaskList tiers[NUM_TIERS], Lau
   hipError_t e = hipSuccess;
   // heavy tiers first so their long tasks overlap with nothing idle at the end
   if (tiers[TIER_BIG].count) {
-    auto k = k_tc_block<2048, STRAT, 512>;
+    auto k = k_tc_block<2048, 512, STRAT, 512, 2>;
     int grid = persistent_grid(k, l.num_cu, tiers[TIER_BIG].count, 1, 512);
     l.begin("tc_block_big", tiers[TIER_BIG].count);
     k<<<grid, 512, 0, l.stream>>>(g.rowptr, g.colidx, tiers[TIER_BIG].tasks, (int)tiers[TIER_BIG].count,
@@ -176,7 +178,7 @@ hipError_t launch_tc_strat(const DevCSR &g, const TaskList tiers[NUM_TIERS], Lau
     if ((e = hipGetLastError()) != hipSuccess) return e;
   }
   if (tiers[TIER_B2048].count) {
-    auto k = k_tc_block<2048, STRAT, 512>;
+    auto k = k_tc_block<2048, 512, STRAT, 512, 2>;
     int grid = persistent_grid(k, l.num_cu, tiers[TIER_B2048].count, 1, 512);
     l.begin("tc_block_2048", tiers[TIER_B2048].count);
     k<<<grid, 512, 0, l.stream>>>(g.rowptr, g.colidx, tiers[TIER_B2048].tasks,
@@ -185,7 +187,7 @@ hipError_t launch_tc_strat(const DevCSR &g, const TaskList tiers[NUM_TIERS], Lau
     if ((e = hipGetLastError()) != hipSuccess) return e;
   }
   if (tiers[TIER_W256].count) {
-    auto k = k_tc_wave<256, STRAT, 1>;
+    auto k = k_tc_wave<256, STRAT, 1, 2>;
     int grid = persistent_grid(k, l.num_cu, tiers[TIER_W256].count, 4);
     l.begin("tc_wave_256", tiers[TIER_W256].count);
     k<<<grid, 256, 0, l.stream>>>(g.rowptr, g.colidx, tiers[TIER_W256].tasks,
@@ -194,7 +196,7 @@ hipError_t launch_tc_strat(const DevCSR &g, const TaskList tiers[NUM_TIERS], Lau
     if ((e = hipGetLastError()) != hipSuccess) return e;
   }
   if (tiers[TIER_W64].count) {
-    auto k = k_tc_wave<64, STRAT, 8>;
+    auto k = k_tc_wave<64, STRAT, 8, 2>;
     int grid = persistent_grid(k, l.num_cu, tiers[TIER_W64].count, 32);
     l.begin("tc_wave_64", tiers[TIER_W64].count);
     k<<<grid, 256, 0, l.stream>>>(g.rowptr, g.colidx, tiers[TIER_W64].tasks, (int)tiers[TIER_W64].count,

@@ -2,13 +2,16 @@
 //
 // A task probes (or aggregates) the elements of many short sorted lists
 // N(v), v in some vertex list.  Rather than giving each list its own lanes
-// (most lists are far shorter than 64), the lists are concatenated into one
-// index space [0, L): a wave walks 64 consecutive indices per window, and
-// each lane finds its list from a 64-bit mask of list ends inside the
-// window.  A wave owns a contiguous range of windows, so the first list of
-// the next window follows from the last lane of the current one (one LDS
-// read) instead of a fresh binary search, and U windows are mapped before
-// their U global loads are issued together.
+// (most lists are far shorter than 64), the non-empty lists are
+// concatenated into one index space [0, L): a wave walks 64 consecutive
+// indices per window.  Lane -> list is resolved per window without any
+// cross-lane reduction: the lanes that hold the ends of the next 64 lists
+// scatter a per-window stamp into a 64-entry LDS flag row at the position
+// where their list ends, one ballot over the row marks the lanes that open
+// a new list, and mbcnt turns it into each lane's list index.  The first
+// list of the next window is the current one plus the number of lists that
+// end inside this window (one more ballot).  A wave owns a contiguous range
+// of windows and maps U windows before issuing their U global loads.
 #pragma once
 
 #include <climits>
@@ -17,22 +20,19 @@
 
 namespace xgm {
 
-template <int PCAP>
+template <int PCAP, bool IDX = true>
 struct Segs {
-  uint32_t off[PCAP];  // first element of each non-empty list (index into colidx; nnz < 2^31)
-  int pre[PCAP];       // inclusive prefix of the lengths
-  int idx[PCAP];       // position of the owning vertex in the vertex list
+  static constexpr bool kIdx = IDX;
+  int2 os[PCAP + 1];       // .x = first element (index into colidx; nnz < 2^31), .y = start in [0, L); os[ns].y = L
+  int idx[IDX ? PCAP : 1]; // position of the owning vertex in the vertex list (if kept)
 };
 
-// Build the table for verts[k0, k1) with one wave.  keep(v, len) selects
-// lists; bound >= 0 cuts every list to its elements < bound (one binary
-// search per list).  Returns the number of non-empty lists and L.
-template <int PCAP, class Keep>
-__device__ __forceinline__ void wave_segments(Segs<PCAP> &t, const int64_t *__restrict__ rowptr,
-                                              const int32_t *__restrict__ colidx, const int *verts, int k0,
-                                              int k1, int bound, Keep keep, int lane, int &ns_out,
-                                              int &L_out) {
-  int ns = 0, carry = 0;
+// Scan of (len, keep) over verts[k0, k1) that compacts the non-empty lists
+// into t starting at slot ns_in with flattened start L_in.  One wave.
+template <class Seg, class Keep>
+__device__ __forceinline__ void wave_segments_range(Seg &t, const int64_t *__restrict__ rowptr,
+                                                    const int32_t *__restrict__ colidx, const int *verts, int k0,
+                                                    int k1, int bound, Keep keep, int lane, int &ns, int &L) {
   for (int k = k0; k < k1; k += 64) {
     const int kk = k + lane;
     int len = 0;
@@ -44,23 +44,95 @@ __device__ __forceinline__ void wave_segments(Segs<PCAP> &t, const int64_t *__re
       if (!keep(v, len)) len = 0;
       else if (bound >= 0) len = lower_bound_dev(colidx + vb, len, bound);
     }
-    const int incl = wave_incl_scan(len, lane) + carry;
+    const int incl = wave_incl_scan(len, lane) + L;
     const u64 m = __ballot(len > 0);
     if (len > 0) {
       const int pos = ns + __popcll(m & lanemask_lt(lane));
-      t.off[pos] = (uint32_t)vb;
-      t.pre[pos] = incl;
-      t.idx[pos] = kk;
+      t.os[pos] = make_int2((int)vb, incl - len);
+      if constexpr (Seg::kIdx) t.idx[pos] = kk;
     }
     ns += __popcll(m);
-    carry = __shfl(incl, 63, 64);
+    L = __shfl(incl, 63, 64);
   }
+}
+
+// Build the table for verts[k0, k1) with one wave.  keep(v, len) selects
+// lists; bound >= 0 cuts every list to its elements < bound.
+template <class Seg, class Keep>
+__device__ __forceinline__ void wave_segments(Seg &t, const int64_t *__restrict__ rowptr,
+                                              const int32_t *__restrict__ colidx, const int *verts, int k0,
+                                              int k1, int bound, Keep keep, int lane, int &ns_out,
+                                              int &L_out) {
+  int ns = 0, L = 0;
+  wave_segments_range(t, rowptr, colidx, verts, k0, k1, bound, keep, lane, ns, L);
+  if (lane == 0) t.os[ns] = make_int2(0, L);
   ns_out = ns;
-  L_out = carry;
+  L_out = L;
+}
+
+// Block-cooperative build: every thread loads the (offset, length) of its
+// share of verts[k0, k1) (k1 - k0 <= PCAP), then wave 0 compacts the
+// non-empty lists in LDS.  tmp holds PCAP ints.  All threads call; the
+// result is visible to all threads on return.
+template <int NT, class Seg, class Keep>
+__device__ __forceinline__ void block_segments(Seg &t, int *tmp, const int64_t *__restrict__ rowptr,
+                                               const int32_t *__restrict__ colidx, const int *verts, int k0,
+                                               int k1, int bound, Keep keep, int *sh_ns, int *sh_L) {
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int n = k1 - k0;
+  for (int q = tid; q < n; q += NT) {
+    const int v = verts[k0 + q];
+    const int64_t vb = rowptr[v];
+    int len = (int)(rowptr[v + 1] - vb);
+    if (!keep(v, len)) len = 0;
+    else if (bound >= 0) len = lower_bound_dev(colidx + vb, len, bound);
+    t.os[q].x = (int)vb;
+    tmp[q] = len;
+  }
+  __syncthreads();
+  if (tid < 64) {
+    int ns = 0, L = 0;
+    for (int k = 0; k < n; k += 64) {
+      const int kk = k + lane;
+      int len = 0, vb = 0;
+      if (kk < n) {
+        len = tmp[kk];
+        vb = t.os[kk].x;
+      }
+      const int incl = wave_incl_scan(len, lane) + L;
+      const u64 m = __ballot(len > 0);
+      if (len > 0) {  // in place: pos <= kk and the whole chunk was read above
+        const int pos = ns + __popcll(m & lanemask_lt(lane));
+        t.os[pos] = make_int2(vb, incl - len);
+        if constexpr (Seg::kIdx) t.idx[pos] = k0 + kk;
+      }
+      ns += __popcll(m);
+      L = __shfl(incl, 63, 64);
+    }
+    if (lane == 0) {
+      t.os[ns] = make_int2(0, L);
+      *sh_ns = ns;
+      *sh_L = L;
+    }
+  }
+  __syncthreads();
 }
 
 struct KeepAll {
   __device__ bool operator()(int, int) const { return true; }
+};
+
+// Per-wave state of the window mapping: a 64-entry LDS flag row (set to -1
+// once) and a running stamp.
+struct WaveFlags {
+  int *row;
+  int stamp;
+  __device__ __forceinline__ void init(int *r, int lane) {
+    row = r;
+    stamp = 0;
+    row[lane] = -1;
+    wave_sync();
+  }
 };
 
 // Lane view of one window: segment j, element w (valid if inside the
@@ -73,40 +145,47 @@ struct WindowLane {
   u64 heads;
 };
 
-// Walk flattened elements [lo, hi) with one wave (lo a multiple of 64 or
-// the start of the range), U windows per step.  f(const WindowLane&) is
-// called by all 64 lanes for every window (invalid lanes included, so f
-// may use ballots).
-template <int U, int PCAP, class F>
-__device__ __forceinline__ void wave_walk(const Segs<PCAP> &t, const int32_t *__restrict__ colidx, int ns,
-                                          int lo, int hi, int lane, F &&f) {
+// Walk flattened elements [lo, hi) with one wave, U windows per step.
+// f(const WindowLane&) is called by all 64 lanes for every window (invalid
+// lanes included, so f may use ballots).
+template <int U, class Seg, class F>
+__device__ __forceinline__ void wave_walk(const Seg &t, const int32_t *__restrict__ colidx, int ns,
+                                          int lo, int hi, int lane, WaveFlags &fl, F &&f) {
   if (lo >= hi) return;
-  int jl = upper_bound_dev(t.pre, ns, lo);  // segment holding element lo
+  // first segment whose end exceeds lo (ends are starts of the next one)
+  int jl;
+  {
+    int a = 0, b = ns;  // find last j with os[j].y <= lo
+    while (b - a > 1) {
+      const int mid = (a + b) >> 1;
+      if (t.os[mid].y <= lo) a = mid; else b = mid;
+    }
+    jl = a;
+  }
   for (int base = lo; base < hi; base += 64 * U) {
     WindowLane r[U];
     int64_t pos[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       const int b = base + 64 * u;
-      const int e = (jl + lane < ns) ? t.pre[jl + lane] : INT_MAX;
-      const int p = e - b;  // >= 1 for in-range windows
-      const u64 ends = wave_or_u64((p > 0 && p < 64) ? (1ull << p) : 0ull);
-      const int j = jl + __popcll(ends & lanemask_le(lane));
+      const int e = (jl + lane < ns) ? t.os[jl + lane + 1].y : INT_MAX;
+      const int p = e - b;  // >= 1 inside the range
+      ++fl.stamp;
+      if (p > 0 && p < 64) fl.row[p] = fl.stamp;
+      wave_sync();
+      const u64 heads = __ballot(fl.row[lane] == fl.stamp);
+      const int j = jl + __popcll(heads & lanemask_le(lane));
       const int i = b + lane;
       r[u].valid = i < hi;
       r[u].next_valid = i + 1 < hi;
       r[u].j = j;
-      r[u].heads = ends | 1ull;
+      r[u].heads = heads | 1ull;
       pos[u] = 0;
       if (r[u].valid) {
-        const int st = j > 0 ? t.pre[j - 1] : 0;
-        pos[u] = (int64_t)t.off[j] + (i - st);
+        const int2 s = t.os[j];
+        pos[u] = (int64_t)(uint32_t)s.x + (i - s.y);
       }
-      // first segment of the next window
-      if (u + 1 < U || base + 64 * U < hi) {
-        const int j63 = __shfl(j, 63, 64);
-        jl = (j63 < ns && t.pre[j63] <= b + 64) ? j63 + 1 : j63;
-      }
+      jl += __popcll(__ballot(p <= 64));
     }
 #pragma unroll
     for (int u = 0; u < U; ++u) r[u].w = r[u].valid ? colidx[pos[u]] : 0;
