@@ -6,13 +6,16 @@
 // the longer (codes/GraphMiner/src/triangle/gpu_kernels/bs_warp_edge.cuh:1-18,
 // xgminer/include/glumin/set_intersect.cuh:440-465).
 //
-// Design here: one task per vertex u.  N+(u) is staged once in LDS (as an
-// open-addressing hash, or as the sorted list for binary search) and every
-// element w of every N+(v), v in N+(u), is probed against it.  The probes of
-// all the v-segments are flattened into one index space [0, L(u)), so a
-// wave64 keeps all 64 lanes busy even when the segments are short; lane ->
-// segment is resolved per 64-wide window from a bitmask of segment ends.
-// Global reads are the concatenated segments, coalesced within a segment.
+// Design here: every DAG edge u->v is charged to the endpoint t whose
+// out-list is longer (ties to v), and only the shorter list is streamed:
+// per edge the work is min(d+(u), d+(v)) probes instead of d+(v).  A task
+// item (t, k0, k1) stages N+(t) once in LDS (an open-addressing hash at load
+// <= 1/4, or the sorted list for binary search; in HBM above the LDS cap)
+// and probes every element of N+(x) for the partners x in positions
+// [k0, k1) of the virtual list N+(t) ++ N-(t) (N- = in-neighbours, from the
+// reverse DAG) that were charged to t.  The probes of all partner lists are
+// flattened into one index space so a wave64 keeps all lanes busy; global
+// reads are the concatenated partner lists, coalesced within a list.
 #include <climits>
 
 #include "device_common.h"
@@ -42,12 +45,27 @@ __device__ __forceinline__ int member(const int *key, int hbits, const int32_t *
   return sorted_contains(key, du, w);
 }
 
+// Partner k of t in the virtual list N+(t) ++ N-(t), and whether the edge
+// is charged to t: out-partner x iff d+(x) < d+(t), in-partner iff d+(x) <= d+(t).
+struct TcVerts {
+  const int32_t *out, *in;
+  int dout;
+  __device__ __forceinline__ int operator()(int k) const { return k < dout ? out[k] : in[k - dout]; }
+};
+struct TcKeep {
+  int dout;
+  __device__ __forceinline__ bool operator()(int k, int, int len) const { return k < dout ? len < dout : len <= dout; }
+};
+
 // ---------------------------------------------------------------------------
-// Wave-per-task kernel: 4 independent waves per block, each owning a Stage.
-template <int CAP, int STRAT, int CLAIM, int HSHIFT>
+// Wave-per-item kernel: 4 independent waves per block, each owning a Stage.
+// Items (t, k0, k1, -) with d+(t) <= CAP and k1 - k0 <= CAP.
+template <int CAP, int STRAT, int HSHIFT>
 __global__ __launch_bounds__(256) void k_tc_wave(const int64_t *__restrict__ rowptr,
                                                  const int32_t *__restrict__ colidx,
-                                                 const int32_t *__restrict__ tasks, int ntasks,
+                                                 const int64_t *__restrict__ rrowptr,
+                                                 const int32_t *__restrict__ rcolidx,
+                                                 const int4 *__restrict__ items, int nitems,
                                                  int *__restrict__ queue, u64 *__restrict__ out) {
   __shared__ Stage<CAP, CAP, STRAT, HSHIFT> stage[4];
   __shared__ int flagrow[4][64];
@@ -57,49 +75,49 @@ __global__ __launch_bounds__(256) void k_tc_wave(const int64_t *__restrict__ row
   fl.init(flagrow[wave], lane);
   u64 acc = 0;
   for (;;) {
-    int t0 = 0;
-    if (lane == 0) t0 = atomicAdd(queue, CLAIM);
-    t0 = __shfl(t0, 0, 64);
-    if (t0 >= ntasks) break;
-    const int t1 = min(t0 + CLAIM, ntasks);
-    for (int t = t0; t < t1; ++t) {
-      const int u = tasks[t];
-      const int64_t b = rowptr[u];
-      const int du = (int)(rowptr[u + 1] - b);
-      const int32_t *nu = colidx + b;
-      int hbits = 0;
-      if (STRAT == STRAT_HASH) {
-        hbits = hash_bits_for(du, HSHIFT);
-        for (int k = lane; k < (1 << hbits); k += 64) s.key[k] = -1;
-        wave_sync();
-        for (int k = lane; k < du; k += 64) hash_insert(s.key, hbits, nu[k]);
-      } else {
-        for (int k = lane; k < du; k += 64) s.key[k] = nu[k];
-      }
-      int ns, L;
-      wave_segments(s.seg, rowptr, colidx, nu, 0, du, -1, KeepAll(), lane, ns, L);
+    int t = 0;
+    if (lane == 0) t = atomicAdd(queue, 1);
+    t = __shfl(t, 0, 64);
+    if (t >= nitems) break;
+    const int4 it = items[t];
+    const int u = it.x;
+    const int64_t b = rowptr[u];
+    const int du = (int)(rowptr[u + 1] - b);
+    const int32_t *nu = colidx + b;
+    int hbits = 0;
+    if (STRAT == STRAT_HASH) {
+      hbits = hash_bits_for(du, HSHIFT);
+      for (int k = lane; k < (1 << hbits); k += 64) s.key[k] = -1;
       wave_sync();
-      int cnt = 0;
-      wave_walk<UNROLL>(s.seg, colidx, ns, 0, L, lane, fl, [&](const WindowLane &r) {
-        if (r.valid) cnt += member<STRAT>(s.key, hbits, nu, du, true, r.w);
-      });
-      acc += (u64)cnt;
-      wave_sync();  // stage is rewritten by the next task
+      for (int k = lane; k < du; k += 64) hash_insert(s.key, hbits, nu[k]);
+    } else {
+      for (int k = lane; k < du; k += 64) s.key[k] = nu[k];
     }
+    int ns = 0, L = 0;
+    wave_segments_fn(s.seg, rowptr, it.y, it.z, TcVerts{nu, rcolidx + rrowptr[u], du}, TcKeep{du}, lane, ns, L);
+    wave_sync();
+    int cnt = 0;
+    wave_walk<UNROLL>(s.seg, colidx, ns, 0, L, lane, fl, [&](const WindowLane &r) {
+      if (r.valid) cnt += member<STRAT>(s.key, hbits, nu, du, true, r.w);
+    });
+    acc += (u64)cnt;
+    wave_sync();  // stage is rewritten by the next item
   }
   acc = wave_sum_u64(acc);
   if (lane == 0 && acc) atomicAdd(out, acc);
 }
 
 // ---------------------------------------------------------------------------
-// Block-per-task kernel: NT/64 waves share one Stage, build the segment
-// table together and walk contiguous shares of it.  If d+(u) > CAP the
-// membership test binary-searches N+(u) in HBM and the segments are
-// processed in pieces of CAP.
+// Block-per-item kernel: NT/64 waves share one Stage, build the segment
+// table together and walk contiguous shares of it.  If d+(t) > CAP the
+// membership test binary-searches N+(t) in HBM.  Partners are taken in
+// pieces of PCAP.
 template <int CAP, int PCAP, int STRAT, int NT, int HSHIFT>
 __global__ __launch_bounds__(NT) void k_tc_block(const int64_t *__restrict__ rowptr,
                                                  const int32_t *__restrict__ colidx,
-                                                 const int32_t *__restrict__ tasks, int ntasks,
+                                                 const int64_t *__restrict__ rrowptr,
+                                                 const int32_t *__restrict__ rcolidx,
+                                                 const int4 *__restrict__ items, int nitems,
                                                  int *__restrict__ queue, u64 *__restrict__ out) {
   __shared__ Stage<CAP, PCAP, STRAT, HSHIFT> s;
   __shared__ int tmp[PCAP];
@@ -114,8 +132,9 @@ __global__ __launch_bounds__(NT) void k_tc_block(const int64_t *__restrict__ row
     if (threadIdx.x == 0) sh_task = atomicAdd(queue, 1);
     __syncthreads();
     const int t = sh_task;
-    if (t >= ntasks) break;
-    const int u = tasks[t];
+    if (t >= nitems) break;
+    const int4 it = items[t];
+    const int u = it.x;
     const int64_t b = rowptr[u];
     const int du = (int)(rowptr[u + 1] - b);
     const int32_t *nu = colidx + b;
@@ -131,11 +150,12 @@ __global__ __launch_bounds__(NT) void k_tc_block(const int64_t *__restrict__ row
         for (int k = threadIdx.x; k < du; k += NT) s.key[k] = nu[k];
       }
     }
+    const TcVerts vert{nu, rcolidx + rrowptr[u], du};
     int cnt = 0;
-    for (int k0 = 0; k0 < du; k0 += PCAP) {
-      const int k1 = min(du, k0 + PCAP);
+    for (int k0 = it.y; k0 < it.z; k0 += PCAP) {
+      const int k1 = min(it.z, k0 + PCAP);
       __syncthreads();  // previous piece fully consumed
-      block_segments<NT>(s.seg, tmp, rowptr, colidx, nu, k0, k1, -1, KeepAll(), &sh_ns, &sh_L);
+      block_segments_fn<NT>(s.seg, tmp, rowptr, k0, k1, vert, TcKeep{du}, &sh_ns, &sh_L);
       int lo, hi;
       wave_share(sh_L, wave, NW, lo, hi);
       wave_walk<UNROLL>(s.seg, colidx, sh_ns, lo, hi, lane, fl, [&](const WindowLane &r) {
@@ -164,47 +184,32 @@ int persistent_grid(K kernel, int num_cu, int64_t tasks, int tasks_per_block, in
   return (int)std::max<int64_t>(1, std::min(want, cap));
 }
 
+template <class K>
+hipError_t run_tc(const char *name, K kernel, int nt, const DevCSR &g, const TaskList &items, int *queue, Launch &l) {
+  if (!items.count) return hipSuccess;
+  int grid = persistent_grid(kernel, l.num_cu, items.count, nt / 64 > 1 && nt == 256 ? 4 : 1, nt);
+  l.begin(name, items.count);
+  kernel<<<grid, nt, 0, l.stream>>>(g.rowptr, g.colidx, g.rrowptr, g.rcolidx, (const int4 *)items.tasks,
+                                    (int)items.count, queue, l.d_out);
+  l.end();
+  return hipGetLastError();
+}
+
 template <int STRAT>
 hipError_t launch_tc_strat(const DevCSR &g, const TaskList tiers[NUM_TIERS], Launch &l, int *d_queues) {
-  hipError_t e = hipSuccess;
-  // heavy tiers first so their long tasks overlap with nothing idle at the end
-  if (tiers[TIER_BIG].count) {
-    auto k = k_tc_block<2048, 512, STRAT, 512, 2>;
-    int grid = persistent_grid(k, l.num_cu, tiers[TIER_BIG].count, 1, 512);
-    l.begin("tc_block_big", tiers[TIER_BIG].count);
-    k<<<grid, 512, 0, l.stream>>>(g.rowptr, g.colidx, tiers[TIER_BIG].tasks, (int)tiers[TIER_BIG].count,
-                                  d_queues + TIER_BIG, l.d_out);
-    l.end();
-    if ((e = hipGetLastError()) != hipSuccess) return e;
-  }
-  if (tiers[TIER_B2048].count) {
-    auto k = k_tc_block<2048, 512, STRAT, 512, 2>;
-    int grid = persistent_grid(k, l.num_cu, tiers[TIER_B2048].count, 1, 512);
-    l.begin("tc_block_2048", tiers[TIER_B2048].count);
-    k<<<grid, 512, 0, l.stream>>>(g.rowptr, g.colidx, tiers[TIER_B2048].tasks,
-                                  (int)tiers[TIER_B2048].count, d_queues + TIER_B2048, l.d_out);
-    l.end();
-    if ((e = hipGetLastError()) != hipSuccess) return e;
-  }
-  if (tiers[TIER_W256].count) {
-    auto k = k_tc_wave<256, STRAT, 1, 2>;
-    int grid = persistent_grid(k, l.num_cu, tiers[TIER_W256].count, 4);
-    l.begin("tc_wave_256", tiers[TIER_W256].count);
-    k<<<grid, 256, 0, l.stream>>>(g.rowptr, g.colidx, tiers[TIER_W256].tasks,
-                                  (int)tiers[TIER_W256].count, d_queues + TIER_W256, l.d_out);
-    l.end();
-    if ((e = hipGetLastError()) != hipSuccess) return e;
-  }
-  if (tiers[TIER_W64].count) {
-    auto k = k_tc_wave<64, STRAT, 8, 2>;
-    int grid = persistent_grid(k, l.num_cu, tiers[TIER_W64].count, 32);
-    l.begin("tc_wave_64", tiers[TIER_W64].count);
-    k<<<grid, 256, 0, l.stream>>>(g.rowptr, g.colidx, tiers[TIER_W64].tasks, (int)tiers[TIER_W64].count,
-                                  d_queues + TIER_W64, l.d_out);
-    l.end();
-    if ((e = hipGetLastError()) != hipSuccess) return e;
-  }
-  return e;
+  hipError_t e;
+  // heavy tiers first so their long items overlap with nothing idle at the end
+  if ((e = run_tc("tc_block_big", k_tc_block<2048, 512, STRAT, 512, 2>, 512, g, tiers[TIER_BIG], d_queues + TIER_BIG,
+                  l)))
+    return e;
+  if ((e = run_tc("tc_block_2048", k_tc_block<2048, 512, STRAT, 512, 2>, 512, g, tiers[TIER_B2048],
+                  d_queues + TIER_B2048, l)))
+    return e;
+  if ((e = run_tc("tc_wave_256", k_tc_wave<256, STRAT, 2>, 256, g, tiers[TIER_W256], d_queues + TIER_W256, l)))
+    return e;
+  if ((e = run_tc("tc_wave_64", k_tc_wave<64, STRAT, 2>, 256, g, tiers[TIER_W64], d_queues + TIER_W64, l)))
+    return e;
+  return hipSuccess;
 }
 
 }  // namespace

@@ -18,6 +18,9 @@ struct DevCSR {
   const int64_t *rowptr = nullptr;
   const int32_t *colidx = nullptr;
   int64_t n = 0, nnz = 0;
+  // reverse (in-neighbour) lists of a DAG; null for undirected graphs
+  const int64_t *rrowptr = nullptr;
+  const int32_t *rcolidx = nullptr;
 };
 
 // A list of vertex tasks (int32 ids) resident in HBM, ordered by
@@ -38,12 +41,15 @@ struct Launch {
   std::function<void()> end;
 };
 
-// Degree tiers of vertex tasks for the DAG patterns (out-degree d+).
+// Tiers of triangle-count items by the staged out-degree d+(t).
 enum TcTier { TIER_W64 = 0, TIER_W256 = 1, TIER_B2048 = 2, TIER_BIG = 3, NUM_TIERS = 4 };
 constexpr int kTierMaxDeg[NUM_TIERS] = {64, 256, 2048, 1 << 30};
+// wave tiers take at most this many partners per item (one segment table)
+constexpr int kTcWaveCap[2] = {64, 256};
 
-// Triangle counting over a degree-oriented DAG.  strategy: 1 = LDS hash,
-// 2 = LDS binary search.  Adds the count into l.d_out.
+// Triangle counting over a degree-oriented DAG with its reverse lists.
+// tiers[] hold int4 items (t, k0, k1, 0) (TaskList.count = items).
+// strategy: 1 = LDS hash, 2 = LDS binary search.  Adds into l.d_out.
 hipError_t launch_tc(const DevCSR &g, const TaskList tiers[NUM_TIERS], int strategy, Launch &l,
                      int *d_queues /* NUM_TIERS counters */);
 

@@ -56,6 +56,34 @@ __device__ __forceinline__ void wave_segments_range(Seg &t, const int64_t *__res
   }
 }
 
+// Same as wave_segments_range over a virtual vertex list: vert(k) gives
+// the k-th vertex and keep(k, v, len) selects it.
+template <class Seg, class VertAt, class KeepK>
+__device__ __forceinline__ void wave_segments_fn(Seg &t, const int64_t *__restrict__ rowptr, int k0, int k1,
+                                                 VertAt vert, KeepK keep, int lane, int &ns, int &L) {
+  for (int k = k0; k < k1; k += 64) {
+    const int kk = k + lane;
+    int len = 0;
+    int64_t vb = 0;
+    if (kk < k1) {
+      const int v = vert(kk);
+      vb = rowptr[v];
+      len = (int)(rowptr[v + 1] - vb);
+      if (!keep(kk, v, len)) len = 0;
+    }
+    const int incl = wave_incl_scan(len, lane) + L;
+    const u64 m = __ballot(len > 0);
+    if (len > 0) {
+      const int pos = ns + __popcll(m & lanemask_lt(lane));
+      t.os[pos] = make_int2((int)vb, incl - len);
+      if constexpr (Seg::kIdx) t.idx[pos] = kk;
+    }
+    ns += __popcll(m);
+    L = __shfl(incl, 63, 64);
+  }
+  if (lane == 0) t.os[ns] = make_int2(0, L);
+}
+
 // Build the table for verts[k0, k1) with one wave.  keep(v, len) selects
 // lists; bound >= 0 cuts every list to its elements < bound.
 template <class Seg, class Keep>
@@ -102,6 +130,49 @@ __device__ __forceinline__ void block_segments(Seg &t, int *tmp, const int64_t *
       const int incl = wave_incl_scan(len, lane) + L;
       const u64 m = __ballot(len > 0);
       if (len > 0) {  // in place: pos <= kk and the whole chunk was read above
+        const int pos = ns + __popcll(m & lanemask_lt(lane));
+        t.os[pos] = make_int2(vb, incl - len);
+        if constexpr (Seg::kIdx) t.idx[pos] = k0 + kk;
+      }
+      ns += __popcll(m);
+      L = __shfl(incl, 63, 64);
+    }
+    if (lane == 0) {
+      t.os[ns] = make_int2(0, L);
+      *sh_ns = ns;
+      *sh_L = L;
+    }
+  }
+  __syncthreads();
+}
+
+// Block-cooperative build over a virtual vertex list (see wave_segments_fn).
+template <int NT, class Seg, class VertAt, class KeepK>
+__device__ __forceinline__ void block_segments_fn(Seg &t, int *tmp, const int64_t *__restrict__ rowptr, int k0,
+                                                  int k1, VertAt vert, KeepK keep, int *sh_ns, int *sh_L) {
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int n = k1 - k0;
+  for (int q = tid; q < n; q += NT) {
+    const int v = vert(k0 + q);
+    const int64_t vb = rowptr[v];
+    int len = (int)(rowptr[v + 1] - vb);
+    if (!keep(k0 + q, v, len)) len = 0;
+    t.os[q].x = (int)vb;
+    tmp[q] = len;
+  }
+  __syncthreads();
+  if (tid < 64) {
+    int ns = 0, L = 0;
+    for (int k = 0; k < n; k += 64) {
+      const int kk = k + lane;
+      int len = 0, vb = 0;
+      if (kk < n) {
+        len = tmp[kk];
+        vb = t.os[kk].x;
+      }
+      const int incl = wave_incl_scan(len, lane) + L;
+      const u64 m = __ballot(len > 0);
+      if (len > 0) {
         const int pos = ns + __popcll(m & lanemask_lt(lane));
         t.os[pos] = make_int2(vb, incl - len);
         if constexpr (Seg::kIdx) t.idx[pos] = k0 + kk;

@@ -53,6 +53,10 @@ struct xgm_dgraph {
   uint32_t flags = 0;
   int64_t *d_rowptr = nullptr;
   int32_t *d_colidx = nullptr;
+  int64_t *d_rrowptr = nullptr;  // reverse DAG (in-neighbour lists), DAG graphs only
+  int32_t *d_rcolidx = nullptr;
+  std::vector<int64_t> h_rrowptr;
+  std::vector<int32_t> h_rcolidx;
   int max_out = 0;
   std::vector<int64_t> h_rowptr;  // host copy of the CSR (task preparation)
   std::vector<int32_t> h_colidx;
@@ -85,6 +89,8 @@ struct xgm_dgraph {
     for (auto &kv : task_cache) (void)hipFree(kv.second.first);
     (void)hipFree(d_rowptr);
     (void)hipFree(d_colidx);
+    (void)hipFree(d_rrowptr);
+    (void)hipFree(d_rcolidx);
     (void)hipFree(d_scratch);
   }
 };
@@ -387,6 +393,69 @@ int cache_parts(xgm_ctx *ctx, xgm_dgraph *g, const std::string &key, const std::
 
 bool cached(const xgm_dgraph *g, const std::string &key) { return g->task_cache.count(key) != 0; }
 
+// Reverse (in-neighbour) lists of a DAG, each sorted.
+void build_reverse(const std::vector<int64_t> &rp, const std::vector<int32_t> &ci, int64_t n,
+                   std::vector<int64_t> &rrp, std::vector<int32_t> &rci) {
+  std::vector<int64_t> indeg(n, 0);
+  parallel_for((int64_t)ci.size(), 1 << 16, [&](int64_t b, int64_t e) {
+    for (int64_t k = b; k < e; ++k) __atomic_fetch_add(&indeg[ci[k]], 1, __ATOMIC_RELAXED);
+  });
+  exclusive_scan(indeg, rrp);
+  rci.resize(ci.size());
+  std::vector<int64_t> pos(rrp.begin(), rrp.end() - 1);
+  parallel_for(n, 4096, [&](int64_t b, int64_t e) {
+    for (int64_t u = b; u < e; ++u)
+      for (int64_t k = rp[u]; k < rp[u + 1]; ++k)
+        rci[__atomic_fetch_add(&pos[ci[k]], 1, __ATOMIC_RELAXED)] = (int32_t)u;
+  });
+  parallel_for(n, 4096, [&](int64_t b, int64_t e) {
+    for (int64_t v = b; v < e; ++v) std::sort(rci.begin() + rrp[v], rci.begin() + rrp[v + 1]);
+  });
+}
+
+// Triangle-count items (t, k0, k1, 0): every DAG edge is charged to the
+// endpoint with the longer out-list (ties to the head); an item covers the
+// positions [k0, k1) of N+(t) ++ N-(t), cut at ~kSlice probes or at the
+// wave tiers' segment capacity.  Tiers by d+(t); items ordered by
+// decreasing work inside a tier.
+void build_tc_items(const xgm_dgraph *g, std::vector<std::vector<int32_t>> &parts) {
+  const int64_t n = g->n;
+  const auto &dp = g->dplus;
+  const int64_t kSlice = 1 << 16;
+  struct It { int64_t work; int32_t t, k0, k1; };
+  std::vector<std::vector<It>> tiers(NUM_TIERS);
+  for (int64_t t = 0; t < n; ++t) {
+    const int d = dp[t];
+    if (d < 1) continue;
+    int tier = 0;
+    while (tier < NUM_TIERS - 1 && d > kTierMaxDeg[tier]) ++tier;
+    const int cap = tier < 2 ? kTcWaveCap[tier] : INT32_MAX;
+    const int32_t *out = g->h_colidx.data() + g->h_rowptr[t];
+    const int32_t *in = g->h_rcolidx.data() + g->h_rrowptr[t];
+    const int din = (int)(g->h_rrowptr[t + 1] - g->h_rrowptr[t]);
+    const int P = d + din;
+    int64_t acc = 0;
+    int k0 = 0;
+    for (int k = 0; k < P; ++k) {
+      const int x = k < d ? out[k] : in[k - d];
+      const bool charged = k < d ? dp[x] < d : dp[x] <= d;
+      if (charged) acc += dp[x];
+      if (acc >= kSlice || k + 1 - k0 >= cap) {
+        if (acc) tiers[tier].push_back({acc, (int32_t)t, k0, k + 1});
+        k0 = k + 1;
+        acc = 0;
+      }
+    }
+    if (acc) tiers[tier].push_back({acc, (int32_t)t, k0, P});
+  }
+  parts.assign(NUM_TIERS, {});
+  for (int t = 0; t < NUM_TIERS; ++t) {
+    auto &v = tiers[t];
+    std::stable_sort(v.begin(), v.end(), [](const It &a, const It &b) { return a.work > b.work; });
+    for (auto &it : v) parts[t].insert(parts[t].end(), {it.t, it.k0, it.k1, 0});
+  }
+}
+
 int check_device(xgm_ctx *ctx) {
   hipDeviceProp_t p;
   HIP_TRY(ctx, hipGetDeviceProperties(&p, ctx->device));
@@ -466,7 +535,24 @@ int xgm_graph_upload(xgm_ctx *ctx, const xgm_csr *g, xgm_dgraph **out) {
   } catch (const std::bad_alloc &) {
     return XGM_ENOMEM;
   }
-  if (g->flags & XGM_CSR_DAG) build_dag_order(*g, d->order, d->dplus, d->max_out);
+  if (g->flags & XGM_CSR_DAG) {
+    build_dag_order(*g, d->order, d->dplus, d->max_out);
+    try {
+      build_reverse(d->h_rowptr, d->h_colidx, g->n, d->h_rrowptr, d->h_rcolidx);
+    } catch (const std::bad_alloc &) {
+      return XGM_ENOMEM;
+    }
+    if ((e = hipMalloc(&d->d_rrowptr, (g->n + 1) * sizeof(int64_t))) != hipSuccess)
+      return cleanup(ctx->hip_fail(e, "hipMalloc(rrowptr)"));
+    if ((e = hipMalloc(&d->d_rcolidx, std::max<int64_t>(1, g->nnz) * sizeof(int32_t))) != hipSuccess)
+      return cleanup(ctx->hip_fail(e, "hipMalloc(rcolidx)"));
+    if ((e = hipMemcpyAsync(d->d_rrowptr, d->h_rrowptr.data(), (g->n + 1) * sizeof(int64_t), hipMemcpyHostToDevice,
+                            ctx->stream)) != hipSuccess)
+      return cleanup(ctx->hip_fail(e, "H2D rrowptr"));
+    if (g->nnz && (e = hipMemcpyAsync(d->d_rcolidx, d->h_rcolidx.data(), g->nnz * sizeof(int32_t),
+                                      hipMemcpyHostToDevice, ctx->stream)) != hipSuccess)
+      return cleanup(ctx->hip_fail(e, "H2D rcolidx"));
+  }
   else if (g->flags & XGM_CSR_SYMMETRIC) build_undirected_order(*g, d->order, d->dplus, d->work);
   if ((e = hipStreamSynchronize(ctx->stream)) != hipSuccess) return cleanup(ctx->hip_fail(e, "upload sync"));
   *out = d.release();
@@ -493,6 +579,8 @@ int xgm_count(xgm_ctx *ctx, const xgm_dgraph *g, int pattern, const xgm_opts *op
   dg.colidx = g->d_colidx;
   dg.n = g->n;
   dg.nnz = g->nnz;
+  dg.rrowptr = g->d_rrowptr;
+  dg.rcolidx = g->d_rcolidx;
   HIP_TRY(ctx, launch_zero(ctx->d_out, kScratchU64, ctx->stream));
   HIP_TRY(ctx, launch_zero(ctx->d_queues, kScratchQueues / 2, ctx->stream));
   Launch l = make_launch(ctx, 0);
@@ -506,8 +594,9 @@ int xgm_count(xgm_ctx *ctx, const xgm_dgraph *g, int pattern, const xgm_opts *op
       if (strategy != XGM_STRAT_AUTO && strategy != XGM_STRAT_HASH && strategy != XGM_STRAT_BSEARCH)
         return ctx->fail(XGM_EUNSUPPORTED, "strategy not available for TRIANGLE");
       int s = (strategy == XGM_STRAT_BSEARCH) ? 2 : 1;
-      std::vector<int> hi(kTierMaxDeg, kTierMaxDeg + NUM_TIERS);
-      if ((rc = tasks_by_ranges(ctx, gm, "tc", 2, hi, tl))) return rc;
+      std::vector<std::vector<int32_t>> parts;
+      if (!cached(gm, "tc_items")) build_tc_items(gm, parts);
+      if ((rc = cache_parts(ctx, gm, "tc_items", parts, 4, tl))) return rc;
       TaskList tiers[NUM_TIERS];
       for (int t = 0; t < NUM_TIERS; ++t) tiers[t] = tl[t];
       e = launch_tc(dg, tiers, s, l, ctx->d_queues);
