@@ -235,3 +235,28 @@ def test_sgl_citeseer(xgm, ctx):
     g = xgm.load_csr(CITESEER)
     assert xgm.SglSolver(g, "diamond") == 3730
     assert xgm.SglSolver(g, "rectangle") == 6059
+
+
+# ------------------------------------------- the reference's own drivers on libxgm
+
+_REF = os.path.join(os.path.dirname(__file__), "..", "oracle", "_ref")
+
+
+@pytest.mark.skipif(not os.path.exists(os.path.join(_REF, "triangle_xgm")),
+                    reason="GraphMiner drivers linked with libxgm not built (needs the reference tree)")
+@pytest.mark.parametrize("exe,args,pattern,want", [
+    ("triangle_xgm", [], r"total_num_triangles = (\d+)", 1166),
+    ("clique_xgm", ["4"], r"num_4-cliques = (\d+)", 255),
+    ("clique_xgm", ["5"], r"num_5-cliques = (\d+)", 46),
+    ("sgl_xgm", ["diamond"], r"total_num = (\d+)", 3730),
+    ("sgl_xgm", ["rectangle"], r"total_num = (\d+)", 6059),
+])
+def test_graphminer_drivers_linked_to_libxgm(exe, args, pattern, want):
+    """GraphMiner's main.cc/graph.cc (orientation, file loading) with the
+    solver entry points resolved to libxgm (x-gminer_amd/integration)."""
+    import re
+    import subprocess
+    _say("graphminer driver", exe, *args)
+    out = subprocess.run([os.path.join(_REF, exe), CITESEER] + args, capture_output=True, text=True,
+                         timeout=120, check=True).stdout
+    assert int(re.search(pattern, out).group(1)) == want
