@@ -59,7 +59,8 @@ enum xgm_strategy {
   XGM_STRAT_LUT = 3,    /* cliques: vertex tasks with a bitmap LUT in LDS, edge LUT above lut_max */
   XGM_STRAT_LIST = 4,   /* cliques: edge tasks, nested list intersections (G2Miner plain);
                            4-cycle: bounded pairwise intersections in the reference loop order */
-  XGM_STRAT_ELUT = 5    /* cliques: edge tasks, bitmap LUT over N+(v0) ∩ N+(v1) */
+  XGM_STRAT_ELUT = 5,   /* cliques: edge tasks, bitmap LUT over N+(v0) ∩ N+(v1) */
+  XGM_STRAT_LUT_RANK = 6 /* cliques: vertex LUT with rows in DAG order (upper-triangular) */
 };
 
 typedef struct xgm_opts {

@@ -95,7 +95,7 @@ def test_profile_reports_kernels(xgm, ctx):
 # ---------------------------------------------------------------- k-cliques
 
 CLQ = {"clique4": (2, 4), "clique5": (3, 5)}
-CLQ_STRATS = {"auto": 0, "lut": 3, "list": 4, "elut": 5}
+CLQ_STRATS = {"auto": 0, "lut": 3, "list": 4, "elut": 5, "lut_rank": 6}
 
 
 @pytest.mark.parametrize("n", [4, 6, 40, 100, 300, 600])
@@ -260,3 +260,20 @@ def test_graphminer_drivers_linked_to_libxgm(exe, args, pattern, want):
     out = subprocess.run([os.path.join(_REF, exe), CITESEER] + args, capture_output=True, text=True,
                          timeout=120, check=True).stdout
     assert int(re.search(pattern, out).group(1)) == want
+
+
+def test_lut_rank_rejects_foreign_orientation(xgm, ctx):
+    """An id-oriented DAG (u -> v iff v > u) is acyclic but not in (degree, id)
+    order: the rank LUT must refuse it rather than miscount; LUT accepts it."""
+    import numpy as np
+    _say("lut_rank foreign orientation")
+    g = random_graph(xgm, 300, 4000, 9)
+    src = np.repeat(np.arange(g.n), g.degree())
+    keep = g.colidx > src
+    rp = np.concatenate([[0], np.cumsum(np.bincount(src[keep], minlength=g.n))]).astype(np.int64)
+    d = xgm.Graph.from_arrays(rp, g.colidx[keep], xgm.CSR_SORTED | xgm.CSR_DAG)
+    import oracle
+    want = oracle.count("clique4", d.rowptr, d.colidx)[0]
+    assert ctx.count(d, 2, 3) == want
+    with pytest.raises(xgm.XGMError):
+        ctx.count(d, 2, 6)

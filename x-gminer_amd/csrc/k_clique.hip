@@ -176,6 +176,161 @@ __device__ u64 block_lut_count(const int64_t *__restrict__ rowptr, const int32_t
   return c;
 }
 
+// Triangular variant of lut_count: rows are indexed in DAG (degree, id)
+// order, so row a only holds bits b > a and every AND below starts at the
+// word holding bit b+1 (LDS words below it are never read).
+template <int W>
+__device__ __forceinline__ u64 lut_count_tri(const u64 *rows, int s, int weff, int depth, int tid, int nthreads) {
+  u64 acc = 0;
+  // items (a, q) with q >= a / 64 only; enumerate the triangle row by row
+  const int items = s * weff;
+  for (int t = tid; t < items; t += nthreads) {
+    const int a = t / weff, q = t - a * weff;
+    if (q < (a >> 6)) continue;
+    u64 bits = rows[a * W + q];
+    if (depth == 2) {
+      acc += __popcll(bits);
+      continue;
+    }
+    while (bits) {
+      const int j = q * 64 + __builtin_ctzll(bits);
+      bits &= bits - 1;
+      const int ws = (j + 1) >> 6;
+      if (depth == 3) {
+#pragma unroll
+        for (int w = 0; w < W; ++w)
+          if (w >= ws && w < weff) acc += __popcll(rows[a * W + w] & rows[j * W + w]);
+      } else {
+        u64 tw[W];
+#pragma unroll
+        for (int w = 0; w < W; ++w) tw[w] = (w >= ws && w < weff) ? (rows[a * W + w] & rows[j * W + w]) : 0ull;
+#pragma unroll
+        for (int w2 = 0; w2 < W; ++w2) {
+          u64 tb = tw[w2];
+          while (tb) {
+            const int k = w2 * 64 + __builtin_ctzll(tb);
+            tb &= tb - 1;
+            const int ks = (k + 1) >> 6;
+#pragma unroll
+            for (int w = 0; w < W; ++w)
+              if (w >= ks && w < weff) acc += __popcll(tw[w] & rows[k * W + w]);
+          }
+        }
+      }
+    }
+  }
+  return acc;
+}
+
+// In-LDS bitonic sort of n <= SMAX 64-bit keys with an int payload (padded
+// to a power of two with ~0); all threads call.
+template <int SMAX, int NT>
+__device__ __forceinline__ void block_bitonic_sort(u64 *key, int *val, int n) {
+  int m = 1;
+  while (m < n) m <<= 1;
+  for (int i = n + threadIdx.x; i < m; i += NT) key[i] = ~0ull;
+  __syncthreads();
+  for (int size = 2; size <= m; size <<= 1) {
+    for (int stride = size >> 1; stride > 0; stride >>= 1) {
+      for (int i = threadIdx.x; i < m; i += NT) {
+        const int j = i ^ stride;
+        if (j > i) {
+          const bool up = (i & size) == 0;
+          const u64 a = key[i], b = key[j];
+          if ((a > b) == up) {
+            key[i] = b;
+            key[j] = a;
+            const int t = val[i];
+            val[i] = val[j];
+            val[j] = t;
+          }
+        }
+      }
+      __syncthreads();
+    }
+  }
+}
+
+// Rank-ordered vertex LUT (strategy LUT_RANK): X = N+(v0) stays id-sorted
+// for lookup; perm[i] = position of X[i] in (undirected degree, id) order,
+// which is the DAG orientation order, so the LUT is upper-triangular.
+template <int SMAX, int W, int NT>
+struct RStage {
+  int x[SMAX];
+  int perm[SMAX];
+  u64 rows[SMAX * W];
+  union {  // the sort keys are dead once perm is built, before the segments
+    struct {
+      u64 key[SMAX];
+      int val[SMAX];
+    };
+    Segs<SMAX> seg;
+  };
+  int flagrow[NT / 64][64];
+  int ns, L, item;
+};
+
+template <int SMAX, int W, int NT>
+__global__ __launch_bounds__(NT) void k_clique_vlut_rank(const int64_t *__restrict__ rowptr,
+                                                         const int32_t *__restrict__ colidx,
+                                                         const int64_t *__restrict__ rrowptr,
+                                                         const int32_t *__restrict__ tasks, int ntasks, int min_s,
+                                                         int depth, int *__restrict__ queue, u64 *__restrict__ out,
+                                                         u64 *__restrict__ err) {
+  __shared__ RStage<SMAX, W, NT> S;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  WaveFlags fl;
+  fl.init(S.flagrow[wave], lane);
+  u64 acc = 0;
+  for (;;) {
+    if (tid == 0) S.item = atomicAdd(queue, 1);
+    __syncthreads();
+    const int t = S.item;
+    if (t >= ntasks) break;
+    const int v0 = tasks[t];
+    const int64_t b = rowptr[v0];
+    const int s = (int)(rowptr[v0 + 1] - b);
+    if (s >= min_s) {
+      const int weff = (s + 63) >> 6;
+      for (int q = tid; q < s; q += NT) {
+        const int x = colidx[b + q];
+        S.x[q] = x;
+        const u64 deg = (u64)((rowptr[x + 1] - rowptr[x]) + (rrowptr[x + 1] - rrowptr[x]));
+        S.key[q] = (deg << 32) | (u64)(uint32_t)x;  // (undirected degree, id): the orientation order
+        S.val[q] = q;
+      }
+      for (int q = tid; q < s * W; q += NT) S.rows[q] = 0ull;
+      __syncthreads();
+      block_bitonic_sort<SMAX, NT>(S.key, S.val, s);
+      for (int r = tid; r < s; r += NT) S.perm[S.val[r]] = r;
+      __syncthreads();  // keys are overwritten by the segment table
+      if (wave == 0) {
+        int ns, L;
+        wave_segments(S.seg, rowptr, colidx, S.x, 0, s, -1, KeepAll(), lane, ns, L);
+        if (lane == 0) { S.ns = ns; S.L = L; }
+      }
+      __syncthreads();
+      int lo, hi;
+      wave_share(S.L, wave, NT / 64, lo, hi);
+      wave_walk<4>(S.seg, colidx, S.ns, lo, hi, lane, fl, [&](const WindowLane &r) {
+        if (r.valid) {
+          const int pos = lower_bound_dev(S.x, s, r.w);
+          if (pos < s && S.x[pos] == r.w) {
+            const int col = S.perm[pos], row = S.perm[S.seg.idx[r.j]];
+            if (col <= row) atomicOr(err, 1ull);  // DAG not in (degree, id) orientation
+            atomicOr(&S.rows[row * W + (col >> 6)], 1ull << (col & 63));
+          }
+        }
+      });
+      __syncthreads();
+      acc += lut_count_tri<W>(S.rows, s, weff, depth, tid, NT);
+    }
+    __syncthreads();
+  }
+  acc = wave_sum_u64(acc);
+  if (lane == 0 && acc) atomicAdd(out, acc);
+}
+
 // Vertex-task LUT kernel: one block (NT threads) per vertex v0 with
 // min_s <= d+(v0) <= SMAX.  depth = k - 1 (K4 -> triangles, K5 -> 4-cliques).
 template <int SMAX, int W, int NT>
@@ -287,6 +442,18 @@ hipError_t run_vlut(const char *name, const DevCSR &g, const TaskList &tl, int k
 }
 
 template <int SMAX, int W, int NT>
+hipError_t run_vlut_rank(const char *name, const DevCSR &g, const TaskList &tl, int k, int *queue, Launch &l) {
+  if (!tl.count) return hipSuccess;
+  auto kern = k_clique_vlut_rank<SMAX, W, NT>;
+  int grid = grid_for(kern, NT, 0, l.num_cu, tl.count);
+  l.begin(name, tl.count);
+  kern<<<grid, NT, 0, l.stream>>>(g.rowptr, g.colidx, g.rrowptr, tl.tasks, (int)tl.count, k - 1, k - 1, queue,
+                                  l.d_out, l.d_err);
+  l.end();
+  return hipGetLastError();
+}
+
+template <int SMAX, int W, int NT>
 hipError_t run_edge(const char *name, const DevCSR &g, const EdgeItems &ei, int k, int *scratch, int md,
                     int *queue, Launch &l) {
   if (!ei.count) return hipSuccess;
@@ -328,6 +495,16 @@ hipError_t launch_clique(const DevCSR &g, const CliqueTasks &ct, int k, int *scr
     if ((e = run_edge<128, 0, 64>(k4 ? "k4_edge_list_128" : "k5_edge_list_128", g, ct.edge[0], k, scratch, md,
                                   d_queues + 6, l)))
       return e;
+  }
+  if (ct.rank) {
+    if ((e = run_vlut_rank<1024, 16, 1024>(k4 ? "k4_rlut_1024" : "k5_rlut_1024", g, ct.lut[3], k, d_queues + 1, l)))
+      return e;
+    if ((e = run_vlut_rank<512, 8, 256>(k4 ? "k4_rlut_512" : "k5_rlut_512", g, ct.lut[2], k, d_queues + 2, l)))
+      return e;
+    if ((e = run_vlut_rank<256, 4, 256>(k4 ? "k4_rlut_256" : "k5_rlut_256", g, ct.lut[1], k, d_queues + 3, l)))
+      return e;
+    if ((e = run_vlut_rank<64, 1, 64>(k4 ? "k4_rlut_64" : "k5_rlut_64", g, ct.lut[0], k, d_queues + 4, l))) return e;
+    return hipSuccess;
   }
   if ((e = run_vlut<1024, 16, 1024>(k4 ? "k4_lut_1024" : "k5_lut_1024", g, ct.lut[3], k, d_queues + 1, l))) return e;
   if ((e = run_vlut<512, 8, 256>(k4 ? "k4_lut_512" : "k5_lut_512", g, ct.lut[2], k, d_queues + 2, l))) return e;

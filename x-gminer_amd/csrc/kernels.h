@@ -35,6 +35,7 @@ struct Launch {
   hipStream_t stream = nullptr;
   int num_cu = 256;
   u64 *d_out = nullptr;   // accumulator slot the launch adds into (pre-zeroed)
+  u64 *d_err = nullptr;   // error word kernels may OR into (pre-zeroed)
   int *d_queue = nullptr; // work counter the launch claims from (pre-zeroed)
   // Bracket one kernel launch with events: begin(name, tasks) / end().
   std::function<void(const char *, int64_t)> begin;
@@ -66,6 +67,7 @@ struct CliqueTasks {
   TaskList lut[4];
   EdgeItems edge[2];
   bool edge_lut = true;
+  bool rank = false;  // vertex LUT rows in (degree, id) order (triangular)
 };
 
 // k-clique counting (k = 4 or 5) over a DAG.  scratch holds

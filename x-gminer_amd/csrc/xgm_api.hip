@@ -108,6 +108,7 @@ Launch make_launch(xgm_ctx *ctx, int slot) {
   l.stream = ctx->stream;
   l.num_cu = ctx->num_cu;
   l.d_out = ctx->d_out + slot;
+  l.d_err = ctx->d_out + (kScratchU64 - 1);
   l.d_queue = ctx->d_queues;
   l.begin = [ctx](const char *name, int64_t tasks) {
     int i = ctx->prof.n_kernels;
@@ -606,7 +607,7 @@ int xgm_count(xgm_ctx *ctx, const xgm_dgraph *g, int pattern, const xgm_opts *op
     case XGM_CLIQUE5: {
       if (!(g->flags & XGM_CSR_DAG)) return ctx->fail(XGM_EINVAL, "CLIQUE needs a DAG-oriented CSR");
       if (strategy != XGM_STRAT_AUTO && strategy != XGM_STRAT_LUT && strategy != XGM_STRAT_LIST &&
-          strategy != XGM_STRAT_ELUT)
+          strategy != XGM_STRAT_ELUT && strategy != XGM_STRAT_LUT_RANK)
         return ctx->fail(XGM_EUNSUPPORTED, "strategy not available for CLIQUE");
       const int k = (pattern == XGM_CLIQUE4) ? 4 : 5;
       int lut_max = (opts && opts->lut_max_degree > 0) ? std::min(opts->lut_max_degree, 1024) : 1024;
@@ -620,6 +621,7 @@ int xgm_count(xgm_ctx *ctx, const xgm_dgraph *g, int pattern, const xgm_opts *op
       CliqueTasks ct;
       for (int t = 0; t < 4; ++t) ct.lut[t] = tl[t];
       ct.edge_lut = (strategy != XGM_STRAT_LIST);
+      ct.rank = (strategy == XGM_STRAT_LUT_RANK);
       std::vector<int32_t> hubs;
       for (int32_t u : gm->order)
         if (gm->dplus[u] > lut_max && gm->dplus[u] >= k - 1) hubs.push_back(u);
@@ -773,9 +775,12 @@ int xgm_count(xgm_ctx *ctx, const xgm_dgraph *g, int pattern, const xgm_opts *op
       return ctx->fail(XGM_EUNSUPPORTED, "pattern not supported");
   }
   if (e != hipSuccess) return ctx->hip_fail(e, "kernel launch");
-  u64 h = 0;
-  HIP_TRY(ctx, hipMemcpyAsync(&h, ctx->d_out, sizeof(u64), hipMemcpyDeviceToHost, ctx->stream));
+  u64 hv[kScratchU64];
+  HIP_TRY(ctx, hipMemcpyAsync(hv, ctx->d_out, sizeof(hv), hipMemcpyDeviceToHost, ctx->stream));
   HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  const u64 h = hv[0];
+  if (hv[kScratchU64 - 1])
+    return ctx->fail(XGM_EINVAL, "DAG is not in (degree, id) orientation; use XGM_STRAT_LUT");
   float total = 0.f;
   for (int i = 0; i < ctx->prof.n_kernels; ++i) {
     float ms = 0.f;
