@@ -113,6 +113,37 @@ typedef struct xgm_profile {
 } xgm_profile;
 int xgm_last_profile(const xgm_ctx *ctx, xgm_profile *out);
 
+/* ---------------------------------------------------------------------
+ * Learned scheduler.  No counterpart in the reference (which uses fixed
+ * degree thresholds, xgminer/include/common.h:61,114-116).  A 16-64-64-5
+ * MLP predicts per-task log-costs of the clique strategies; it is trained
+ * and evaluated on the GPU by the library's own BF16-MFMA kernels and only
+ * chooses schedules: counts are identical under every schedule.
+ * --------------------------------------------------------------------- */
+#define XGM_MLP_IN 16
+#define XGM_MLP_OUT 5
+#define XGM_MLP_PARAMS 5573 /* W1[16x64] b1[64] W2[64x64] b2[64] W3[64x5] b3[5] */
+typedef struct xgm_mlp xgm_mlp;
+int xgm_mlp_create(int device, uint64_t seed, xgm_mlp **out);
+void xgm_mlp_free(xgm_mlp *m);
+const char *xgm_mlp_last_error(const xgm_mlp *m);
+int xgm_mlp_set_params(xgm_mlp *m, const float *params);
+int xgm_mlp_get_params(const xgm_mlp *m, float *params);
+/* Y[n x 5] = model(X[n x 16]) (host arrays, batched on the device). */
+int xgm_mlp_predict(xgm_mlp *m, const float *X, int64_t n, float *Y);
+/* Masked mean-squared-error training (Adam, shuffled mini-batches);
+ * T, M are n x 5 targets and 0/1 masks.  *final_loss = mean batch loss of
+ * the last epoch. */
+int xgm_mlp_train(xgm_mlp *m, const float *X, const float *T, const float *M, int64_t n, int epochs, int batch,
+                  float lr, uint64_t seed, float *final_loss);
+/* One full-batch loss and gradient (XGM_MLP_PARAMS floats), for checking
+ * the backward pass against a CPU reference. */
+int xgm_mlp_grads(xgm_mlp *m, const float *X, const float *T, const float *M, int64_t n, float *grads,
+                  float *loss);
+/* C = A*B through the MFMA tile GEMM (A, B rounded to BF16), for checking
+ * the matrix-core operand layout. */
+int xgm_mfma_gemm_check(int device, int M, int N, int K, const float *A, const float *B, float *C);
+
 #ifdef __cplusplus
 }
 #endif

@@ -190,3 +190,37 @@ def test_restatement_matches_reference_binaries(xgm, orc):
     mine = _counts(orc, g)
     for p in mine:
         assert orc.ref_count_graph(p, g)[0] == mine[p], p
+
+
+def test_mlp_reference_gradients_vs_finite_differences():
+    """Pins oracle/mlp_ref.py: its analytic backward pass (rounding off)
+    equals central finite differences of its own loss, parameter by
+    parameter on a sample of every tensor."""
+    from oracle import mlp_ref
+    rng = np.random.default_rng(0)
+    p = rng.standard_normal(mlp_ref.NPARAM) * 0.3
+    X = rng.standard_normal((40, 16))
+    T = rng.standard_normal((40, 5))
+    M = (rng.random((40, 5)) < 0.8).astype(np.float64)
+    _, g = mlp_ref.loss_and_grads(p, X, T, M, rounding=False)
+    h = 1e-6
+    o = 0
+    for name, shape in mlp_ref.SIZES:
+        k = int(np.prod(shape))
+        for i in rng.choice(k, size=min(k, 12), replace=False):
+            q = p.copy(); q[o + i] += h
+            lp = mlp_ref.loss(q, X, T, M, rounding=False)
+            q[o + i] -= 2 * h
+            lm = mlp_ref.loss(q, X, T, M, rounding=False)
+            fd = (lp - lm) / (2 * h)
+            assert abs(fd - g[o + i]) <= 1e-5 * max(1.0, abs(fd)), (name, i, fd, g[o + i])
+        o += k
+
+
+def test_bf16_rounding_known_values():
+    from oracle.mlp_ref import bf16
+    # exactly representable, round-half-to-even, and carry into the exponent
+    assert bf16(1.0) == 1.0 and bf16(-2.5) == -2.5
+    assert bf16(1.0 + 2 ** -8) == 1.0                 # halfway, even neighbour 1.0
+    assert bf16(1.0 + 3 * 2 ** -8) == 1.0 + 2 ** -6   # halfway, rounds up to even
+    assert bf16(1.0 + 2 ** -7 + 2 ** -9) == 1.0 + 2 ** -7

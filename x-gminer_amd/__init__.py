@@ -95,6 +95,18 @@ def _load():
         "xgm_count": (C.c_int, [VP, VP, C.c_int, P(_Opts), P(U64)]),
         "xgm_count_csr": (C.c_int, [VP, P(_Csr), C.c_int, P(_Opts), P(U64)]),
         "xgm_last_profile": (C.c_int, [VP, P(_Profile)]),
+        "xgm_mlp_create": (C.c_int, [C.c_int, U64, P(VP)]),
+        "xgm_mlp_free": (None, [VP]),
+        "xgm_mlp_last_error": (C.c_char_p, [VP]),
+        "xgm_mlp_set_params": (C.c_int, [VP, P(C.c_float)]),
+        "xgm_mlp_get_params": (C.c_int, [VP, P(C.c_float)]),
+        "xgm_mlp_predict": (C.c_int, [VP, P(C.c_float), I64, P(C.c_float)]),
+        "xgm_mlp_train": (C.c_int, [VP, P(C.c_float), P(C.c_float), P(C.c_float), I64, C.c_int, C.c_int,
+                                    C.c_float, U64, P(C.c_float)]),
+        "xgm_mlp_grads": (C.c_int, [VP, P(C.c_float), P(C.c_float), P(C.c_float), I64, P(C.c_float),
+                                    P(C.c_float)]),
+        "xgm_mfma_gemm_check": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, P(C.c_float), P(C.c_float),
+                                          P(C.c_float)]),
     }
     for name, (res, args) in sigs.items():
         fn = getattr(lib, name)
@@ -307,3 +319,80 @@ def SglSolver(g: Graph, pattern: str) -> int:
     if pid is None:
         raise XGMError(XGM_EUNSUPPORTED, f"pattern {pattern!r} not supported")
     return _ctx().count(g, pid)
+
+
+# --------------------------------------------------------------------------
+# learned scheduler cost model (16-64-64-5 MLP on the matrix cores)
+
+MLP_IN, MLP_OUT, MLP_PARAMS = 16, 5, 5573
+
+
+def _f32(a, shape=None):
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    if shape is not None:
+        a = a.reshape(shape)
+    return a, a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+class MLP:
+    def __init__(self, device: int = 0, seed: int = 1):
+        h = C.c_void_p()
+        _check(lib.xgm_mlp_create(device, seed, C.byref(h)))
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None) is not None:
+            lib.xgm_mlp_free(self._h)
+            self._h = None
+
+    __del__ = close
+
+    def _chk(self, rc):
+        if rc != XGM_OK:
+            raise XGMError(rc, lib.xgm_strerror(rc).decode() + " (" + lib.xgm_mlp_last_error(self._h).decode() + ")")
+
+    @property
+    def params(self):
+        out = np.zeros(MLP_PARAMS, np.float32)
+        self._chk(lib.xgm_mlp_get_params(self._h, out.ctypes.data_as(C.POINTER(C.c_float))))
+        return out
+
+    @params.setter
+    def params(self, p):
+        p, pp = _f32(p, (MLP_PARAMS,))
+        self._chk(lib.xgm_mlp_set_params(self._h, pp))
+
+    def predict(self, X):
+        X, px = _f32(X, (-1, MLP_IN))
+        Y = np.zeros((len(X), MLP_OUT), np.float32)
+        self._chk(lib.xgm_mlp_predict(self._h, px, len(X), Y.ctypes.data_as(C.POINTER(C.c_float))))
+        return Y
+
+    def train(self, X, T, M, epochs=20, batch=4096, lr=1e-3, seed=7):
+        X, px = _f32(X, (-1, MLP_IN))
+        T, pt = _f32(T, (-1, MLP_OUT))
+        M, pm = _f32(M, (-1, MLP_OUT))
+        loss = C.c_float()
+        self._chk(lib.xgm_mlp_train(self._h, px, pt, pm, len(X), epochs, batch, lr, seed, C.byref(loss)))
+        return float(loss.value)
+
+    def grads(self, X, T, M):
+        X, px = _f32(X, (-1, MLP_IN))
+        T, pt = _f32(T, (-1, MLP_OUT))
+        M, pm = _f32(M, (-1, MLP_OUT))
+        g = np.zeros(MLP_PARAMS, np.float32)
+        loss = C.c_float()
+        self._chk(lib.xgm_mlp_grads(self._h, px, pt, pm, len(X), g.ctypes.data_as(C.POINTER(C.c_float)),
+                                    C.byref(loss)))
+        return float(loss.value), g
+
+
+def mfma_gemm(A, B, device=0):
+    A, pa = _f32(A)
+    B, pb = _f32(B)
+    M, K = A.shape
+    K2, N = B.shape
+    assert K == K2
+    Cm = np.zeros((M, N), np.float32)
+    _check(lib.xgm_mfma_gemm_check(device, M, N, K, pa, pb, Cm.ctypes.data_as(C.POINTER(C.c_float))))
+    return Cm
