@@ -140,6 +140,27 @@ int xgm_mlp_train(xgm_mlp *m, const float *X, const float *T, const float *M, in
  * the backward pass against a CPU reference. */
 int xgm_mlp_grads(xgm_mlp *m, const float *X, const float *T, const float *M, int64_t n, float *grads,
                   float *loss);
+/* Clique schedules: one code per vertex -- 0 id-ordered LUT, 1 rank-ordered
+ * LUT, 2 edge LUT, 3 edge nested lists, 4 rank-ordered LUT one capacity bin
+ * up; -1 for vertices that are not tasks (d+ < k-1).  LUT codes on vertices
+ * with d+ > 1024 run as edge LUT.  Every schedule gives the same count. */
+typedef struct xgm_sched xgm_sched;
+/* Degree-threshold codes of a strategy (XGM_STRAT_*; AUTO = LUT up to
+ * lut_max_degree, edge LUT above). */
+int xgm_sched_heuristic(const xgm_dgraph *g, int pattern, int strategy, int lut_max_degree, int8_t *codes);
+/* Per-vertex scheduler features (n x XGM_MLP_IN floats, computed on the GPU):
+ * log out-degree, log sum / max of 2-hop out-degrees, sampled neighbour
+ * overlap, LUT fill density, log estimated sub-DAG edges / triangles /
+ * 4-cliques, capacity bin, row words, log in-degree, bias. */
+int xgm_sched_features(xgm_ctx *ctx, const xgm_dgraph *g, int pattern, float *feats);
+/* Learned codes: features, model inference and argmin on the GPU. */
+int xgm_sched_learned(xgm_ctx *ctx, const xgm_dgraph *g, int pattern, xgm_mlp *m, int8_t *codes);
+int xgm_sched_create(xgm_ctx *ctx, const xgm_dgraph *g, int pattern, const int8_t *codes, xgm_sched **out);
+void xgm_sched_free(xgm_sched *s);
+/* Count with an explicit schedule.  vertex_us (optional, n floats) receives
+ * the wall-clock microseconds spent on each vertex's task(s). */
+int xgm_count_sched(xgm_ctx *ctx, const xgm_dgraph *g, const xgm_sched *s, float *vertex_us, uint64_t *out);
+
 /* C = A*B through the MFMA tile GEMM (A, B rounded to BF16), for checking
  * the matrix-core operand layout. */
 int xgm_mfma_gemm_check(int device, int M, int N, int K, const float *A, const float *B, float *C);

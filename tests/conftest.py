@@ -20,8 +20,9 @@ CITESEER = os.path.join(GOLDEN, "citeseer", "graph")
 def load_xgm():
     if "xgminer_amd" in sys.modules:
         return sys.modules["xgminer_amd"]
+    pkg = os.path.join(ROOT, "x-gminer_amd")
     spec = importlib.util.spec_from_file_location(
-        "xgminer_amd", os.path.join(ROOT, "x-gminer_amd", "__init__.py"))
+        "xgminer_amd", os.path.join(pkg, "__init__.py"), submodule_search_locations=[pkg])
     mod = importlib.util.module_from_spec(spec)
     sys.modules["xgminer_amd"] = mod
     spec.loader.exec_module(mod)

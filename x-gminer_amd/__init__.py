@@ -107,6 +107,12 @@ def _load():
                                     P(C.c_float)]),
         "xgm_mfma_gemm_check": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, P(C.c_float), P(C.c_float),
                                           P(C.c_float)]),
+        "xgm_sched_heuristic": (C.c_int, [VP, C.c_int, C.c_int, C.c_int, P(C.c_int8)]),
+        "xgm_sched_features": (C.c_int, [VP, VP, C.c_int, P(C.c_float)]),
+        "xgm_sched_learned": (C.c_int, [VP, VP, C.c_int, VP, P(C.c_int8)]),
+        "xgm_sched_create": (C.c_int, [VP, VP, C.c_int, P(C.c_int8), P(VP)]),
+        "xgm_sched_free": (None, [VP]),
+        "xgm_count_sched": (C.c_int, [VP, VP, VP, P(C.c_float), P(U64)]),
     }
     for name, (res, args) in sigs.items():
         fn = getattr(lib, name)
@@ -273,7 +279,55 @@ class Context:
         return p.total_ms, ks
 
 
+class Schedule:
+    """Per-vertex clique strategy codes (0 LUT, 1 rank LUT, 2 edge LUT, 3 edge
+    lists, 4 rank LUT one bin up, -1 no task) uploaded as task lists."""
+
+    CODES = ("lut", "lut_rank", "edge_lut", "edge_list", "lut_rank_up")
+
+    def __init__(self, dg: "DeviceGraph", pattern: int, codes):
+        self.codes = np.ascontiguousarray(codes, dtype=np.int8)
+        h = C.c_void_p()
+        _check(lib.xgm_sched_create(dg.ctx._h, dg._h, pattern, self.codes.ctypes.data_as(C.POINTER(C.c_int8)),
+                                    C.byref(h)), dg.ctx._h)
+        self._h, self.dg, self.pattern = h, dg, pattern
+
+    def close(self):
+        if getattr(self, "_h", None) is not None:
+            lib.xgm_sched_free(self._h)
+            self._h = None
+
+    __del__ = close
+
+    def count(self, log=False):
+        """Returns count, or (count, per-vertex microseconds) with log=True."""
+        out = C.c_uint64()
+        us = np.zeros(self.dg.host.n, np.float32) if log else None
+        _check(lib.xgm_count_sched(self.dg.ctx._h, self.dg._h, self._h,
+                                   us.ctypes.data_as(C.POINTER(C.c_float)) if log else None, C.byref(out)),
+               self.dg.ctx._h)
+        return (int(out.value), us) if log else int(out.value)
+
+
 class DeviceGraph:
+    def sched_heuristic(self, pattern, strategy=STRAT_AUTO, lut_max_degree=0):
+        codes = np.zeros(self.host.n, np.int8)
+        _check(lib.xgm_sched_heuristic(self._h, pattern, strategy, lut_max_degree,
+                                       codes.ctypes.data_as(C.POINTER(C.c_int8))))
+        return codes
+
+    def sched_features(self, pattern):
+        F = np.zeros((self.host.n, MLP_IN), np.float32)
+        _check(lib.xgm_sched_features(self.ctx._h, self._h, pattern, F.ctypes.data_as(C.POINTER(C.c_float))),
+               self.ctx._h)
+        return F
+
+    def sched_learned(self, pattern, model: "MLP"):
+        codes = np.zeros(self.host.n, np.int8)
+        _check(lib.xgm_sched_learned(self.ctx._h, self._h, pattern, model._h,
+                                     codes.ctypes.data_as(C.POINTER(C.c_int8))), self.ctx._h)
+        return codes
+
     def __init__(self, ctx: Context, g: Graph):
         h = C.c_void_p()
         _check(lib.xgm_graph_upload(ctx._h, g.csr, C.byref(h)), ctx._h)

@@ -276,7 +276,7 @@ __global__ __launch_bounds__(NT) void k_clique_vlut_rank(const int64_t *__restri
                                                          const int64_t *__restrict__ rrowptr,
                                                          const int32_t *__restrict__ tasks, int ntasks, int min_s,
                                                          int depth, int *__restrict__ queue, u64 *__restrict__ out,
-                                                         u64 *__restrict__ err) {
+                                                         u64 *__restrict__ err, u64 *__restrict__ tlog) {
   __shared__ RStage<SMAX, W, NT> S;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   WaveFlags fl;
@@ -287,6 +287,7 @@ __global__ __launch_bounds__(NT) void k_clique_vlut_rank(const int64_t *__restri
     __syncthreads();
     const int t = S.item;
     if (t >= ntasks) break;
+    const u64 t_start = tlog ? (u64)wall_clock64() : 0;
     const int v0 = tasks[t];
     const int64_t b = rowptr[v0];
     const int s = (int)(rowptr[v0 + 1] - b);
@@ -326,6 +327,7 @@ __global__ __launch_bounds__(NT) void k_clique_vlut_rank(const int64_t *__restri
       acc += lut_count_tri<W>(S.rows, s, weff, depth, tid, NT);
     }
     __syncthreads();
+    if (tlog && tid == 0) tlog[t] = (u64)wall_clock64() - t_start;
   }
   acc = wave_sum_u64(acc);
   if (lane == 0 && acc) atomicAdd(out, acc);
@@ -337,7 +339,8 @@ template <int SMAX, int W, int NT>
 __global__ __launch_bounds__(NT) void k_clique_vlut(const int64_t *__restrict__ rowptr,
                                                     const int32_t *__restrict__ colidx,
                                                     const int32_t *__restrict__ tasks, int ntasks, int min_s,
-                                                    int depth, int *__restrict__ queue, u64 *__restrict__ out) {
+                                                    int depth, int *__restrict__ queue, u64 *__restrict__ out,
+                                                    u64 *__restrict__ tlog) {
   __shared__ CStage<SMAX, W, NT> S;
   const int tid = threadIdx.x;
   WaveFlags fl;
@@ -348,6 +351,7 @@ __global__ __launch_bounds__(NT) void k_clique_vlut(const int64_t *__restrict__ 
     __syncthreads();
     const int t = S.item;
     if (t >= ntasks) break;
+    const u64 t_start = tlog ? (u64)wall_clock64() : 0;
     const int v0 = tasks[t];
     const int64_t b = rowptr[v0];
     const int s = (int)(rowptr[v0 + 1] - b);
@@ -356,6 +360,7 @@ __global__ __launch_bounds__(NT) void k_clique_vlut(const int64_t *__restrict__ 
       acc += block_lut_count<SMAX, W, NT>(rowptr, colidx, S, s, depth, fl);
     }
     __syncthreads();
+    if (tlog && tid == 0) tlog[t] = (u64)wall_clock64() - t_start;
   }
   acc = wave_sum_u64(acc);
   if ((tid & 63) == 0 && acc) atomicAdd(out, acc);
@@ -372,7 +377,8 @@ __global__ __launch_bounds__(NT) void k_clique_edge(const int64_t *__restrict__ 
                                                     const int32_t *__restrict__ colidx,
                                                     const int2 *__restrict__ items, int nitems, int k,
                                                     int *__restrict__ scratch, int md,
-                                                    int *__restrict__ queue, u64 *__restrict__ out) {
+                                                    int *__restrict__ queue, u64 *__restrict__ out,
+                                                    u64 *__restrict__ tlog) {
   __shared__ CStage<SMAX, W, NT> S;
   const int tid = threadIdx.x;
   WaveFlags fl;
@@ -385,6 +391,7 @@ __global__ __launch_bounds__(NT) void k_clique_edge(const int64_t *__restrict__ 
     __syncthreads();
     const int t = S.item;
     if (t >= nitems) break;
+    const u64 t_start = tlog ? (u64)wall_clock64() : 0;
     const int2 it = items[t];
     const int64_t b0 = rowptr[it.x], b1 = rowptr[it.y];
     const int d0 = (int)(rowptr[it.x + 1] - b0), d1 = (int)(rowptr[it.y + 1] - b1);
@@ -417,6 +424,7 @@ __global__ __launch_bounds__(NT) void k_clique_edge(const int64_t *__restrict__ 
       }
     }
     __syncthreads();
+    if (tlog && tid == 0) tlog[t] = (u64)wall_clock64() - t_start;
   }
   acc = wave_sum_u64(acc);
   if ((tid & 63) == 0 && acc) atomicAdd(out, acc);
@@ -431,37 +439,38 @@ int grid_for(K kernel, int nt, size_t shmem, int num_cu, int64_t tasks) {
 }
 
 template <int SMAX, int W, int NT>
-hipError_t run_vlut(const char *name, const DevCSR &g, const TaskList &tl, int k, int *queue, Launch &l) {
+hipError_t run_vlut(const char *name, const DevCSR &g, const TaskList &tl, int k, int *queue, u64 *tlog, Launch &l) {
   if (!tl.count) return hipSuccess;
   auto kern = k_clique_vlut<SMAX, W, NT>;
   int grid = grid_for(kern, NT, 0, l.num_cu, tl.count);
   l.begin(name, tl.count);
-  kern<<<grid, NT, 0, l.stream>>>(g.rowptr, g.colidx, tl.tasks, (int)tl.count, k - 1, k - 1, queue, l.d_out);
+  kern<<<grid, NT, 0, l.stream>>>(g.rowptr, g.colidx, tl.tasks, (int)tl.count, k - 1, k - 1, queue, l.d_out, tlog);
   l.end();
   return hipGetLastError();
 }
 
 template <int SMAX, int W, int NT>
-hipError_t run_vlut_rank(const char *name, const DevCSR &g, const TaskList &tl, int k, int *queue, Launch &l) {
+hipError_t run_vlut_rank(const char *name, const DevCSR &g, const TaskList &tl, int k, int *queue, u64 *tlog,
+                         Launch &l) {
   if (!tl.count) return hipSuccess;
   auto kern = k_clique_vlut_rank<SMAX, W, NT>;
   int grid = grid_for(kern, NT, 0, l.num_cu, tl.count);
   l.begin(name, tl.count);
   kern<<<grid, NT, 0, l.stream>>>(g.rowptr, g.colidx, g.rrowptr, tl.tasks, (int)tl.count, k - 1, k - 1, queue,
-                                  l.d_out, l.d_err);
+                                  l.d_out, l.d_err, tlog);
   l.end();
   return hipGetLastError();
 }
 
 template <int SMAX, int W, int NT>
 hipError_t run_edge(const char *name, const DevCSR &g, const EdgeItems &ei, int k, int *scratch, int md,
-                    int *queue, Launch &l) {
+                    int *queue, u64 *tlog, Launch &l) {
   if (!ei.count) return hipSuccess;
   auto kern = k_clique_edge<SMAX, W, NT>;
   int grid = grid_for(kern, NT, 0, l.num_cu, ei.count);
   l.begin(name, ei.count);
   kern<<<grid, NT, 0, l.stream>>>(g.rowptr, g.colidx, (const int2 *)ei.items, (int)ei.count, k, scratch, md, queue,
-                                  l.d_out);
+                                  l.d_out, tlog);
   l.end();
   return hipGetLastError();
 }
@@ -481,36 +490,134 @@ hipError_t launch_clique(const DevCSR &g, const CliqueTasks &ct, int k, int *scr
   hipError_t e;
   const bool k4 = (k == 4);
   // edge items first: they carry the hub vertices
-  if (ct.edge_lut) {
-    if ((e = run_edge<1024, 16, 256>(k4 ? "k4_edge_lut_1024" : "k5_edge_lut_1024", g, ct.edge[1], k, scratch, md,
-                                     d_queues + 5, l)))
-      return e;
-    if ((e = run_edge<128, 2, 64>(k4 ? "k4_edge_lut_128" : "k5_edge_lut_128", g, ct.edge[0], k, scratch, md,
-                                  d_queues + 6, l)))
-      return e;
-  } else {
-    if ((e = run_edge<1024, 0, 256>(k4 ? "k4_edge_list_1024" : "k5_edge_list_1024", g, ct.edge[1], k, scratch, md,
-                                    d_queues + 5, l)))
-      return e;
-    if ((e = run_edge<128, 0, 64>(k4 ? "k4_edge_list_128" : "k5_edge_list_128", g, ct.edge[0], k, scratch, md,
-                                  d_queues + 6, l)))
-      return e;
-  }
-  if (ct.rank) {
-    if ((e = run_vlut_rank<1024, 16, 1024>(k4 ? "k4_rlut_1024" : "k5_rlut_1024", g, ct.lut[3], k, d_queues + 1, l)))
-      return e;
-    if ((e = run_vlut_rank<512, 8, 256>(k4 ? "k4_rlut_512" : "k5_rlut_512", g, ct.lut[2], k, d_queues + 2, l)))
-      return e;
-    if ((e = run_vlut_rank<256, 4, 256>(k4 ? "k4_rlut_256" : "k5_rlut_256", g, ct.lut[1], k, d_queues + 3, l)))
-      return e;
-    if ((e = run_vlut_rank<64, 1, 64>(k4 ? "k4_rlut_64" : "k5_rlut_64", g, ct.lut[0], k, d_queues + 4, l))) return e;
-    return hipSuccess;
-  }
-  if ((e = run_vlut<1024, 16, 1024>(k4 ? "k4_lut_1024" : "k5_lut_1024", g, ct.lut[3], k, d_queues + 1, l))) return e;
-  if ((e = run_vlut<512, 8, 256>(k4 ? "k4_lut_512" : "k5_lut_512", g, ct.lut[2], k, d_queues + 2, l))) return e;
-  if ((e = run_vlut<256, 4, 256>(k4 ? "k4_lut_256" : "k5_lut_256", g, ct.lut[1], k, d_queues + 3, l))) return e;
-  if ((e = run_vlut<64, 1, 64>(k4 ? "k4_lut_64" : "k5_lut_64", g, ct.lut[0], k, d_queues + 4, l))) return e;
+  if ((e = run_edge<1024, 16, 256>(k4 ? "k4_edge_lut_1024" : "k5_edge_lut_1024", g, ct.elut[1], k, scratch, md,
+                                   d_queues + 8, ct.log_elut[1], l)))
+    return e;
+  if ((e = run_edge<128, 2, 64>(k4 ? "k4_edge_lut_128" : "k5_edge_lut_128", g, ct.elut[0], k, scratch, md,
+                                d_queues + 9, ct.log_elut[0], l)))
+    return e;
+  if ((e = run_edge<1024, 0, 256>(k4 ? "k4_edge_list_1024" : "k5_edge_list_1024", g, ct.list[1], k, scratch, md,
+                                  d_queues + 10, ct.log_list[1], l)))
+    return e;
+  if ((e = run_edge<128, 0, 64>(k4 ? "k4_edge_list_128" : "k5_edge_list_128", g, ct.list[0], k, scratch, md,
+                                d_queues + 11, ct.log_list[0], l)))
+    return e;
+  if ((e = run_vlut_rank<1024, 16, 1024>(k4 ? "k4_rlut_1024" : "k5_rlut_1024", g, ct.rlut[3], k, d_queues + 4,
+                                         ct.log_rlut[3], l)))
+    return e;
+  if ((e = run_vlut<1024, 16, 1024>(k4 ? "k4_lut_1024" : "k5_lut_1024", g, ct.lut[3], k, d_queues + 0, ct.log_lut[3],
+                                    l)))
+    return e;
+  if ((e = run_vlut_rank<512, 8, 256>(k4 ? "k4_rlut_512" : "k5_rlut_512", g, ct.rlut[2], k, d_queues + 5,
+                                      ct.log_rlut[2], l)))
+    return e;
+  if ((e = run_vlut<512, 8, 256>(k4 ? "k4_lut_512" : "k5_lut_512", g, ct.lut[2], k, d_queues + 1, ct.log_lut[2], l)))
+    return e;
+  if ((e = run_vlut_rank<256, 4, 256>(k4 ? "k4_rlut_256" : "k5_rlut_256", g, ct.rlut[1], k, d_queues + 6,
+                                      ct.log_rlut[1], l)))
+    return e;
+  if ((e = run_vlut<256, 4, 256>(k4 ? "k4_lut_256" : "k5_lut_256", g, ct.lut[1], k, d_queues + 2, ct.log_lut[1], l)))
+    return e;
+  if ((e = run_vlut_rank<64, 1, 64>(k4 ? "k4_rlut_64" : "k5_rlut_64", g, ct.rlut[0], k, d_queues + 7, ct.log_rlut[0],
+                                    l)))
+    return e;
+  if ((e = run_vlut<64, 1, 64>(k4 ? "k4_lut_64" : "k5_lut_64", g, ct.lut[0], k, d_queues + 3, ct.log_lut[0], l)))
+    return e;
   return hipSuccess;
+}
+
+// Per-vertex scheduler features (XGM_MLP_IN floats per vertex; zeros for
+// vertices that are not k-clique tasks).  One thread per vertex.
+__global__ void k_clique_features(const int64_t *__restrict__ rowptr, const int32_t *__restrict__ colidx,
+                                  const int64_t *__restrict__ rrowptr, int64_t n, int k, float *__restrict__ F) {
+  const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= n) return;
+  float *f = F + v * 16;
+  const int64_t b = rowptr[v];
+  const int d = (int)(rowptr[v + 1] - b);
+  if (d < k - 1) {
+#pragma unroll
+    for (int i = 0; i < 16; ++i) f[i] = 0.f;
+    return;
+  }
+  const int32_t *X = colidx + b;
+  double sum = 0;
+  int mx = 0;
+  for (int i = 0; i < d; ++i) {
+    const int x = X[i];
+    const int dx = (int)(rowptr[x + 1] - rowptr[x]);
+    sum += dx;
+    mx = max(mx, dx);
+  }
+  // sampled neighbour overlap: |N+(x) ∩ N+(v)| for up to 4 evenly spaced x
+  const int ns = min(4, d);
+  double ov = 0, dxs = 0;
+  for (int q = 0; q < ns; ++q) {
+    const int x = X[(int64_t)q * d / ns];
+    const int32_t *Y = colidx + rowptr[x];
+    const int dx = (int)(rowptr[x + 1] - rowptr[x]);
+    int i = 0, j = 0, c = 0;
+    while (i < d && j < dx) {
+      const int a = X[i], bb = Y[j];
+      c += (a == bb);
+      i += (a <= bb);
+      j += (bb <= a);
+    }
+    ov += c;
+    dxs += dx;
+  }
+  const float rho = (float)(ov / ((double)ns * d));  // row density of the LUT
+  const float r2 = (float)(ov / fmax(1.0, dxs));     // share of a neighbour's list inside N+(v)
+  const float dd = (float)d;
+  const float s = 1.f / 24.f;                         // keeps log2 features near [0, 1]
+  f[0] = log2f(1.f + dd) * s * 2.f;
+  f[1] = log2f(1.f + (float)sum) * s;
+  f[2] = log2f(1.f + (float)mx) * s * 2.f;
+  f[3] = r2;
+  f[4] = rho;
+  f[5] = log2f(1.f + dd * dd * rho * 0.5f) * s;
+  f[6] = log2f(1.f + dd * dd * dd * rho * rho * rho / 6.f) * s;
+  f[7] = log2f(1.f + dd * dd * dd * dd * rho * rho * rho * rho * rho * rho / 24.f) * s;
+  f[8] = d <= 64 ? 1.f : 0.f;
+  f[9] = (d > 64 && d <= 256) ? 1.f : 0.f;
+  f[10] = (d > 256 && d <= 512) ? 1.f : 0.f;
+  f[11] = (d > 512 && d <= 1024) ? 1.f : 0.f;
+  f[12] = (float)((d + 63) >> 6) / 16.f;
+  f[13] = log2f(1.f + (float)(rrowptr[v + 1] - rrowptr[v])) * s;
+  f[14] = dd / 1024.f;
+  f[15] = 1.f;
+}
+
+__global__ void k_sched_argmin(const int64_t *__restrict__ rowptr, int64_t n, const float *__restrict__ F,
+                               const float *__restrict__ Y, int8_t *__restrict__ codes) {
+  const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= n) return;
+  if (F[v * 16 + 15] == 0.f) {
+    codes[v] = -1;
+    return;
+  }
+  const int d = (int)(rowptr[v + 1] - rowptr[v]);
+  const float *y = Y + v * 5;
+  int best = 2;
+  float bv = y[2];
+  for (int c = 0; c < 5; ++c) {
+    const bool ok = (c == 2 || c == 3) || d <= 1024;  // LUT codes only up to 1024 rows
+    if (ok && y[c] < bv) { bv = y[c]; best = c; }
+  }
+  codes[v] = (int8_t)best;
+}
+
+hipError_t launch_sched_argmin(const DevCSR &g, const float *F, const float *Y, int8_t *codes, hipStream_t s) {
+  if (!g.n) return hipSuccess;
+  k_sched_argmin<<<(unsigned)((g.n + 255) / 256), 256, 0, s>>>(g.rowptr, g.n, F, Y, codes);
+  return hipGetLastError();
+}
+
+hipError_t launch_clique_features(const DevCSR &g, int k, float *F, hipStream_t s) {
+  const int64_t n = g.n;
+  if (!n) return hipSuccess;
+  k_clique_features<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(g.rowptr, g.colidx, g.rrowptr, n, k, F);
+  return hipGetLastError();
 }
 
 }  // namespace xgm

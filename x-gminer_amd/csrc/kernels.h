@@ -11,6 +11,8 @@
 
 typedef unsigned long long u64;
 
+struct xgm_mlp;
+
 namespace xgm {
 
 // Device-resident CSR (HBM): int64 rowptr[n+1], int32 colidx[nnz].
@@ -60,20 +62,23 @@ struct EdgeItems {
   int64_t count = 0;
 };
 
-// Tasks of a k-clique count: vertex LUT tiers by d+(v0) (<=64, <=256, <=512,
-// <=lut_max) and edge tasks split by min(d+(v0), d+(v1)) (<=128, larger),
-// run with the LUT (edge_lut) or with nested list intersections.
+// Tasks of a k-clique count.  Vertex tasks run the id-ordered LUT (lut) or
+// the rank-ordered LUT (rlut) in bins by capacity (64/256/512/1024 rows);
+// edge tasks (v0, v1) run with the LUT (elut) or nested lists (list), split
+// by min(d+(v0), d+(v1)) <= 128.  Optional per-item wall-clock logs
+// (100 MHz ticks) have the same shapes.
 struct CliqueTasks {
-  TaskList lut[4];
-  EdgeItems edge[2];
-  bool edge_lut = true;
-  bool rank = false;  // vertex LUT rows in (degree, id) order (triangular)
+  TaskList lut[4], rlut[4];
+  EdgeItems elut[2], list[2];
+  u64 *log_lut[4] = {}, *log_rlut[4] = {}, *log_elut[2] = {}, *log_list[2] = {};
 };
 
 // k-clique counting (k = 4 or 5) over a DAG.  scratch holds
 // clique_edge_grid() * 2 * md ints (md = max out-degree).  Adds into l.d_out.
 hipError_t launch_clique(const DevCSR &g, const CliqueTasks &ct, int k, int *scratch, int md, Launch &l,
-                         int *d_queues /* 7 counters */);
+                         int *d_queues /* 12 counters */);
+// Scheduler features of every vertex (n x 16 floats) for k-clique tasks.
+hipError_t launch_clique_features(const DevCSR &g, int k, float *F, hipStream_t s);
 int clique_edge_grid(int num_cu);
 
 // Edge-induced diamond over an undirected CSR; items are int4 (owner t,
@@ -93,6 +98,12 @@ int cycle4_dense_grid(int num_cu, int64_t tasks);
 // Edge-induced 4-cycle, bounded pairwise intersections in the reference's
 // loop order (original ids).
 hipError_t launch_cycle4_pair(const DevCSR &g, const TaskList &tl, Launch &l, int *d_queues);
+
+// Scheduler cost model inference on device arrays (n x 16 -> n x 5).
+int mlp_predict_device(xgm_mlp *m, const float *dX, int64_t n, float *dY);
+// codes[v] = argmin over the valid clique strategy codes of the predicted
+// cost Y[v] (-1 where F marks v as no task).
+hipError_t launch_sched_argmin(const DevCSR &g, const float *F, const float *Y, int8_t *codes, hipStream_t s);
 
 // Small utility kernel: zero `n` 64-bit words (graph-capture safe, unlike memset nodes).
 hipError_t launch_zero(void *ptr, size_t n_u64, hipStream_t s);

@@ -288,6 +288,31 @@ int upload(xgm_mlp *md, void **dst, const void *src, size_t bytes) {
 
 }  // namespace
 
+namespace xgm {
+
+// Inference on device arrays (X: n x 16 FP32, Y: n x 5 FP32), used by the
+// scheduler without host round trips.  Synchronous.
+int mlp_predict_device(xgm_mlp *md, const float *dX, int64_t n, float *dY) {
+  if (!md || n < 0) return XGM_EINVAL;
+  if (!n) return XGM_OK;
+  MLP_TRY(md, hipSetDevice(md->device));
+  const int64_t chunk = 1 << 16;
+  int rc = ensure_ws(md, std::min(n, chunk));
+  if (rc) return rc;
+  for (int64_t r0 = 0; r0 < n; r0 += chunk) {
+    const int B = (int)std::min(chunk, n - r0);
+    k_gather<<<blocks_for((int64_t)B * IN), 256, 0, md->stream>>>(B, nullptr, dX + r0 * IN, nullptr, nullptr, md->xb,
+                                                                   nullptr, nullptr);
+    forward(md, B);
+    MLP_TRY(md, hipMemcpyAsync(dY + r0 * OUT, md->y, (size_t)B * OUT * sizeof(float), hipMemcpyDeviceToDevice,
+                               md->stream));
+  }
+  MLP_TRY(md, hipStreamSynchronize(md->stream));
+  return XGM_OK;
+}
+
+}  // namespace xgm
+
 extern "C" {
 
 int xgm_mlp_create(int device, uint64_t seed, xgm_mlp **out) {

@@ -46,6 +46,53 @@ struct xgm_ctx {
   }
 };
 
+namespace {
+constexpr int kClqLists = 12;
+constexpr int kBinCap[4] = {64, 256, 512, 1024};
+}  // namespace
+
+// Lists of a clique schedule: 0-3 id-ordered LUT bins, 4-7 rank-ordered
+// LUT bins, 8-9 edge LUT (min out-degree <= 128 / larger), 10-11 edge list.
+
+struct xgm_sched {
+  int pattern = 0, k = 0;
+  int device = 0;
+  int32_t *d_buf = nullptr;
+  int64_t off[kClqLists] = {}, cnt[kClqLists] = {};  // in items; edge lists hold 2 ints per item
+  std::vector<int32_t> owner[kClqLists];              // item -> vertex (for the timing log)
+  ~xgm_sched() {
+    (void)hipSetDevice(device);
+    (void)hipFree(d_buf);
+  }
+  CliqueTasks tasks(u64 *log_base) const {
+    CliqueTasks ct;
+    for (int i = 0; i < 4; ++i) {
+      ct.lut[i] = TaskList{d_buf + off[i], cnt[i]};
+      ct.rlut[i] = TaskList{d_buf + off[4 + i], cnt[4 + i]};
+    }
+    for (int i = 0; i < 2; ++i) {
+      ct.elut[i] = EdgeItems{d_buf + off[8 + i], cnt[8 + i]};
+      ct.list[i] = EdgeItems{d_buf + off[10 + i], cnt[10 + i]};
+    }
+    if (log_base) {
+      int64_t o = 0;
+      u64 **dst[kClqLists] = {&ct.log_lut[0], &ct.log_lut[1], &ct.log_lut[2], &ct.log_lut[3],
+                              &ct.log_rlut[0], &ct.log_rlut[1], &ct.log_rlut[2], &ct.log_rlut[3],
+                              &ct.log_elut[0], &ct.log_elut[1], &ct.log_list[0], &ct.log_list[1]};
+      for (int i = 0; i < kClqLists; ++i) {
+        *dst[i] = log_base + o;
+        o += cnt[i];
+      }
+    }
+    return ct;
+  }
+  int64_t items() const {
+    int64_t t = 0;
+    for (int i = 0; i < kClqLists; ++i) t += cnt[i];
+    return t;
+  }
+};
+
 struct xgm_dgraph {
   xgm_ctx *ctx = nullptr;
   int device = 0;
@@ -71,6 +118,7 @@ struct xgm_dgraph {
   // per-vertex wedge work, built on first 4-cycle count
   std::vector<int64_t> work;
   bool relabelled = false;
+  std::map<std::string, std::unique_ptr<xgm_sched>> scheds;  // cached clique schedules
   int *d_dense = nullptr;          // per-block 4-cycle counters, zero between counts
   size_t dense_ints = 0;
   int64_t *d_rl_rowptr = nullptr;
@@ -457,6 +505,83 @@ void build_tc_items(const xgm_dgraph *g, std::vector<std::vector<int32_t>> &part
   }
 }
 
+// codes[v]: 0 LUT, 1 rank LUT, 2 edge LUT, 3 edge list, 4 rank LUT one bin
+// up; vertices with d+ < k-1 are not tasks; LUT codes on hubs (d+ > 1024)
+// and unknown codes fall back to the edge LUT.
+int build_clique_sched(xgm_ctx *ctx, const xgm_dgraph *g, int pattern, const int8_t *codes, xgm_sched *sc) {
+  const int k = pattern == XGM_CLIQUE4 ? 4 : 5;
+  sc->pattern = pattern;
+  sc->k = k;
+  sc->device = ctx->device;
+  std::vector<int32_t> L[kClqLists];
+  for (int32_t v : g->order) {
+    const int d = g->dplus[v];
+    if (d < k - 1) continue;
+    int c = codes ? codes[v] : 0;
+    if (c < 0 || c > 4) c = 2;
+    if (d > kBinCap[3] && (c == 0 || c == 1 || c == 4)) c = 2;
+    int bin = 0;
+    while (d > kBinCap[bin]) ++bin;
+    if (c == 0) { L[bin].push_back(v); sc->owner[bin].push_back(v); continue; }
+    if (c == 1 || c == 4) {
+      const int b = (c == 4) ? std::min(bin + 1, 3) : bin;
+      L[4 + b].push_back(v);
+      sc->owner[4 + b].push_back(v);
+      continue;
+    }
+    const int base = (c == 2) ? 8 : 10;
+    for (int64_t e = g->h_rowptr[v]; e < g->h_rowptr[v + 1]; ++e) {
+      const int32_t v1 = g->h_colidx[e];
+      if (g->dplus[v1] < k - 2) continue;
+      const int li = base + (std::min(d, g->dplus[v1]) <= 128 ? 0 : 1);
+      L[li].push_back(v);
+      L[li].push_back(v1);
+      sc->owner[li].push_back(v);
+    }
+  }
+  std::vector<int32_t> all;
+  for (int i = 0; i < kClqLists; ++i) {
+    const int stride = i >= 8 ? 2 : 1;
+    sc->off[i] = (int64_t)all.size();
+    sc->cnt[i] = (int64_t)L[i].size() / stride;
+    all.insert(all.end(), L[i].begin(), L[i].end());
+  }
+  HIP_TRY(ctx, hipMalloc(&sc->d_buf, std::max<size_t>(2, all.size()) * sizeof(int32_t)));
+  if (!all.empty()) HIP_TRY(ctx, hipMemcpy(sc->d_buf, all.data(), all.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+  return XGM_OK;
+}
+
+// Heuristic codes: id-ordered LUT up to lut_max out-degree, edge LUT above
+// (plain degree thresholds, the reference's fixed-threshold scheme).
+void heuristic_codes(const xgm_dgraph *g, int k, int strategy, int lut_max, int8_t *codes) {
+  for (int64_t v = 0; v < g->n; ++v) {
+    const int d = g->dplus[v];
+    int8_t c = -1;
+    if (d >= k - 1) {
+      switch (strategy) {
+        case XGM_STRAT_LIST: c = 3; break;
+        case XGM_STRAT_ELUT: c = 2; break;
+        case XGM_STRAT_LUT_RANK: c = d <= lut_max ? 1 : 2; break;
+        default: c = d <= lut_max ? 0 : 2; break;
+      }
+    }
+    codes[v] = c;
+  }
+}
+
+int run_clique_sched(xgm_ctx *ctx, xgm_dgraph *gm, const xgm_sched *sc, const DevCSR &dg, Launch &l, u64 *log) {
+  const int64_t edges = sc->cnt[8] + sc->cnt[9] + sc->cnt[10] + sc->cnt[11];
+  if (edges) {
+    size_t need = (size_t)clique_edge_grid(ctx->num_cu) * 2 * std::max(1, gm->max_out);
+    int rc = ensure_scratch(ctx, gm, need);
+    if (rc) return rc;
+  }
+  CliqueTasks ct = sc->tasks(log);
+  hipError_t e = launch_clique(dg, ct, sc->k, gm->d_scratch, std::max(1, gm->max_out), l, ctx->d_queues);
+  if (e != hipSuccess) return ctx->hip_fail(e, "clique launch");
+  return XGM_OK;
+}
+
 int check_device(xgm_ctx *ctx) {
   hipDeviceProp_t p;
   HIP_TRY(ctx, hipGetDeviceProperties(&p, ctx->device));
@@ -610,27 +735,19 @@ int xgm_count(xgm_ctx *ctx, const xgm_dgraph *g, int pattern, const xgm_opts *op
           strategy != XGM_STRAT_ELUT && strategy != XGM_STRAT_LUT_RANK)
         return ctx->fail(XGM_EUNSUPPORTED, "strategy not available for CLIQUE");
       const int k = (pattern == XGM_CLIQUE4) ? 4 : 5;
-      int lut_max = (opts && opts->lut_max_degree > 0) ? std::min(opts->lut_max_degree, 1024) : 1024;
-      if (strategy == XGM_STRAT_LIST || strategy == XGM_STRAT_ELUT) lut_max = 0;
-      // vertex LUT ranges: (.,64] (64,256] (256,512] (512,lut_max]; above -> edge items
-      std::vector<int> hi;
-      for (int b : {64, 256, 512, 1024}) hi.push_back(std::min(b, lut_max));
-      hi.push_back(INT32_MAX);
-      std::string key = "clique" + std::to_string(k) + "_" + std::to_string(lut_max);
-      if ((rc = tasks_by_ranges(ctx, gm, key, k - 1, hi, tl))) return rc;
-      CliqueTasks ct;
-      for (int t = 0; t < 4; ++t) ct.lut[t] = tl[t];
-      ct.edge_lut = (strategy != XGM_STRAT_LIST);
-      ct.rank = (strategy == XGM_STRAT_LUT_RANK);
-      std::vector<int32_t> hubs;
-      for (int32_t u : gm->order)
-        if (gm->dplus[u] > lut_max && gm->dplus[u] >= k - 1) hubs.push_back(u);
-      if ((rc = edge_items(ctx, gm, key + "_edges", hubs, k - 2, 128, ct.edge))) return rc;
-      if (ct.edge[0].count + ct.edge[1].count) {
-        size_t need = (size_t)clique_edge_grid(ctx->num_cu) * 2 * std::max(1, g->max_out);
-        if ((rc = ensure_scratch(ctx, gm, need))) return rc;
+      const int lut_max = (opts && opts->lut_max_degree > 0) ? std::min(opts->lut_max_degree, 1024) : 1024;
+      const std::string key = "csched" + std::to_string(k) + "_" + std::to_string(strategy) + "_" +
+                              std::to_string(lut_max);
+      auto it = gm->scheds.find(key);
+      if (it == gm->scheds.end()) {
+        std::vector<int8_t> codes(g->n);
+        heuristic_codes(gm, k, strategy, lut_max, codes.data());
+        std::unique_ptr<xgm_sched> sc(new (std::nothrow) xgm_sched);
+        if (!sc) return XGM_ENOMEM;
+        if ((rc = build_clique_sched(ctx, gm, pattern, codes.data(), sc.get()))) return rc;
+        it = gm->scheds.emplace(key, std::move(sc)).first;
       }
-      e = launch_clique(dg, ct, k, gm->d_scratch, std::max(1, g->max_out), l, ctx->d_queues);
+      if ((rc = run_clique_sched(ctx, gm, it->second.get(), dg, l, nullptr))) return rc;
       break;
     }
     case XGM_DIAMOND_EDGE: {
@@ -806,6 +923,140 @@ int xgm_count_csr(xgm_ctx *ctx, const xgm_csr *g, int pattern, const xgm_opts *o
 int xgm_last_profile(const xgm_ctx *ctx, xgm_profile *out) {
   if (!ctx || !out) return XGM_EINVAL;
   *out = ctx->prof;
+  return XGM_OK;
+}
+
+
+// ------------------------------------------------------------ schedules
+
+static DevCSR dev_view(const xgm_dgraph *g) {
+  DevCSR dg;
+  dg.rowptr = g->d_rowptr;
+  dg.colidx = g->d_colidx;
+  dg.n = g->n;
+  dg.nnz = g->nnz;
+  dg.rrowptr = g->d_rrowptr;
+  dg.rcolidx = g->d_rcolidx;
+  return dg;
+}
+
+static int clique_k(int pattern) { return pattern == XGM_CLIQUE4 ? 4 : pattern == XGM_CLIQUE5 ? 5 : 0; }
+
+int xgm_sched_heuristic(const xgm_dgraph *g, int pattern, int strategy, int lut_max_degree, int8_t *codes) {
+  if (!g || !codes || !clique_k(pattern) || !(g->flags & XGM_CSR_DAG)) return XGM_EINVAL;
+  const int lut_max = lut_max_degree > 0 ? std::min(lut_max_degree, 1024) : 1024;
+  heuristic_codes(g, clique_k(pattern), strategy, lut_max, codes);
+  return XGM_OK;
+}
+
+int xgm_sched_features(xgm_ctx *ctx, const xgm_dgraph *g, int pattern, float *feats) {
+  if (!ctx || !g || !feats || !clique_k(pattern) || !(g->flags & XGM_CSR_DAG)) return XGM_EINVAL;
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  float *dF = nullptr;
+  HIP_TRY(ctx, hipMalloc(&dF, std::max<int64_t>(1, g->n) * 16 * sizeof(float)));
+  hipError_t e = launch_clique_features(dev_view(g), clique_k(pattern), dF, ctx->stream);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(feats, dF, g->n * 16 * sizeof(float), hipMemcpyDeviceToHost, ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  (void)hipFree(dF);
+  if (e != hipSuccess) return ctx->hip_fail(e, "features");
+  return XGM_OK;
+}
+
+int xgm_sched_learned(xgm_ctx *ctx, const xgm_dgraph *g, int pattern, xgm_mlp *m, int8_t *codes) {
+  if (!ctx || !g || !m || !codes || !clique_k(pattern) || !(g->flags & XGM_CSR_DAG)) return XGM_EINVAL;
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  float *dF = nullptr, *dY = nullptr;
+  int8_t *dC = nullptr;
+  auto done = [&](int rc) {
+    (void)hipFree(dF);
+    (void)hipFree(dY);
+    (void)hipFree(dC);
+    return rc;
+  };
+  const int64_t n = std::max<int64_t>(1, g->n);
+  hipError_t e;
+  if ((e = hipMalloc(&dF, n * 16 * sizeof(float))) != hipSuccess) return done(ctx->hip_fail(e, "hipMalloc(F)"));
+  if ((e = hipMalloc(&dY, n * 5 * sizeof(float))) != hipSuccess) return done(ctx->hip_fail(e, "hipMalloc(Y)"));
+  if ((e = hipMalloc(&dC, n)) != hipSuccess) return done(ctx->hip_fail(e, "hipMalloc(codes)"));
+  const DevCSR dg = dev_view(g);
+  if ((e = launch_clique_features(dg, clique_k(pattern), dF, ctx->stream)) != hipSuccess ||
+      (e = hipStreamSynchronize(ctx->stream)) != hipSuccess)
+    return done(ctx->hip_fail(e, "features"));
+  int rc = mlp_predict_device(m, dF, g->n, dY);
+  if (rc) return done(ctx->fail(rc, std::string("model: ") + xgm_mlp_last_error(m)));
+  if ((e = launch_sched_argmin(dg, dF, dY, dC, ctx->stream)) != hipSuccess ||
+      (e = hipMemcpyAsync(codes, dC, g->n, hipMemcpyDeviceToHost, ctx->stream)) != hipSuccess ||
+      (e = hipStreamSynchronize(ctx->stream)) != hipSuccess)
+    return done(ctx->hip_fail(e, "argmin"));
+  return done(XGM_OK);
+}
+
+int xgm_sched_create(xgm_ctx *ctx, const xgm_dgraph *g, int pattern, const int8_t *codes, xgm_sched **out) {
+  if (!ctx || !g || !codes || !out || !clique_k(pattern) || !(g->flags & XGM_CSR_DAG)) return XGM_EINVAL;
+  *out = nullptr;
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  std::unique_ptr<xgm_sched> sc(new (std::nothrow) xgm_sched);
+  if (!sc) return XGM_ENOMEM;
+  int rc = build_clique_sched(ctx, g, pattern, codes, sc.get());
+  if (rc) return rc;
+  *out = sc.release();
+  return XGM_OK;
+}
+
+void xgm_sched_free(xgm_sched *s) { delete s; }
+
+int xgm_count_sched(xgm_ctx *ctx, const xgm_dgraph *g, const xgm_sched *s, float *vertex_us, uint64_t *out) {
+  if (!ctx || !g || !s || !out) return XGM_EINVAL;
+  if (g->ctx != ctx) return ctx->fail(XGM_EINVAL, "graph belongs to another context");
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  xgm_dgraph *gm = const_cast<xgm_dgraph *>(g);
+  ctx->prof = xgm_profile{};
+  HIP_TRY(ctx, launch_zero(ctx->d_out, kScratchU64, ctx->stream));
+  HIP_TRY(ctx, launch_zero(ctx->d_queues, kScratchQueues / 2, ctx->stream));
+  u64 *dlog = nullptr;
+  const int64_t items = s->items();
+  if (vertex_us) {
+    HIP_TRY(ctx, hipMalloc(&dlog, std::max<int64_t>(1, items) * sizeof(u64)));
+    HIP_TRY(ctx, launch_zero(dlog, std::max<int64_t>(1, items), ctx->stream));
+  }
+  Launch l = make_launch(ctx, 0);
+  int rc = run_clique_sched(ctx, gm, s, dev_view(g), l, dlog);
+  if (rc) {
+    (void)hipFree(dlog);
+    return rc;
+  }
+  u64 hv[kScratchU64];
+  hipError_t e = hipMemcpyAsync(hv, ctx->d_out, sizeof(hv), hipMemcpyDeviceToHost, ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  if (e != hipSuccess) {
+    (void)hipFree(dlog);
+    return ctx->hip_fail(e, "count");
+  }
+  float total = 0.f;
+  for (int i = 0; i < ctx->prof.n_kernels; ++i) {
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, ctx->ev[2 * i], ctx->ev[2 * i + 1]);
+    ctx->prof.k[i].ms = ms;
+  }
+  if (ctx->prof.n_kernels > 0)
+    (void)hipEventElapsedTime(&total, ctx->ev[0], ctx->ev[2 * ctx->prof.n_kernels - 1]);
+  ctx->prof.total_ms = total;
+  if (vertex_us) {
+    std::vector<u64> h(std::max<int64_t>(1, items));
+    e = hipMemcpy(h.data(), dlog, h.size() * sizeof(u64), hipMemcpyDeviceToHost);
+    (void)hipFree(dlog);
+    if (e != hipSuccess) return ctx->hip_fail(e, "D2H log");
+    std::fill(vertex_us, vertex_us + g->n, 0.f);
+    int64_t o = 0;
+    for (int i = 0; i < kClqLists; ++i) {
+      for (int64_t j = 0; j < s->cnt[i]; ++j) vertex_us[s->owner[i][j]] += (float)h[o + j] * 0.01f;  // 100 MHz ticks
+      o += s->cnt[i];
+    }
+  }
+  if (hv[kScratchU64 - 1])
+    return ctx->fail(XGM_EINVAL, "DAG is not in (degree, id) orientation; use XGM_STRAT_LUT");
+  *out = (uint64_t)hv[0];
   return XGM_OK;
 }
 
