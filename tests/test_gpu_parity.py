@@ -277,3 +277,30 @@ def test_lut_rank_rejects_foreign_orientation(xgm, ctx):
     assert ctx.count(d, 2, 3) == want
     with pytest.raises(xgm.XGMError):
         ctx.count(d, 2, 6)
+
+
+@pytest.mark.parametrize("pattern,name", [(2, "clique4"), (3, "clique5")])
+def test_schedules_every_code_and_mixtures_exact(xgm, ctx, pattern, name):
+    """Every per-vertex strategy code, and random mixtures of codes, count the
+    same: the scheduler cannot change a count."""
+    import numpy as np
+    from golden.make_golden import SMALL, make_graph
+    _say("schedules", name)
+    gold = _gold()["rmat12_s3"]["counts"][name]
+    d = make_graph(xgm, SMALL["rmat12_s3"][0], SMALL["rmat12_s3"][1]).orient()
+    dg = ctx.upload(d)
+    task = np.diff(d.rowptr) >= (pattern + 1)  # d+ >= k - 1
+    for c in range(5):
+        s = xgm.Schedule(dg, pattern, np.where(task, c, -1))
+        n, us = s.count(log=True)
+        assert n == gold, c
+        assert np.all(us >= 0) and us[task].sum() > 0
+        s.close()
+    rng = np.random.default_rng(pattern)
+    for _ in range(3):
+        s = xgm.Schedule(dg, pattern, np.where(task, rng.integers(0, 5, d.n), -1))
+        assert s.count() == gold
+        s.close()
+    heur = dg.sched_heuristic(pattern)
+    assert xgm.Schedule(dg, pattern, heur).count() == gold
+    dg.close()

@@ -50,7 +50,8 @@ def main():
             g = xgm.gen_chung_lu(size, 20.0, 2.1, 5000.0 if size < 1e6 else 50000.0, seed)
         d = g.orient()
         dg = ctx.upload(d)
-        X, T, M, cnt = scheduler.collect(dg, xgm.CLIQUE5, max_rows=300000, seed=seed)
+        codes = range(5) if (kind != "rmat" or size <= 21) else (0, 1, 2, 4)  # edge lists are slow at scale 22
+        X, T, M, cnt = scheduler.collect(dg, xgm.CLIQUE5, codes=codes, max_rows=300000, seed=seed)
         dg.close()
         Xs.append(X); Ts.append(T); Ms.append(M)
         print(json.dumps(dict(stage="collect", graph=f"{kind}{size}_s{seed}", tasks=int(len(X)), k5=cnt,
