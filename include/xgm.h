@@ -158,7 +158,8 @@ int xgm_sched_learned(xgm_ctx *ctx, const xgm_dgraph *g, int pattern, xgm_mlp *m
 int xgm_sched_create(xgm_ctx *ctx, const xgm_dgraph *g, int pattern, const int8_t *codes, xgm_sched **out);
 void xgm_sched_free(xgm_sched *s);
 /* Count with an explicit schedule.  vertex_us (optional, n floats) receives
- * the wall-clock microseconds spent on each vertex's task(s). */
+ * each vertex's share of the kernel time: the wall-clock microseconds of its
+ * task(s) divided by the resident blocks of the kernel that ran them. */
 int xgm_count_sched(xgm_ctx *ctx, const xgm_dgraph *g, const xgm_sched *s, float *vertex_us, uint64_t *out);
 
 /* C = A*B through the MFMA tile GEMM (A, B rounded to BF16), for checking

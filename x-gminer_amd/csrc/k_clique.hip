@@ -477,6 +477,27 @@ hipError_t run_edge(const char *name, const DevCSR &g, const EdgeItems &ei, int 
 
 }  // namespace
 
+// Resident blocks of the kernel that runs schedule list `li` (0-3 LUT bins,
+// 4-7 rank LUT bins, 8-9 edge LUT, 10-11 edge lists): a task's share of
+// that kernel's time is its wall time divided by this.
+int clique_list_capacity(int num_cu, int li) {
+  const int64_t inf = 1 << 30;
+  switch (li) {
+    case 0: return grid_for(k_clique_vlut<64, 1, 64>, 64, 0, num_cu, inf);
+    case 1: return grid_for(k_clique_vlut<256, 4, 256>, 256, 0, num_cu, inf);
+    case 2: return grid_for(k_clique_vlut<512, 8, 256>, 256, 0, num_cu, inf);
+    case 3: return grid_for(k_clique_vlut<1024, 16, 1024>, 1024, 0, num_cu, inf);
+    case 4: return grid_for(k_clique_vlut_rank<64, 1, 64>, 64, 0, num_cu, inf);
+    case 5: return grid_for(k_clique_vlut_rank<256, 4, 256>, 256, 0, num_cu, inf);
+    case 6: return grid_for(k_clique_vlut_rank<512, 8, 256>, 256, 0, num_cu, inf);
+    case 7: return grid_for(k_clique_vlut_rank<1024, 16, 1024>, 1024, 0, num_cu, inf);
+    case 8: return grid_for(k_clique_edge<128, 2, 64>, 64, 0, num_cu, inf);
+    case 9: return grid_for(k_clique_edge<1024, 16, 256>, 256, 0, num_cu, inf);
+    case 10: return grid_for(k_clique_edge<128, 0, 64>, 64, 0, num_cu, inf);
+    default: return grid_for(k_clique_edge<1024, 0, 256>, 256, 0, num_cu, inf);
+  }
+}
+
 int clique_edge_grid(int num_cu) {
   int a = grid_for(k_clique_edge<128, 2, 64>, 64, 0, num_cu, 1 << 30);
   int b = grid_for(k_clique_edge<1024, 16, 256>, 256, 0, num_cu, 1 << 30);

@@ -80,6 +80,7 @@ hipError_t launch_clique(const DevCSR &g, const CliqueTasks &ct, int k, int *scr
 // Scheduler features of every vertex (n x 16 floats) for k-clique tasks.
 hipError_t launch_clique_features(const DevCSR &g, int k, float *F, hipStream_t s);
 int clique_edge_grid(int num_cu);
+int clique_list_capacity(int num_cu, int list);
 
 // Edge-induced diamond over an undirected CSR; items are int4 (owner t,
 // k0, k1, 0) (TaskList.tasks points at 4*count ints, count = items) in

@@ -1050,7 +1050,10 @@ int xgm_count_sched(xgm_ctx *ctx, const xgm_dgraph *g, const xgm_sched *s, float
     std::fill(vertex_us, vertex_us + g->n, 0.f);
     int64_t o = 0;
     for (int i = 0; i < kClqLists; ++i) {
-      for (int64_t j = 0; j < s->cnt[i]; ++j) vertex_us[s->owner[i][j]] += (float)h[o + j] * 0.01f;  // 100 MHz ticks
+      // 100 MHz ticks -> microseconds, divided by the kernel's resident
+      // blocks: the task's share of its kernel's duration
+      const float scale = 0.01f / (float)std::max(1, clique_list_capacity(ctx->num_cu, i));
+      for (int64_t j = 0; j < s->cnt[i]; ++j) vertex_us[s->owner[i][j]] += (float)h[o + j] * scale;
       o += s->cnt[i];
     }
   }

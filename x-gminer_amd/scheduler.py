@@ -3,8 +3,9 @@
 3 edge nested lists, 4 rank-ordered LUT one capacity bin up).
 
 * ``collect``: per-vertex features (computed on the GPU) and per-vertex
-  wall-clock cost under every code, from instrumented runs of the plain
-  kernels.  Targets are log2(1 + microseconds); rows that a code cannot run
+  cost under every code, from instrumented runs of the plain kernels: the
+  task's wall time divided by the resident blocks of its kernel, i.e. its
+  share of the kernel duration.  Targets are log2(1 + ns); rows a code cannot run
   (LUT codes on vertices above 1024 out-neighbours) are masked.
 * ``train``: fits the 16-64-64-5 MLP on the GPU (the library's own BF16-MFMA
   forward/backward/Adam kernels).
@@ -47,7 +48,7 @@ def collect(dg, pattern=CLIQUE5, codes=range(NCODES), max_rows=None, seed=0):
             count = n
         elif n != count:
             raise RuntimeError(f"count differs between schedules: {n} vs {count} (code {c})")
-        T[:, c] = np.log2(1.0 + us)
+        T[:, c] = np.log2(1.0 + 1000.0 * us)  # kernel-time share in ns
         valid = task & ((c in (2, 3)) | (dplus <= 1024))
         Mk[:, c] = valid
     rows = np.nonzero(task)[0]

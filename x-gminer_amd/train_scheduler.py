@@ -62,9 +62,9 @@ def main():
     pred = model.predict(X)
     choice = np.argmin(np.where(M > 0, pred, np.inf), axis=1)
     best = scheduler.oracle_best(T, M)
-    cost = lambda ch: float(np.sum(2.0 ** T[np.arange(len(T)), ch] - 1.0))
+    cost = lambda ch: float(np.sum(2.0 ** T[np.arange(len(T)), ch] - 1.0) * 1e-6)  # ms of kernel time
     print(json.dumps(dict(stage="train", rows=int(len(X)), final_loss=loss, seconds=round(time.time() - t0, 1),
-                          train_cost_us=dict(model=cost(choice), best=cost(best),
+                          train_cost_ms=dict(model=cost(choice), best=cost(best),
                                              **{f"code{c}": cost(np.where(M[:, c] > 0, c, 2)) for c in range(5)}))),
           flush=True)
     out = args.out or scheduler.DEFAULT_MODEL
