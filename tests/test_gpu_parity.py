@@ -304,3 +304,35 @@ def test_schedules_every_code_and_mixtures_exact(xgm, ctx, pattern, name):
     heur = dg.sched_heuristic(pattern)
     assert xgm.Schedule(dg, pattern, heur).count() == gold
     dg.close()
+
+
+def _hub_graph(xgm, nb=1100, nl=1200, inner=10, seed=3):
+    """v (id 0) -> B (nb vertices) <- L (nl vertices), B x L complete, B
+    sparse inside: v and every L vertex get out-degree nb > 1024 under the
+    degree orientation, with sparse sub-DAGs."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    B = 1 + np.arange(nb)
+    L = 1 + nb + np.arange(nl)
+    src = [np.zeros(nb, np.int64), np.repeat(B, nl), np.repeat(B, inner)]
+    dst = [B, np.tile(L, nb), rng.choice(B, nb * inner)]
+    return xgm.from_edges(1 + nb + nl, np.concatenate(src), np.concatenate(dst))
+
+
+def test_clique_hubs_hbm_lut(xgm, orc, ctx):
+    import numpy as np
+    _say("clique hubs hbm lut")
+    d = _hub_graph(xgm).orient()
+    assert int(np.diff(d.rowptr).max()) > 1024
+    want4 = orc.count("clique4", d.rowptr, d.colidx)[0]
+    want5 = orc.count("clique5", d.rowptr, d.colidx)[0]
+    dg = ctx.upload(d)
+    for strat in (0, 6, 5):
+        assert ctx.count(dg, 2, strat) == want4, strat
+        assert ctx.count(dg, 3, strat) == want5, strat
+    task = np.diff(d.rowptr) >= 4
+    for c in (1, 4):
+        s = xgm.Schedule(dg, 3, np.where(task, c, -1))
+        assert s.count() == want5
+        s.close()
+    dg.close()

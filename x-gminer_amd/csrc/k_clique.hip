@@ -333,6 +333,129 @@ __global__ __launch_bounds__(NT) void k_clique_vlut_rank(const int64_t *__restri
   if (lane == 0 && acc) atomicAdd(out, acc);
 }
 
+// Triangular counting over rows in HBM with a runtime word count (no
+// per-thread row copies): K5 terms recompute row_a & row_j per word.
+__device__ __forceinline__ u64 lut_count_tri_wide(const u64 *__restrict__ rows, int s, int weff, int depth, int tid,
+                                                  int nthreads) {
+  u64 acc = 0;
+  const int items = s * weff;
+  for (int t = tid; t < items; t += nthreads) {
+    const int a = t / weff, q = t - a * weff;
+    if (q < (a >> 6)) continue;
+    const u64 *ra = rows + (size_t)a * weff;
+    u64 bits = ra[q];
+    if (depth == 2) {
+      acc += __popcll(bits);
+      continue;
+    }
+    while (bits) {
+      const int j = q * 64 + __builtin_ctzll(bits);
+      bits &= bits - 1;
+      const u64 *rj = rows + (size_t)j * weff;
+      const int ws = (j + 1) >> 6;
+      if (depth == 3) {
+        for (int w = ws; w < weff; ++w) acc += __popcll(ra[w] & rj[w]);
+      } else {
+        for (int w2 = ws; w2 < weff; ++w2) {
+          u64 tb = ra[w2] & rj[w2];
+          while (tb) {
+            const int k = w2 * 64 + __builtin_ctzll(tb);
+            tb &= tb - 1;
+            const u64 *rk = rows + (size_t)k * weff;
+            for (int w = (k + 1) >> 6; w < weff; ++w) acc += __popcll(ra[w] & rj[w] & rk[w]);
+          }
+        }
+      }
+    }
+  }
+  return acc;
+}
+
+// Rank-ordered vertex LUT for hubs (1024 < d+(v0) <= GMAX): the staging
+// (ids, ranks, sort keys, segment table) stays in LDS, the s x s-bit rows
+// live in this block's HBM slot (GMAX * GMAX / 8 bytes).
+template <int GMAX, int NT>
+struct GStage {
+  int x[GMAX];
+  int perm[GMAX];
+  union {
+    struct {
+      u64 key[GMAX];
+      int val[GMAX];
+    };
+    Segs<GMAX> seg;
+  };
+  int flagrow[NT / 64][64];
+  int ns, L, item;
+};
+
+template <int GMAX, int NT>
+__global__ __launch_bounds__(NT) void k_clique_vlut_rank_hbm(const int64_t *__restrict__ rowptr,
+                                                             const int32_t *__restrict__ colidx,
+                                                             const int64_t *__restrict__ rrowptr,
+                                                             const int32_t *__restrict__ tasks, int ntasks, int min_s,
+                                                             int depth, u64 *__restrict__ lut_slots,
+                                                             int *__restrict__ queue, u64 *__restrict__ out,
+                                                             u64 *__restrict__ err, u64 *__restrict__ tlog) {
+  __shared__ GStage<GMAX, NT> S;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  WaveFlags fl;
+  fl.init(S.flagrow[wave], lane);
+  u64 *rows = lut_slots + (size_t)blockIdx.x * ((size_t)GMAX * GMAX / 64);
+  u64 acc = 0;
+  for (;;) {
+    if (tid == 0) S.item = atomicAdd(queue, 1);
+    __syncthreads();
+    const int t = S.item;
+    if (t >= ntasks) break;
+    const u64 t_start = tlog ? (u64)wall_clock64() : 0;
+    const int v0 = tasks[t];
+    const int64_t b = rowptr[v0];
+    const int s = (int)(rowptr[v0 + 1] - b);
+    if (s >= min_s && s <= GMAX) {
+      const int weff = (s + 63) >> 6;
+      for (int q = tid; q < s; q += NT) {
+        const int x = colidx[b + q];
+        S.x[q] = x;
+        const u64 deg = (u64)((rowptr[x + 1] - rowptr[x]) + (rrowptr[x + 1] - rrowptr[x]));
+        S.key[q] = (deg << 32) | (u64)(uint32_t)x;
+        S.val[q] = q;
+      }
+      for (int q = tid; q < s * weff; q += NT) rows[q] = 0ull;
+      __syncthreads();
+      block_bitonic_sort<GMAX, NT>(S.key, S.val, s);
+      for (int r = tid; r < s; r += NT) S.perm[S.val[r]] = r;
+      __syncthreads();
+      if (wave == 0) {
+        int ns, L;
+        wave_segments(S.seg, rowptr, colidx, S.x, 0, s, -1, KeepAll(), lane, ns, L);
+        if (lane == 0) { S.ns = ns; S.L = L; }
+      }
+      __threadfence_block();
+      __syncthreads();
+      int lo, hi;
+      wave_share(S.L, wave, NT / 64, lo, hi);
+      wave_walk<4>(S.seg, colidx, S.ns, lo, hi, lane, fl, [&](const WindowLane &r) {
+        if (r.valid) {
+          const int pos = lower_bound_dev(S.x, s, r.w);
+          if (pos < s && S.x[pos] == r.w) {
+            const int col = S.perm[pos], row = S.perm[S.seg.idx[r.j]];
+            if (col <= row) atomicOr(err, 1ull);
+            atomicOr(&rows[(size_t)row * weff + (col >> 6)], 1ull << (col & 63));
+          }
+        }
+      });
+      __threadfence_block();
+      __syncthreads();
+      acc += lut_count_tri_wide(rows, s, weff, depth, tid, NT);
+    }
+    __syncthreads();
+    if (tlog && tid == 0) tlog[t] = (u64)wall_clock64() - t_start;
+  }
+  acc = wave_sum_u64(acc);
+  if (lane == 0 && acc) atomicAdd(out, acc);
+}
+
 // Vertex-task LUT kernel: one block (NT threads) per vertex v0 with
 // min_s <= d+(v0) <= SMAX.  depth = k - 1 (K4 -> triangles, K5 -> 4-cliques).
 template <int SMAX, int W, int NT>
@@ -462,6 +585,18 @@ hipError_t run_vlut_rank(const char *name, const DevCSR &g, const TaskList &tl, 
   return hipGetLastError();
 }
 
+hipError_t run_vlut_rank_hbm(const char *name, const DevCSR &g, const TaskList &tl, int k, u64 *slots, int *queue,
+                             u64 *tlog, Launch &l) {
+  if (!tl.count) return hipSuccess;
+  auto kern = k_clique_vlut_rank_hbm<kHubLutMax, 1024>;
+  int grid = std::min(grid_for(kern, 1024, 0, l.num_cu, tl.count), clique_hub_slots(l.num_cu));
+  l.begin(name, tl.count);
+  kern<<<grid, 1024, 0, l.stream>>>(g.rowptr, g.colidx, g.rrowptr, tl.tasks, (int)tl.count, k - 1, k - 1, slots,
+                                    queue, l.d_out, l.d_err, tlog);
+  l.end();
+  return hipGetLastError();
+}
+
 template <int SMAX, int W, int NT>
 hipError_t run_edge(const char *name, const DevCSR &g, const EdgeItems &ei, int k, int *scratch, int md,
                     int *queue, u64 *tlog, Launch &l) {
@@ -494,9 +629,14 @@ int clique_list_capacity(int num_cu, int li) {
     case 8: return grid_for(k_clique_edge<128, 2, 64>, 64, 0, num_cu, inf);
     case 9: return grid_for(k_clique_edge<1024, 16, 256>, 256, 0, num_cu, inf);
     case 10: return grid_for(k_clique_edge<128, 0, 64>, 64, 0, num_cu, inf);
-    default: return grid_for(k_clique_edge<1024, 0, 256>, 256, 0, num_cu, inf);
+    case 11: return grid_for(k_clique_edge<1024, 0, 256>, 256, 0, num_cu, inf);
+    default:
+      return std::min(grid_for(k_clique_vlut_rank_hbm<kHubLutMax, 1024>, 1024, 0, num_cu, inf),
+                      clique_hub_slots(num_cu));
   }
 }
+
+int clique_hub_slots(int num_cu) { return num_cu; }
 
 int clique_edge_grid(int num_cu) {
   int a = grid_for(k_clique_edge<128, 2, 64>, 64, 0, num_cu, 1 << 30);
@@ -522,6 +662,9 @@ hipError_t launch_clique(const DevCSR &g, const CliqueTasks &ct, int k, int *scr
     return e;
   if ((e = run_edge<128, 0, 64>(k4 ? "k4_edge_list_128" : "k5_edge_list_128", g, ct.list[0], k, scratch, md,
                                 d_queues + 11, ct.log_list[0], l)))
+    return e;
+  if ((e = run_vlut_rank_hbm(k4 ? "k4_rlut_hbm" : "k5_rlut_hbm", g, ct.rlut_hbm, k, ct.hub_slots, d_queues + 12,
+                             ct.log_rlut_hbm, l)))
     return e;
   if ((e = run_vlut_rank<1024, 16, 1024>(k4 ? "k4_rlut_1024" : "k5_rlut_1024", g, ct.rlut[3], k, d_queues + 4,
                                          ct.log_rlut[3], l)))
@@ -622,7 +765,8 @@ __global__ void k_sched_argmin(const int64_t *__restrict__ rowptr, int64_t n, co
   int best = 2;
   float bv = y[2];
   for (int c = 0; c < 5; ++c) {
-    const bool ok = (c == 2 || c == 3) || d <= 1024;  // LUT codes only up to 1024 rows
+    // id-ordered LUT up to 1024 rows; rank LUT up to kHubLutMax (HBM rows above 1024)
+    const bool ok = (c == 2 || c == 3) || (c == 0 ? d <= 1024 : d <= kHubLutMax);
     if (ok && y[c] < bv) { bv = y[c]; best = c; }
   }
   codes[v] = (int8_t)best;

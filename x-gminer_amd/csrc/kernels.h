@@ -70,17 +70,21 @@ struct EdgeItems {
 struct CliqueTasks {
   TaskList lut[4], rlut[4];
   EdgeItems elut[2], list[2];
-  u64 *log_lut[4] = {}, *log_rlut[4] = {}, *log_elut[2] = {}, *log_list[2] = {};
+  TaskList rlut_hbm;        // rank-ordered LUT with rows in HBM (1024 < d+ <= kHubLutMax)
+  u64 *hub_slots = nullptr; // clique_hub_slots() * kHubLutMax^2 / 64 words
+  u64 *log_lut[4] = {}, *log_rlut[4] = {}, *log_elut[2] = {}, *log_list[2] = {}, *log_rlut_hbm = nullptr;
 };
+constexpr int kHubLutMax = 4096;
 
 // k-clique counting (k = 4 or 5) over a DAG.  scratch holds
 // clique_edge_grid() * 2 * md ints (md = max out-degree).  Adds into l.d_out.
 hipError_t launch_clique(const DevCSR &g, const CliqueTasks &ct, int k, int *scratch, int md, Launch &l,
-                         int *d_queues /* 12 counters */);
+                         int *d_queues /* 13 counters */);
 // Scheduler features of every vertex (n x 16 floats) for k-clique tasks.
 hipError_t launch_clique_features(const DevCSR &g, int k, float *F, hipStream_t s);
 int clique_edge_grid(int num_cu);
-int clique_list_capacity(int num_cu, int list);
+int clique_list_capacity(int num_cu, int list);  // list 12 = rank LUT in HBM
+int clique_hub_slots(int num_cu);
 
 // Edge-induced diamond over an undirected CSR; items are int4 (owner t,
 // k0, k1, 0) (TaskList.tasks points at 4*count ints, count = items) in

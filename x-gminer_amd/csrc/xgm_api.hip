@@ -47,12 +47,13 @@ struct xgm_ctx {
 };
 
 namespace {
-constexpr int kClqLists = 12;
+constexpr int kClqLists = 13;
 constexpr int kBinCap[4] = {64, 256, 512, 1024};
 }  // namespace
 
 // Lists of a clique schedule: 0-3 id-ordered LUT bins, 4-7 rank-ordered
-// LUT bins, 8-9 edge LUT (min out-degree <= 128 / larger), 10-11 edge list.
+// LUT bins, 8-9 edge LUT (min out-degree <= 128 / larger), 10-11 edge list,
+// 12 rank-ordered LUT with rows in HBM (1024 < d+ <= kHubLutMax).
 
 struct xgm_sched {
   int pattern = 0, k = 0;
@@ -74,11 +75,13 @@ struct xgm_sched {
       ct.elut[i] = EdgeItems{d_buf + off[8 + i], cnt[8 + i]};
       ct.list[i] = EdgeItems{d_buf + off[10 + i], cnt[10 + i]};
     }
+    ct.rlut_hbm = TaskList{d_buf + off[12], cnt[12]};
     if (log_base) {
       int64_t o = 0;
       u64 **dst[kClqLists] = {&ct.log_lut[0], &ct.log_lut[1], &ct.log_lut[2], &ct.log_lut[3],
                               &ct.log_rlut[0], &ct.log_rlut[1], &ct.log_rlut[2], &ct.log_rlut[3],
-                              &ct.log_elut[0], &ct.log_elut[1], &ct.log_list[0], &ct.log_list[1]};
+                              &ct.log_elut[0], &ct.log_elut[1], &ct.log_list[0], &ct.log_list[1],
+                              &ct.log_rlut_hbm};
       for (int i = 0; i < kClqLists; ++i) {
         *dst[i] = log_base + o;
         o += cnt[i];
@@ -119,6 +122,7 @@ struct xgm_dgraph {
   std::vector<int64_t> work;
   bool relabelled = false;
   std::map<std::string, std::unique_ptr<xgm_sched>> scheds;  // cached clique schedules
+  u64 *d_hub_lut = nullptr;        // per-block HBM LUT rows for clique hubs
   int *d_dense = nullptr;          // per-block 4-cycle counters, zero between counts
   size_t dense_ints = 0;
   int64_t *d_rl_rowptr = nullptr;
@@ -132,6 +136,7 @@ struct xgm_dgraph {
   ~xgm_dgraph() {
     (void)hipFree(d_dense);
     (void)hipFree(d_hub);
+    (void)hipFree(d_hub_lut);
     (void)hipFree(d_rl_rowptr);
     (void)hipFree(d_rl_colidx);
     for (auto &kv : task_cache) (void)hipFree(kv.second.first);
@@ -506,8 +511,9 @@ void build_tc_items(const xgm_dgraph *g, std::vector<std::vector<int32_t>> &part
 }
 
 // codes[v]: 0 LUT, 1 rank LUT, 2 edge LUT, 3 edge list, 4 rank LUT one bin
-// up; vertices with d+ < k-1 are not tasks; LUT codes on hubs (d+ > 1024)
-// and unknown codes fall back to the edge LUT.
+// up; vertices with d+ < k-1 are not tasks.  Rank codes on hubs
+// (1024 < d+ <= kHubLutMax) use the HBM-row LUT; other LUT codes on hubs and
+// unknown codes fall back to the edge LUT.
 int build_clique_sched(xgm_ctx *ctx, const xgm_dgraph *g, int pattern, const int8_t *codes, xgm_sched *sc) {
   const int k = pattern == XGM_CLIQUE4 ? 4 : 5;
   sc->pattern = pattern;
@@ -519,6 +525,11 @@ int build_clique_sched(xgm_ctx *ctx, const xgm_dgraph *g, int pattern, const int
     if (d < k - 1) continue;
     int c = codes ? codes[v] : 0;
     if (c < 0 || c > 4) c = 2;
+    if (d > kBinCap[3] && (c == 1 || c == 4) && d <= kHubLutMax) {
+      L[12].push_back(v);
+      sc->owner[12].push_back(v);
+      continue;
+    }
     if (d > kBinCap[3] && (c == 0 || c == 1 || c == 4)) c = 2;
     int bin = 0;
     while (d > kBinCap[bin]) ++bin;
@@ -561,7 +572,7 @@ void heuristic_codes(const xgm_dgraph *g, int k, int strategy, int lut_max, int8
       switch (strategy) {
         case XGM_STRAT_LIST: c = 3; break;
         case XGM_STRAT_ELUT: c = 2; break;
-        case XGM_STRAT_LUT_RANK: c = d <= lut_max ? 1 : 2; break;
+        case XGM_STRAT_LUT_RANK: c = (d <= lut_max || d > 1024) ? 1 : 2; break;  // hubs: HBM-row LUT
         default: c = d <= lut_max ? 0 : 2; break;
       }
     }
@@ -576,7 +587,12 @@ int run_clique_sched(xgm_ctx *ctx, xgm_dgraph *gm, const xgm_sched *sc, const De
     int rc = ensure_scratch(ctx, gm, need);
     if (rc) return rc;
   }
+  if (sc->cnt[12] && !gm->d_hub_lut) {
+    const size_t words = (size_t)clique_hub_slots(ctx->num_cu) * ((size_t)kHubLutMax * kHubLutMax / 64);
+    HIP_TRY(ctx, hipMalloc(&gm->d_hub_lut, words * sizeof(u64)));
+  }
   CliqueTasks ct = sc->tasks(log);
+  ct.hub_slots = gm->d_hub_lut;
   hipError_t e = launch_clique(dg, ct, sc->k, gm->d_scratch, std::max(1, gm->max_out), l, ctx->d_queues);
   if (e != hipSuccess) return ctx->hip_fail(e, "clique launch");
   return XGM_OK;

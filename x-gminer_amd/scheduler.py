@@ -2,6 +2,9 @@
 ``xgm_sched_create`` (0 id-ordered LUT, 1 rank-ordered LUT, 2 edge LUT,
 3 edge nested lists, 4 rank-ordered LUT one capacity bin up).
 
+Codes 1 and 4 on vertices with 1024 < d+ <= 4096 run the rank-ordered LUT
+with its rows in HBM; the plain heuristic sends those vertices to the edge LUT.
+
 * ``collect``: per-vertex features (computed on the GPU) and per-vertex
   cost under every code, from instrumented runs of the plain kernels: the
   task's wall time divided by the resident blocks of its kernel, i.e. its
@@ -49,7 +52,8 @@ def collect(dg, pattern=CLIQUE5, codes=range(NCODES), max_rows=None, seed=0):
         elif n != count:
             raise RuntimeError(f"count differs between schedules: {n} vs {count} (code {c})")
         T[:, c] = np.log2(1.0 + 1000.0 * us)  # kernel-time share in ns
-        valid = task & ((c in (2, 3)) | (dplus <= 1024))
+        limit = 1024 if c == 0 else 4096  # id-ordered LUT in LDS; rank LUT up to 4096 rows (HBM above 1024)
+        valid = task & ((c in (2, 3)) | (dplus <= limit))
         Mk[:, c] = valid
     rows = np.nonzero(task)[0]
     if max_rows and len(rows) > max_rows:
