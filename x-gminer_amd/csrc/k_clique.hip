@@ -422,6 +422,7 @@ __global__ __launch_bounds__(NT) void k_clique_vlut_rank_hbm(const int64_t *__re
         S.val[q] = q;
       }
       for (int q = tid; q < s * weff; q += NT) rows[q] = 0ull;
+      __threadfence();  // zero fill reaches L2 before the atomics below
       __syncthreads();
       block_bitonic_sort<GMAX, NT>(S.key, S.val, s);
       for (int r = tid; r < s; r += NT) S.perm[S.val[r]] = r;
@@ -431,7 +432,6 @@ __global__ __launch_bounds__(NT) void k_clique_vlut_rank_hbm(const int64_t *__re
         wave_segments(S.seg, rowptr, colidx, S.x, 0, s, -1, KeepAll(), lane, ns, L);
         if (lane == 0) { S.ns = ns; S.L = L; }
       }
-      __threadfence_block();
       __syncthreads();
       int lo, hi;
       wave_share(S.L, wave, NT / 64, lo, hi);
@@ -445,7 +445,10 @@ __global__ __launch_bounds__(NT) void k_clique_vlut_rank_hbm(const int64_t *__re
           }
         }
       });
-      __threadfence_block();
+      // The bits were set by atomics performed in L2; an agent-scope fence
+      // (acquire invalidates this CU's L1) is needed before plain loads, or
+      // lines cached by the zero fill or an earlier task read stale.
+      __threadfence();
       __syncthreads();
       acc += lut_count_tri_wide(rows, s, weff, depth, tid, NT);
     }

@@ -178,8 +178,7 @@ __global__ __launch_bounds__(NT) void k_cycle4_wedge(const int64_t *__restrict__
       (void)ns;
     }
     if (DENSE) {
-      // reset the counters this task touched
-      __threadfence_block();
+      // reset the counters this task touched (its atomics have all returned)
       for (int k0 = 0; k0 < nlow; k0 += PCAP) {
         const int k1 = min(nlow, k0 + PCAP);
         __syncthreads();
@@ -196,7 +195,7 @@ __global__ __launch_bounds__(NT) void k_cycle4_wedge(const int64_t *__restrict__
           if (r.valid) cnt[r.w] = 0;
         });
       }
-      __threadfence_block();
+      __threadfence();  // the zeros reach L2 before the next task's atomics
     }
     __syncthreads();
   }
