@@ -552,7 +552,7 @@ int build_clique_sched(xgm_ctx *ctx, const xgm_dgraph *g, int pattern, const int
   }
   std::vector<int32_t> all;
   for (int i = 0; i < kClqLists; ++i) {
-    const int stride = i >= 8 ? 2 : 1;
+    const int stride = (i >= 8 && i <= 11) ? 2 : 1;  // edge lists hold (v0, v1) pairs
     sc->off[i] = (int64_t)all.size();
     sc->cnt[i] = (int64_t)L[i].size() / stride;
     all.insert(all.end(), L[i].begin(), L[i].end());
