@@ -65,9 +65,13 @@ enum xgm_strategy {
 
 typedef struct xgm_opts {
   int strategy;          /* xgm_strategy; 0 = auto */
-  int lut_max_degree;    /* LUT path for tasks with out-degree <= this; 0 = 1024 */
+  int lut_max_degree;    /* cliques: vertex LUT for out-degree <= this; 0 = 1024 */
   uint32_t flags;        /* reserved, must be 0 */
-  int reserved;
+  int hash_shift;        /* triangles: LDS hash load factor 2^-shift, 1 or 2 (LDS per wave); 0 = 2 */
+  int wave_max_small;    /* triangles: staged out-degree up to this runs one wave per item in the
+                            64-slot kernel (1..64); 0 = 64 */
+  int wave_max_large;    /* triangles: ... in the 256-slot wave kernel (wave_max_small..256), larger
+                            staged lists run one block per item; 0 = 256 */
 } xgm_opts;
 
 typedef struct xgm_ctx xgm_ctx;
@@ -97,6 +101,29 @@ int xgm_count(xgm_ctx *ctx, const xgm_dgraph *g, int pattern, const xgm_opts *op
 /* One-shot convenience: upload, count, free. */
 int xgm_count_csr(xgm_ctx *ctx, const xgm_csr *g, int pattern, const xgm_opts *opts,
                   uint64_t *out);
+
+/* ---- batches of instances (disjoint union) ---------------------------
+ * Upload n_inst graphs of one kind (same flags) as one device graph; every
+ * count on it can return one uint64 per instance.  Replaces running the
+ * reference solver once per graph. */
+int xgm_graph_upload_batch(xgm_ctx *ctx, const xgm_csr *graphs, int n_inst, xgm_dgraph **out);
+int xgm_graph_instances(const xgm_dgraph *g);
+/* out[i] = count of `pattern` in instance i (n_inst entries). */
+int xgm_count_instances(xgm_ctx *ctx, const xgm_dgraph *g, int pattern, const xgm_opts *opts, uint64_t *out);
+
+/* A plan captures the counts of several patterns -- triangles / cliques on a
+ * DAG batch, diamonds / 4-cycles on an undirected batch (either may be NULL
+ * if unused) -- into one hipGraph of persistent work-queue kernels that is
+ * replayed by xgm_plan_run.  opts: one per pattern (NULL = auto). */
+typedef struct xgm_plan xgm_plan;
+int xgm_plan_create(xgm_ctx *ctx, const xgm_dgraph *dag, const xgm_dgraph *und, const int *patterns, int n_pat,
+                    const xgm_opts *opts, xgm_plan **out);
+/* out: n_pat x n_inst counts (pattern-major); *ms = replay time (HIP events). */
+int xgm_plan_run(xgm_ctx *ctx, xgm_plan *plan, uint64_t *out, float *ms);
+void xgm_plan_free(xgm_plan *plan);
+/* Autotuner: times the candidate option sets of `pattern` on g (every
+ * candidate gives the same count) and writes the fastest into *best. */
+int xgm_autotune(xgm_ctx *ctx, const xgm_dgraph *g, int pattern, int reps, xgm_opts *best, float *best_ms);
 
 /* Timing of the most recent xgm_count on ctx, from HIP events recorded on
  * the stream the kernels run on. */

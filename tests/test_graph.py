@@ -99,3 +99,21 @@ def test_validate_rejects(xgm):
     asym = xgm.Graph.from_arrays(np.array([0, 1, 1], np.int64), np.array([1], np.int32),
                                  xgm.CSR_SORTED | xgm.CSR_SYMMETRIC)
     assert not asym.validate()
+
+
+def test_batch_workload_spec(xgm):
+    """256 instances, seeds 100..355: ER n in [1e3, 2e5], d in [4, 64];
+    RMAT scale in 10..18 -- deterministic across calls."""
+    from xgminer_amd import workloads
+    s = workloads.batch_specs()
+    assert [x[0] for x in s] == list(range(100, 356))
+    assert sum(x[1] == "er" for x in s) == 128 and sum(x[1] == "rmat" for x in s) == 128
+    for seed, kind, p in s:
+        if kind == "er":
+            assert 1000 <= p["n"] <= 200000 and 4.0 <= p["avg_degree"] <= 64.0
+        else:
+            assert 10 <= p["scale"] <= 18
+    assert workloads.batch_specs() == s
+    assert len({x[2]["scale"] for x in s if x[1] == "rmat"}) == 9
+    small = workloads.batch_specs(2)
+    assert [x[0] for x in small] == [100, 101, 228, 229]

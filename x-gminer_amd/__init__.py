@@ -56,7 +56,15 @@ class _Stats(C.Structure):
 
 class _Opts(C.Structure):
     _fields_ = [("strategy", C.c_int), ("lut_max_degree", C.c_int),
-                ("flags", C.c_uint32), ("reserved", C.c_int)]
+                ("flags", C.c_uint32), ("hash_shift", C.c_int),
+                ("wave_max_small", C.c_int), ("wave_max_large", C.c_int)]
+
+    def as_dict(self):
+        return {k: int(getattr(self, k)) for k, _ in self._fields_ if k != "flags"}
+
+
+def _opts(strategy=0, lut_max_degree=0, hash_shift=0, wave_max_small=0, wave_max_large=0):
+    return _Opts(strategy, lut_max_degree, 0, hash_shift, wave_max_small, wave_max_large)
 
 
 class _KTime(C.Structure):
@@ -95,6 +103,13 @@ def _load():
         "xgm_count": (C.c_int, [VP, VP, C.c_int, P(_Opts), P(U64)]),
         "xgm_count_csr": (C.c_int, [VP, P(_Csr), C.c_int, P(_Opts), P(U64)]),
         "xgm_last_profile": (C.c_int, [VP, P(_Profile)]),
+        "xgm_graph_upload_batch": (C.c_int, [VP, P(_Csr), C.c_int, P(VP)]),
+        "xgm_graph_instances": (C.c_int, [VP]),
+        "xgm_count_instances": (C.c_int, [VP, VP, C.c_int, P(_Opts), P(U64)]),
+        "xgm_plan_create": (C.c_int, [VP, VP, VP, P(C.c_int), C.c_int, P(_Opts), P(VP)]),
+        "xgm_plan_run": (C.c_int, [VP, VP, P(U64), P(C.c_float)]),
+        "xgm_plan_free": (None, [VP]),
+        "xgm_autotune": (C.c_int, [VP, VP, C.c_int, C.c_int, P(_Opts), P(C.c_float)]),
         "xgm_mlp_create": (C.c_int, [C.c_int, U64, P(VP)]),
         "xgm_mlp_free": (None, [VP]),
         "xgm_mlp_last_error": (C.c_char_p, [VP]),
@@ -263,8 +278,29 @@ class Context:
     def upload(self, g: Graph) -> "DeviceGraph":
         return DeviceGraph(self, g)
 
-    def count(self, g, pattern: int, strategy: int = STRAT_AUTO, lut_max_degree: int = 0) -> int:
-        opts = _Opts(strategy, lut_max_degree, 0, 0)
+    def upload_batch(self, graphs) -> "DeviceGraph":
+        """Disjoint union of graphs sharing one kind (DAG or undirected)."""
+        return DeviceGraph(self, list(graphs))
+
+    def count_instances(self, dg: "DeviceGraph", pattern: int, **opts) -> np.ndarray:
+        """One count per instance of a batch (one entry for a single graph)."""
+        out = np.zeros(dg.n_inst, np.uint64)
+        o = _opts(**opts)
+        _check(lib.xgm_count_instances(self._h, dg._h, pattern, C.byref(o),
+                                       out.ctypes.data_as(C.POINTER(C.c_uint64))), self._h)
+        return out
+
+    def autotune(self, dg: "DeviceGraph", pattern: int, reps: int = 3):
+        """Fastest option set for pattern on dg: (opts dict, ms)."""
+        o, ms = _Opts(), C.c_float()
+        _check(lib.xgm_autotune(self._h, dg._h, pattern, reps, C.byref(o), C.byref(ms)), self._h)
+        return o.as_dict(), float(ms.value)
+
+    def plan(self, patterns, dag=None, und=None, opts=None) -> "Plan":
+        return Plan(self, patterns, dag, und, opts)
+
+    def count(self, g, pattern: int, strategy: int = STRAT_AUTO, lut_max_degree: int = 0, **kw) -> int:
+        opts = _opts(strategy, lut_max_degree, **kw)
         out = C.c_uint64()
         if isinstance(g, DeviceGraph):
             _check(lib.xgm_count(self._h, g._h, pattern, C.byref(opts), C.byref(out)), self._h)
@@ -302,36 +338,78 @@ class Schedule:
     def count(self, log=False):
         """Returns count, or (count, per-vertex microseconds) with log=True."""
         out = C.c_uint64()
-        us = np.zeros(self.dg.host.n, np.float32) if log else None
+        us = np.zeros(self.dg.n, np.float32) if log else None
         _check(lib.xgm_count_sched(self.dg.ctx._h, self.dg._h, self._h,
                                    us.ctypes.data_as(C.POINTER(C.c_float)) if log else None, C.byref(out)),
                self.dg.ctx._h)
         return (int(out.value), us) if log else int(out.value)
 
 
+class Plan:
+    """Counts of several patterns over resident batches captured as one
+    hipGraph (xgm_plan_create); run() replays it."""
+
+    def __init__(self, ctx: Context, patterns, dag=None, und=None, opts=None):
+        pats = (C.c_int * len(patterns))(*patterns)
+        oa = None
+        if opts is not None:
+            oa = (_Opts * len(patterns))(*[_opts(**o) for o in opts])
+        h = C.c_void_p()
+        _check(lib.xgm_plan_create(ctx._h, dag._h if dag else None, und._h if und else None, pats, len(patterns),
+                                   oa, C.byref(h)), ctx._h)
+        self._h, self.ctx, self.patterns = h, ctx, list(patterns)
+        self.n_inst = (dag or und).n_inst
+
+    def run(self):
+        """(counts[n_patterns, n_instances], replay ms)."""
+        out = np.zeros((len(self.patterns), self.n_inst), np.uint64)
+        ms = C.c_float()
+        _check(lib.xgm_plan_run(self.ctx._h, self._h, out.ctypes.data_as(C.POINTER(C.c_uint64)), C.byref(ms)),
+               self.ctx._h)
+        return out, float(ms.value)
+
+    def close(self):
+        if getattr(self, "_h", None) is not None:
+            lib.xgm_plan_free(self._h)
+            self._h = None
+
+    __del__ = close
+
+
 class DeviceGraph:
     def sched_heuristic(self, pattern, strategy=STRAT_AUTO, lut_max_degree=0):
-        codes = np.zeros(self.host.n, np.int8)
+        codes = np.zeros(self.n, np.int8)
         _check(lib.xgm_sched_heuristic(self._h, pattern, strategy, lut_max_degree,
                                        codes.ctypes.data_as(C.POINTER(C.c_int8))))
         return codes
 
     def sched_features(self, pattern):
-        F = np.zeros((self.host.n, MLP_IN), np.float32)
+        F = np.zeros((self.n, MLP_IN), np.float32)
         _check(lib.xgm_sched_features(self.ctx._h, self._h, pattern, F.ctypes.data_as(C.POINTER(C.c_float))),
                self.ctx._h)
         return F
 
     def sched_learned(self, pattern, model: "MLP"):
-        codes = np.zeros(self.host.n, np.int8)
+        codes = np.zeros(self.n, np.int8)
         _check(lib.xgm_sched_learned(self.ctx._h, self._h, pattern, model._h,
                                      codes.ctypes.data_as(C.POINTER(C.c_int8))), self.ctx._h)
         return codes
 
-    def __init__(self, ctx: Context, g: Graph):
+    def __init__(self, ctx: Context, g):
         h = C.c_void_p()
-        _check(lib.xgm_graph_upload(ctx._h, g.csr, C.byref(h)), ctx._h)
-        self._h, self.ctx, self.host = h, ctx, g
+        if isinstance(g, list):  # batch of instances (disjoint union)
+            arr = (_Csr * len(g))(*[x._csr for x in g])
+            _check(lib.xgm_graph_upload_batch(ctx._h, arr, len(g), C.byref(h)), ctx._h)
+            self.instances = g
+            self.host = None
+            self.n = sum(x.n for x in g)
+        else:
+            _check(lib.xgm_graph_upload(ctx._h, g.csr, C.byref(h)), ctx._h)
+            self.instances = None
+            self.host = g
+            self.n = g.n
+        self._h, self.ctx = h, ctx
+        self.n_inst = int(lib.xgm_graph_instances(h))
         ctx._graphs.add(self)
 
     def close(self):

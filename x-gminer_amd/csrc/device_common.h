@@ -117,3 +117,17 @@ __device__ __forceinline__ int hash_bits_for(int n, int shift = 1) {
   while ((1 << b) < (n << shift)) ++b;
   return b;
 }
+
+// Add a task's per-lane partial count.  Without slots it joins the
+// kernel-wide accumulator (flushed once at the end); with slots (batched
+// instances) the wave reduces it and adds it to out[slots[v]].  Every lane
+// of the wave must call.
+__device__ __forceinline__ void task_flush(u64 &acc, u64 part, const int32_t *__restrict__ slots, int v,
+                                           u64 *__restrict__ out, int lane) {
+  if (!slots) {
+    acc += part;
+    return;
+  }
+  part = wave_sum_u64(part);
+  if (lane == 0 && part) atomicAdd(out + slots[v], part);
+}

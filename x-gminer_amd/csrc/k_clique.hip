@@ -276,7 +276,8 @@ __global__ __launch_bounds__(NT) void k_clique_vlut_rank(const int64_t *__restri
                                                          const int64_t *__restrict__ rrowptr,
                                                          const int32_t *__restrict__ tasks, int ntasks, int min_s,
                                                          int depth, int *__restrict__ queue, u64 *__restrict__ out,
-                                                         u64 *__restrict__ err, u64 *__restrict__ tlog) {
+                                                         u64 *__restrict__ err, u64 *__restrict__ tlog,
+                                                         const int32_t *__restrict__ slots) {
   __shared__ RStage<SMAX, W, NT> S;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   WaveFlags fl;
@@ -288,6 +289,7 @@ __global__ __launch_bounds__(NT) void k_clique_vlut_rank(const int64_t *__restri
     const int t = S.item;
     if (t >= ntasks) break;
     const u64 t_start = tlog ? (u64)wall_clock64() : 0;
+    u64 part = 0;
     const int v0 = tasks[t];
     const int64_t b = rowptr[v0];
     const int s = (int)(rowptr[v0 + 1] - b);
@@ -324,8 +326,9 @@ __global__ __launch_bounds__(NT) void k_clique_vlut_rank(const int64_t *__restri
         }
       });
       __syncthreads();
-      acc += lut_count_tri<W>(S.rows, s, weff, depth, tid, NT);
+      part += lut_count_tri<W>(S.rows, s, weff, depth, tid, NT);
     }
+    task_flush(acc, part, slots, v0, out, tid & 63);
     __syncthreads();
     if (tlog && tid == 0) tlog[t] = (u64)wall_clock64() - t_start;
   }
@@ -396,7 +399,8 @@ __global__ __launch_bounds__(NT) void k_clique_vlut_rank_hbm(const int64_t *__re
                                                              const int32_t *__restrict__ tasks, int ntasks, int min_s,
                                                              int depth, u64 *__restrict__ lut_slots,
                                                              int *__restrict__ queue, u64 *__restrict__ out,
-                                                             u64 *__restrict__ err, u64 *__restrict__ tlog) {
+                                                             u64 *__restrict__ err, u64 *__restrict__ tlog,
+                                                             const int32_t *__restrict__ slots) {
   __shared__ GStage<GMAX, NT> S;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   WaveFlags fl;
@@ -409,6 +413,7 @@ __global__ __launch_bounds__(NT) void k_clique_vlut_rank_hbm(const int64_t *__re
     const int t = S.item;
     if (t >= ntasks) break;
     const u64 t_start = tlog ? (u64)wall_clock64() : 0;
+    u64 part = 0;
     const int v0 = tasks[t];
     const int64_t b = rowptr[v0];
     const int s = (int)(rowptr[v0 + 1] - b);
@@ -450,8 +455,9 @@ __global__ __launch_bounds__(NT) void k_clique_vlut_rank_hbm(const int64_t *__re
       // lines cached by the zero fill or an earlier task read stale.
       __threadfence();
       __syncthreads();
-      acc += lut_count_tri_wide(rows, s, weff, depth, tid, NT);
+      part += lut_count_tri_wide(rows, s, weff, depth, tid, NT);
     }
+    task_flush(acc, part, slots, v0, out, tid & 63);
     __syncthreads();
     if (tlog && tid == 0) tlog[t] = (u64)wall_clock64() - t_start;
   }
@@ -466,7 +472,7 @@ __global__ __launch_bounds__(NT) void k_clique_vlut(const int64_t *__restrict__ 
                                                     const int32_t *__restrict__ colidx,
                                                     const int32_t *__restrict__ tasks, int ntasks, int min_s,
                                                     int depth, int *__restrict__ queue, u64 *__restrict__ out,
-                                                    u64 *__restrict__ tlog) {
+                                                    u64 *__restrict__ tlog, const int32_t *__restrict__ slots) {
   __shared__ CStage<SMAX, W, NT> S;
   const int tid = threadIdx.x;
   WaveFlags fl;
@@ -478,13 +484,15 @@ __global__ __launch_bounds__(NT) void k_clique_vlut(const int64_t *__restrict__ 
     const int t = S.item;
     if (t >= ntasks) break;
     const u64 t_start = tlog ? (u64)wall_clock64() : 0;
+    u64 part = 0;
     const int v0 = tasks[t];
     const int64_t b = rowptr[v0];
     const int s = (int)(rowptr[v0 + 1] - b);
     if (s >= min_s) {
       for (int q = tid; q < s; q += NT) S.x[q] = colidx[b + q];
-      acc += block_lut_count<SMAX, W, NT>(rowptr, colidx, S, s, depth, fl);
+      part += block_lut_count<SMAX, W, NT>(rowptr, colidx, S, s, depth, fl);
     }
+    task_flush(acc, part, slots, v0, out, tid & 63);
     __syncthreads();
     if (tlog && tid == 0) tlog[t] = (u64)wall_clock64() - t_start;
   }
@@ -504,7 +512,7 @@ __global__ __launch_bounds__(NT) void k_clique_edge(const int64_t *__restrict__ 
                                                     const int2 *__restrict__ items, int nitems, int k,
                                                     int *__restrict__ scratch, int md,
                                                     int *__restrict__ queue, u64 *__restrict__ out,
-                                                    u64 *__restrict__ tlog) {
+                                                    u64 *__restrict__ tlog, const int32_t *__restrict__ slots) {
   __shared__ CStage<SMAX, W, NT> S;
   const int tid = threadIdx.x;
   WaveFlags fl;
@@ -518,7 +526,9 @@ __global__ __launch_bounds__(NT) void k_clique_edge(const int64_t *__restrict__ 
     const int t = S.item;
     if (t >= nitems) break;
     const u64 t_start = tlog ? (u64)wall_clock64() : 0;
+    u64 part = 0;
     const int2 it = items[t];
+    const int v0 = it.x;
     const int64_t b0 = rowptr[it.x], b1 = rowptr[it.y];
     const int d0 = (int)(rowptr[it.x + 1] - b0), d1 = (int)(rowptr[it.y + 1] - b1);
     if (d0 >= k - 2 && d1 >= k - 2) {
@@ -528,13 +538,13 @@ __global__ __launch_bounds__(NT) void k_clique_edge(const int64_t *__restrict__ 
         bool done = false;
         if constexpr (W > 0) {
           if (w1 == S.x) {
-            acc += block_lut_count<SMAX, W, NT>(rowptr, colidx, S, n1, k - 2, fl);
+            part += block_lut_count<SMAX, W, NT>(rowptr, colidx, S, n1, k - 2, fl);
             done = true;
           }
         }
         if (done) {
         } else if (k == 4) {
-          acc += block_sum_intersections<NT, SMAX>(rowptr, colidx, w1, n1, Member{w1, n1}, S, fl);
+          part += block_sum_intersections<NT, SMAX>(rowptr, colidx, w1, n1, Member{w1, n1}, S, fl);
         } else {
           for (int c = 0; c < n1; ++c) {
             const int w = w1[c];
@@ -544,11 +554,12 @@ __global__ __launch_bounds__(NT) void k_clique_edge(const int64_t *__restrict__ 
             int *w2 = (min(n1, dw) <= SMAX) ? S.x2 : g_w2;
             const int n2 = block_intersect_ordered<NT>(colidx + bw, dw, Member{w1, n1}, w2, S);
             if (n2 < 2) continue;
-            acc += block_sum_intersections<NT, SMAX>(rowptr, colidx, w2, n2, Member{w2, n2}, S, fl);
+            part += block_sum_intersections<NT, SMAX>(rowptr, colidx, w2, n2, Member{w2, n2}, S, fl);
           }
         }
       }
     }
+    task_flush(acc, part, slots, v0, out, tid & 63);
     __syncthreads();
     if (tlog && tid == 0) tlog[t] = (u64)wall_clock64() - t_start;
   }
@@ -570,7 +581,8 @@ hipError_t run_vlut(const char *name, const DevCSR &g, const TaskList &tl, int k
   auto kern = k_clique_vlut<SMAX, W, NT>;
   int grid = grid_for(kern, NT, 0, l.num_cu, tl.count);
   l.begin(name, tl.count);
-  kern<<<grid, NT, 0, l.stream>>>(g.rowptr, g.colidx, tl.tasks, (int)tl.count, k - 1, k - 1, queue, l.d_out, tlog);
+  kern<<<grid, NT, 0, l.stream>>>(g.rowptr, g.colidx, tl.tasks, (int)tl.count, k - 1, k - 1, queue, l.d_out, tlog,
+                                  l.slots);
   l.end();
   return hipGetLastError();
 }
@@ -583,7 +595,7 @@ hipError_t run_vlut_rank(const char *name, const DevCSR &g, const TaskList &tl, 
   int grid = grid_for(kern, NT, 0, l.num_cu, tl.count);
   l.begin(name, tl.count);
   kern<<<grid, NT, 0, l.stream>>>(g.rowptr, g.colidx, g.rrowptr, tl.tasks, (int)tl.count, k - 1, k - 1, queue,
-                                  l.d_out, l.d_err, tlog);
+                                  l.d_out, l.d_err, tlog, l.slots);
   l.end();
   return hipGetLastError();
 }
@@ -595,7 +607,7 @@ hipError_t run_vlut_rank_hbm(const char *name, const DevCSR &g, const TaskList &
   int grid = std::min(grid_for(kern, 1024, 0, l.num_cu, tl.count), clique_hub_slots(l.num_cu));
   l.begin(name, tl.count);
   kern<<<grid, 1024, 0, l.stream>>>(g.rowptr, g.colidx, g.rrowptr, tl.tasks, (int)tl.count, k - 1, k - 1, slots,
-                                    queue, l.d_out, l.d_err, tlog);
+                                    queue, l.d_out, l.d_err, tlog, l.slots);
   l.end();
   return hipGetLastError();
 }
@@ -608,7 +620,7 @@ hipError_t run_edge(const char *name, const DevCSR &g, const EdgeItems &ei, int 
   int grid = grid_for(kern, NT, 0, l.num_cu, ei.count);
   l.begin(name, ei.count);
   kern<<<grid, NT, 0, l.stream>>>(g.rowptr, g.colidx, (const int2 *)ei.items, (int)ei.count, k, scratch, md, queue,
-                                  l.d_out, tlog);
+                                  l.d_out, tlog, l.slots);
   l.end();
   return hipGetLastError();
 }

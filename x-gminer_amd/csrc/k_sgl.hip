@@ -42,7 +42,8 @@ template <int HCAP, int PCAP, int NT, bool LDS_SET>
 __global__ __launch_bounds__(NT) void k_diamond(const int64_t *__restrict__ rowptr,
                                                 const int32_t *__restrict__ colidx,
                                                 const int4 *__restrict__ items, int ntasks,
-                                                int *__restrict__ queue, u64 *__restrict__ out) {
+                                                int *__restrict__ queue, u64 *__restrict__ out,
+                                                const int32_t *__restrict__ slots) {
   __shared__ DiaStage<HCAP, PCAP> S;
   __shared__ int flagrow[NT / 64][64];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -56,6 +57,7 @@ __global__ __launch_bounds__(NT) void k_diamond(const int64_t *__restrict__ rowp
     if (t >= ntasks) break;
     const int4 it = items[t];
     const int v = it.x;
+    u64 part = 0;
     const int64_t bt = rowptr[v];
     const int dt = (int)(rowptr[v + 1] - bt);
     const int32_t *nt = colidx + bt;
@@ -92,9 +94,10 @@ __global__ __launch_bounds__(NT) void k_diamond(const int64_t *__restrict__ rowp
       __syncthreads();
       for (int q = tid; q < ns; q += NT) {
         const u64 c = (u64)S.cnt[q];
-        acc += c * (c - 1) / 2;  // c = 0 gives 0 in modular arithmetic
+        part += c * (c - 1) / 2;  // c = 0 gives 0 in modular arithmetic
       }
     }
+    task_flush(acc, part, slots, v, out, lane);
     __syncthreads();
   }
   acc = wave_sum_u64(acc);
@@ -126,21 +129,24 @@ __device__ __forceinline__ int hash_bump(int *key, int *val, int bits, int w) {
   }
 }
 
-// Block per task x of the relabelled graph.  DENSE: counters in
-// dense + blockIdx.x * n (HBM), reset after the task; else LDS hash sized
-// for wedge work <= HSLOTS / 2.
+// Block per task x of the relabelled graph.  DENSE: counters in the row
+// dense + blockIdx.x * stride (HBM), indexed by w - lo where lo is the first
+// id of x's instance (inst_lo[slots[x]]; 0 without instances), reset after
+// the task; else LDS hash sized for wedge work <= HSLOTS / 2.
 template <int HSLOTS, int PCAP, int NT, bool DENSE>
 __global__ __launch_bounds__(NT) void k_cycle4_wedge(const int64_t *__restrict__ rowptr,
                                                      const int32_t *__restrict__ colidx,
                                                      const int32_t *__restrict__ tasks, int ntasks,
-                                                     int *__restrict__ dense, int64_t n,
-                                                     int *__restrict__ queue, u64 *__restrict__ out) {
+                                                     int *__restrict__ dense, int64_t stride,
+                                                     const int64_t *__restrict__ inst_lo,
+                                                     int *__restrict__ queue, u64 *__restrict__ out,
+                                                     const int32_t *__restrict__ slots) {
   __shared__ WedgeStage<DENSE ? 1 : HSLOTS, PCAP> S;
   __shared__ int flagrow[NT / 64][64];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   WaveFlags fl;
   fl.init(flagrow[wave], lane);
-  int *cnt = DENSE ? dense + (size_t)blockIdx.x * (size_t)n : nullptr;
+  int *const row = DENSE ? dense + (size_t)blockIdx.x * (size_t)stride : nullptr;
   u64 acc = 0;
   for (;;) {
     if (tid == 0) S.task = atomicAdd(queue, 1);
@@ -148,9 +154,11 @@ __global__ __launch_bounds__(NT) void k_cycle4_wedge(const int64_t *__restrict__
     const int t = S.task;
     if (t >= ntasks) break;
     const int x = tasks[t];
+    u64 part = 0;
     const int64_t bx = rowptr[x];
     const int32_t *nx = colidx + bx;
     const int nlow = lower_bound_dev(nx, (int)(rowptr[x + 1] - bx), x);  // |N(x) ∩ [0, x)|
+    int *cnt = DENSE ? row - (inst_lo ? inst_lo[slots[x]] : 0) : nullptr;
     int hb = 0;
     if (!DENSE) {
       hb = 1;
@@ -172,7 +180,7 @@ __global__ __launch_bounds__(NT) void k_cycle4_wedge(const int64_t *__restrict__
       wave_walk<4>(S.seg, colidx, ns, lo, hi, lane, fl, [&](const WindowLane &r) {
         if (r.valid) {
           const int old = DENSE ? atomicAdd(&cnt[r.w], 1) : hash_bump(S.key, S.val, hb, r.w);
-          acc += (u64)old;
+          part += (u64)old;
         }
       });
       (void)ns;
@@ -197,6 +205,7 @@ __global__ __launch_bounds__(NT) void k_cycle4_wedge(const int64_t *__restrict__
       }
       __threadfence();  // the zeros reach L2 before the next task's atomics
     }
+    task_flush(acc, part, slots, x, out, lane);
     __syncthreads();
   }
   acc = wave_sum_u64(acc);
@@ -205,15 +214,17 @@ __global__ __launch_bounds__(NT) void k_cycle4_wedge(const int64_t *__restrict__
 
 // Hub tasks of the wedge strategy, split into items (x, k0, k1, slot): the
 // slice [k0, k1) of x's lower neighbours.  All items of one hub add into
-// the same HBM counter array hub + slot * n with device-scope atomics, so
-// the returned previous values still sum to sum_w C(c_x(w), 2).  The
-// counter region is zeroed before the launch.
+// the same HBM counter row hub + slot * stride (indexed like the dense rows)
+// with device-scope atomics, so the returned previous values still sum to
+// sum_w C(c_x(w), 2).  The counter region is zeroed before the launch.
 template <int PCAP, int NT>
 __global__ __launch_bounds__(NT) void k_cycle4_hub(const int64_t *__restrict__ rowptr,
                                                    const int32_t *__restrict__ colidx,
                                                    const int4 *__restrict__ items, int nitems,
-                                                   int *__restrict__ hub, int64_t n,
-                                                   int *__restrict__ queue, u64 *__restrict__ out) {
+                                                   int *__restrict__ hub, int64_t stride,
+                                                   const int64_t *__restrict__ inst_lo,
+                                                   int *__restrict__ queue, u64 *__restrict__ out,
+                                                   const int32_t *__restrict__ slots) {
   __shared__ Segs<PCAP, false> seg;
   __shared__ int flagrow[NT / 64][64];
   __shared__ int sh_item, sh_ns, sh_L;
@@ -228,8 +239,9 @@ __global__ __launch_bounds__(NT) void k_cycle4_hub(const int64_t *__restrict__ r
     if (t >= nitems) break;
     const int4 it = items[t];
     const int x = it.x;
+    u64 part = 0;
     const int32_t *nx = colidx + rowptr[x];
-    int *cnt = hub + (size_t)it.w * (size_t)n;
+    int *cnt = hub + (size_t)it.w * (size_t)stride - (inst_lo ? inst_lo[slots[x]] : 0);
     for (int k0 = it.y; k0 < it.z; k0 += PCAP) {
       const int k1 = min(it.z, k0 + PCAP);
       __syncthreads();
@@ -242,9 +254,10 @@ __global__ __launch_bounds__(NT) void k_cycle4_hub(const int64_t *__restrict__ r
       int lo, hi;
       wave_share(sh_L, wave, NT / 64, lo, hi);
       wave_walk<4>(seg, colidx, sh_ns, lo, hi, lane, fl, [&](const WindowLane &r) {
-        if (r.valid) acc += (u64)atomicAdd(&cnt[r.w], 1);
+        if (r.valid) part += (u64)atomicAdd(&cnt[r.w], 1);
       });
     }
+    task_flush(acc, part, slots, x, out, lane);
     __syncthreads();
   }
   acc = wave_sum_u64(acc);
@@ -262,7 +275,8 @@ struct PairStage {
 __global__ __launch_bounds__(256) void k_cycle4_pair(const int64_t *__restrict__ rowptr,
                                                      const int32_t *__restrict__ colidx,
                                                      const int32_t *__restrict__ tasks, int ntasks,
-                                                     int *__restrict__ queue, u64 *__restrict__ out) {
+                                                     int *__restrict__ queue, u64 *__restrict__ out,
+                                                     const int32_t *__restrict__ slots) {
   __shared__ PairStage S;
   __shared__ int flagrow[4][64];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -275,6 +289,7 @@ __global__ __launch_bounds__(256) void k_cycle4_pair(const int64_t *__restrict__
     const int t = S.task;
     if (t >= ntasks) break;
     const int v0 = tasks[t];
+    u64 part = 0;
     const int64_t b0 = rowptr[v0];
     const int32_t *n0 = colidx + b0;
     const int nlow = lower_bound_dev(n0, (int)(rowptr[v0 + 1] - b0), v0);  // v1, v2 < v0
@@ -307,10 +322,11 @@ __global__ __launch_bounds__(256) void k_cycle4_pair(const int64_t *__restrict__
         int lo, hi;
         wave_share(L, wave, 4, lo, hi);
         wave_walk<4>(S.seg, colidx, ns, lo, hi, lane, fl, [&](const WindowLane &r) {
-          if (r.valid) acc += lds ? hash_contains(S.key, hb, r.w) : sorted_contains(n1, s1, r.w);
+          if (r.valid) part += lds ? hash_contains(S.key, hb, r.w) : sorted_contains(n1, s1, r.w);
         });
       }
     }
+    task_flush(acc, part, slots, v0, out, lane);
     __syncthreads();
   }
   acc = wave_sum_u64(acc);
@@ -332,7 +348,7 @@ hipError_t run_diamond(const char *name, const DevCSR &g, const TaskList &tl, in
   auto k = k_diamond<HCAP, PCAP, NT, LDS_SET>;
   int grid = grid_of(k, NT, l.num_cu, tl.count);
   l.begin(name, tl.count);
-  k<<<grid, NT, 0, l.stream>>>(g.rowptr, g.colidx, (const int4 *)tl.tasks, (int)tl.count, queue, l.d_out);
+  k<<<grid, NT, 0, l.stream>>>(g.rowptr, g.colidx, (const int4 *)tl.tasks, (int)tl.count, queue, l.d_out, l.slots);
   l.end();
   return hipGetLastError();
 }
@@ -343,7 +359,8 @@ hipError_t run_wedge_lds(const char *name, const DevCSR &g, const TaskList &tl, 
   auto k = k_cycle4_wedge<HSLOTS, PCAP, NT, false>;
   int grid = grid_of(k, NT, l.num_cu, tl.count);
   l.begin(name, tl.count);
-  k<<<grid, NT, 0, l.stream>>>(g.rowptr, g.colidx, tl.tasks, (int)tl.count, nullptr, g.n, queue, l.d_out);
+  k<<<grid, NT, 0, l.stream>>>(g.rowptr, g.colidx, tl.tasks, (int)tl.count, nullptr, 0, nullptr, queue, l.d_out,
+                               l.slots);
   l.end();
   return hipGetLastError();
 }
@@ -364,14 +381,14 @@ int cycle4_dense_grid(int num_cu, int64_t tasks) {
 }
 
 hipError_t launch_cycle4_wedge(const DevCSR &g, const TaskList tl[3], int *dense, const TaskList &hub_items,
-                               int *hub, Launch &l, int *d_queues) {
+                               int *hub, const CounterRows &rows, Launch &l, int *d_queues) {
   hipError_t e;
   if (hub_items.count) {
     auto k = k_cycle4_hub<1024, 256>;
     int grid = grid_of(k, 256, l.num_cu, hub_items.count);
     l.begin("cycle4_wedge_hub", hub_items.count);
-    k<<<grid, 256, 0, l.stream>>>(g.rowptr, g.colidx, (const int4 *)hub_items.tasks, (int)hub_items.count, hub, g.n,
-                                  d_queues + 3, l.d_out);
+    k<<<grid, 256, 0, l.stream>>>(g.rowptr, g.colidx, (const int4 *)hub_items.tasks, (int)hub_items.count, hub,
+                                  rows.stride, rows.inst_lo, d_queues + 3, l.d_out, l.slots);
     l.end();
     if ((e = hipGetLastError())) return e;
   }
@@ -379,8 +396,8 @@ hipError_t launch_cycle4_wedge(const DevCSR &g, const TaskList tl[3], int *dense
     int grid = cycle4_dense_grid(l.num_cu, tl[2].count);
     l.begin("cycle4_wedge_hbm", tl[2].count);
     k_cycle4_wedge<1, 1024, 256, true><<<grid, 256, 0, l.stream>>>(g.rowptr, g.colidx, tl[2].tasks,
-                                                                    (int)tl[2].count, dense, g.n, d_queues + 2,
-                                                                    l.d_out);
+                                                                    (int)tl[2].count, dense, rows.stride, rows.inst_lo,
+                                                                    d_queues + 2, l.d_out, l.slots);
     l.end();
     if ((e = hipGetLastError())) return e;
   }
@@ -393,7 +410,7 @@ hipError_t launch_cycle4_pair(const DevCSR &g, const TaskList &tl, Launch &l, in
   if (!tl.count) return hipSuccess;
   int grid = grid_of(k_cycle4_pair, 256, l.num_cu, tl.count);
   l.begin("cycle4_pair", tl.count);
-  k_cycle4_pair<<<grid, 256, 0, l.stream>>>(g.rowptr, g.colidx, tl.tasks, (int)tl.count, d_queues, l.d_out);
+  k_cycle4_pair<<<grid, 256, 0, l.stream>>>(g.rowptr, g.colidx, tl.tasks, (int)tl.count, d_queues, l.d_out, l.slots);
   l.end();
   return hipGetLastError();
 }

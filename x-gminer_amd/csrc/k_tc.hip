@@ -66,7 +66,8 @@ __global__ __launch_bounds__(256) void k_tc_wave(const int64_t *__restrict__ row
                                                  const int64_t *__restrict__ rrowptr,
                                                  const int32_t *__restrict__ rcolidx,
                                                  const int4 *__restrict__ items, int nitems,
-                                                 int *__restrict__ queue, u64 *__restrict__ out) {
+                                                 int *__restrict__ queue, u64 *__restrict__ out,
+                                                 const int32_t *__restrict__ slots) {
   __shared__ Stage<CAP, CAP, STRAT, HSHIFT> stage[4];
   __shared__ int flagrow[4][64];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -100,7 +101,7 @@ __global__ __launch_bounds__(256) void k_tc_wave(const int64_t *__restrict__ row
     wave_walk<UNROLL>(s.seg, colidx, ns, 0, L, lane, fl, [&](const WindowLane &r) {
       if (r.valid) cnt += member<STRAT>(s.key, hbits, nu, du, true, r.w);
     });
-    acc += (u64)cnt;
+    task_flush(acc, (u64)cnt, slots, u, out, lane);
     wave_sync();  // stage is rewritten by the next item
   }
   acc = wave_sum_u64(acc);
@@ -118,7 +119,8 @@ __global__ __launch_bounds__(NT) void k_tc_block(const int64_t *__restrict__ row
                                                  const int64_t *__restrict__ rrowptr,
                                                  const int32_t *__restrict__ rcolidx,
                                                  const int4 *__restrict__ items, int nitems,
-                                                 int *__restrict__ queue, u64 *__restrict__ out) {
+                                                 int *__restrict__ queue, u64 *__restrict__ out,
+                                                 const int32_t *__restrict__ slots) {
   __shared__ Stage<CAP, PCAP, STRAT, HSHIFT> s;
   __shared__ int tmp[PCAP];
   __shared__ int flagrow[NT / 64][64];
@@ -162,7 +164,7 @@ __global__ __launch_bounds__(NT) void k_tc_block(const int64_t *__restrict__ row
         if (r.valid) cnt += member<STRAT>(s.key, hbits, nu, du, lds_member, r.w);
       });
     }
-    acc += (u64)cnt;
+    task_flush(acc, (u64)cnt, slots, u, out, lane);
     __syncthreads();  // sh_task and stage reused by the next claim
   }
   acc = wave_sum_u64(acc);
@@ -190,24 +192,24 @@ hipError_t run_tc(const char *name, K kernel, int nt, const DevCSR &g, const Tas
   int grid = persistent_grid(kernel, l.num_cu, items.count, nt / 64 > 1 && nt == 256 ? 4 : 1, nt);
   l.begin(name, items.count);
   kernel<<<grid, nt, 0, l.stream>>>(g.rowptr, g.colidx, g.rrowptr, g.rcolidx, (const int4 *)items.tasks,
-                                    (int)items.count, queue, l.d_out);
+                                    (int)items.count, queue, l.d_out, l.slots);
   l.end();
   return hipGetLastError();
 }
 
-template <int STRAT>
+template <int STRAT, int HS>
 hipError_t launch_tc_strat(const DevCSR &g, const TaskList tiers[NUM_TIERS], Launch &l, int *d_queues) {
   hipError_t e;
   // heavy tiers first so their long items overlap with nothing idle at the end
-  if ((e = run_tc("tc_block_big", k_tc_block<2048, 512, STRAT, 512, 2>, 512, g, tiers[TIER_BIG], d_queues + TIER_BIG,
+  if ((e = run_tc("tc_block_big", k_tc_block<2048, 512, STRAT, 512, HS>, 512, g, tiers[TIER_BIG], d_queues + TIER_BIG,
                   l)))
     return e;
-  if ((e = run_tc("tc_block_2048", k_tc_block<2048, 512, STRAT, 512, 2>, 512, g, tiers[TIER_B2048],
+  if ((e = run_tc("tc_block_2048", k_tc_block<2048, 512, STRAT, 512, HS>, 512, g, tiers[TIER_B2048],
                   d_queues + TIER_B2048, l)))
     return e;
-  if ((e = run_tc("tc_wave_256", k_tc_wave<256, STRAT, 2>, 256, g, tiers[TIER_W256], d_queues + TIER_W256, l)))
+  if ((e = run_tc("tc_wave_256", k_tc_wave<256, STRAT, HS>, 256, g, tiers[TIER_W256], d_queues + TIER_W256, l)))
     return e;
-  if ((e = run_tc("tc_wave_64", k_tc_wave<64, STRAT, 2>, 256, g, tiers[TIER_W64], d_queues + TIER_W64, l)))
+  if ((e = run_tc("tc_wave_64", k_tc_wave<64, STRAT, HS>, 256, g, tiers[TIER_W64], d_queues + TIER_W64, l)))
     return e;
   return hipSuccess;
 }
@@ -221,10 +223,11 @@ hipError_t launch_zero(void *ptr, size_t n_u64, hipStream_t s) {
   return hipGetLastError();
 }
 
-hipError_t launch_tc(const DevCSR &g, const TaskList tiers[NUM_TIERS], int strategy, Launch &l,
+hipError_t launch_tc(const DevCSR &g, const TaskList tiers[NUM_TIERS], int strategy, int hash_shift, Launch &l,
                      int *d_queues) {
-  if (strategy == STRAT_BSEARCH) return launch_tc_strat<STRAT_BSEARCH>(g, tiers, l, d_queues);
-  return launch_tc_strat<STRAT_HASH>(g, tiers, l, d_queues);
+  if (strategy == STRAT_BSEARCH) return launch_tc_strat<STRAT_BSEARCH, 2>(g, tiers, l, d_queues);
+  if (hash_shift == 1) return launch_tc_strat<STRAT_HASH, 1>(g, tiers, l, d_queues);
+  return launch_tc_strat<STRAT_HASH, 2>(g, tiers, l, d_queues);
 }
 
 }  // namespace xgm

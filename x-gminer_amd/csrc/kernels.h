@@ -38,6 +38,8 @@ struct Launch {
   int num_cu = 256;
   u64 *d_out = nullptr;   // accumulator slot the launch adds into (pre-zeroed)
   u64 *d_err = nullptr;   // error word kernels may OR into (pre-zeroed)
+  // batched instances: per-vertex output slot (d_out[slots[v]]), null = one count
+  const int32_t *slots = nullptr;
   int *d_queue = nullptr; // work counter the launch claims from (pre-zeroed)
   // Bracket one kernel launch with events: begin(name, tasks) / end().
   std::function<void(const char *, int64_t)> begin;
@@ -53,7 +55,7 @@ constexpr int kTcWaveCap[2] = {64, 256};
 // Triangle counting over a degree-oriented DAG with its reverse lists.
 // tiers[] hold int4 items (t, k0, k1, 0) (TaskList.count = items).
 // strategy: 1 = LDS hash, 2 = LDS binary search.  Adds into l.d_out.
-hipError_t launch_tc(const DevCSR &g, const TaskList tiers[NUM_TIERS], int strategy, Launch &l,
+hipError_t launch_tc(const DevCSR &g, const TaskList tiers[NUM_TIERS], int strategy, int hash_shift, Launch &l,
                      int *d_queues /* NUM_TIERS counters */);
 
 // Edge tasks (v0, v1) of a k-clique count, as int32 pairs in HBM.
@@ -91,13 +93,22 @@ int clique_hub_slots(int num_cu);
 // owner-degree tiers (<=64, <=1024, <=8192, larger).
 hipError_t launch_diamond(const DevCSR &g, const TaskList tl[4], Launch &l, int *d_queues);
 
+// HBM counter rows of the 4-cycle wedge kernels: one row of `stride` ints
+// spans the vertex ids of one instance (every instance occupies the
+// contiguous ids [inst_lo[i], inst_lo[i] + n_i), n_i <= stride); inst_lo is
+// null for a single graph (stride = n).
+struct CounterRows {
+  int64_t stride = 0;
+  const int64_t *inst_lo = nullptr;
+};
+
 // Edge-induced 4-cycle, wedge aggregation over the (degree, id)-relabelled
 // CSR; vertex tasks by wedge work (<=512, <=4096, larger); dense holds
-// cycle4_dense_grid() * n ints, all zero on entry (left zero on exit).
+// cycle4_dense_grid() rows, all zero on entry (left zero on exit).
 // hub_items are int4 (x, k0, k1, slot) slices of hub tasks adding into
-// hub + slot * n, zeroed by the caller before the launch.
+// row `slot` of hub, zeroed by the caller before the launch.
 hipError_t launch_cycle4_wedge(const DevCSR &g, const TaskList tl[3], int *dense, const TaskList &hub_items,
-                               int *hub, Launch &l, int *d_queues);
+                               int *hub, const CounterRows &rows, Launch &l, int *d_queues);
 int cycle4_dense_grid(int num_cu, int64_t tasks);
 
 // Edge-induced 4-cycle, bounded pairwise intersections in the reference's

@@ -132,11 +132,24 @@ struct xgm_dgraph {
   std::vector<int64_t> rl_rowptr;  // host copy of the relabelled CSR
   std::vector<int32_t> rl_colidx;
   int *d_hub = nullptr;            // per-hub 4-cycle counters (slots * n ints)
+  int nhubs = 0;
+  // batched instances (disjoint union): instance of every vertex, first id
+  // of every instance (n_inst + 1 bounds), largest instance
+  int n_inst = 1;
+  std::vector<int32_t> h_slot;
+  std::vector<int64_t> inst_base;
+  int64_t max_inst_n = 0;
+  int32_t *d_slot = nullptr;
+  int64_t *d_inst_lo = nullptr;
+  u64 *d_inst_out = nullptr;       // per-instance counts
   size_t hub_ints = 0;
   ~xgm_dgraph() {
     (void)hipFree(d_dense);
     (void)hipFree(d_hub);
     (void)hipFree(d_hub_lut);
+    (void)hipFree(d_slot);
+    (void)hipFree(d_inst_lo);
+    (void)hipFree(d_inst_out);
     (void)hipFree(d_rl_rowptr);
     (void)hipFree(d_rl_colidx);
     for (auto &kv : task_cache) (void)hipFree(kv.second.first);
@@ -285,16 +298,25 @@ void build_undirected_order(const xgm_csr &g, std::vector<int32_t> &order, std::
   std::stable_sort(order.begin(), order.end(), [&](int32_t a, int32_t b) { return work[a] > work[b]; });
 }
 
-// Relabel vertices by ascending (degree, id): the returned CSR lists are
-// sorted in the new ids; wedge work of x = sum over lower neighbours v < x of
-// |N(v) ∩ [0, x)|.  Counts of isomorphism-invariant patterns are unchanged.
-void relabel_by_degree(const xgm_csr &g, std::vector<int64_t> &rp, std::vector<int32_t> &ci,
-                       std::vector<int32_t> &order, std::vector<int64_t> &wwork) {
+// Relabel vertices by ascending (degree, id) inside each id range
+// [bounds[i], bounds[i+1]) (one range per instance of a batch; empty bounds =
+// the whole graph), so every instance keeps its contiguous ids: the returned
+// CSR lists are sorted in the new ids; wedge work of x = sum over lower
+// neighbours v < x of |N(v) ∩ [0, x)|.  Counts of isomorphism-invariant
+// patterns are unchanged.
+void relabel_by_degree(const xgm_csr &g, const std::vector<int64_t> &bounds, std::vector<int64_t> &rp,
+                       std::vector<int32_t> &ci, std::vector<int32_t> &order, std::vector<int64_t> &wwork) {
   const int64_t n = g.n;
-  std::vector<int32_t> byrank(n), rank(n);
+  std::vector<int32_t> rank(n), byrank(n);
   std::iota(byrank.begin(), byrank.end(), 0);
   auto deg = [&](int32_t v) { return g.rowptr[v + 1] - g.rowptr[v]; };
-  std::stable_sort(byrank.begin(), byrank.end(), [&](int32_t a, int32_t b) { return deg(a) < deg(b); });
+  const std::vector<int64_t> whole{0, n};
+  const auto &bd = bounds.empty() ? whole : bounds;
+  parallel_for((int64_t)bd.size() - 1, 1, [&](int64_t b, int64_t e) {
+    for (int64_t i = b; i < e; ++i)
+      std::stable_sort(byrank.begin() + bd[i], byrank.begin() + bd[i + 1],
+                       [&](int32_t a, int32_t c) { return deg(a) < deg(c); });
+  });
   for (int64_t r = 0; r < n; ++r) rank[byrank[r]] = (int32_t)r;
   std::vector<int64_t> cnt(n);
   for (int64_t r = 0; r < n; ++r) cnt[r] = deg(byrank[r]);
@@ -472,7 +494,7 @@ void build_reverse(const std::vector<int64_t> &rp, const std::vector<int32_t> &c
 // positions [k0, k1) of N+(t) ++ N-(t), cut at ~kSlice probes or at the
 // wave tiers' segment capacity.  Tiers by d+(t); items ordered by
 // decreasing work inside a tier.
-void build_tc_items(const xgm_dgraph *g, std::vector<std::vector<int32_t>> &parts) {
+void build_tc_items(const xgm_dgraph *g, int max_small, int max_large, std::vector<std::vector<int32_t>> &parts) {
   const int64_t n = g->n;
   const auto &dp = g->dplus;
   const int64_t kSlice = 1 << 16;
@@ -481,8 +503,8 @@ void build_tc_items(const xgm_dgraph *g, std::vector<std::vector<int32_t>> &part
   for (int64_t t = 0; t < n; ++t) {
     const int d = dp[t];
     if (d < 1) continue;
-    int tier = 0;
-    while (tier < NUM_TIERS - 1 && d > kTierMaxDeg[tier]) ++tier;
+    const int tier = d <= max_small ? TIER_W64 : d <= max_large ? TIER_W256 : d <= kTierMaxDeg[TIER_B2048] ? TIER_B2048
+                                                                                                       : TIER_BIG;
     const int cap = tier < 2 ? kTcWaveCap[tier] : INT32_MAX;
     const int32_t *out = g->h_colidx.data() + g->h_rowptr[t];
     const int32_t *in = g->h_rcolidx.data() + g->h_rrowptr[t];
@@ -593,7 +615,7 @@ int run_clique_sched(xgm_ctx *ctx, xgm_dgraph *gm, const xgm_sched *sc, const De
   }
   CliqueTasks ct = sc->tasks(log);
   ct.hub_slots = gm->d_hub_lut;
-  hipError_t e = launch_clique(dg, ct, sc->k, gm->d_scratch, std::max(1, gm->max_out), l, ctx->d_queues);
+  hipError_t e = launch_clique(dg, ct, sc->k, gm->d_scratch, std::max(1, gm->max_out), l, l.d_queue);
   if (e != hipSuccess) return ctx->hip_fail(e, "clique launch");
   return XGM_OK;
 }
@@ -709,13 +731,12 @@ void xgm_graph_free(xgm_dgraph *g) {
   delete g;
 }
 
-int xgm_count(xgm_ctx *ctx, const xgm_dgraph *g, int pattern, const xgm_opts *opts, uint64_t *out) {
-  if (!ctx || !g || !out) return XGM_EINVAL;
-  if (g->ctx != ctx) return ctx->fail(XGM_EINVAL, "graph belongs to another context");
-  if (opts && opts->flags != 0) return ctx->fail(XGM_EINVAL, "xgm_opts.flags must be 0");
+// Enqueue the kernels of one pattern count on ctx->stream (no host sync,
+// no D2H): task lists, scratch and relabelled copies are built and cached
+// on first use, so a second call enqueues kernels only (capturable).
+static int dispatch_pattern(xgm_ctx *ctx, const xgm_dgraph *g, int pattern, const xgm_opts *opts, Launch &l) {
   const int strategy = opts ? opts->strategy : XGM_STRAT_AUTO;
-  HIP_TRY(ctx, hipSetDevice(ctx->device));
-  ctx->prof = xgm_profile{};
+  if (g->d_slot && l.slots != g->d_slot) return ctx->fail(XGM_EINVAL, "a batch counts with its instance slots");
   DevCSR dg;
   dg.rowptr = g->d_rowptr;
   dg.colidx = g->d_colidx;
@@ -723,9 +744,6 @@ int xgm_count(xgm_ctx *ctx, const xgm_dgraph *g, int pattern, const xgm_opts *op
   dg.nnz = g->nnz;
   dg.rrowptr = g->d_rrowptr;
   dg.rcolidx = g->d_rcolidx;
-  HIP_TRY(ctx, launch_zero(ctx->d_out, kScratchU64, ctx->stream));
-  HIP_TRY(ctx, launch_zero(ctx->d_queues, kScratchQueues / 2, ctx->stream));
-  Launch l = make_launch(ctx, 0);
   hipError_t e = hipSuccess;
   xgm_dgraph *gm = const_cast<xgm_dgraph *>(g);
   std::vector<TaskList> tl;
@@ -736,12 +754,17 @@ int xgm_count(xgm_ctx *ctx, const xgm_dgraph *g, int pattern, const xgm_opts *op
       if (strategy != XGM_STRAT_AUTO && strategy != XGM_STRAT_HASH && strategy != XGM_STRAT_BSEARCH)
         return ctx->fail(XGM_EUNSUPPORTED, "strategy not available for TRIANGLE");
       int s = (strategy == XGM_STRAT_BSEARCH) ? 2 : 1;
+      const int hs = (opts && opts->hash_shift == 1) ? 1 : 2;
+      const int ws = (opts && opts->wave_max_small > 0) ? std::min(opts->wave_max_small, kTcWaveCap[0]) : kTcWaveCap[0];
+      const int wl = (opts && opts->wave_max_large > 0) ? std::max(ws, std::min(opts->wave_max_large, kTcWaveCap[1]))
+                                                        : kTcWaveCap[1];
+      const std::string key = "tc_items_" + std::to_string(ws) + "_" + std::to_string(wl);
       std::vector<std::vector<int32_t>> parts;
-      if (!cached(gm, "tc_items")) build_tc_items(gm, parts);
-      if ((rc = cache_parts(ctx, gm, "tc_items", parts, 4, tl))) return rc;
+      if (!cached(gm, key)) build_tc_items(gm, ws, wl, parts);
+      if ((rc = cache_parts(ctx, gm, key, parts, 4, tl))) return rc;
       TaskList tiers[NUM_TIERS];
       for (int t = 0; t < NUM_TIERS; ++t) tiers[t] = tl[t];
-      e = launch_tc(dg, tiers, s, l, ctx->d_queues);
+      e = launch_tc(dg, tiers, s, hs, l, l.d_queue);
       break;
     }
     case XGM_CLIQUE4:
@@ -801,7 +824,7 @@ int xgm_count(xgm_ctx *ctx, const xgm_dgraph *g, int pattern, const xgm_opts *op
       }
       if ((rc = cache_parts(ctx, gm, "diamond", parts, 4, tl))) return rc;
       TaskList t4[4] = {tl[0], tl[1], tl[2], tl[3]};
-      e = launch_diamond(dg, t4, l, ctx->d_queues);
+      e = launch_diamond(dg, t4, l, l.d_queue);
       break;
     }
     case XGM_CYCLE4_EDGE: {
@@ -819,14 +842,16 @@ int xgm_count(xgm_ctx *ctx, const xgm_dgraph *g, int pattern, const xgm_opts *op
         }
         if ((rc = tasks_split(ctx, gm, "cycle4_pair", ord, [](int32_t) { return (int64_t)0; }, {INT64_MAX}, tl)))
           return rc;
-        e = launch_cycle4_pair(dg, tl[0], l, ctx->d_queues);
+        e = launch_cycle4_pair(dg, tl[0], l, l.d_queue);
         break;
       }
       if (strategy != XGM_STRAT_AUTO && strategy != XGM_STRAT_HASH)
         return ctx->fail(XGM_EUNSUPPORTED, "strategy not available for CYCLE4_EDGE");
       if (!gm->relabelled) {
         xgm_csr hv{g->n, g->nnz, gm->h_rowptr.data(), gm->h_colidx.data(), g->flags};
-        relabel_by_degree(hv, gm->rl_rowptr, gm->rl_colidx, gm->rl_order, gm->rl_work);
+        // instances keep their id ranges, so the vertex -> instance slots
+        // hold for the relabelled ids too
+        relabel_by_degree(hv, gm->inst_base, gm->rl_rowptr, gm->rl_colidx, gm->rl_order, gm->rl_work);
         HIP_TRY(ctx, hipMalloc(&gm->d_rl_rowptr, gm->rl_rowptr.size() * 8));
         HIP_TRY(ctx, hipMalloc(&gm->d_rl_colidx, std::max<size_t>(1, gm->rl_colidx.size()) * 4));
         HIP_TRY(ctx, hipMemcpy(gm->d_rl_rowptr, gm->rl_rowptr.data(), gm->rl_rowptr.size() * 8, hipMemcpyHostToDevice));
@@ -837,9 +862,11 @@ int xgm_count(xgm_ctx *ctx, const xgm_dgraph *g, int pattern, const xgm_opts *op
       // vertex tasks by wedge work: <=512, <=4096, <=2^18 (per-block HBM
       // counters); above that (at most kMaxHubs tasks) hubs split into
       // slices of ~2^16 wedges sharing one counter array per hub
+      // counter rows span one instance (the largest) or the whole graph
+      const int64_t stride = gm->n_inst > 1 ? gm->max_inst_n : g->n;
       const int64_t kHubWork = 1 << 18, kSlice = 1 << 16;
-      const int kMaxHubs = 512;
-      std::vector<std::vector<int32_t>> parts(4), hubp(1);
+      const int kMaxHubs = (int)std::max<int64_t>(1, std::min<int64_t>(512, ((int64_t)1 << 31) / std::max<int64_t>(1, stride)));
+      std::vector<std::vector<int32_t>> parts(3), hubp(1);
       int nhubs = 0;
       if (!cached(gm, "cycle4_wedge")) {
         const auto &rp = gm->rl_rowptr;
@@ -867,16 +894,14 @@ int xgm_count(xgm_ctx *ctx, const xgm_dgraph *g, int pattern, const xgm_opts *op
           }
           parts[w <= 512 ? 0 : w <= 4096 ? 1 : 2].push_back(x);
         }
-        parts[3].push_back(nhubs);  // remembered in the cache
+        gm->nhubs = nhubs;
       }
       std::vector<TaskList> hl;
       if ((rc = cache_parts(ctx, gm, "cycle4_wedge", parts, 1, tl))) return rc;
       if ((rc = cache_parts(ctx, gm, "cycle4_hubs", hubp, 4, hl))) return rc;
       TaskList t3[3] = {tl[0], tl[1], tl[2]};
       if (hl[0].count) {
-        int32_t nh = 0;
-        HIP_TRY(ctx, hipMemcpy(&nh, tl[3].tasks, sizeof(int32_t), hipMemcpyDeviceToHost));
-        size_t need = ((size_t)nh * (size_t)g->n + 1) & ~(size_t)1;
+        size_t need = ((size_t)gm->nhubs * (size_t)stride + 1) & ~(size_t)1;
         if (gm->hub_ints < need) {
           (void)hipFree(gm->d_hub);
           gm->d_hub = nullptr;
@@ -887,7 +912,7 @@ int xgm_count(xgm_ctx *ctx, const xgm_dgraph *g, int pattern, const xgm_opts *op
         HIP_TRY(ctx, launch_zero(gm->d_hub, need / 2, ctx->stream));
       }
       if (t3[2].count) {
-        size_t need = (size_t)cycle4_dense_grid(ctx->num_cu, t3[2].count) * (size_t)g->n;
+        size_t need = (size_t)cycle4_dense_grid(ctx->num_cu, t3[2].count) * (size_t)stride;
         need = (need + 1) & ~(size_t)1;  // whole 64-bit words for launch_zero
         if (gm->dense_ints < need) {
           (void)hipFree(gm->d_dense);
@@ -901,30 +926,24 @@ int xgm_count(xgm_ctx *ctx, const xgm_dgraph *g, int pattern, const xgm_opts *op
       DevCSR rg = dg;
       rg.rowptr = gm->d_rl_rowptr;
       rg.colidx = gm->d_rl_colidx;
-      e = launch_cycle4_wedge(rg, t3, gm->d_dense, hl[0], gm->d_hub, l, ctx->d_queues);
+      CounterRows rows;
+      rows.stride = stride;
+      rows.inst_lo = l.slots ? gm->d_inst_lo : nullptr;
+      e = launch_cycle4_wedge(rg, t3, gm->d_dense, hl[0], gm->d_hub, rows, l, l.d_queue);
       break;
     }
     default:
       return ctx->fail(XGM_EUNSUPPORTED, "pattern not supported");
   }
   if (e != hipSuccess) return ctx->hip_fail(e, "kernel launch");
-  u64 hv[kScratchU64];
-  HIP_TRY(ctx, hipMemcpyAsync(hv, ctx->d_out, sizeof(hv), hipMemcpyDeviceToHost, ctx->stream));
-  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
-  const u64 h = hv[0];
-  if (hv[kScratchU64 - 1])
-    return ctx->fail(XGM_EINVAL, "DAG is not in (degree, id) orientation; use XGM_STRAT_LUT");
-  float total = 0.f;
-  for (int i = 0; i < ctx->prof.n_kernels; ++i) {
-    float ms = 0.f;
-    HIP_TRY(ctx, hipEventElapsedTime(&ms, ctx->ev[2 * i], ctx->ev[2 * i + 1]));
-    ctx->prof.k[i].ms = ms;
-  }
-  if (ctx->prof.n_kernels > 0)
-    HIP_TRY(ctx, hipEventElapsedTime(&total, ctx->ev[0], ctx->ev[2 * ctx->prof.n_kernels - 1]));
-  ctx->prof.total_ms = total;
-  *out = (uint64_t)h;
   return XGM_OK;
+}
+
+static int count_into(xgm_ctx *ctx, const xgm_dgraph *g, int pattern, const xgm_opts *opts, uint64_t *out,
+                      bool per_inst);
+
+int xgm_count(xgm_ctx *ctx, const xgm_dgraph *g, int pattern, const xgm_opts *opts, uint64_t *out) {
+  return count_into(ctx, g, pattern, opts, out, false);
 }
 
 int xgm_count_csr(xgm_ctx *ctx, const xgm_csr *g, int pattern, const xgm_opts *opts, uint64_t *out) {
@@ -939,6 +958,288 @@ int xgm_count_csr(xgm_ctx *ctx, const xgm_csr *g, int pattern, const xgm_opts *o
 int xgm_last_profile(const xgm_ctx *ctx, xgm_profile *out) {
   if (!ctx || !out) return XGM_EINVAL;
   *out = ctx->prof;
+  return XGM_OK;
+}
+
+// ------------------------------------------------------------ batches
+
+int xgm_graph_upload_batch(xgm_ctx *ctx, const xgm_csr *graphs, int n_inst, xgm_dgraph **out) {
+  if (!ctx || !graphs || !out || n_inst < 1) return XGM_EINVAL;
+  *out = nullptr;
+  const uint32_t flags = graphs[0].flags;
+  int64_t n = 0, nnz = 0, max_n = 0;
+  for (int i = 0; i < n_inst; ++i) {
+    if (graphs[i].flags != flags) return ctx->fail(XGM_EINVAL, "instances must share their CSR flags");
+    if (xgm_csr_validate(&graphs[i]) != XGM_OK)
+      return ctx->fail(XGM_EINVAL, "instance " + std::to_string(i) + " failed CSR validation");
+    n += graphs[i].n;
+    nnz += graphs[i].nnz;
+    max_n = std::max(max_n, graphs[i].n);
+  }
+  if (n >= (int64_t)INT32_MAX || nnz >= (int64_t)INT32_MAX)
+    return ctx->fail(XGM_EUNSUPPORTED, "batch with >= 2^31 vertices or edges");
+  // disjoint union: instance i's ids shift by the vertices before it, which
+  // keeps every list sorted and every degree (so the orientation) unchanged
+  std::vector<int64_t> rp, base(n_inst + 1, 0);
+  std::vector<int32_t> ci, slot;
+  try {
+    rp.resize(n + 1);
+    ci.resize(nnz);
+    slot.resize(n);
+  } catch (const std::bad_alloc &) {
+    return XGM_ENOMEM;
+  }
+  int64_t vo = 0, eo = 0;
+  rp[0] = 0;
+  for (int i = 0; i < n_inst; ++i) {
+    const xgm_csr &g = graphs[i];
+    base[i] = vo;
+    for (int64_t v = 0; v < g.n; ++v) {
+      rp[vo + v + 1] = eo + g.rowptr[v + 1];
+      slot[vo + v] = i;
+    }
+    const int32_t off = (int32_t)vo;
+    for (int64_t k = 0; k < g.nnz; ++k) ci[eo + k] = g.colidx[k] + off;
+    vo += g.n;
+    eo += g.nnz;
+  }
+  base[n_inst] = vo;
+  xgm_csr u{n, nnz, rp.data(), ci.data(), flags};
+  xgm_dgraph *d = nullptr;
+  int rc = xgm_graph_upload(ctx, &u, &d);
+  if (rc) return rc;
+  std::unique_ptr<xgm_dgraph> dg(d);
+  dg->n_inst = n_inst;
+  dg->h_slot = std::move(slot);
+  dg->inst_base = std::move(base);
+  dg->max_inst_n = max_n;
+  HIP_TRY(ctx, hipMalloc(&dg->d_slot, std::max<int64_t>(1, n) * sizeof(int32_t)));
+  HIP_TRY(ctx, hipMalloc(&dg->d_inst_lo, (n_inst + 1) * sizeof(int64_t)));
+  HIP_TRY(ctx, hipMalloc(&dg->d_inst_out, ((n_inst + 1) & ~1) * sizeof(u64) + sizeof(u64)));
+  if (n) HIP_TRY(ctx, hipMemcpy(dg->d_slot, dg->h_slot.data(), n * sizeof(int32_t), hipMemcpyHostToDevice));
+  HIP_TRY(ctx, hipMemcpy(dg->d_inst_lo, dg->inst_base.data(), (n_inst + 1) * sizeof(int64_t), hipMemcpyHostToDevice));
+  *out = dg.release();
+  return XGM_OK;
+}
+
+int xgm_graph_instances(const xgm_dgraph *g) { return g ? g->n_inst : 0; }
+
+}  // extern "C"
+
+// Zero ctx scratch + the per-instance outputs, dispatch, copy back.  A
+// batch always counts per instance (its 4-cycle counter rows are indexed
+// within an instance); per_inst = false returns the sum.
+static int count_into(xgm_ctx *ctx, const xgm_dgraph *g, int pattern, const xgm_opts *opts, uint64_t *out,
+                      bool per_inst) {
+  if (!ctx || !g || !out) return XGM_EINVAL;
+  if (g->ctx != ctx) return ctx->fail(XGM_EINVAL, "graph belongs to another context");
+  if (opts && opts->flags != 0) return ctx->fail(XGM_EINVAL, "xgm_opts.flags must be 0");
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  ctx->prof = xgm_profile{};
+  HIP_TRY(ctx, launch_zero(ctx->d_out, kScratchU64, ctx->stream));
+  HIP_TRY(ctx, launch_zero(ctx->d_queues, kScratchQueues / 2, ctx->stream));
+  Launch l = make_launch(ctx, 0);
+  const int ni = g->d_slot ? g->n_inst : 1;
+  if (g->d_slot) {
+    HIP_TRY(ctx, launch_zero(g->d_inst_out, ni, ctx->stream));
+    l.d_out = g->d_inst_out;
+    l.slots = g->d_slot;
+  }
+  int rc = dispatch_pattern(ctx, g, pattern, opts, l);
+  if (rc) return rc;
+  u64 err = 0;
+  std::vector<u64> h(ni);
+  HIP_TRY(ctx, hipMemcpyAsync(h.data(), l.d_out, ni * sizeof(u64), hipMemcpyDeviceToHost, ctx->stream));
+  HIP_TRY(ctx, hipMemcpyAsync(&err, l.d_err, sizeof(u64), hipMemcpyDeviceToHost, ctx->stream));
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  if (err) return ctx->fail(XGM_EINVAL, "DAG is not in (degree, id) orientation; use XGM_STRAT_LUT");
+  float total = 0.f;
+  for (int i = 0; i < ctx->prof.n_kernels; ++i) {
+    float ms = 0.f;
+    HIP_TRY(ctx, hipEventElapsedTime(&ms, ctx->ev[2 * i], ctx->ev[2 * i + 1]));
+    ctx->prof.k[i].ms = ms;
+  }
+  if (ctx->prof.n_kernels > 0)
+    HIP_TRY(ctx, hipEventElapsedTime(&total, ctx->ev[0], ctx->ev[2 * ctx->prof.n_kernels - 1]));
+  ctx->prof.total_ms = total;
+  if (per_inst) std::memcpy(out, h.data(), ni * sizeof(u64));
+  else out[0] = std::accumulate(h.begin(), h.end(), (u64)0);
+  return XGM_OK;
+}
+
+struct xgm_plan {
+  int device = 0;
+  int n_inst = 1;
+  std::vector<int> patterns;
+  u64 *d_out = nullptr;  // n_pat * n_inst counts, then the error word
+  int *d_queues = nullptr;
+  size_t out_words = 0, queue_words = 0;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  hipEvent_t ev[2] = {nullptr, nullptr};
+  ~xgm_plan() {
+    (void)hipSetDevice(device);
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    for (auto e : ev)
+      if (e) (void)hipEventDestroy(e);
+    (void)hipFree(d_out);
+    (void)hipFree(d_queues);
+  }
+};
+
+extern "C" {
+
+int xgm_count_instances(xgm_ctx *ctx, const xgm_dgraph *g, int pattern, const xgm_opts *opts, uint64_t *out) {
+  return count_into(ctx, g, pattern, opts, out, true);
+}
+
+static bool dag_pattern(int p) { return p == XGM_TRIANGLE || p == XGM_CLIQUE4 || p == XGM_CLIQUE5; }
+
+int xgm_plan_create(xgm_ctx *ctx, const xgm_dgraph *dag, const xgm_dgraph *und, const int *patterns, int n_pat,
+                    const xgm_opts *opts, xgm_plan **out) {
+  if (!ctx || !patterns || n_pat < 1 || n_pat > 16 || !out) return XGM_EINVAL;
+  *out = nullptr;
+  int n_inst = -1;
+  for (int i = 0; i < n_pat; ++i) {
+    const xgm_dgraph *g = dag_pattern(patterns[i]) ? dag : und;
+    if (!g) return ctx->fail(XGM_EINVAL, "plan pattern " + std::to_string(patterns[i]) + " has no graph");
+    if (g->ctx != ctx) return ctx->fail(XGM_EINVAL, "graph belongs to another context");
+    if (n_inst >= 0 && g->n_inst != n_inst) return ctx->fail(XGM_EINVAL, "batches differ in instance count");
+    n_inst = g->n_inst;
+    if (opts && opts[i].flags != 0) return ctx->fail(XGM_EINVAL, "xgm_opts.flags must be 0");
+  }
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  // first counts build and upload every task list and scratch buffer, so
+  // the captured dispatch enqueues kernels only
+  std::vector<uint64_t> tmp(n_inst);
+  for (int i = 0; i < n_pat; ++i) {
+    const xgm_dgraph *g = dag_pattern(patterns[i]) ? dag : und;
+    int rc = count_into(ctx, g, patterns[i], opts ? &opts[i] : nullptr, tmp.data(), true);
+    if (rc) return rc;
+  }
+  std::unique_ptr<xgm_plan> p(new (std::nothrow) xgm_plan);
+  if (!p) return XGM_ENOMEM;
+  p->device = ctx->device;
+  p->n_inst = n_inst;
+  p->patterns.assign(patterns, patterns + n_pat);
+  p->out_words = (((size_t)n_pat * n_inst + 1) + 1) & ~(size_t)1;
+  p->queue_words = (size_t)n_pat * kScratchQueues / 2;
+  HIP_TRY(ctx, hipMalloc(&p->d_out, p->out_words * sizeof(u64)));
+  HIP_TRY(ctx, hipMalloc(&p->d_queues, p->queue_words * sizeof(u64)));
+  HIP_TRY(ctx, hipEventCreate(&p->ev[0]));
+  HIP_TRY(ctx, hipEventCreate(&p->ev[1]));
+  HIP_TRY(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+  int rc = XGM_OK;
+  hipError_t e = launch_zero(p->d_out, p->out_words, ctx->stream);
+  if (e == hipSuccess) e = launch_zero(p->d_queues, p->queue_words, ctx->stream);
+  if (e != hipSuccess) rc = ctx->hip_fail(e, "plan zero");
+  for (int i = 0; i < n_pat && rc == XGM_OK; ++i) {
+    const xgm_dgraph *g = dag_pattern(patterns[i]) ? dag : und;
+    Launch l = make_launch(ctx, 0);
+    l.begin = [](const char *, int64_t) {};
+    l.end = []() {};
+    l.d_out = p->d_out + (size_t)i * n_inst;
+    l.d_err = p->d_out + (size_t)n_pat * n_inst;
+    l.d_queue = p->d_queues + (size_t)i * kScratchQueues;
+    l.slots = g->d_slot;
+    rc = dispatch_pattern(ctx, g, patterns[i], opts ? &opts[i] : nullptr, l);
+  }
+  hipGraph_t graph = nullptr;
+  e = hipStreamEndCapture(ctx->stream, &graph);
+  if (rc) {
+    if (graph) (void)hipGraphDestroy(graph);
+    return rc;
+  }
+  if (e != hipSuccess) return ctx->hip_fail(e, "plan capture");
+  p->graph = graph;
+  HIP_TRY(ctx, hipGraphInstantiate(&p->exec, p->graph, nullptr, nullptr, 0));
+  *out = p.release();
+  return XGM_OK;
+}
+
+int xgm_plan_run(xgm_ctx *ctx, xgm_plan *plan, uint64_t *out, float *ms) {
+  if (!ctx || !plan || !out) return XGM_EINVAL;
+  if (plan->device != ctx->device) return ctx->fail(XGM_EINVAL, "plan belongs to another device");
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  const size_t nc = plan->patterns.size() * (size_t)plan->n_inst;
+  std::vector<u64> h(nc + 1);
+  HIP_TRY(ctx, hipEventRecord(plan->ev[0], ctx->stream));
+  HIP_TRY(ctx, hipGraphLaunch(plan->exec, ctx->stream));
+  HIP_TRY(ctx, hipEventRecord(plan->ev[1], ctx->stream));
+  HIP_TRY(ctx, hipMemcpyAsync(h.data(), plan->d_out, (nc + 1) * sizeof(u64), hipMemcpyDeviceToHost, ctx->stream));
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  if (h[nc]) return ctx->fail(XGM_EINVAL, "DAG is not in (degree, id) orientation; use XGM_STRAT_LUT");
+  std::memcpy(out, h.data(), nc * sizeof(u64));
+  if (ms) HIP_TRY(ctx, hipEventElapsedTime(ms, plan->ev[0], plan->ev[1]));
+  return XGM_OK;
+}
+
+void xgm_plan_free(xgm_plan *plan) { delete plan; }
+
+int xgm_autotune(xgm_ctx *ctx, const xgm_dgraph *g, int pattern, int reps, xgm_opts *best, float *best_ms) {
+  if (!ctx || !g || !best || reps < 1) return XGM_EINVAL;
+  std::vector<xgm_opts> cand;
+  auto mk = [](int strat, int lut, int hs, int ws, int wl) {
+    xgm_opts o{};
+    o.strategy = strat;
+    o.lut_max_degree = lut;
+    o.hash_shift = hs;
+    o.wave_max_small = ws;
+    o.wave_max_large = wl;
+    return o;
+  };
+  switch (pattern) {
+    case XGM_TRIANGLE:
+      for (int hs : {2, 1})
+        for (auto t : {std::make_pair(64, 256), std::make_pair(32, 256), std::make_pair(64, 128),
+                       std::make_pair(32, 128), std::make_pair(16, 64)})
+          cand.push_back(mk(XGM_STRAT_HASH, 0, hs, t.first, t.second));
+      cand.push_back(mk(XGM_STRAT_BSEARCH, 0, 0, 0, 0));
+      break;
+    case XGM_CLIQUE4:
+    case XGM_CLIQUE5:
+      cand.push_back(mk(XGM_STRAT_AUTO, 0, 0, 0, 0));
+      cand.push_back(mk(XGM_STRAT_LUT_RANK, 0, 0, 0, 0));
+      cand.push_back(mk(XGM_STRAT_LUT, 1024, 0, 0, 0));
+      cand.push_back(mk(XGM_STRAT_LUT, 256, 0, 0, 0));
+      cand.push_back(mk(XGM_STRAT_ELUT, 0, 0, 0, 0));
+      break;
+    case XGM_DIAMOND_EDGE:
+      cand.push_back(mk(XGM_STRAT_AUTO, 0, 0, 0, 0));
+      break;
+    case XGM_CYCLE4_EDGE:
+      cand.push_back(mk(XGM_STRAT_AUTO, 0, 0, 0, 0));
+      cand.push_back(mk(XGM_STRAT_LIST, 0, 0, 0, 0));
+      break;
+    default:
+      return ctx->fail(XGM_EUNSUPPORTED, "pattern not supported");
+  }
+  const int ni = g->n_inst;
+  std::vector<uint64_t> ref, got(ni);
+  float bms = 0.f;
+  int bi = -1;
+  for (size_t c = 0; c < cand.size(); ++c) {
+    int rc = count_into(ctx, g, pattern, &cand[c], got.data(), true);  // builds the candidate's task lists
+    if (rc == XGM_EINVAL && cand[c].strategy == XGM_STRAT_LUT_RANK) continue;  // foreign orientation
+    if (rc) return rc;
+    if (ref.empty()) ref = got;
+    else if (got != ref)
+      return ctx->fail(XGM_EINVAL, "autotune candidates disagree (candidate " + std::to_string(c) + ")");
+    float t = 0.f;
+    for (int r = 0; r < reps; ++r) {
+      if ((rc = count_into(ctx, g, pattern, &cand[c], got.data(), true))) return rc;
+      const float ms = ctx->prof.total_ms;
+      t = r == 0 ? ms : std::min(t, ms);
+    }
+    if (bi < 0 || t < bms) {
+      bi = (int)c;
+      bms = t;
+    }
+  }
+  if (bi < 0) return ctx->fail(XGM_EINVAL, "no autotune candidate ran");
+  *best = cand[bi];
+  if (best_ms) *best_ms = bms;
   return XGM_OK;
 }
 

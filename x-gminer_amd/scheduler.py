@@ -39,8 +39,8 @@ def collect(dg, pattern=CLIQUE5, codes=range(NCODES), max_rows=None, seed=0):
     F = dg.sched_features(pattern)
     task = F[:, 15] > 0
     dplus = np.diff(dg.host.rowptr)
-    T = np.zeros((dg.host.n, NCODES), np.float32)
-    Mk = np.zeros((dg.host.n, NCODES), np.float32)
+    T = np.zeros((dg.n, NCODES), np.float32)
+    Mk = np.zeros((dg.n, NCODES), np.float32)
     count = None
     for c in codes:
         cv = np.where(task, c, -1).astype(np.int8)
