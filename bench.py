@@ -2,6 +2,10 @@
 (a,b,c,d) = (0.57,0.19,0.19,0.05), seed 2, degree-oriented int32 CSR) on one
 MI355X.  A step is one full count of the pattern over the resident graph.
 
+--workload batch256: the 256-instance batch (x-gminer_amd/workloads.py);
+a step is one replay of the hipGraph plan counting {triangle, 4-clique,
+diamond, 4-cycle} on every instance (1,024 counts) with autotuned options.
+
 Prints one JSON line (rank 0): metric/value/unit, ms_per_step, the roofline
 of the dominant kernel (algorithmic bytes / HIP-event duration vs 8 TB/s),
 and the reference CPU path timed on this host's cores (cpu_baseline).
@@ -29,7 +33,11 @@ WORKLOADS = {
     "rmat23": ("rmat", dict(scale=23, seed=4), "rmat23_s4", "dag"),
     "chunglu": ("cl", dict(n=4000000, avg_degree=20.0, beta=2.1, w_max=50000.0, seed=3), "chunglu4m_s3",
                 "undirected"),
+    "batch256": ("batch", {}, None, "both"),
 }
+BATCH_PATTERNS = ("triangle", "clique4", "diamond", "rectangle")
+DIAMOND_TIERS = [("diamond_wave_64", 1, 64), ("diamond_block_1024", 65, 1024), ("diamond_block_8192", 1025, 8192),
+                 ("diamond_hbm_set", 8193, 1 << 40)]
 # tier bounds of the triangle kernels (out-degree), mirrored from kernels.h
 TC_TIERS = [("tc_wave_64", 1, 64), ("tc_wave_256", 65, 256), ("tc_block_2048", 257, 2048),
             ("tc_block_big", 2049, 1 << 40)]
@@ -56,6 +64,172 @@ def tc_alg_bytes_per_tier(d):
         m = (t_deg >= lo) & (t_deg <= hi)
         out[name] = int(per_e[m].sum())
     return out, int(per_e.sum())
+
+
+def diamond_alg_bytes_per_tier(g):
+    """Algorithmic bytes of edge-induced diamond counting on an undirected
+    CSR: 4*(d(v0) + d(v1)) + 40 per undirected edge, attributed to the kernel
+    tier of its owner (the (degree, id)-larger endpoint)."""
+    deg = np.diff(g.rowptr).astype(np.int64)
+    src = np.repeat(np.arange(g.n, dtype=np.int64), deg)
+    dst = g.colidx.astype(np.int64)
+    keep = src < dst
+    src, dst = src[keep], dst[keep]
+    ds, dd = deg[src], deg[dst]
+    owner_deg = np.maximum(ds, dd)
+    per_e = 4 * (ds + dd) + 40
+    out = {name: int(per_e[(owner_deg >= lo) & (owner_deg <= hi)].sum()) for name, lo, hi in DIAMOND_TIERS}
+    return out, int(per_e.sum())
+
+
+def union(xgm, graphs):
+    """Disjoint union of host graphs (the layout xgm_graph_upload_batch builds)."""
+    offs = np.cumsum([0] + [g.n for g in graphs])
+    eoffs = np.cumsum([0] + [g.nnz for g in graphs])
+    rp = np.concatenate([g.rowptr[:-1] + e for g, e in zip(graphs, eoffs)] + [np.array([eoffs[-1]])])
+    ci = np.concatenate([g.colidx + np.int32(o) for g, o in zip(graphs, offs)])
+    return xgm.Graph.from_arrays(rp.astype(np.int64), ci.astype(np.int32), graphs[0].flags)
+
+
+def cpu_baseline_batch(specs, inst, threads, budget_s=20.0):
+    """Reference GraphMiner OpenMP binaries on a bounded sample of the batch:
+    instances taken alternately from the ER and RMAT halves, each counted for
+    all four patterns, until the counting loops have run budget_s seconds."""
+    import oracle
+    if not all(oracle.ref_available(p) for p in BATCH_PATTERNS):
+        return None
+    er = [i for i, s in enumerate(specs) if s[1] == "er"]
+    rm = [i for i, s in enumerate(specs) if s[1] == "rmat"]
+    order = [x for pair in zip(er, rm) for x in pair]
+    tot_t, tot_e, used = 0.0, 0, []
+    for i in order:
+        g = inst[i]
+        for p in BATCH_PATTERNS:
+            _, t = oracle.ref_count_graph(p, g, threads=threads, timeout=600)
+            tot_t += t
+            tot_e += g.nnz // 2
+        used.append(specs[i][0])
+        log(f"cpu baseline: seed {specs[i][0]} done, {tot_t:.1f}s")
+        if tot_t >= budget_s:
+            break
+    return dict(value=tot_e / tot_t, unit="|E|/s", cores=threads, kind="reference",
+                sample=f"{len(used)} batch instances (seeds {used}), all four patterns each, "
+                       "GraphMiner tc/clique/sgl OpenMP binaries (counting loops only)",
+                seconds=round(tot_t, 3))
+
+
+def bench_batch(args, xgm):
+    from xgminer_amd import workloads
+    t0 = time.time()
+    specs = workloads.batch_specs()
+    inst = workloads.batch_graphs(xgm, specs)
+    dags = [g.orient() for g in inst]
+    msum = sum(g.nnz // 2 for g in inst)
+    log(f"batch: {len(inst)} instances, sum m={msum} ({time.time()-t0:.1f}s)")
+    gpath = os.path.join(ROOT, "tests", "golden", "batch_counts.json")
+    golden = None
+    if os.path.exists(gpath):
+        rows = json.load(open(gpath))["instances"]
+        if len(rows) == len(specs) and all(r["seed"] == s[0] for r, s in zip(rows, specs)):
+            if all(f"{g.hash():016x}" == r["hash"] for g, r in zip(inst, rows)):
+                golden = np.array([[r["counts"][p] for r in rows] for p in BATCH_PATTERNS], np.uint64)
+    ctx = xgm.Context(0)
+    t0 = time.time()
+    dag = ctx.upload_batch(dags)
+    und = ctx.upload_batch(inst)
+    log(f"upload {time.time()-t0:.2f}s")
+    pids = [PATTERN_IDS[p] for p in BATCH_PATTERNS]
+    on = {"triangle": dag, "clique4": dag, "diamond": und, "rectangle": und}
+
+    t0 = time.time()
+    plain = ctx.plan(pids, dag=dag, und=und)
+    plain_counts, _ = plain.run()
+    log(f"heuristic plan ready ({time.time()-t0:.1f}s)")
+    t0 = time.time()
+    tuned = {}
+    for p in BATCH_PATTERNS:
+        best, ms = ctx.autotune(on[p], PATTERN_IDS[p], reps=2)
+        tuned[p] = dict(opts=best, ms=round(ms, 3))
+        log(f"autotune {p}: {best} {ms:.2f} ms")
+    tune_s = time.time() - t0
+    plan = ctx.plan(pids, dag=dag, und=und, opts=[tuned[p]["opts"] for p in BATCH_PATTERNS])
+
+    def timed(pl, steps, warmup):
+        counts = None
+        for _ in range(warmup):
+            counts, _ = pl.run()
+        evs = []
+        t = time.perf_counter()
+        for _ in range(steps):
+            c, ms = pl.run()
+            evs.append(ms)
+            if counts is not None and not np.array_equal(c, counts):
+                raise SystemExit("non-deterministic batch counts")
+            counts = c
+        return counts, (time.perf_counter() - t) * 1e3 / steps, float(np.mean(evs))
+
+    counts, ms_step, ev_ms = timed(plan, args.steps, args.warmup)
+    pcounts, pms_step, pev_ms = timed(plain, args.steps, args.warmup)
+    log(f"tuned plan {ms_step:.3f} ms/step (graph {ev_ms:.3f}); heuristic plan {pms_step:.3f} ms/step")
+    if not np.array_equal(counts, pcounts) or not np.array_equal(counts, plain_counts):
+        raise SystemExit("tuned and heuristic plans disagree")
+    if golden is not None and not np.array_equal(counts, golden):
+        bad = np.argwhere(counts != golden)[:5].tolist()
+        raise SystemExit(f"BATCH COUNT MISMATCH vs golden at {bad}")
+
+    # per-pattern eager counts with the tuned options: kernel durations (HIP
+    # events on the library stream) for the roofline of the dominant kernel
+    kernels = {}
+    for p in BATCH_PATTERNS:
+        ctx.count_instances(on[p], PATTERN_IDS[p], **tuned[p]["opts"])
+        _, ks = ctx.last_profile()
+        for k in ks:
+            kernels[k.name] = kernels.get(k.name, 0.0) + k.ms
+    dom = max(kernels, key=kernels.get)
+    roof = None
+    alg = {}
+    if dom.startswith("tc_"):
+        alg, _ = tc_alg_bytes_per_tier(union(xgm, dags))
+    elif dom.startswith("diamond_"):
+        alg, _ = diamond_alg_bytes_per_tier(union(xgm, inst))
+    if dom in alg:
+        ach = alg[dom] / (kernels[dom] * 1e-3) / 1e9
+        roof = dict(bound="hbm", achieved=round(ach, 1), peak=HBM_PEAK_GBS, unit="GB/s",
+                    frac=round(ach / HBM_PEAK_GBS, 4), traffic=None, kernel=dom,
+                    alg_bytes_per_launch=alg[dom], avg_launch_ms=round(kernels[dom], 4))
+    cpu = None if args.no_cpu else cpu_baseline_batch(specs, inst, args.cpu_threads)
+    out = {
+        "metric": "|E|/s on batch256 (sum of m over instances x 4 patterns per step)",
+        "value": 4 * msum / (ms_step * 1e-3),
+        "unit": "|E|/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int32",
+        "data": "synthetic",
+        "config": {"workload": "256 instances (128 Erdos-Renyi n log-uniform [1e3, 2e5], avg degree U[4, 64]; "
+                               "128 R-MAT scale U{10..18}, edge factor 16; seeds 100..355) x {triangle, "
+                               "4-clique, diamond, 4-cycle}, one hipGraph replay per step",
+                   "instances": len(inst), "sum_undirected_edges": msum, "parallelism": "single GPU"},
+        "counts_total": {p: int(counts[i].sum()) for i, p in enumerate(BATCH_PATTERNS)},
+        "count_matches_golden": True if golden is not None else None,
+        "graph_replay_ms": round(ev_ms, 4),
+        "heuristic_plan_ms_per_step": round(pms_step, 4),
+        "tuned_vs_heuristic": round(pms_step / ms_step, 4),
+        "autotune": tuned,
+        "autotune_s": round(tune_s, 1),
+        "kernels_ms": {k: round(v, 4) for k, v in kernels.items()},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+    for x in (plan, plain):
+        x.close()
+    ctx.close()
 
 
 def cpu_baseline(g, golden_count, threads):
@@ -87,6 +261,8 @@ def main():
 
     from conftest import load_xgm
     xgm = load_xgm()
+    if args.workload == "batch256":
+        return bench_batch(args, xgm)
     t0 = time.time()
     kind, params, gname, on = WORKLOADS[args.workload]
     if kind == "rmat":

@@ -122,7 +122,11 @@ int xgm_plan_create(xgm_ctx *ctx, const xgm_dgraph *dag, const xgm_dgraph *und, 
 int xgm_plan_run(xgm_ctx *ctx, xgm_plan *plan, uint64_t *out, float *ms);
 void xgm_plan_free(xgm_plan *plan);
 /* Autotuner: times the candidate option sets of `pattern` on g (every
- * candidate gives the same count) and writes the fastest into *best. */
+ * candidate gives the same count; a disagreement is an error) and writes the
+ * fastest into *best.  Candidates: triangles -- LDS hash load 1/2 or 1/4 x
+ * five wave/block tier splits, and binary search; cliques -- auto, rank LUT,
+ * id-ordered LUT with threshold 1024 or 256, edge LUT; 4-cycles -- the wedge
+ * aggregation only.  Each candidate runs 1 + reps times. */
 int xgm_autotune(xgm_ctx *ctx, const xgm_dgraph *g, int pattern, int reps, xgm_opts *best, float *best_ms);
 
 /* Timing of the most recent xgm_count on ctx, from HIP events recorded on

@@ -1209,8 +1209,10 @@ int xgm_autotune(xgm_ctx *ctx, const xgm_dgraph *g, int pattern, int reps, xgm_o
       cand.push_back(mk(XGM_STRAT_AUTO, 0, 0, 0, 0));
       break;
     case XGM_CYCLE4_EDGE:
+      // the pair strategy (reference loop order) does one bounded
+      // intersection per pair of lower neighbours, never less work than the
+      // wedge aggregation and O(d^2) intersections on a hub: not a candidate
       cand.push_back(mk(XGM_STRAT_AUTO, 0, 0, 0, 0));
-      cand.push_back(mk(XGM_STRAT_LIST, 0, 0, 0, 0));
       break;
     default:
       return ctx->fail(XGM_EUNSUPPORTED, "pattern not supported");
