@@ -72,6 +72,8 @@ typedef struct xgm_opts {
                             64-slot kernel (1..64); 0 = 64 */
   int wave_max_large;    /* triangles: ... in the 256-slot wave kernel (wave_max_small..256), larger
                             staged lists run one block per item; 0 = 256 */
+  int hub_rows;          /* 4-cycles: HBM counter rows shared by hub tasks in one round (each row
+                            spans the largest instance); 0 = as many as fit in 8 GiB */
 } xgm_opts;
 
 typedef struct xgm_ctx xgm_ctx;

@@ -83,6 +83,9 @@ def test_batch_per_instance_counts(xgm, ctx, mixed):
         assert np.array_equal(got, _col(want, p)), p
         # second call: the HBM counter rows were left zero by the first
         assert np.array_equal(ctx.count_instances(und, pid), _col(want, p)), p + " (repeat)"
+    _say("batch rectangle hub rounds")
+    for rows in (1, 2, 7):
+        assert np.array_equal(ctx.count_instances(und, 5, hub_rows=rows), _col(want, "rectangle")), rows
     _say("batch rectangle pair strategy")
     assert np.array_equal(ctx.count_instances(und, 5, strategy=4), _col(want, "rectangle"))
     _say("batch clique strategies")
@@ -110,7 +113,7 @@ def test_batch_plan_replays(xgm, ctx, mixed):
         assert ctx.count(dag, 1) == int(expect[0].sum())
     plan.close()
     _say("plan with options")
-    opts = [dict(strategy=2), dict(strategy=6), {}, dict(strategy=4)]
+    opts = [dict(strategy=2), dict(strategy=6), {}, dict(hub_rows=3)]
     plan = ctx.plan(pats, dag=dag, und=und, opts=opts)
     for rep in range(2):
         got, _ = plan.run()
@@ -145,6 +148,17 @@ def test_autotune_picks_an_exact_candidate(xgm, ctx, mixed):
         assert np.array_equal(ctx.count_instances(g, pid, **best), _col(want, p)), (p, best)
     dag.close()
     und.close()
+
+
+def test_cycle4_hub_rounds_single_graph(xgm, ctx):
+    """K_{520,520}: the top ranks have wedge work 520 * (x - 520) > 2^18,
+    so about 15 hub tasks; every row cap gives the closed form."""
+    _say("cycle4 hub rounds single graph")
+    dg = ctx.upload(complete_bipartite(xgm, 520, 520))
+    want = math.comb(520, 2) ** 2
+    for rows in (0, 1, 4, 15, 100):
+        assert ctx.count(dg, 5, hub_rows=rows) == want, rows
+    dg.close()
 
 
 def test_tc_tile_options_exact(xgm, orc, ctx):

@@ -380,17 +380,26 @@ int cycle4_dense_grid(int num_cu, int64_t tasks) {
   return grid_of(k_cycle4_wedge<1, 1024, 256, true>, 256, num_cu, tasks, 1);
 }
 
-hipError_t launch_cycle4_wedge(const DevCSR &g, const TaskList tl[3], int *dense, const TaskList &hub_items,
-                               int *hub, const CounterRows &rows, Launch &l, int *d_queues) {
+hipError_t launch_cycle4_wedge(const DevCSR &g, const TaskList tl[3], int *dense, const TaskList *hub_rounds,
+                               int n_rounds, int *hub, size_t hub_round_u64, const CounterRows &rows, Launch &l,
+                               int *d_queues) {
   hipError_t e;
-  if (hub_items.count) {
+  int64_t hub_items = 0;
+  for (int r = 0; r < n_rounds; ++r) hub_items += hub_rounds[r].count;
+  if (hub_items) {
     auto k = k_cycle4_hub<1024, 256>;
-    int grid = grid_of(k, 256, l.num_cu, hub_items.count);
-    l.begin("cycle4_wedge_hub", hub_items.count);
-    k<<<grid, 256, 0, l.stream>>>(g.rowptr, g.colidx, (const int4 *)hub_items.tasks, (int)hub_items.count, hub,
-                                  rows.stride, rows.inst_lo, d_queues + 3, l.d_out, l.slots);
+    l.begin("cycle4_wedge_hub", hub_items);
+    for (int r = 0; r < n_rounds; ++r) {
+      const TaskList &it = hub_rounds[r];
+      if (!it.count) continue;
+      if ((e = launch_zero(hub, hub_round_u64, l.stream))) return e;
+      if ((e = launch_zero(d_queues + 4, 1, l.stream))) return e;
+      int grid = grid_of(k, 256, l.num_cu, it.count);
+      k<<<grid, 256, 0, l.stream>>>(g.rowptr, g.colidx, (const int4 *)it.tasks, (int)it.count, hub, rows.stride,
+                                    rows.inst_lo, d_queues + 4, l.d_out, l.slots);
+      if ((e = hipGetLastError())) return e;
+    }
     l.end();
-    if ((e = hipGetLastError())) return e;
   }
   if (tl[2].count) {
     int grid = cycle4_dense_grid(l.num_cu, tl[2].count);

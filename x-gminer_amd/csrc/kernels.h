@@ -105,10 +105,12 @@ struct CounterRows {
 // Edge-induced 4-cycle, wedge aggregation over the (degree, id)-relabelled
 // CSR; vertex tasks by wedge work (<=512, <=4096, larger); dense holds
 // cycle4_dense_grid() rows, all zero on entry (left zero on exit).
-// hub_items are int4 (x, k0, k1, slot) slices of hub tasks adding into
-// row `slot` of hub, zeroed by the caller before the launch.
-hipError_t launch_cycle4_wedge(const DevCSR &g, const TaskList tl[3], int *dense, const TaskList &hub_items,
-                               int *hub, const CounterRows &rows, Launch &l, int *d_queues);
+// hub_rounds[r] are int4 (x, k0, k1, slot) slices of hub tasks adding into
+// row `slot` of hub; the first hub_round_u64 words of hub are zeroed before
+// every round.  Uses queue counters 0-2 and 4-5 of d_queues.
+hipError_t launch_cycle4_wedge(const DevCSR &g, const TaskList tl[3], int *dense, const TaskList *hub_rounds,
+                               int n_rounds, int *hub, size_t hub_round_u64, const CounterRows &rows, Launch &l,
+                               int *d_queues);
 int cycle4_dense_grid(int num_cu, int64_t tasks);
 
 // Edge-induced 4-cycle, bounded pairwise intersections in the reference's

@@ -131,7 +131,7 @@ struct xgm_dgraph {
   std::vector<int64_t> rl_work;
   std::vector<int64_t> rl_rowptr;  // host copy of the relabelled CSR
   std::vector<int32_t> rl_colidx;
-  int *d_hub = nullptr;            // per-hub 4-cycle counters (slots * n ints)
+  int *d_hub = nullptr;            // 4-cycle hub counter rows (hub_rows * stride ints)
   int nhubs = 0;
   // batched instances (disjoint union): instance of every vertex, first id
   // of every instance (n_inst + 1 bounds), largest instance
@@ -860,48 +860,58 @@ static int dispatch_pattern(xgm_ctx *ctx, const xgm_dgraph *g, int pattern, cons
         gm->relabelled = true;
       }
       // vertex tasks by wedge work: <=512, <=4096, <=2^18 (per-block HBM
-      // counters); above that (at most kMaxHubs tasks) hubs split into
-      // slices of ~2^16 wedges sharing one counter array per hub
+      // counter rows); above that hubs, split into slices of ~2^16 wedges
+      // that share one counter row per hub.  Hubs take the rows in rounds
+      // of at most hub_rows (kHubBudgetInts counters in total), the rows
+      // zeroed before every round.
       // counter rows span one instance (the largest) or the whole graph
       const int64_t stride = gm->n_inst > 1 ? gm->max_inst_n : g->n;
-      const int64_t kHubWork = 1 << 18, kSlice = 1 << 16;
-      const int kMaxHubs = (int)std::max<int64_t>(1, std::min<int64_t>(512, ((int64_t)1 << 31) / std::max<int64_t>(1, stride)));
+      const int64_t kHubWork = 1 << 18, kSlice = 1 << 16, kHubBudgetInts = (int64_t)1 << 31;
+      const int64_t row_cap = std::max<int64_t>(1, kHubBudgetInts / std::max<int64_t>(1, stride));
+      const int64_t rows_max = (opts && opts->hub_rows > 0) ? std::min<int64_t>(opts->hub_rows, row_cap) : row_cap;
+      const std::string hkey = "cycle4_hubs_" + std::to_string(rows_max);
       std::vector<std::vector<int32_t>> parts(3), hubp(1);
-      int nhubs = 0;
-      if (!cached(gm, "cycle4_wedge")) {
+      if (!cached(gm, "cycle4_wedge") || !cached(gm, hkey)) {
         const auto &rp = gm->rl_rowptr;
         const auto &ci = gm->rl_colidx;
+        std::vector<int32_t> hubs;
         for (int32_t x : gm->rl_order) {
           const int64_t w = gm->rl_work[x];
-          if (w > kHubWork && nhubs < kMaxHubs) {
-            const int32_t *nx = ci.data() + rp[x];
-            const int nlow = (int)(std::lower_bound(nx, nx + (rp[x + 1] - rp[x]), x) - nx);
-            int64_t acc = 0;
-            int k0 = 0;
-            for (int k = 0; k < nlow; ++k) {
-              const int32_t v = nx[k];
-              const int32_t *nv = ci.data() + rp[v];
-              acc += std::lower_bound(nv, nv + (rp[v + 1] - rp[v]), x) - nv;
-              if (acc >= kSlice || k + 1 - k0 >= 4096) {
-                hubp[0].insert(hubp[0].end(), {x, k0, k + 1, nhubs});
-                k0 = k + 1;
-                acc = 0;
-              }
-            }
-            if (k0 < nlow) hubp[0].insert(hubp[0].end(), {x, k0, nlow, nhubs});
-            ++nhubs;
-            continue;
-          }
-          parts[w <= 512 ? 0 : w <= 4096 ? 1 : 2].push_back(x);
+          if (w > kHubWork) hubs.push_back(x);
+          else parts[w <= 512 ? 0 : w <= 4096 ? 1 : 2].push_back(x);
         }
-        gm->nhubs = nhubs;
+        const int64_t R = std::min<int64_t>((int64_t)hubs.size(), rows_max);
+        if (R > 0) hubp.assign((hubs.size() + R - 1) / R, {});
+        for (size_t h = 0; h < hubs.size(); ++h) {
+          const int32_t x = hubs[h];
+          auto &P = hubp[h / R];
+          const int32_t slot = (int32_t)(h % R);
+          const int32_t *nx = ci.data() + rp[x];
+          const int nlow = (int)(std::lower_bound(nx, nx + (rp[x + 1] - rp[x]), x) - nx);
+          int64_t acc = 0;
+          int k0 = 0;
+          for (int k = 0; k < nlow; ++k) {
+            const int32_t v = nx[k];
+            const int32_t *nv = ci.data() + rp[v];
+            acc += std::lower_bound(nv, nv + (rp[v + 1] - rp[v]), x) - nv;
+            if (acc >= kSlice || k + 1 - k0 >= 4096) {
+              P.insert(P.end(), {x, k0, k + 1, slot});
+              k0 = k + 1;
+              acc = 0;
+            }
+          }
+          if (k0 < nlow) P.insert(P.end(), {x, k0, nlow, slot});
+        }
+        gm->nhubs = (int)hubs.size();
       }
       std::vector<TaskList> hl;
       if ((rc = cache_parts(ctx, gm, "cycle4_wedge", parts, 1, tl))) return rc;
-      if ((rc = cache_parts(ctx, gm, "cycle4_hubs", hubp, 4, hl))) return rc;
+      if ((rc = cache_parts(ctx, gm, hkey, hubp, 4, hl))) return rc;
       TaskList t3[3] = {tl[0], tl[1], tl[2]};
-      if (hl[0].count) {
-        size_t need = ((size_t)gm->nhubs * (size_t)stride + 1) & ~(size_t)1;
+      const int64_t hub_rows = std::min<int64_t>(gm->nhubs, rows_max);
+      const size_t hub_round_u64 = ((size_t)hub_rows * (size_t)stride + 1) / 2;
+      if (gm->nhubs) {
+        const size_t need = 2 * hub_round_u64;
         if (gm->hub_ints < need) {
           (void)hipFree(gm->d_hub);
           gm->d_hub = nullptr;
@@ -909,7 +919,6 @@ static int dispatch_pattern(xgm_ctx *ctx, const xgm_dgraph *g, int pattern, cons
           HIP_TRY(ctx, hipMalloc(&gm->d_hub, need * sizeof(int)));
           gm->hub_ints = need;
         }
-        HIP_TRY(ctx, launch_zero(gm->d_hub, need / 2, ctx->stream));
       }
       if (t3[2].count) {
         size_t need = (size_t)cycle4_dense_grid(ctx->num_cu, t3[2].count) * (size_t)stride;
@@ -929,7 +938,8 @@ static int dispatch_pattern(xgm_ctx *ctx, const xgm_dgraph *g, int pattern, cons
       CounterRows rows;
       rows.stride = stride;
       rows.inst_lo = l.slots ? gm->d_inst_lo : nullptr;
-      e = launch_cycle4_wedge(rg, t3, gm->d_dense, hl[0], gm->d_hub, rows, l, l.d_queue);
+      e = launch_cycle4_wedge(rg, t3, gm->d_dense, hl.data(), (int)hl.size(), gm->d_hub, hub_round_u64, rows, l,
+                              l.d_queue);
       break;
     }
     default:
