@@ -104,6 +104,27 @@ int xgm_count(xgm_ctx *ctx, const xgm_dgraph *g, int pattern, const xgm_opts *op
 int xgm_count_csr(xgm_ctx *ctx, const xgm_csr *g, int pattern, const xgm_opts *opts,
                   uint64_t *out);
 
+/* ---- set operations on neighbour lists --------------------------------
+ * For every pair i, A = N(a[i]) and B = N(b[i]) (the lists of g), both
+ * restricted to elements < bounds[i] (bounds may be NULL: no bound):
+ *   XGM_SETOP_INTERSECT  |A ∩ B|          intersect_num_bs_cache (+ upper bound),
+ *                                        xgminer/include/glumin/set_intersect.cuh:440-465, 596-637
+ *   XGM_SETOP_DIFF       |A \ B|          GLUMIN difference_num_bs_cache (no owner
+ *                                        exclusion), set_difference.cuh:20-35, 110-129
+ *   XGM_SETOP_DIFF_EXCL  |A \ B \ {b[i]}| GraphMiner VertexSet::difference_num,
+ *                                        codes/GraphMiner/src/common/VertexSet.cc:69-88
+ * counts[i] receives the size.  With values != NULL the sets are also
+ * written in increasing order (intersect_bs_cache, set_intersect.cuh:99-132;
+ * difference_set_bs_cache, set_difference.cuh:218-241): offsets (n_pairs + 1,
+ * caller-allocated) receives the exclusive scan of counts and
+ * values[offsets[i] .. offsets[i+1]) set i.  If offsets[n_pairs] > values_cap
+ * values is not written and XGM_EINVAL is returned (offsets still filled).
+ * a, b, bounds, counts, offsets and values are host arrays. */
+enum { XGM_SETOP_INTERSECT = 1, XGM_SETOP_DIFF = 2, XGM_SETOP_DIFF_EXCL = 3 };
+int xgm_setop_pairs(xgm_ctx *ctx, const xgm_dgraph *g, int op, const int32_t *a, const int32_t *b,
+                    const int32_t *bounds, int64_t n_pairs, uint32_t *counts, int64_t *offsets, int32_t *values,
+                    int64_t values_cap);
+
 /* ---- batches of instances (disjoint union) ---------------------------
  * Upload n_inst graphs of one kind (same flags) as one device graph; every
  * count on it can return one uint64 per instance.  Replaces running the
@@ -112,6 +133,11 @@ int xgm_graph_upload_batch(xgm_ctx *ctx, const xgm_csr *graphs, int n_inst, xgm_
 int xgm_graph_instances(const xgm_dgraph *g);
 /* out[i] = count of `pattern` in instance i (n_inst entries). */
 int xgm_count_instances(xgm_ctx *ctx, const xgm_dgraph *g, int pattern, const xgm_opts *opts, uint64_t *out);
+/* One call for many (graph, pattern) instances: gs[i] is an undirected
+ * sorted CSR (XGM_CSR_SORTED | XGM_CSR_SYMMETRIC), patterns[i] its pattern;
+ * triangle and clique instances are oriented by the library.  Instances of
+ * one pattern are counted as one batch.  out[i] = count of instance i. */
+int xgm_count_batch(xgm_ctx *ctx, const xgm_csr *gs, const int *patterns, int n_inst, uint64_t *out);
 
 /* A plan captures the counts of several patterns -- triangles / cliques on a
  * DAG batch, diamonds / 4-cycles on an undirected batch (either may be NULL

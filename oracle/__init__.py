@@ -119,6 +119,17 @@ def count(pattern: str, rowptr, colidx, v_begin=0, v_end=-1, threads=0):
     return int(c), int(b.value)
 
 
+def cycle4_wedge(rowptr, colidx, threads=0):
+    """Edge-induced 4-cycles of an undirected CSR by wedge aggregation (a
+    second CPU algorithm for large golden counts; pinned against
+    count("rectangle") in tests/test_oracle.py)."""
+    rp, prp = _p64(rowptr); ci, pci = _p32(colidx)
+    fn = lib().orc_cycle4_wedge
+    fn.restype = C.c_uint64
+    fn.argtypes = [C.c_int64, C.POINTER(C.c_int64), C.POINTER(C.c_int32), C.c_int]
+    return int(fn(len(rp) - 1, prp, pci, threads))
+
+
 # ---- the reference's own binaries (oracle/_ref) ----
 _REF_CMD = {
     "triangle": ("tc_omp_base", [], r"total_num_triangles = (\d+)"),

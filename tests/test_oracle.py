@@ -173,6 +173,30 @@ def test_golden_reference_counts(xgm, orc):
         assert _counts(orc, g) == gold[name]["counts"], name
 
 
+def test_cycle4_wedge_matches_restatement(xgm, orc):
+    """The wedge-aggregation 4-cycle counter (large golden counts) equals the
+    rectangle.h restatement, the reference's golden counts, brute force and
+    closed forms."""
+    from golden.make_golden import SMALL, make_graph
+    gold = json.load(open(os.path.join(GOLDEN, "counts.json")))
+    for name in ("rmat10_s7", "rmat12_s3", "er2000_d16_s5", "chunglu20k_s9"):
+        g = make_graph(xgm, *SMALL[name][:2])
+        assert orc.cycle4_wedge(g.rowptr, g.colidx, 4) == gold[name]["counts"]["rectangle"], name
+    for seed in (11, 12, 13):
+        g = random_graph(xgm, 16, 60, seed)
+        assert orc.cycle4_wedge(g.rowptr, g.colidx) == _brute(g)["rectangle"]
+    for seed in (41, 42):
+        g = random_graph(xgm, 2000, 30000, seed)
+        assert orc.cycle4_wedge(g.rowptr, g.colidx) == orc.count("rectangle", g.rowptr, g.colidx)[0]
+    for a, b in ((2, 2), (3, 5), (40, 70)):
+        g = complete_bipartite(xgm, a, b)
+        assert orc.cycle4_wedge(g.rowptr, g.colidx) == math.comb(a, 2) * math.comb(b, 2)
+    for n in (4, 9, 30):
+        g = complete_graph(xgm, n)
+        assert orc.cycle4_wedge(g.rowptr, g.colidx) == 3 * math.comb(n, 4)
+    assert orc.cycle4_wedge(xgm.load_csr(CITESEER).rowptr, xgm.load_csr(CITESEER).colidx) == 6059
+
+
 def test_algorithmic_bytes_tc_closed_form(xgm, orc):
     """B_TC = 4*sum_v d+(v)*(d+(v)+d-(v)) + 40*m on the DAG."""
     g = random_graph(xgm, 500, 5000, seed=5)

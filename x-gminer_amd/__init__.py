@@ -35,6 +35,7 @@ PATTERN_NAMES = {TRIANGLE: "triangle", CLIQUE4: "clique4", CLIQUE5: "clique5",
                  DIAMOND_EDGE: "diamond", CYCLE4_EDGE: "rectangle"}
 # strategies
 STRAT_AUTO, STRAT_HASH, STRAT_BSEARCH, STRAT_LUT, STRAT_LIST, STRAT_ELUT, STRAT_LUT_RANK = 0, 1, 2, 3, 4, 5, 6
+SETOP_INTERSECT, SETOP_DIFF, SETOP_DIFF_EXCL = 1, 2, 3
 
 
 class XGMError(RuntimeError):
@@ -110,6 +111,9 @@ def _load():
         "xgm_plan_run": (C.c_int, [VP, VP, P(U64), P(C.c_float)]),
         "xgm_plan_free": (None, [VP]),
         "xgm_autotune": (C.c_int, [VP, VP, C.c_int, C.c_int, P(_Opts), P(C.c_float)]),
+        "xgm_count_batch": (C.c_int, [VP, P(_Csr), P(C.c_int), C.c_int, P(U64)]),
+        "xgm_setop_pairs": (C.c_int, [VP, VP, C.c_int, P(I32), P(I32), P(I32), I64, P(C.c_uint32), P(I64),
+                                      P(I32), I64]),
         "xgm_mlp_create": (C.c_int, [C.c_int, U64, P(VP)]),
         "xgm_mlp_free": (None, [VP]),
         "xgm_mlp_last_error": (C.c_char_p, [VP]),
@@ -295,6 +299,40 @@ class Context:
         o, ms = _Opts(), C.c_float()
         _check(lib.xgm_autotune(self._h, dg._h, pattern, reps, C.byref(o), C.byref(ms)), self._h)
         return o.as_dict(), float(ms.value)
+
+    def count_batch(self, graphs, patterns) -> np.ndarray:
+        """One count per (undirected graph, pattern) pair (xgm_count_batch)."""
+        arr = (_Csr * len(graphs))(*[g._csr for g in graphs])
+        pats = (C.c_int * len(patterns))(*patterns)
+        out = np.zeros(len(graphs), np.uint64)
+        _check(lib.xgm_count_batch(self._h, arr, pats, len(graphs), out.ctypes.data_as(C.POINTER(C.c_uint64))),
+               self._h)
+        return out
+
+    def setop_pairs(self, dg: "DeviceGraph", op: int, a, b, bounds=None, materialise=False):
+        """Set operation op (SETOP_*) on N(a[i]), N(b[i]) below bounds[i]:
+        counts, or (counts, offsets, values) with materialise=True."""
+        a = np.ascontiguousarray(a, np.int32)
+        b = np.ascontiguousarray(b, np.int32)
+        n = len(a)
+        pi32 = lambda x: x.ctypes.data_as(C.POINTER(C.c_int32))
+        bd = None if bounds is None else np.ascontiguousarray(bounds, np.int32)
+        pbd = pi32(bd) if bd is not None else None
+        counts = np.zeros(n, np.uint32)
+        pc = counts.ctypes.data_as(C.POINTER(C.c_uint32))
+        if not materialise:
+            _check(lib.xgm_setop_pairs(self._h, dg._h, op, pi32(a), pi32(b), pbd, n, pc, None, None, 0), self._h)
+            return counts
+        offsets = np.zeros(n + 1, np.int64)
+        po = offsets.ctypes.data_as(C.POINTER(C.c_int64))
+        probe = np.zeros(1, np.int32)
+        rc = lib.xgm_setop_pairs(self._h, dg._h, op, pi32(a), pi32(b), pbd, n, pc, po, pi32(probe), 0)
+        if rc != XGM_OK and offsets[-1] == 0:   # a failure other than "values_cap too small"
+            _check(rc, self._h)
+        vals = np.zeros(max(1, int(offsets[-1])), np.int32)
+        _check(lib.xgm_setop_pairs(self._h, dg._h, op, pi32(a), pi32(b), pbd, n, pc, po, pi32(vals), len(vals)),
+               self._h)
+        return counts, offsets, vals[:int(offsets[-1])]
 
     def plan(self, patterns, dag=None, und=None, opts=None) -> "Plan":
         return Plan(self, patterns, dag, und, opts)

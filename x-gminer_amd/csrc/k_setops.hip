@@ -1,0 +1,125 @@
+// Batched set operations on the neighbour lists of a resident graph: for
+// every pair (a, b) with A = N(a), B = N(b), truncated below an optional
+// bound, the intersection count, the difference count with or without the
+// owner exclusion, or the ordered result set.
+//
+// Reference semantics: intersect_num_bs_cache (+ upper bound) and
+// intersect_bs_cache (xgminer/include/glumin/set_intersect.cuh:99-132,
+// 440-465, 596-637); difference_num_bs_cache without owner exclusion
+// (set_difference.cuh:20-35, 110-129) and difference_set_bs_cache
+// (:218-241, 308-333); GraphMiner's VertexSet::difference_num, which also
+// drops the owner vid(B) = b (codes/GraphMiner/src/common/VertexSet.cc:69-88).
+//
+// A wave per pair.  The list searched into (L) sits in LDS when it has at
+// most kStage elements, else a 64-sample LDS index narrows a binary search
+// in HBM to |L|/64 elements.  The streamed list is read 64 elements per
+// window; results are written in order with a ballot prefix.
+#include <algorithm>
+
+#include "device_common.h"
+#include "kernels.h"
+
+namespace xgm {
+namespace {
+
+constexpr int kStage = 2048;
+constexpr int kSetopWaves = 4;  // waves per block
+
+struct WaveStage {
+  int list[kStage];
+  int sample[64];
+};
+
+// Membership of key in L (length ll, sorted): LDS copy when staged, else
+// the sample index then HBM.
+__device__ __forceinline__ bool in_list(const WaveStage &S, bool staged, const int32_t *__restrict__ L, int ll,
+                                        int key) {
+  if (ll <= 0) return false;
+  if (staged) {
+    const int i = lower_bound_dev(S.list, ll, key);
+    return i < ll && S.list[i] == key;
+  }
+  // sample[j] = L[j * ll / 64]: the key lies in [L(j), L(j+1)) of the last
+  // sample <= key
+  const int j = upper_bound_dev(S.sample, 64, key) - 1;
+  if (j < 0) return false;
+  const int lo = (int)((int64_t)j * ll / 64);
+  const int hi = j == 63 ? ll : (int)((int64_t)(j + 1) * ll / 64);
+  const int i = lo + lower_bound_dev(L + lo, hi - lo, key);
+  return i < hi && L[i] == key;
+}
+
+__device__ __forceinline__ void stage(WaveStage &S, const int32_t *__restrict__ L, int ll, int lane, bool &staged) {
+  staged = ll <= kStage;
+  if (staged) {
+    for (int i = lane; i < ll; i += 64) S.list[i] = L[i];
+  } else {
+    S.sample[lane] = L[(int64_t)lane * ll / 64];
+  }
+  wave_sync();
+}
+
+__global__ __launch_bounds__(64 * kSetopWaves) void k_setop_pairs(
+    const int64_t *__restrict__ rowptr, const int32_t *__restrict__ colidx, const int32_t *__restrict__ pa,
+    const int32_t *__restrict__ pb, const int32_t *__restrict__ bounds, int64_t npairs, int op,
+    uint32_t *__restrict__ counts, const int64_t *__restrict__ offsets, int32_t *__restrict__ values) {
+  __shared__ WaveStage stages[kSetopWaves];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  WaveStage &S = stages[wave];
+  const int64_t nw = (int64_t)gridDim.x * kSetopWaves;
+  for (int64_t p = (int64_t)blockIdx.x * kSetopWaves + wave; p < npairs; p += nw) {
+    const int a = pa[p], b = pb[p];
+    const int ub = bounds ? bounds[p] : INT32_MAX;
+    const int32_t *A = colidx + rowptr[a];
+    const int32_t *B = colidx + rowptr[b];
+    // truncate both lists below the bound
+    const int la = lower_bound_dev(A, (int)(rowptr[a + 1] - rowptr[a]), ub);
+    const int lb = lower_bound_dev(B, (int)(rowptr[b + 1] - rowptr[b]), ub);
+    const bool intersect = op == SETOP_INTERSECT;
+    // intersections stream the shorter list; differences stream A
+    const bool swap = intersect && la > lb;
+    const int32_t *T = swap ? B : A;   // streamed
+    const int lt = swap ? lb : la;
+    const int32_t *L = swap ? A : B;   // searched
+    const int ll = swap ? la : lb;
+    bool staged = true;
+    wave_sync();  // the previous pair's reads of S are done
+    if (ll > 0) stage(S, L, ll, lane, staged);
+    uint32_t hits = 0;
+    int64_t out = values ? offsets[p] : 0;
+    for (int k0 = 0; k0 < lt; k0 += 64) {
+      const int k = k0 + lane;
+      bool keep = false;
+      if (k < lt) {
+        const int x = T[k];
+        const bool found = in_list(S, staged, L, ll, x);
+        keep = intersect ? found : (!found && !(op == SETOP_DIFF_EXCL && x == b));
+      }
+      const u64 m = __ballot(keep);
+      if (values && keep) values[out + __popcll(m & lanemask_lt(lane))] = T[k];
+      out += __popcll(m);
+      hits += (uint32_t)__popcll(m);
+    }
+    if (lane == 0) counts[p] = hits;
+  }
+}
+
+}  // namespace
+
+hipError_t launch_setop_pairs(const DevCSR &g, int op, const int32_t *a, const int32_t *b, const int32_t *bounds,
+                              int64_t npairs, uint32_t *counts, const int64_t *offsets, int32_t *values, Launch &l) {
+  if (npairs <= 0) return hipSuccess;
+  int per_cu = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_setop_pairs, 64 * kSetopWaves, 0) != hipSuccess ||
+      per_cu < 1)
+    per_cu = 1;
+  const int64_t want = (npairs + kSetopWaves - 1) / kSetopWaves;
+  const int grid = (int)std::max<int64_t>(1, std::min<int64_t>(want, (int64_t)per_cu * l.num_cu));
+  l.begin(values ? "setop_pairs_materialise" : "setop_pairs_count", npairs);
+  k_setop_pairs<<<grid, 64 * kSetopWaves, 0, l.stream>>>(g.rowptr, g.colidx, a, b, bounds, npairs, op, counts,
+                                                         offsets, values);
+  l.end();
+  return hipGetLastError();
+}
+
+}  // namespace xgm

@@ -124,6 +124,13 @@ int mlp_predict_device(xgm_mlp *m, const float *dX, int64_t n, float *dY);
 hipError_t launch_sched_argmin(const DevCSR &g, const float *F, const float *Y, int8_t *codes, hipStream_t s);
 
 // Small utility kernel: zero `n` 64-bit words (graph-capture safe, unlike memset nodes).
+// Batched set operations on neighbour lists (a wave per pair): op is one
+// of SETOP_*; counts[i] always written; values (at offsets[i]) in order when
+// non-null.  Pair ids must be < g.n (checked by the caller).
+enum SetOp { SETOP_INTERSECT = 1, SETOP_DIFF = 2, SETOP_DIFF_EXCL = 3 };
+hipError_t launch_setop_pairs(const DevCSR &g, int op, const int32_t *a, const int32_t *b, const int32_t *bounds,
+                              int64_t npairs, uint32_t *counts, const int64_t *offsets, int32_t *values, Launch &l);
+
 hipError_t launch_zero(void *ptr, size_t n_u64, hipStream_t s);
 
 }  // namespace xgm

@@ -731,6 +731,8 @@ void xgm_graph_free(xgm_dgraph *g) {
   delete g;
 }
 
+static DevCSR dev_view(const xgm_dgraph *g);
+
 // Enqueue the kernels of one pattern count on ctx->stream (no host sync,
 // no D2H): task lists, scratch and relabelled copies are built and cached
 // on first use, so a second call enqueues kernels only (capturable).
@@ -1105,6 +1107,112 @@ int xgm_count_instances(xgm_ctx *ctx, const xgm_dgraph *g, int pattern, const xg
 }
 
 static bool dag_pattern(int p) { return p == XGM_TRIANGLE || p == XGM_CLIQUE4 || p == XGM_CLIQUE5; }
+
+int xgm_count_batch(xgm_ctx *ctx, const xgm_csr *gs, const int *patterns, int n_inst, uint64_t *out) {
+  if (!ctx || !gs || !patterns || !out || n_inst < 1) return XGM_EINVAL;
+  std::map<int, std::vector<int>> by_pat;
+  for (int i = 0; i < n_inst; ++i) {
+    if ((gs[i].flags & (XGM_CSR_SORTED | XGM_CSR_SYMMETRIC)) != (XGM_CSR_SORTED | XGM_CSR_SYMMETRIC))
+      return ctx->fail(XGM_EINVAL, "xgm_count_batch takes undirected sorted CSRs");
+    if (patterns[i] < XGM_TRIANGLE || patterns[i] > XGM_CYCLE4_EDGE)
+      return ctx->fail(XGM_EUNSUPPORTED, "pattern not supported");
+    by_pat[patterns[i]].push_back(i);
+  }
+  for (auto &kv : by_pat) {
+    const int p = kv.first;
+    const std::vector<int> &idx = kv.second;
+    std::vector<xgm_csr> views(idx.size());
+    std::vector<std::unique_ptr<xgm_hgraph, void (*)(xgm_hgraph *)>> owned;
+    for (size_t j = 0; j < idx.size(); ++j) {
+      if (!dag_pattern(p)) {
+        views[j] = gs[idx[j]];
+        continue;
+      }
+      xgm_hgraph *h = nullptr;
+      int rc = xgm_orient_dag(&gs[idx[j]], &h);
+      if (rc) return ctx->fail(rc, "orientation of instance " + std::to_string(idx[j]) + " failed");
+      owned.emplace_back(h, xgm_hgraph_free);
+      xgm_hgraph_view(h, &views[j]);
+    }
+    xgm_dgraph *dg = nullptr;
+    int rc = xgm_graph_upload_batch(ctx, views.data(), (int)views.size(), &dg);
+    if (rc) return rc;
+    std::unique_ptr<xgm_dgraph, void (*)(xgm_dgraph *)> hold(dg, xgm_graph_free);
+    std::vector<uint64_t> tmp(idx.size());
+    if ((rc = count_into(ctx, dg, p, nullptr, tmp.data(), true))) return rc;
+    for (size_t j = 0; j < idx.size(); ++j) out[idx[j]] = tmp[j];
+  }
+  return XGM_OK;
+}
+
+int xgm_setop_pairs(xgm_ctx *ctx, const xgm_dgraph *g, int op, const int32_t *a, const int32_t *b,
+                    const int32_t *bounds, int64_t n_pairs, uint32_t *counts, int64_t *offsets, int32_t *values,
+                    int64_t values_cap) {
+  if (!ctx || !g || n_pairs < 0 || (n_pairs && (!a || !b || !counts))) return XGM_EINVAL;
+  if (g->ctx != ctx) return ctx->fail(XGM_EINVAL, "graph belongs to another context");
+  if (op != XGM_SETOP_INTERSECT && op != XGM_SETOP_DIFF && op != XGM_SETOP_DIFF_EXCL)
+    return ctx->fail(XGM_EINVAL, "unknown set operation");
+  if (values && !offsets) return ctx->fail(XGM_EINVAL, "values need offsets");
+  for (int64_t i = 0; i < n_pairs; ++i)
+    if (a[i] < 0 || a[i] >= g->n || b[i] < 0 || b[i] >= g->n)
+      return ctx->fail(XGM_EINVAL, "pair " + std::to_string(i) + " names a vertex outside the graph");
+  if (n_pairs == 0) {
+    if (offsets) offsets[0] = 0;
+    return XGM_OK;
+  }
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  // device copies of the pair arrays, freed on every return path
+  struct Bufs {
+    std::vector<void *> p;
+    ~Bufs() {
+      for (void *x : p) (void)hipFree(x);
+    }
+  } bufs;
+  auto dev = [&](void **d, size_t bytes) {
+    hipError_t e = hipMalloc(d, std::max<size_t>(bytes, 4));
+    if (e == hipSuccess) bufs.p.push_back(*d);
+    return e;
+  };
+  int32_t *da = nullptr, *db = nullptr, *dbound = nullptr, *dval = nullptr;
+  uint32_t *dcnt = nullptr;
+  int64_t *doff = nullptr;
+  const size_t n = (size_t)n_pairs;
+  HIP_TRY(ctx, dev((void **)&da, n * 4));
+  HIP_TRY(ctx, dev((void **)&db, n * 4));
+  HIP_TRY(ctx, dev((void **)&dcnt, n * 4));
+  HIP_TRY(ctx, hipMemcpyAsync(da, a, n * 4, hipMemcpyHostToDevice, ctx->stream));
+  HIP_TRY(ctx, hipMemcpyAsync(db, b, n * 4, hipMemcpyHostToDevice, ctx->stream));
+  if (bounds) {
+    HIP_TRY(ctx, dev((void **)&dbound, n * 4));
+    HIP_TRY(ctx, hipMemcpyAsync(dbound, bounds, n * 4, hipMemcpyHostToDevice, ctx->stream));
+  }
+  ctx->prof = xgm_profile{};
+  Launch l = make_launch(ctx, 0);
+  const DevCSR dg = dev_view(g);
+  HIP_TRY(ctx, launch_setop_pairs(dg, op, da, db, dbound, n_pairs, dcnt, nullptr, nullptr, l));
+  HIP_TRY(ctx, hipMemcpyAsync(counts, dcnt, n * 4, hipMemcpyDeviceToHost, ctx->stream));
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  if (values) {
+    offsets[0] = 0;
+    for (size_t i = 0; i < n; ++i) offsets[i + 1] = offsets[i] + counts[i];
+    const int64_t total = offsets[n];
+    if (total > values_cap)
+      return ctx->fail(XGM_EINVAL, "values_cap " + std::to_string(values_cap) + " < " + std::to_string(total));
+    HIP_TRY(ctx, dev((void **)&doff, (n + 1) * 8));
+    HIP_TRY(ctx, dev((void **)&dval, (size_t)total * 4));
+    HIP_TRY(ctx, hipMemcpyAsync(doff, offsets, (n + 1) * 8, hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(ctx, launch_setop_pairs(dg, op, da, db, dbound, n_pairs, dcnt, doff, dval, l));
+    if (total) HIP_TRY(ctx, hipMemcpyAsync(values, dval, (size_t)total * 4, hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  }
+  for (int i = 0; i < ctx->prof.n_kernels; ++i) {
+    float ms = 0.f;
+    HIP_TRY(ctx, hipEventElapsedTime(&ms, ctx->ev[2 * i], ctx->ev[2 * i + 1]));
+    ctx->prof.k[i].ms = ms;
+    ctx->prof.total_ms += ms;
+  }
+  return XGM_OK;
+}
 
 int xgm_plan_create(xgm_ctx *ctx, const xgm_dgraph *dag, const xgm_dgraph *und, const int *patterns, int n_pat,
                     const xgm_opts *opts, xgm_plan **out) {
