@@ -25,6 +25,7 @@ sys.path.insert(0, os.path.join(ROOT, "tests"))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+COPY_BYTES = 1 << 30  # copy kernel size for bandwidth and counter calibration
 PATTERN_IDS = {"triangle": 1, "clique4": 2, "clique5": 3, "diamond": 4, "rectangle": 5}
 # workload name -> (generator, params, golden key, graph the patterns run on)
 WORKLOADS = {
@@ -232,6 +233,65 @@ def bench_batch(args, xgm):
     ctx.close()
 
 
+# kernel symbol prefix of each profiled launch name (rocprof sees symbols)
+KERNEL_SYMBOL = {"tc_block_2048": "k_tc_block<", "tc_block_big": "k_tc_block<", "tc_wave_256": "k_tc_wave<256",
+                 "tc_wave_64": "k_tc_wave<64,"}
+
+
+def _pmc_per_dispatch(counter, argv, symbol, timeout=240):
+    """Run `python3 bench.py argv` under rocprofv3 --pmc counter and return
+    the counter's mean value per dispatch of kernels whose name contains
+    `symbol` (KiB for FETCH_SIZE / WRITE_SIZE), or None."""
+    import csv
+    import glob
+    import shutil
+    exe = shutil.which("rocprofv3")
+    if not exe:
+        return None
+    with tempfile.TemporaryDirectory(prefix="xgm_pmc_") as d:
+        cmd = [exe, "--pmc", counter, "--kernel-trace", "--output-format", "csv", "-d", d, "-o", "run", "--",
+               sys.executable, os.path.abspath(__file__)] + argv
+        try:
+            subprocess.run(cmd, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL, timeout=timeout,
+                           cwd=tempfile.gettempdir())
+        except subprocess.TimeoutExpired:
+            log(f"rocprofv3 --pmc {counter} timed out")
+            return None
+        files = glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True)
+        vals = {}
+        for f in files:
+            for row in csv.DictReader(open(f)):
+                if symbol in row.get("Kernel_Name", "") and row.get("Counter_Name") == counter:
+                    key = row.get("Dispatch_Id") or row.get("Correlation_Id") or str(len(vals))
+                    vals[key] = vals.get(key, 0.0) + float(row["Counter_Value"])
+        if not vals:
+            return None
+        return float(np.mean(list(vals.values())))
+
+
+def measure_traffic(args, dom):
+    """HBM bytes per launch of kernel `dom` from two rocprofv3 --pmc runs
+    (FETCH_SIZE, WRITE_SIZE), each scaled by what the same counter reads for
+    the plain copy kernel over a known size (bytes / (counter KiB * 1024))."""
+    sym = KERNEL_SYMBOL.get(dom)
+    if sym is None:
+        return None, None
+    run = ["--workload", args.workload, "--pattern", args.pattern, "--strategy", str(args.strategy),
+           "--steps", "1", "--warmup", "0", "--no-cpu", "--no-traffic"]
+    cal = ["--copy-calibrate"]
+    out = {}
+    for c in ("FETCH_SIZE", "WRITE_SIZE"):
+        log(f"traffic: rocprofv3 --pmc {c}")
+        k = _pmc_per_dispatch(c, run, sym)
+        ref = _pmc_per_dispatch(c, cal, "k_copy_u4")
+        if k is None or ref is None or ref <= 0:
+            return None, None
+        factor = COPY_BYTES / (ref * 1024.0)
+        out[c] = dict(kib_per_dispatch=round(k, 1), copy_kib=round(ref, 1), factor=round(factor, 4),
+                      bytes=k * 1024.0 * factor)
+    return int(out["FETCH_SIZE"]["bytes"] + out["WRITE_SIZE"]["bytes"]), out
+
+
 def cpu_baseline(g, golden_count, threads):
     import oracle
     if not oracle.ref_available("triangle"):
@@ -252,6 +312,9 @@ def main():
     ap.add_argument("--workload", default="rmat22", choices=list(WORKLOADS))
     ap.add_argument("--strategy", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 --pmc traffic runs")
+    ap.add_argument("--copy-calibrate", action="store_true",
+                    help="only run the plain copy kernel over 1 GiB (counter calibration)")
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
     args = ap.parse_args()
     if args.gpus != 1:
@@ -261,6 +324,11 @@ def main():
 
     from conftest import load_xgm
     xgm = load_xgm()
+    if args.copy_calibrate:
+        ctx = xgm.Context(0)
+        print(json.dumps({"copy_bytes": COPY_BYTES, "gbps": ctx.copy_bandwidth(COPY_BYTES, 1)}))
+        ctx.close()
+        return
     if args.workload == "batch256":
         return bench_batch(args, xgm)
     t0 = time.time()
@@ -318,7 +386,7 @@ def main():
         raise SystemExit(f"COUNT MISMATCH vs golden: {count} != {golden}")
 
     roof = None
-    extra = {"upload_s": round(upload_s, 3)}
+    extra = {"upload_s": round(upload_s, 3), "hbm_copy_gbps": round(ctx.copy_bandwidth(COPY_BYTES, 5), 1)}
     avg = {k: float(np.mean(v)) for k, v in ktimes.items()}
     extra["kernels_ms"] = {k: round(v, 4) for k, v in avg.items()}
     if args.pattern == "triangle":
@@ -328,6 +396,10 @@ def main():
         roof = dict(bound="hbm", achieved=round(ach, 1), peak=HBM_PEAK_GBS, unit="GB/s",
                     frac=round(ach / HBM_PEAK_GBS, 4), traffic=None, kernel=dom,
                     alg_bytes_per_launch=per_tier[dom], avg_launch_ms=round(avg[dom], 4))
+        if not args.no_traffic:
+            traffic, detail = measure_traffic(args, dom)
+            roof["traffic"] = traffic
+            extra["traffic_counters"] = detail
         pat_ach = btot / (float(np.mean(totals)) * 1e-3) / 1e9
         extra["pattern_roofline"] = dict(alg_bytes=btot, kernel_ms=round(float(np.mean(totals)), 4),
                                          achieved=round(pat_ach, 1), frac=round(pat_ach / HBM_PEAK_GBS, 4))

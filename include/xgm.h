@@ -157,6 +157,13 @@ void xgm_plan_free(xgm_plan *plan);
  * aggregation only.  Each candidate runs 1 + reps times. */
 int xgm_autotune(xgm_ctx *ctx, const xgm_dgraph *g, int pattern, int reps, xgm_opts *best, float *best_ms);
 
+/* Measured HBM bandwidth: a plain streaming copy kernel (k_copy_u4) over
+ * `bytes` (rounded down to 16 B) timed with HIP events, best of `reps`;
+ * *gbps = 2 * bytes / time (read + write).  No counterpart in the
+ * reference: the reference point for the roofline and for scaling HBM
+ * traffic counters. */
+int xgm_copy_bandwidth(xgm_ctx *ctx, size_t bytes, int reps, float *gbps);
+
 /* Timing of the most recent xgm_count on ctx, from HIP events recorded on
  * the stream the kernels run on. */
 #define XGM_MAX_PROFILE_KERNELS 16

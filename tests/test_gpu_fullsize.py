@@ -1,0 +1,67 @@
+"""Parity at the full benchmark sizes: R-MAT-22 (seed 2) triangles and
+4-cliques against the reference binaries' counts, Chung-Lu 4M (seed 3)
+diamonds and 4-cycles against the oracle's, each graph tied to its count by
+its CSR hash.  5-cliques on R-MAT-22 (no golden count yet) are checked for
+agreement across every strategy, which share no kernel."""
+import json
+import os
+import sys
+
+import pytest
+
+from conftest import GOLDEN
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(*a):
+    print("[case]", *a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def ctx(xgm):
+    c = xgm.Context(0)
+    yield c
+    c.close()
+
+
+def _gold(name):
+    return json.load(open(os.path.join(GOLDEN, "counts.json")))[name]
+
+
+@pytest.mark.timeout(600)
+def test_rmat22_full(xgm, ctx):
+    from xgminer_amd import workloads
+    gold = _gold("rmat22_s2")
+    _say("rmat22 generate")
+    g = workloads.rmat(xgm, 22, 2)
+    assert f"{g.hash():016x}" == gold["hash"]
+    dg = ctx.upload(g.orient())
+    _say("rmat22 triangle")
+    assert ctx.count(dg, 1) == gold["counts"]["triangle"]
+    _say("rmat22 clique4")
+    for strat in (0, 6):
+        assert ctx.count(dg, 2, strat) == gold["counts"]["clique4"], strat
+    _say("rmat22 clique5")
+    k5 = {s: ctx.count(dg, 3, s) for s in (0, 6)}
+    assert len(set(k5.values())) == 1, k5
+    if "clique5" in gold["counts"]:
+        assert k5[0] == gold["counts"]["clique5"]
+    dg.close()
+
+
+@pytest.mark.timeout(600)
+def test_chunglu4m_full(xgm, ctx):
+    from xgminer_amd import workloads
+    gold = _gold("chunglu4m_s3")
+    _say("chunglu4m generate")
+    g = workloads.chung_lu_4m(xgm)
+    assert f"{g.hash():016x}" == gold["hash"]
+    dg = ctx.upload(g)
+    _say("chunglu4m diamond")
+    assert ctx.count(dg, 4) == gold["counts"]["diamond"]
+    _say("chunglu4m rectangle")
+    assert ctx.count(dg, 5) == gold["counts"]["rectangle"]
+    assert ctx.count(dg, 5, hub_rows=64) == gold["counts"]["rectangle"]
+    dg.close()

@@ -112,6 +112,7 @@ def _load():
         "xgm_plan_free": (None, [VP]),
         "xgm_autotune": (C.c_int, [VP, VP, C.c_int, C.c_int, P(_Opts), P(C.c_float)]),
         "xgm_count_batch": (C.c_int, [VP, P(_Csr), P(C.c_int), C.c_int, P(U64)]),
+        "xgm_copy_bandwidth": (C.c_int, [VP, C.c_size_t, C.c_int, P(C.c_float)]),
         "xgm_setop_pairs": (C.c_int, [VP, VP, C.c_int, P(I32), P(I32), P(I32), I64, P(C.c_uint32), P(I64),
                                       P(I32), I64]),
         "xgm_mlp_create": (C.c_int, [C.c_int, U64, P(VP)]),
@@ -299,6 +300,12 @@ class Context:
         o, ms = _Opts(), C.c_float()
         _check(lib.xgm_autotune(self._h, dg._h, pattern, reps, C.byref(o), C.byref(ms)), self._h)
         return o.as_dict(), float(ms.value)
+
+    def copy_bandwidth(self, nbytes: int = 1 << 30, reps: int = 5) -> float:
+        """Measured HBM GB/s of a plain copy kernel (read + write bytes)."""
+        g = C.c_float()
+        _check(lib.xgm_copy_bandwidth(self._h, nbytes, reps, C.byref(g)), self._h)
+        return float(g.value)
 
     def count_batch(self, graphs, patterns) -> np.ndarray:
         """One count per (undirected graph, pattern) pair (xgm_count_batch)."""

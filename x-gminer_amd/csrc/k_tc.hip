@@ -223,6 +223,20 @@ hipError_t launch_zero(void *ptr, size_t n_u64, hipStream_t s) {
   return hipGetLastError();
 }
 
+// Plain streaming copy of n 16-byte words (grid-stride, one load and one
+// store per word): the reference point for measured HBM bandwidth and for
+// scaling the FETCH_SIZE / WRITE_SIZE counters.
+__global__ __launch_bounds__(256) void k_copy_u4(int4 *__restrict__ dst, const int4 *__restrict__ src, size_t n) {
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) dst[i] = src[i];
+}
+
+hipError_t launch_copy(void *dst, const void *src, size_t n16, int num_cu, hipStream_t s) {
+  const int grid = num_cu * 8;
+  k_copy_u4<<<grid, 256, 0, s>>>((int4 *)dst, (const int4 *)src, n16);
+  return hipGetLastError();
+}
+
 hipError_t launch_tc(const DevCSR &g, const TaskList tiers[NUM_TIERS], int strategy, int hash_shift, Launch &l,
                      int *d_queues) {
   if (strategy == STRAT_BSEARCH) return launch_tc_strat<STRAT_BSEARCH, 2>(g, tiers, l, d_queues);

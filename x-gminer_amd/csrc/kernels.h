@@ -132,5 +132,6 @@ hipError_t launch_setop_pairs(const DevCSR &g, int op, const int32_t *a, const i
                               int64_t npairs, uint32_t *counts, const int64_t *offsets, int32_t *values, Launch &l);
 
 hipError_t launch_zero(void *ptr, size_t n_u64, hipStream_t s);
+hipError_t launch_copy(void *dst, const void *src, size_t n16, int num_cu, hipStream_t s);
 
 }  // namespace xgm

@@ -967,6 +967,31 @@ int xgm_count_csr(xgm_ctx *ctx, const xgm_csr *g, int pattern, const xgm_opts *o
   return rc;
 }
 
+int xgm_copy_bandwidth(xgm_ctx *ctx, size_t bytes, int reps, float *gbps) {
+  if (!ctx || !gbps || reps < 1 || bytes < 16) return XGM_EINVAL;
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  const size_t n16 = bytes / 16;
+  void *a = nullptr, *b = nullptr;
+  HIP_TRY(ctx, hipMalloc(&a, n16 * 16));
+  hipError_t e = hipMalloc(&b, n16 * 16);
+  float best = 0.f;
+  if (e == hipSuccess) e = launch_zero(a, n16 * 2, ctx->stream);
+  for (int r = 0; r <= reps && e == hipSuccess; ++r) {  // first launch is a warm-up
+    e = hipEventRecord(ctx->ev[0], ctx->stream);
+    if (e == hipSuccess) e = launch_copy(b, a, n16, ctx->num_cu, ctx->stream);
+    if (e == hipSuccess) e = hipEventRecord(ctx->ev[1], ctx->stream);
+    if (e == hipSuccess) e = hipEventSynchronize(ctx->ev[1]);
+    float ms = 0.f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]);
+    if (e == hipSuccess && r > 0 && (best == 0.f || ms < best)) best = ms;
+  }
+  (void)hipFree(a);
+  (void)hipFree(b);
+  if (e != hipSuccess) return ctx->hip_fail(e, "copy bandwidth");
+  *gbps = (float)(2.0 * (double)(n16 * 16) / (best * 1e-3) / 1e9);
+  return XGM_OK;
+}
+
 int xgm_last_profile(const xgm_ctx *ctx, xgm_profile *out) {
   if (!ctx || !out) return XGM_EINVAL;
   *out = ctx->prof;
