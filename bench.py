@@ -292,6 +292,69 @@ def measure_traffic(args, dom):
     return int(out["FETCH_SIZE"]["bytes"] + out["WRITE_SIZE"]["bytes"]), out
 
 
+def bench_sched(args, xgm, d, m, desc, golden):
+    """Learned vs heuristic clique schedules on one graph: the learned
+    schedule is timed over the K steps; the heuristic (and the all-rank-LUT
+    schedule) over --baseline-steps steps, every count checked equal."""
+    from xgminer_amd import scheduler
+    pid = PATTERN_IDS[args.pattern]
+    ctx = xgm.Context(0)
+    dg = ctx.upload(d)
+    t0 = time.time()
+    model = scheduler.load()
+    learned = scheduler.schedule(dg, model, pid)
+    prep_s = time.time() - t0
+    plans = {"learned": learned, "heuristic": dg.sched_heuristic(pid), "rank_all": dg.sched_heuristic(pid, 6)}
+    res, count = {}, None
+    for name, codes in plans.items():
+        s = xgm.Schedule(dg, pid, codes)
+        steps = args.steps if name == "learned" else args.baseline_steps
+        for i in range(args.warmup if name == "learned" else 0):
+            s.count()
+        times, t = [], time.perf_counter()
+        for i in range(steps):
+            c = s.count()
+            times.append(ctx.last_profile()[0])
+            log(f"{name} step {i}: count={c} kernel ms={times[-1]:.1f}")
+            if count is not None and c != count:
+                raise SystemExit(f"schedules disagree: {name} {c} vs {count}")
+            count = c
+        wall_ms = (time.perf_counter() - t) * 1e3 / steps
+        res[name] = dict(ms_per_step=round(wall_ms, 3), kernel_ms=round(float(np.median(times)), 3),
+                         code_hist=np.bincount(codes[codes >= 0], minlength=5).tolist())
+        s.close()
+    if golden is not None and count != golden:
+        raise SystemExit(f"COUNT MISMATCH vs golden: {count} != {golden}")
+    ms = res["learned"]["ms_per_step"]
+    out = {
+        "metric": f"|E|/s on {args.workload} ({args.pattern}, learned schedule)",
+        "value": m / (ms * 1e-3),
+        "unit": "|E|/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int32",
+        "data": "synthetic",
+        "config": {"workload": f"{args.pattern} count on {desc}, degree-oriented DAG; learned vs heuristic schedule",
+                   "n": d.n, "undirected_edges": m, "parallelism": "single GPU"},
+        "count": count,
+        "count_matches_golden": (golden == count) if golden is not None else None,
+        "speedup_vs_heuristic": round(res["heuristic"]["ms_per_step"] / ms, 3),
+        "speedup_vs_rank_all": round(res["rank_all"]["ms_per_step"] / ms, 3),
+        "schedules": res,
+        "schedule_prep_s": round(prep_s, 3),
+        "roofline": None,
+        "cpu_baseline": None,
+    }
+    print(json.dumps(out))
+    dg.close()
+    ctx.close()
+
+
 def cpu_baseline(g, golden_count, threads):
     import oracle
     if not oracle.ref_available("triangle"):
@@ -313,6 +376,9 @@ def main():
     ap.add_argument("--strategy", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 --pmc traffic runs")
+    ap.add_argument("--sched", default="", choices=["", "compare"],
+                    help="clique patterns: time the learned schedule against the heuristic and rank-LUT ones")
+    ap.add_argument("--baseline-steps", type=int, default=1)
     ap.add_argument("--copy-calibrate", action="store_true",
                     help="only run the plain copy kernel over 1 GiB (counter calibration)")
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
@@ -348,6 +414,9 @@ def main():
     golden = None
     if gname in gold and gold[gname]["hash"] == f"{g.hash():016x}":
         golden = gold[gname]["counts"].get(args.pattern)
+
+    if args.sched == "compare":
+        return bench_sched(args, xgm, d, m, desc, golden)
 
     # xgm_count is synchronous on the library's own stream (it waits for its
     # kernels and the D2H copy of the count), so no extra device barrier is

@@ -221,7 +221,12 @@ int xgm_sched_heuristic(const xgm_dgraph *g, int pattern, int strategy, int lut_
  * overlap, LUT fill density, log estimated sub-DAG edges / triangles /
  * 4-cliques, capacity bin, row words, log in-degree, bias. */
 int xgm_sched_features(xgm_ctx *ctx, const xgm_dgraph *g, int pattern, float *feats);
-/* Learned codes: features, model inference and argmin on the GPU. */
+/* Learned codes: features, model inference and argmin on the GPU (the
+ * model predicts each task's share of its kernel's time under every code),
+ * then a critical-path guard on the host: the lists run as consecutive
+ * kernels of about max(sum of shares, longest task) each, so a vertex task
+ * predicted to be its kernel's tail moves to the edge LUT (single-edge
+ * items) when that lowers the estimate. */
 int xgm_sched_learned(xgm_ctx *ctx, const xgm_dgraph *g, int pattern, xgm_mlp *m, int8_t *codes);
 int xgm_sched_create(xgm_ctx *ctx, const xgm_dgraph *g, int pattern, const int8_t *codes, xgm_sched **out);
 void xgm_sched_free(xgm_sched *s);

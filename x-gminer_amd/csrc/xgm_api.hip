@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <memory>
@@ -1425,6 +1426,57 @@ int xgm_sched_features(xgm_ctx *ctx, const xgm_dgraph *g, int pattern, float *fe
   return XGM_OK;
 }
 
+// List a vertex task runs in under code c (kernel and bin), or -1 for the
+// edge codes, whose items are single DAG edges.  Mirrors build_clique_sched.
+static int vertex_list(int d, int c) {
+  if (c == 2 || c == 3) return -1;
+  if (d > kBinCap[3]) return (c == 1 || c == 4) && d <= kHubLutMax ? 12 : -1;
+  int bin = 0;
+  while (d > kBinCap[bin]) ++bin;
+  if (c == 0) return bin;
+  return 4 + (c == 4 ? std::min(bin + 1, 3) : bin);
+}
+
+// The model predicts each task's share of its kernel's time (log2(1 +
+// 1000 us)); the argmin minimises the sum of shares.  The lists run as
+// consecutive persistent kernels, so a kernel lasts about max(sum of its
+// shares, longest task), a task's wall time being its share x the kernel's
+// resident blocks.  Per vertex-task list, while the longest task is the
+// kernel's tail, moving it to the edge LUT (single-edge items, no tail)
+// is taken when it lowers the estimate: the list's time falls to max(rest
+// of the shares, next-longest task) and the edge kernel gains the task's
+// edge-LUT share.
+static void critical_path_guard(xgm_ctx *ctx, const xgm_dgraph *g, const std::vector<float> &Y, int8_t *codes) {
+  auto share = [&](int64_t v, int c) { return (std::exp2((double)Y[(size_t)v * 5 + c]) - 1.0) / 1000.0; };
+  struct ListTime {
+    double sum = 0.0;
+    std::vector<std::pair<double, int32_t>> walls;  // (wall, vertex)
+  } lists[kClqLists];
+  for (int64_t v = 0; v < g->n; ++v) {
+    const int c = codes[v];
+    if (c < 0 || c > 4) continue;
+    const int li = vertex_list(g->dplus[v], c);
+    if (li < 0) continue;
+    const double sh = share(v, c);
+    lists[li].sum += sh;
+    lists[li].walls.push_back({sh * clique_list_capacity(ctx->num_cu, li), (int32_t)v});
+  }
+  for (auto &L : lists) {
+    std::sort(L.walls.begin(), L.walls.end(), [](const auto &x, const auto &y) { return x.first > y.first; });
+    for (size_t i = 0; i < L.walls.size(); ++i) {
+      const double top = L.walls[i].first;
+      if (top <= L.sum) break;  // no longer the tail
+      const int32_t v = L.walls[i].second;
+      const double next = i + 1 < L.walls.size() ? L.walls[i + 1].first : 0.0;
+      const double rest = L.sum - share(v, codes[v]);
+      const double gain = top - std::max(rest, next) - share(v, 2);
+      if (gain <= 0.0) break;
+      codes[v] = 2;
+      L.sum = rest;
+    }
+  }
+}
+
 int xgm_sched_learned(xgm_ctx *ctx, const xgm_dgraph *g, int pattern, xgm_mlp *m, int8_t *codes) {
   if (!ctx || !g || !m || !codes || !clique_k(pattern) || !(g->flags & XGM_CSR_DAG)) return XGM_EINVAL;
   HIP_TRY(ctx, hipSetDevice(ctx->device));
@@ -1447,10 +1499,14 @@ int xgm_sched_learned(xgm_ctx *ctx, const xgm_dgraph *g, int pattern, xgm_mlp *m
     return done(ctx->hip_fail(e, "features"));
   int rc = mlp_predict_device(m, dF, g->n, dY);
   if (rc) return done(ctx->fail(rc, std::string("model: ") + xgm_mlp_last_error(m)));
+  std::vector<float> Y((size_t)g->n * 5);
   if ((e = launch_sched_argmin(dg, dF, dY, dC, ctx->stream)) != hipSuccess ||
       (e = hipMemcpyAsync(codes, dC, g->n, hipMemcpyDeviceToHost, ctx->stream)) != hipSuccess ||
+      (e = hipMemcpyAsync(Y.data(), dY, Y.size() * sizeof(float), hipMemcpyDeviceToHost, ctx->stream)) !=
+          hipSuccess ||
       (e = hipStreamSynchronize(ctx->stream)) != hipSuccess)
     return done(ctx->hip_fail(e, "argmin"));
+  critical_path_guard(ctx, g, Y, codes);
   return done(XGM_OK);
 }
 
