@@ -125,6 +125,17 @@ int xgm_setop_pairs(xgm_ctx *ctx, const xgm_dgraph *g, int op, const int32_t *a,
                     const int32_t *bounds, int64_t n_pairs, uint32_t *counts, int64_t *offsets, int32_t *values,
                     int64_t values_cap);
 
+/* ---- vertex-induced motif census -------------------------------------
+ * g undirected (or a batch of undirected instances).  k = 3: out[2 * i ..] =
+ * {wedges, triangles}; k = 4: out[6 * i ..] = {3-stars, 4-paths, tailed
+ * triangles, 4-cycles, diamonds, 4-cliques} per instance i, in the order and
+ * with the counts of the reference's formula path (GraphMiner
+ * src/motif/omp_formula.cc:8-52, cpu_kernels/automine_formula.h:1-58): the
+ * per-edge sums of that formula on the GPU, 4-cliques and triangles on a
+ * degree-oriented copy (built on first use, kept with g), and the induced
+ * 4-cycles from the edge-induced count (C4 = I(C4) + I(diamond) + 3 K4). */
+int xgm_motifs(xgm_ctx *ctx, const xgm_dgraph *g, int k, uint64_t *out);
+
 /* ---- batches of instances (disjoint union) ---------------------------
  * Upload n_inst graphs of one kind (same flags) as one device graph; every
  * count on it can return one uint64 per instance.  Replaces running the

@@ -130,6 +130,37 @@ def cycle4_wedge(rowptr, colidx, threads=0):
     return int(fn(len(rp) - 1, prp, pci, threads))
 
 
+MOTIF_NAMES = {3: ("wedges", "triangles"),
+               4: ("3-stars", "4-paths", "tailed-triangles", "4-cycles", "diamonds", "4-cliques")}
+
+
+def motif(rowptr, colidx, k, threads=0):
+    """Vertex-induced k-motif census (k = 3 or 4) of an undirected CSR by the
+    reference's formula path; returns the counts in the reference's order."""
+    rp, prp = _p64(rowptr); ci, pci = _p32(colidx)
+    out = (C.c_uint64 * 6)()
+    fn = lib().orc_motif
+    fn.restype = C.c_int
+    fn.argtypes = [C.c_int64, C.POINTER(C.c_int64), C.POINTER(C.c_int32), C.c_int, C.c_int, C.POINTER(C.c_uint64)]
+    if fn(len(rp) - 1, prp, pci, k, threads, out):
+        raise ValueError("k must be 3 or 4")
+    return [int(out[i]) for i in range(len(MOTIF_NAMES[k]))]
+
+
+def ref_motif_graph(g, k, threads=0, timeout=3600):
+    """The reference's motif_omp_formula binary on an undirected graph."""
+    env = dict(os.environ)
+    if threads > 0:
+        env["OMP_NUM_THREADS"] = str(threads)
+    with tempfile.TemporaryDirectory(prefix="xgm_ref_") as d:
+        prefix = os.path.join(d, "graph")
+        g.save(prefix)
+        out = subprocess.run([os.path.join(REF_DIR, "motif_omp_formula"), prefix, str(k)], capture_output=True,
+                             text=True, timeout=timeout, env=env, check=True).stdout
+    got = dict((int(a), int(b)) for a, b in re.findall(r"pattern (\d+): (\d+)", out))
+    return [got[i] for i in range(len(MOTIF_NAMES[k]))]
+
+
 # ---- the reference's own binaries (oracle/_ref) ----
 _REF_CMD = {
     "triangle": ("tc_omp_base", [], r"total_num_triangles = (\d+)"),

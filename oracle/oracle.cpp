@@ -347,4 +347,108 @@ uint64_t orc_cycle4_wedge(int64_t n, const int64_t *rp, const vid *ci, int threa
   return count;
 }
 
+
+// Vertex-induced k-motif census by the reference's formula path:
+// automine_3motif / automine_4motif (src/motif/cpu_kernels/automine_formula.h:1-58)
+// and the post-processing of MotifSolver (src/motif/omp_formula.cc:37-47).
+// k = 3: out = {wedges, triangles}; k = 4: out = {3-stars, 4-paths,
+// tailed triangles, 4-cycles, diamonds, 4-cliques} (the reference's order).
+// GraphMiner set semantics: difference_set drops the owner vid of B
+// (VertexSet.cc:21-67), bounded / intersection with `up` keep x < up.
+int orc_motif(eid n, const eid *rp, const vid *ci, int k, int threads, u64 *out) {
+  if (k != 3 && k != 4) return 1;
+  G g{n, rp, ci};
+  set_threads(threads);
+  const eid md = max_degree(g);
+  u64 c[6] = {0, 0, 0, 0, 0, 0};
+  u64 c0 = 0, c1 = 0, c2 = 0, c3 = 0, c4 = 0, c5 = 0;
+#pragma omp parallel reduction(+ : c0, c1, c2, c3, c4, c5)
+  {
+    std::vector<vid> W(md), N01(md), D(md);
+#pragma omp for schedule(dynamic, 1)
+    for (eid v0 = 0; v0 < n; ++v0) {
+      const vid *y0 = g.N(v0);
+      const eid d0 = g.d(v0);
+      const eid f0 = below(y0, d0, (vid)v0);  // y0f0 = bounded(y0, v0)
+      if (k == 3) c0 += (u64)d0 * (u64)(d0 - 1);
+      for (eid i1 = 0; i1 < f0; ++i1) {
+        const vid v1 = y0[i1];
+        const vid *y1 = g.N(v1);
+        const eid d1 = g.d(v1);
+        if (k == 3) {
+          c1 += isect_num_bound(y0, d0, y1, d1, v1);
+          continue;
+        }
+        const u64 tri = isect_num(y0, d0, y1, d1);
+        c4 += tri * (tri - 1);
+        const u64 staru = (u64)d0 - tri - 1, starv = (u64)d1 - tri - 1;
+        c2 += tri * (staru + starv);
+        c1 += staru * starv;
+        c0 += staru * (staru - 1);
+        c0 += starv * (starv - 1);
+        // y0f0y1f1 = intersection_set(y0f0, y1, v1)
+        eid nw = 0;
+        for (eid a = 0, b = 0; a < f0 && b < d1;) {
+          const vid x = y0[a], y = y1[b];
+          if (x >= v1 || y >= v1) break;
+          if (x <= y) ++a;
+          if (y <= x) ++b;
+          if (x == y) W[nw++] = x;
+        }
+        // n0f0y1 = difference_set(y1, y0): y1 \ y0, without vid(y0) = v0
+        eid nn = 0;
+        {
+          eid a = 0, b = 0;
+          while (a < d1 && b < d0) {
+            const vid x = y1[a], y = y0[b];
+            if (x <= y) ++a;
+            if (y <= x) ++b;
+            if (x < y && x != (vid)v0) N01[nn++] = x;
+          }
+          for (; a < d1; ++a)
+            if (y1[a] != (vid)v0) N01[nn++] = y1[a];
+        }
+        // y0f0n1f1 = difference_set(y0f0, y1, v1): x < v1, x not in y1, x != v1
+        eid nd = 0;
+        {
+          eid a = 0, b = 0;
+          bool stop = false;
+          while (a < f0 && b < d1) {
+            const vid x = y0[a], y = y1[b];
+            if (x >= v1 || y >= v1) { stop = x >= v1; break; }
+            if (x <= y) ++a;
+            if (y <= x) ++b;
+            if (x < y && x != v1) D[nd++] = x;
+          }
+          if (!stop)
+            for (; a < f0; ++a) {
+              if (y0[a] >= v1) break;
+              if (y0[a] != v1) D[nd++] = y0[a];
+            }
+        }
+        for (eid i2 = 0; i2 < nw; ++i2) {
+          const vid v2 = W[i2];
+          c5 += isect_num_bound(W.data(), nw, g.N(v2), g.d(v2), v2);
+        }
+        for (eid i2 = 0; i2 < nd; ++i2) {
+          const vid v2 = D[i2];
+          c3 += isect_num_bound(N01.data(), nn, g.N(v2), g.d(v2), (vid)v0);
+        }
+      }
+    }
+  }
+  c[0] = c0; c[1] = c1; c[2] = c2; c[3] = c3; c[4] = c4; c[5] = c5;
+  if (k == 3) {
+    out[0] = c[0] / 2 - 3 * c[1];
+    out[1] = c[1];
+  } else {
+    c[4] = c[4] / 2 - c[5] * 6;
+    c[2] = c[2] / 2 - c[4] * 2;
+    c[1] = c[1] - c[3] * 4;
+    c[0] = c[0] / 6 - c[2] / 3;
+    for (int i = 0; i < 6; ++i) out[i] = c[i];
+  }
+  return 0;
+}
+
 }  // extern "C"

@@ -197,6 +197,63 @@ def test_cycle4_wedge_matches_restatement(xgm, orc):
     assert orc.cycle4_wedge(xgm.load_csr(CITESEER).rowptr, xgm.load_csr(CITESEER).colidx) == 6059
 
 
+def _brute_motifs(g):
+    """Vertex-induced 3- and 4-motifs by classifying every vertex subset."""
+    n = g.n
+    adj = [set(g.colidx[g.rowptr[u]:g.rowptr[u + 1]].tolist()) for u in range(n)]
+    m3 = [0, 0]
+    for q in itertools.combinations(range(n), 3):
+        ne = sum(y in adj[x] for x, y in itertools.combinations(q, 2))
+        if ne == 2:
+            m3[0] += 1
+        elif ne == 3:
+            m3[1] += 1
+    m4 = [0] * 6   # 3-star, 4-path, tailed triangle, 4-cycle, diamond, 4-clique
+    for q in itertools.combinations(range(n), 4):
+        deg = sorted(sum(y in adj[x] for y in q if y != x) for x in q)
+        ne = sum(deg) // 2
+        if ne == 3 and deg == [1, 1, 1, 3]:
+            m4[0] += 1
+        elif ne == 3 and deg == [1, 1, 2, 2]:
+            m4[1] += 1
+        elif ne == 4 and deg == [1, 2, 2, 3]:
+            m4[2] += 1
+        elif ne == 4 and deg == [2, 2, 2, 2]:
+            m4[3] += 1
+        elif ne == 5:
+            m4[4] += 1
+        elif ne == 6:
+            m4[5] += 1
+    return m3, m4
+
+
+def test_motif_known_answers_and_bruteforce(xgm, orc):
+    """The reference's citeseer motif counts (src/motif/README.md:53-66) and
+    brute force over every 3- and 4-vertex subset of small graphs."""
+    g = xgm.load_csr(CITESEER)
+    assert orc.motif(g.rowptr, g.colidx, 4) == [222630, 111153, 22900, 3094, 2200, 255]
+    assert orc.motif(g.rowptr, g.colidx, 3) == [23380, 1166]
+    for seed in (11, 12, 13, 14):
+        h = random_graph(xgm, 16, 45 + 5 * seed % 20, seed)
+        m3, m4 = _brute_motifs(h)
+        assert orc.motif(h.rowptr, h.colidx, 3, 2) == m3, seed
+        assert orc.motif(h.rowptr, h.colidx, 4, 2) == m4, seed
+    for a, b in ((3, 4), (5, 5)):
+        h = complete_bipartite(xgm, a, b)
+        # K_{a,b}: 3-stars a*C(b,3) + b*C(a,3), 4-paths 0 (every 4-path closes), 4-cycles C(a,2)C(b,2)
+        assert orc.motif(h.rowptr, h.colidx, 4) == [a * math.comb(b, 3) + b * math.comb(a, 3), 0, 0,
+                                                    math.comb(a, 2) * math.comb(b, 2), 0, 0]
+
+
+@pytest.mark.skipif(not os.path.exists(os.path.join(os.path.dirname(__file__), "..", "oracle", "_ref",
+                                                     "motif_omp_formula")), reason="oracle/_ref not built")
+def test_motif_restatement_matches_reference_binary(xgm, orc):
+    for seed in (31, 32):
+        g = random_graph(xgm, 600, 9000, seed)
+        for k in (3, 4):
+            assert orc.motif(g.rowptr, g.colidx, k) == orc.ref_motif_graph(g, k), (seed, k)
+
+
 def test_algorithmic_bytes_tc_closed_form(xgm, orc):
     """B_TC = 4*sum_v d+(v)*(d+(v)+d-(v)) + 40*m on the DAG."""
     g = random_graph(xgm, 500, 5000, seed=5)

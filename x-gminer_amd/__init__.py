@@ -113,6 +113,7 @@ def _load():
         "xgm_autotune": (C.c_int, [VP, VP, C.c_int, C.c_int, P(_Opts), P(C.c_float)]),
         "xgm_count_batch": (C.c_int, [VP, P(_Csr), P(C.c_int), C.c_int, P(U64)]),
         "xgm_copy_bandwidth": (C.c_int, [VP, C.c_size_t, C.c_int, P(C.c_float)]),
+        "xgm_motifs": (C.c_int, [VP, VP, C.c_int, P(U64)]),
         "xgm_setop_pairs": (C.c_int, [VP, VP, C.c_int, P(I32), P(I32), P(I32), I64, P(C.c_uint32), P(I64),
                                       P(I32), I64]),
         "xgm_mlp_create": (C.c_int, [C.c_int, U64, P(VP)]),
@@ -306,6 +307,15 @@ class Context:
         g = C.c_float()
         _check(lib.xgm_copy_bandwidth(self._h, nbytes, reps, C.byref(g)), self._h)
         return float(g.value)
+
+    def motifs(self, dg: "DeviceGraph", k: int) -> np.ndarray:
+        """Vertex-induced k-motif census (k = 3: wedges, triangles; k = 4:
+        3-stars, 4-paths, tailed triangles, 4-cycles, diamonds, 4-cliques),
+        one row per instance."""
+        per = 2 if k == 3 else 6
+        out = np.zeros((dg.n_inst, per), np.uint64)
+        _check(lib.xgm_motifs(self._h, dg._h, k, out.ctypes.data_as(C.POINTER(C.c_uint64))), self._h)
+        return out
 
     def count_batch(self, graphs, patterns) -> np.ndarray:
         """One count per (undirected graph, pattern) pair (xgm_count_batch)."""

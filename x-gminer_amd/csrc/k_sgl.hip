@@ -38,7 +38,12 @@ struct DiaStage {
 
 // Items (t, k0, k1, -): owner t and the slice [k0, k1) of N(t) whose
 // partners this item probes (hub owners are split over many items).
-template <int HCAP, int PCAP, int NT, bool LDS_SET>
+// MOTIF: instead of sum C(c, 2), the four per-edge sums of the reference's
+// 4-motif formula (src/motif/cpu_kernels/automine_formula.h:27-36) with
+// c = |N(t) ∩ N(p)|, su = d(t) - c - 1, sv = d(p) - c - 1, added into
+// out[0..3] (out[4 * slot ..] with instances): sum su(su-1) + sv(sv-1),
+// sum su*sv, sum c*(su + sv), sum c*(c-1).
+template <int HCAP, int PCAP, int NT, bool LDS_SET, bool MOTIF>
 __global__ __launch_bounds__(NT) void k_diamond(const int64_t *__restrict__ rowptr,
                                                 const int32_t *__restrict__ colidx,
                                                 const int4 *__restrict__ items, int ntasks,
@@ -49,7 +54,7 @@ __global__ __launch_bounds__(NT) void k_diamond(const int64_t *__restrict__ rowp
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   WaveFlags fl;
   fl.init(flagrow[wave], lane);
-  u64 acc = 0;
+  u64 acc = 0, macc[4] = {0, 0, 0, 0};
   for (;;) {
     if (tid == 0) S.task = atomicAdd(queue, 1);
     __syncthreads();
@@ -57,7 +62,7 @@ __global__ __launch_bounds__(NT) void k_diamond(const int64_t *__restrict__ rowp
     if (t >= ntasks) break;
     const int4 it = items[t];
     const int v = it.x;
-    u64 part = 0;
+    u64 part = 0, mpart[4] = {0, 0, 0, 0};
     const int64_t bt = rowptr[v];
     const int dt = (int)(rowptr[v + 1] - bt);
     const int32_t *nt = colidx + bt;
@@ -94,14 +99,44 @@ __global__ __launch_bounds__(NT) void k_diamond(const int64_t *__restrict__ rowp
       __syncthreads();
       for (int q = tid; q < ns; q += NT) {
         const u64 c = (u64)S.cnt[q];
-        part += c * (c - 1) / 2;  // c = 0 gives 0 in modular arithmetic
+        if (MOTIF) {
+          const u64 dp = (u64)(S.seg.os[q + 1].y - S.seg.os[q].y);  // partner list, probed whole
+          const u64 su = (u64)dt - c - 1, sv = dp - c - 1;
+          mpart[0] += su * (su - 1) + sv * (sv - 1);
+          mpart[1] += su * sv;
+          mpart[2] += c * (su + sv);
+          mpart[3] += c * (c - 1);
+        } else {
+          part += c * (c - 1) / 2;  // c = 0 gives 0 in modular arithmetic
+        }
       }
     }
-    task_flush(acc, part, slots, v, out, lane);
+    if (MOTIF) {
+      if (!slots) {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) macc[j] += mpart[j];
+      } else {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          const u64 w = wave_sum_u64(mpart[j]);
+          if (lane == 0 && w) atomicAdd(out + 4 * (size_t)slots[v] + j, w);
+        }
+      }
+    } else {
+      task_flush(acc, part, slots, v, out, lane);
+    }
     __syncthreads();
   }
-  acc = wave_sum_u64(acc);
-  if (lane == 0 && acc) atomicAdd(out, acc);
+  if (MOTIF) {
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const u64 w = wave_sum_u64(macc[j]);
+      if (lane == 0 && w) atomicAdd(out + j, w);
+    }
+  } else {
+    acc = wave_sum_u64(acc);
+    if (lane == 0 && acc) atomicAdd(out, acc);
+  }
 }
 
 // ------------------------------------------------------------ 4-cycle WEDGE
@@ -342,10 +377,10 @@ int grid_of(K kernel, int nt, int num_cu, int64_t tasks, int cap_per_cu = 1 << 2
   return (int)std::max<int64_t>(1, std::min<int64_t>(tasks, (int64_t)per_cu * num_cu));
 }
 
-template <int HCAP, int PCAP, int NT, bool LDS_SET>
+template <int HCAP, int PCAP, int NT, bool LDS_SET, bool MOTIF>
 hipError_t run_diamond(const char *name, const DevCSR &g, const TaskList &tl, int *queue, Launch &l) {
   if (!tl.count) return hipSuccess;
-  auto k = k_diamond<HCAP, PCAP, NT, LDS_SET>;
+  auto k = k_diamond<HCAP, PCAP, NT, LDS_SET, MOTIF>;
   int grid = grid_of(k, NT, l.num_cu, tl.count);
   l.begin(name, tl.count);
   k<<<grid, NT, 0, l.stream>>>(g.rowptr, g.colidx, (const int4 *)tl.tasks, (int)tl.count, queue, l.d_out, l.slots);
@@ -367,13 +402,30 @@ hipError_t run_wedge_lds(const char *name, const DevCSR &g, const TaskList &tl, 
 
 }  // namespace
 
-hipError_t launch_diamond(const DevCSR &g, const TaskList tl[4] /* int4 items */, Launch &l, int *d_queues) {
+template <bool MOTIF>
+hipError_t launch_diamond_mode(const DevCSR &g, const TaskList tl[4], Launch &l, int *d_queues) {
   hipError_t e;
-  if ((e = run_diamond<1, 1024, 256, false>("diamond_hbm_set", g, tl[3], d_queues + 3, l))) return e;
-  if ((e = run_diamond<8192, 1024, 256, true>("diamond_block_8192", g, tl[2], d_queues + 2, l))) return e;
-  if ((e = run_diamond<1024, 1024, 256, true>("diamond_block_1024", g, tl[1], d_queues + 1, l))) return e;
-  if ((e = run_diamond<64, 64, 64, true>("diamond_wave_64", g, tl[0], d_queues + 0, l))) return e;
+  if ((e = run_diamond<1, 1024, 256, false, MOTIF>(MOTIF ? "motif_edges_hbm_set" : "diamond_hbm_set", g, tl[3],
+                                                   d_queues + 3, l)))
+    return e;
+  if ((e = run_diamond<8192, 1024, 256, true, MOTIF>(MOTIF ? "motif_edges_block_8192" : "diamond_block_8192", g,
+                                                     tl[2], d_queues + 2, l)))
+    return e;
+  if ((e = run_diamond<1024, 1024, 256, true, MOTIF>(MOTIF ? "motif_edges_block_1024" : "diamond_block_1024", g,
+                                                     tl[1], d_queues + 1, l)))
+    return e;
+  if ((e = run_diamond<64, 64, 64, true, MOTIF>(MOTIF ? "motif_edges_wave_64" : "diamond_wave_64", g, tl[0],
+                                                d_queues + 0, l)))
+    return e;
   return hipSuccess;
+}
+
+hipError_t launch_diamond(const DevCSR &g, const TaskList tl[4] /* int4 items */, Launch &l, int *d_queues) {
+  return launch_diamond_mode<false>(g, tl, l, d_queues);
+}
+
+hipError_t launch_motif_edges(const DevCSR &g, const TaskList tl[4], Launch &l, int *d_queues) {
+  return launch_diamond_mode<true>(g, tl, l, d_queues);
 }
 
 int cycle4_dense_grid(int num_cu, int64_t tasks) {

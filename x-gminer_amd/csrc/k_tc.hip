@@ -28,6 +28,10 @@ namespace {
 constexpr int STRAT_HASH = 1;
 constexpr int STRAT_BSEARCH = 2;
 constexpr int UNROLL = 4;  // windows mapped before their loads are issued
+// partners per segment-table piece of the block kernel: 480 keeps its LDS
+// (hash 32 KB + table + flags) at 40,596 B, so four 512-thread blocks fit
+// a CU's 160 KB and every SIMD runs 8 waves
+constexpr int kBlockPieces = 480;
 
 // LDS state of one staged task: N+(u) as a hash (load <= 1/4) or sorted
 // copy, and the flattened segment table of the lists N+(v), v in N+(u).
@@ -201,10 +205,10 @@ template <int STRAT, int HS>
 hipError_t launch_tc_strat(const DevCSR &g, const TaskList tiers[NUM_TIERS], Launch &l, int *d_queues) {
   hipError_t e;
   // heavy tiers first so their long items overlap with nothing idle at the end
-  if ((e = run_tc("tc_block_big", k_tc_block<2048, 512, STRAT, 512, HS>, 512, g, tiers[TIER_BIG], d_queues + TIER_BIG,
+  if ((e = run_tc("tc_block_big", k_tc_block<2048, kBlockPieces, STRAT, 512, HS>, 512, g, tiers[TIER_BIG], d_queues + TIER_BIG,
                   l)))
     return e;
-  if ((e = run_tc("tc_block_2048", k_tc_block<2048, 512, STRAT, 512, HS>, 512, g, tiers[TIER_B2048],
+  if ((e = run_tc("tc_block_2048", k_tc_block<2048, kBlockPieces, STRAT, 512, HS>, 512, g, tiers[TIER_B2048],
                   d_queues + TIER_B2048, l)))
     return e;
   if ((e = run_tc("tc_wave_256", k_tc_wave<256, STRAT, HS>, 256, g, tiers[TIER_W256], d_queues + TIER_W256, l)))

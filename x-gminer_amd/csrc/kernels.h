@@ -92,6 +92,9 @@ int clique_hub_slots(int num_cu);
 // k0, k1, 0) (TaskList.tasks points at 4*count ints, count = items) in
 // owner-degree tiers (<=64, <=1024, <=8192, larger).
 hipError_t launch_diamond(const DevCSR &g, const TaskList tl[4], Launch &l, int *d_queues);
+// Same items, the four per-edge sums of the 4-motif formula added into
+// l.d_out[0..3] (l.d_out[4 * slot + j] with instance slots).
+hipError_t launch_motif_edges(const DevCSR &g, const TaskList tl[4], Launch &l, int *d_queues);
 
 // HBM counter rows of the 4-cycle wedge kernels: one row of `stride` ints
 // spans the vertex ids of one instance (every instance occupies the

@@ -239,6 +239,21 @@ __device__ __forceinline__ void wave_walk(const Seg &t, const int32_t *__restric
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       const int b = base + 64 * u;
+      const int i = b + lane;
+      r[u].valid = i < hi;
+      r[u].next_valid = i + 1 < hi;
+      pos[u] = 0;
+      // jl is wave-uniform: a window inside one segment (the common case
+      // for long lists) needs no lane -> segment mapping
+      const int2 cur = t.os[jl];
+      const int e0 = t.os[jl + 1].y;
+      if (e0 >= b + 64) {
+        r[u].j = jl;
+        r[u].heads = 1ull;
+        if (r[u].valid) pos[u] = (int64_t)(uint32_t)cur.x + (i - cur.y);
+        if (e0 == b + 64) ++jl;
+        continue;
+      }
       const int e = (jl + lane < ns) ? t.os[jl + lane + 1].y : INT_MAX;
       const int p = e - b;  // >= 1 inside the range
       ++fl.stamp;
@@ -246,12 +261,8 @@ __device__ __forceinline__ void wave_walk(const Seg &t, const int32_t *__restric
       wave_sync();
       const u64 heads = __ballot(fl.row[lane] == fl.stamp);
       const int j = jl + __popcll(heads & lanemask_le(lane));
-      const int i = b + lane;
-      r[u].valid = i < hi;
-      r[u].next_valid = i + 1 < hi;
       r[u].j = j;
       r[u].heads = heads | 1ull;
-      pos[u] = 0;
       if (r[u].valid) {
         const int2 s = t.os[j];
         pos[u] = (int64_t)(uint32_t)s.x + (i - s.y);

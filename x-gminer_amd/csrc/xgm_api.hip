@@ -143,6 +143,8 @@ struct xgm_dgraph {
   int32_t *d_slot = nullptr;
   int64_t *d_inst_lo = nullptr;
   u64 *d_inst_out = nullptr;       // per-instance counts
+  u64 *d_motif = nullptr;          // per-instance 4-motif edge sums (4 per instance)
+  std::unique_ptr<xgm_dgraph> dag; // undirected graphs: degree-oriented copy for motif censuses
   size_t hub_ints = 0;
   ~xgm_dgraph() {
     (void)hipFree(d_dense);
@@ -151,6 +153,7 @@ struct xgm_dgraph {
     (void)hipFree(d_slot);
     (void)hipFree(d_inst_lo);
     (void)hipFree(d_inst_out);
+    (void)hipFree(d_motif);
     (void)hipFree(d_rl_rowptr);
     (void)hipFree(d_rl_colidx);
     for (auto &kv : task_cache) (void)hipFree(kv.second.first);
@@ -734,6 +737,48 @@ void xgm_graph_free(xgm_dgraph *g) {
 
 static DevCSR dev_view(const xgm_dgraph *g);
 
+// Diamond / motif edge items: every undirected edge owned by its (degree,
+// id)-larger endpoint, owners in degree tiers (<= 64, <= 1024, <= 8192,
+// larger), owners above 1024 split into slices of ~64K probed elements so
+// hubs spread over the whole GPU.  Cached as "diamond".
+// Owners of degree <= 2 hold no diamond (c <= 1) and are left out; the
+// motif sums (with_deg2) also need the degree-2 owners.
+static int diamond_lists(xgm_ctx *ctx, xgm_dgraph *gm, std::vector<TaskList> &tl, bool with_deg2 = false) {
+  std::vector<std::vector<int32_t>> parts(4);
+  const char *key = with_deg2 ? "motif_edges" : "diamond";
+  if (!cached(gm, key)) {
+    const auto &deg = gm->dplus;
+    const int64_t kChunk = 1 << 16;
+    std::vector<int32_t> owners = gm->order;  // degree >= 3, by decreasing work
+    if (with_deg2)
+      for (int64_t u = 0; u < gm->n; ++u)
+        if (deg[u] == 2) owners.push_back((int32_t)u);
+    for (int32_t t : owners) {
+      const int d = deg[t];
+      const int tier = d <= 64 ? 0 : d <= 1024 ? 1 : d <= 8192 ? 2 : 3;
+      auto &P = parts[tier];
+      if (tier <= 1) {
+        P.insert(P.end(), {t, 0, d, 0});
+        continue;
+      }
+      const int32_t *nt = gm->h_colidx.data() + gm->h_rowptr[t];
+      int64_t acc = 0;
+      int k0 = 0;
+      for (int k = 0; k < d; ++k) {
+        const int32_t p = nt[k];
+        if (deg[p] < d || (deg[p] == d && p < t)) acc += deg[p];
+        if (acc >= kChunk || k + 1 - k0 >= 4096) {
+          P.insert(P.end(), {t, k0, k + 1, 0});
+          k0 = k + 1;
+          acc = 0;
+        }
+      }
+      if (k0 < d) P.insert(P.end(), {t, k0, d, 0});
+    }
+  }
+  return cache_parts(ctx, gm, key, parts, 4, tl);
+}
+
 // Enqueue the kernels of one pattern count on ctx->stream (no host sync,
 // no D2H): task lists, scratch and relabelled copies are built and cached
 // on first use, so a second call enqueues kernels only (capturable).
@@ -796,36 +841,7 @@ static int dispatch_pattern(xgm_ctx *ctx, const xgm_dgraph *g, int pattern, cons
       if (!(g->flags & XGM_CSR_SYMMETRIC)) return ctx->fail(XGM_EINVAL, "DIAMOND_EDGE needs an undirected CSR");
       if (strategy != XGM_STRAT_AUTO && strategy != XGM_STRAT_HASH)
         return ctx->fail(XGM_EUNSUPPORTED, "strategy not available for DIAMOND_EDGE");
-      std::vector<std::vector<int32_t>> parts(4);
-      if (!cached(gm, "diamond")) {
-        // owners in degree tiers; owners above 1024 split into slices of
-        // ~64K probed elements so hubs spread over the whole GPU
-        const auto &deg = gm->dplus;
-        const int64_t kChunk = 1 << 16;
-        for (int32_t t : gm->order) {
-          const int d = deg[t];
-          const int tier = d <= 64 ? 0 : d <= 1024 ? 1 : d <= 8192 ? 2 : 3;
-          auto &P = parts[tier];
-          if (tier <= 1) {
-            P.insert(P.end(), {t, 0, d, 0});
-            continue;
-          }
-          const int32_t *nt = gm->h_colidx.data() + gm->h_rowptr[t];
-          int64_t acc = 0;
-          int k0 = 0;
-          for (int k = 0; k < d; ++k) {
-            const int32_t p = nt[k];
-            if (deg[p] < d || (deg[p] == d && p < t)) acc += deg[p];
-            if (acc >= kChunk || k + 1 - k0 >= 4096) {
-              P.insert(P.end(), {t, k0, k + 1, 0});
-              k0 = k + 1;
-              acc = 0;
-            }
-          }
-          if (k0 < d) P.insert(P.end(), {t, k0, d, 0});
-        }
-      }
-      if ((rc = cache_parts(ctx, gm, "diamond", parts, 4, tl))) return rc;
+      if ((rc = diamond_lists(ctx, gm, tl))) return rc;
       TaskList t4[4] = {tl[0], tl[1], tl[2], tl[3]};
       e = launch_diamond(dg, t4, l, l.d_queue);
       break;
@@ -1167,6 +1183,101 @@ int xgm_count_batch(xgm_ctx *ctx, const xgm_csr *gs, const int *patterns, int n_
     std::vector<uint64_t> tmp(idx.size());
     if ((rc = count_into(ctx, dg, p, nullptr, tmp.data(), true))) return rc;
     for (size_t j = 0; j < idx.size(); ++j) out[idx[j]] = tmp[j];
+  }
+  return XGM_OK;
+}
+
+// Degree-oriented copy of an undirected device graph (instances keep their
+// slots), built on first use and owned by g.
+static int ensure_dag(xgm_ctx *ctx, xgm_dgraph *g) {
+  if (g->dag) return XGM_OK;
+  xgm_csr u{g->n, g->nnz, g->h_rowptr.data(), g->h_colidx.data(), g->flags};
+  xgm_hgraph *h = nullptr;
+  int rc = xgm_orient_dag(&u, &h);
+  if (rc) return ctx->fail(rc, "orientation failed");
+  std::unique_ptr<xgm_hgraph, void (*)(xgm_hgraph *)> hold(h, xgm_hgraph_free);
+  xgm_csr v;
+  xgm_hgraph_view(h, &v);
+  xgm_dgraph *d = nullptr;
+  if ((rc = xgm_graph_upload(ctx, &v, &d))) return rc;
+  std::unique_ptr<xgm_dgraph> dd(d);
+  if (g->d_slot) {
+    dd->n_inst = g->n_inst;
+    dd->h_slot = g->h_slot;
+    dd->inst_base = g->inst_base;
+    dd->max_inst_n = g->max_inst_n;
+    const size_t ni = (size_t)g->n_inst;
+    HIP_TRY(ctx, hipMalloc(&dd->d_slot, std::max<int64_t>(1, g->n) * sizeof(int32_t)));
+    HIP_TRY(ctx, hipMalloc(&dd->d_inst_lo, (ni + 1) * sizeof(int64_t)));
+    HIP_TRY(ctx, hipMalloc(&dd->d_inst_out, ((ni + 1) & ~(size_t)1) * sizeof(u64) + sizeof(u64)));
+    if (g->n) HIP_TRY(ctx, hipMemcpy(dd->d_slot, g->d_slot, g->n * sizeof(int32_t), hipMemcpyDeviceToDevice));
+    HIP_TRY(ctx, hipMemcpy(dd->d_inst_lo, g->d_inst_lo, (ni + 1) * sizeof(int64_t), hipMemcpyDeviceToDevice));
+  }
+  g->dag = std::move(dd);
+  return XGM_OK;
+}
+
+int xgm_motifs(xgm_ctx *ctx, const xgm_dgraph *g, int k, uint64_t *out) {
+  if (!ctx || !g || !out) return XGM_EINVAL;
+  if (g->ctx != ctx) return ctx->fail(XGM_EINVAL, "graph belongs to another context");
+  if (!(g->flags & XGM_CSR_SYMMETRIC)) return ctx->fail(XGM_EINVAL, "motif census needs an undirected CSR");
+  if (k != 3 && k != 4) return ctx->fail(XGM_EUNSUPPORTED, "motif census for k = 3 or 4");
+  xgm_dgraph *gm = const_cast<xgm_dgraph *>(g);
+  const int ni = g->d_slot ? g->n_inst : 1;
+  int rc = ensure_dag(ctx, gm);
+  if (rc) return rc;
+  std::vector<uint64_t> tri(ni), k4(ni), c4(ni);
+  if (k == 3) {
+    // wedges = sum_v d(d-1) / 2 - 3 * triangles (src/motif/omp_formula.cc:38-39)
+    std::vector<u64> dd(ni, 0);
+    for (int64_t v = 0; v < g->n; ++v) {
+      const u64 d = (u64)(g->h_rowptr[v + 1] - g->h_rowptr[v]);
+      dd[g->d_slot ? g->h_slot[v] : 0] += d * (d - 1);
+    }
+    if ((rc = count_into(ctx, gm->dag.get(), XGM_TRIANGLE, nullptr, tri.data(), true))) return rc;
+    for (int i = 0; i < ni; ++i) {
+      out[2 * i] = dd[i] / 2 - 3 * tri[i];
+      out[2 * i + 1] = tri[i];
+    }
+    return XGM_OK;
+  }
+  // per-edge sums S0..S3 on the GPU (motif mode of the diamond kernels)
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  if (!gm->d_motif) HIP_TRY(ctx, hipMalloc(&gm->d_motif, 4 * (size_t)ni * sizeof(u64)));
+  std::vector<TaskList> tl;
+  if ((rc = diamond_lists(ctx, gm, tl, true))) return rc;
+  ctx->prof = xgm_profile{};
+  HIP_TRY(ctx, launch_zero(ctx->d_out, kScratchU64, ctx->stream));
+  HIP_TRY(ctx, launch_zero(ctx->d_queues, kScratchQueues / 2, ctx->stream));
+  HIP_TRY(ctx, launch_zero(gm->d_motif, 4 * (size_t)ni, ctx->stream));
+  Launch l = make_launch(ctx, 0);
+  l.d_out = gm->d_motif;
+  l.slots = g->d_slot;
+  TaskList t4[4] = {tl[0], tl[1], tl[2], tl[3]};
+  HIP_TRY(ctx, launch_motif_edges(dev_view(g), t4, l, l.d_queue));
+  std::vector<u64> S(4 * (size_t)ni);
+  HIP_TRY(ctx, hipMemcpyAsync(S.data(), gm->d_motif, S.size() * sizeof(u64), hipMemcpyDeviceToHost, ctx->stream));
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  if ((rc = count_into(ctx, gm->dag.get(), XGM_CLIQUE4, nullptr, k4.data(), true))) return rc;
+  if ((rc = count_into(ctx, gm, XGM_CYCLE4_EDGE, nullptr, c4.data(), true))) return rc;
+  // solve for the vertex-induced counts (src/motif/omp_formula.cc:40-45); the
+  // induced 4-cycles come from the edge-induced count:
+  // C4_edge = I(C4) + I(diamond) + 3 K4
+  for (int i = 0; i < ni; ++i) {
+    const u64 *s4 = &S[4 * (size_t)i];
+    const u64 K4 = k4[i];
+    const u64 dia = s4[3] / 2 - 6 * K4;
+    const u64 tail = s4[2] / 2 - 2 * dia;
+    const u64 cyc = c4[i] - dia - 3 * K4;
+    const u64 path = s4[1] - 4 * cyc;
+    const u64 star = s4[0] / 6 - tail / 3;
+    uint64_t *o = out + 6 * (size_t)i;
+    o[0] = star;
+    o[1] = path;
+    o[2] = tail;
+    o[3] = cyc;
+    o[4] = dia;
+    o[5] = K4;
   }
   return XGM_OK;
 }
