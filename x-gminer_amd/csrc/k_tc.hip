@@ -113,6 +113,126 @@ __global__ __launch_bounds__(256) void k_tc_wave(const int64_t *__restrict__ row
 }
 
 // ---------------------------------------------------------------------------
+// Packed wave kernel for the smallest items (d+(t) <= 64, <= 64 partners):
+// a wave claims G items with one atomic and runs them as one task -- the
+// staged lists of all G items go into G hash regions through one flattened
+// insert pass, and the partners of all G items form one segment table -- so
+// the dependent loads of G items (item, rowptr, lists, partner rowptrs,
+// partner lists) are in flight together instead of one item's chain at a
+// time.
+template <int G, int HSHIFT>
+struct PackStage {
+  static constexpr int SLOTS = 64 << HSHIFT;  // per item
+  int key[G * SLOTS];
+  Segs<64 * G, true> seg;                     // idx[] = item of the segment
+  int u[G], k0[G], n[G], du[G], bits[G], cnt[G];
+  int poff[G + 1], doff[G + 1];               // prefix sums of partners / staged lengths
+  int64_t ob[G], ib[G];                        // rowptr[u], rrowptr[u]
+};
+
+template <int G, int HSHIFT>
+__global__ __launch_bounds__(128) void k_tc_wave_packed(const int64_t *__restrict__ rowptr,
+                                                        const int32_t *__restrict__ colidx,
+                                                        const int64_t *__restrict__ rrowptr,
+                                                        const int32_t *__restrict__ rcolidx,
+                                                        const int4 *__restrict__ items, int nitems,
+                                                        int *__restrict__ queue, u64 *__restrict__ out,
+                                                        const int32_t *__restrict__ slots) {
+  using PS = PackStage<G, HSHIFT>;
+  __shared__ PS stage[2];
+  __shared__ int flagrow[2][64];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  PS &s = stage[wave];
+  WaveFlags fl;
+  fl.init(flagrow[wave], lane);
+  u64 acc = 0;
+  for (;;) {
+    int base = 0;
+    if (lane == 0) base = atomicAdd(queue, G);
+    base = __shfl(base, 0, 64);
+    if (base >= nitems) break;
+    const int ng = min(G, nitems - base);
+    // item metadata, one lane per item
+    int n = 0, du = 0;
+    if (lane < ng) {
+      const int4 it = items[base + lane];
+      const int64_t b = rowptr[it.x];
+      du = (int)(rowptr[it.x + 1] - b);
+      n = it.z - it.y;
+      s.u[lane] = it.x;
+      s.k0[lane] = it.y;
+      s.n[lane] = n;
+      s.du[lane] = du;
+      s.ob[lane] = b;
+      s.ib[lane] = rrowptr[it.x];
+      s.bits[lane] = hash_bits_for(du, HSHIFT);
+      s.cnt[lane] = 0;
+    }
+    const int pin = wave_incl_scan(n, lane), din = wave_incl_scan(du, lane);
+    if (lane < ng) {
+      s.poff[lane + 1] = pin;
+      s.doff[lane + 1] = din;
+    }
+    if (lane == 0) { s.poff[0] = 0; s.doff[0] = 0; }
+    for (int k = lane; k < ng * PS::SLOTS; k += 64) s.key[k] = -1;
+    wave_sync();
+    const int P = __shfl(pin, ng - 1, 64), D = __shfl(din, ng - 1, 64);
+    auto item_of = [&](int q, const int *off) {
+      int g = 0;
+#pragma unroll
+      for (int h = 1; h < G; ++h) g += (h < ng && q >= off[h]) ? 1 : 0;
+      return g;
+    };
+    // staged lists N+(u_g) into hash region g
+    for (int q = lane; q < D; q += 64) {
+      const int g = item_of(q, s.doff);
+      hash_insert(s.key + g * PS::SLOTS, s.bits[g], colidx[s.ob[g] + (q - s.doff[g])]);
+    }
+    // one segment table over the partners of all items
+    int ns = 0, L = 0;
+    wave_segments_fn(s.seg, rowptr, 0, P,
+                     [&](int q) {  // partner k of u_g in N+(u_g) ++ N-(u_g)
+                       const int g = item_of(q, s.poff);
+                       const int k = s.k0[g] + q - s.poff[g];
+                       return k < s.du[g] ? colidx[s.ob[g] + k] : rcolidx[s.ib[g] + (k - s.du[g])];
+                     },
+                     [&](int q, int, int len) {  // charged to u_g (the TcKeep rule)
+                       const int g = item_of(q, s.poff);
+                       const int k = s.k0[g] + q - s.poff[g];
+                       return k < s.du[g] ? len < s.du[g] : len <= s.du[g];
+                     },
+                     lane, ns, L);
+    if (lane == 0) s.seg.os[ns] = make_int2(0, L);
+    wave_sync();
+    for (int j = lane; j < ns; j += 64) s.seg.idx[j] = item_of(s.seg.idx[j], s.poff);
+    wave_sync();
+    int cnt = 0;
+    wave_walk<UNROLL>(s.seg, colidx, ns, 0, L, lane, fl, [&](const WindowLane &r) {
+      bool hit = false;
+      if (r.valid) {
+        const int g = s.seg.idx[r.j];
+        hit = hash_contains(s.key + g * PS::SLOTS, s.bits[g], r.w);
+      }
+      if (!slots) {
+        cnt += hit;
+      } else {
+        const int h = segment_run_hits(__ballot(hit), r, lane);
+        if (h) atomicAdd(&s.cnt[s.seg.idx[r.j]], h);
+      }
+    });
+    if (!slots) {
+      acc += (u64)cnt;
+    } else {
+      wave_sync();
+      if (lane < ng && s.cnt[lane]) atomicAdd(out + slots[s.u[lane]], (u64)s.cnt[lane]);
+    }
+    wave_sync();  // stage is rewritten by the next claim
+  }
+  acc = wave_sum_u64(acc);
+  if (lane == 0 && acc) atomicAdd(out, acc);
+}
+
+// ---------------------------------------------------------------------------
 // Block-per-item kernel: NT/64 waves share one Stage, build the segment
 // table together and walk contiguous shares of it.  If d+(t) > CAP the
 // membership test binary-searches N+(t) in HBM.  Partners are taken in
@@ -213,6 +333,21 @@ hipError_t launch_tc_strat(const DevCSR &g, const TaskList tiers[NUM_TIERS], Lau
     return e;
   if ((e = run_tc("tc_wave_256", k_tc_wave<256, STRAT, HS>, 256, g, tiers[TIER_W256], d_queues + TIER_W256, l)))
     return e;
+  if (STRAT == STRAT_HASH) {
+    // smallest items: 8 per wave, two waves per block
+    constexpr int kPack = 8;
+    const TaskList &it = tiers[TIER_W64];
+    if (it.count) {
+      auto k = k_tc_wave_packed<kPack, HS>;
+      const int grid = persistent_grid(k, l.num_cu, it.count, 2 * kPack, 128);
+      l.begin("tc_wave_64", it.count);
+      k<<<grid, 128, 0, l.stream>>>(g.rowptr, g.colidx, g.rrowptr, g.rcolidx, (const int4 *)it.tasks,
+                                    (int)it.count, d_queues + TIER_W64, l.d_out, l.slots);
+      l.end();
+      if ((e = hipGetLastError())) return e;
+    }
+    return hipSuccess;
+  }
   if ((e = run_tc("tc_wave_64", k_tc_wave<64, STRAT, HS>, 256, g, tiers[TIER_W64], d_queues + TIER_W64, l)))
     return e;
   return hipSuccess;
