@@ -139,6 +139,148 @@ __global__ __launch_bounds__(NT) void k_diamond(const int64_t *__restrict__ rowp
   }
 }
 
+// Packed variant for owners of degree <= 64: a wave claims G items with one
+// atomic; the staged lists of all G owners go into G hash regions through
+// one flattened insert pass and the partners of all G owners form one
+// segment table, so the dependent loads of G items are in flight together.
+template <int G>
+struct DiaPack {
+  static constexpr int SLOTS = 128;  // 64 keys at load <= 1/2
+  int key[G * SLOTS];
+  Segs<64 * G, true> seg;            // idx[] = item of the segment
+  int cnt[64 * G];
+  int u[G], k0[G], d[G], bits[G];
+  int poff[G + 1], doff[G + 1];
+  int64_t ob[G];
+  u64 sum[G][4];                      // per-item sums (instance slots only)
+};
+
+template <int G, bool MOTIF>
+__global__ __launch_bounds__(128) void k_diamond_packed(const int64_t *__restrict__ rowptr,
+                                                        const int32_t *__restrict__ colidx,
+                                                        const int4 *__restrict__ items, int ntasks,
+                                                        int *__restrict__ queue, u64 *__restrict__ out,
+                                                        const int32_t *__restrict__ slots) {
+  using PS = DiaPack<G>;
+  __shared__ PS stage[2];
+  __shared__ int flagrow[2][64];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  PS &s = stage[wave];
+  WaveFlags fl;
+  fl.init(flagrow[wave], lane);
+  u64 acc[4] = {0, 0, 0, 0};
+  for (;;) {
+    int base = 0;
+    if (lane == 0) base = atomicAdd(queue, G);
+    base = __shfl(base, 0, 64);
+    if (base >= ntasks) break;
+    const int ng = min(G, ntasks - base);
+    int n = 0, dt = 0;
+    if (lane < ng) {
+      const int4 it = items[base + lane];
+      const int64_t b = rowptr[it.x];
+      dt = (int)(rowptr[it.x + 1] - b);
+      n = it.z - it.y;
+      s.u[lane] = it.x;
+      s.k0[lane] = it.y;
+      s.d[lane] = dt;
+      s.ob[lane] = b;
+      s.bits[lane] = hash_bits_for(dt);
+#pragma unroll
+      for (int j = 0; j < 4; ++j) s.sum[lane][j] = 0;
+    }
+    const int pin = wave_incl_scan(n, lane), din = wave_incl_scan(dt, lane);
+    if (lane < ng) {
+      s.poff[lane + 1] = pin;
+      s.doff[lane + 1] = din;
+    }
+    if (lane == 0) { s.poff[0] = 0; s.doff[0] = 0; }
+    for (int k = lane; k < ng * PS::SLOTS; k += 64) s.key[k] = -1;
+    wave_sync();
+    const int P = __shfl(pin, ng - 1, 64), D = __shfl(din, ng - 1, 64);
+    auto item_of = [&](int q, const int *off) {
+      int g = 0;
+#pragma unroll
+      for (int h = 1; h < G; ++h) g += (h < ng && q >= off[h]) ? 1 : 0;
+      return g;
+    };
+    for (int q = lane; q < D; q += 64) {
+      const int g = item_of(q, s.doff);
+      hash_insert(s.key + g * PS::SLOTS, s.bits[g], colidx[s.ob[g] + (q - s.doff[g])]);
+    }
+    int ns = 0, L = 0;
+    wave_segments_fn(s.seg, rowptr, 0, P,
+                     [&](int q) {
+                       const int g = item_of(q, s.poff);
+                       return colidx[s.ob[g] + s.k0[g] + (q - s.poff[g])];
+                     },
+                     [&](int q, int p, int lp) {  // p owned by u_g: (deg p, p) < (deg u_g, u_g)
+                       const int g = item_of(q, s.poff);
+                       return lp < s.d[g] || (lp == s.d[g] && p < s.u[g]);
+                     },
+                     lane, ns, L);
+    if (lane == 0) s.seg.os[ns] = make_int2(0, L);
+    wave_sync();
+    for (int j = lane; j < ns; j += 64) {
+      s.seg.idx[j] = item_of(s.seg.idx[j], s.poff);
+      s.cnt[j] = 0;
+    }
+    wave_sync();
+    wave_walk<4>(s.seg, colidx, ns, 0, L, lane, fl, [&](const WindowLane &r) {
+      bool hit = false;
+      if (r.valid) {
+        const int g = s.seg.idx[r.j];
+        hit = hash_contains(s.key + g * PS::SLOTS, s.bits[g], r.w);
+      }
+      const int h = segment_run_hits(__ballot(hit), r, lane);
+      if (h) atomicAdd(&s.cnt[r.j], h);
+    });
+    wave_sync();
+    for (int q = lane; q < ns; q += 64) {
+      const u64 c = (u64)s.cnt[q];
+      const int g = s.seg.idx[q];
+      u64 v[4] = {0, 0, 0, 0};
+      if (MOTIF) {
+        const u64 dp = (u64)(s.seg.os[q + 1].y - s.seg.os[q].y);
+        const u64 su = (u64)s.d[g] - c - 1, sv = dp - c - 1;
+        v[0] = su * (su - 1) + sv * (sv - 1);
+        v[1] = su * sv;
+        v[2] = c * (su + sv);
+        v[3] = c * (c - 1);
+      } else {
+        v[0] = c * (c - 1) / 2;
+      }
+      if (!slots) {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[j] += v[j];
+      } else {
+#pragma unroll
+        for (int j = 0; j < (MOTIF ? 4 : 1); ++j)
+          if (v[j]) atomicAdd(&s.sum[g][j], v[j]);
+      }
+    }
+    if (slots) {
+      wave_sync();
+      if (lane < ng) {
+        const int32_t sl = slots[s.u[lane]];
+        if (MOTIF) {
+#pragma unroll
+          for (int j = 0; j < 4; ++j)
+            if (s.sum[lane][j]) atomicAdd(out + 4 * (size_t)sl + j, s.sum[lane][j]);
+        } else if (s.sum[lane][0]) {
+          atomicAdd(out + sl, s.sum[lane][0]);
+        }
+      }
+    }
+    wave_sync();  // stage is rewritten by the next claim
+  }
+#pragma unroll
+  for (int j = 0; j < (MOTIF ? 4 : 1); ++j) {
+    const u64 w = wave_sum_u64(acc[j]);
+    if (lane == 0 && w) atomicAdd(out + j, w);
+  }
+}
+
 // ------------------------------------------------------------ 4-cycle WEDGE
 template <int HSLOTS, int PCAP>
 struct WedgeStage {
@@ -414,9 +556,16 @@ hipError_t launch_diamond_mode(const DevCSR &g, const TaskList tl[4], Launch &l,
   if ((e = run_diamond<1024, 1024, 256, true, MOTIF>(MOTIF ? "motif_edges_block_1024" : "diamond_block_1024", g,
                                                      tl[1], d_queues + 1, l)))
     return e;
-  if ((e = run_diamond<64, 64, 64, true, MOTIF>(MOTIF ? "motif_edges_wave_64" : "diamond_wave_64", g, tl[0],
-                                                d_queues + 0, l)))
-    return e;
+  if (tl[0].count) {  // owners of degree <= 64: 8 per wave, two waves per block
+    constexpr int kPack = 8;
+    auto k = k_diamond_packed<kPack, MOTIF>;
+    const int grid = grid_of(k, 128, l.num_cu, (tl[0].count + 2 * kPack - 1) / (2 * kPack));
+    l.begin(MOTIF ? "motif_edges_wave_64" : "diamond_wave_64", tl[0].count);
+    k<<<grid, 128, 0, l.stream>>>(g.rowptr, g.colidx, (const int4 *)tl[0].tasks, (int)tl[0].count, d_queues + 0,
+                                  l.d_out, l.slots);
+    l.end();
+    if ((e = hipGetLastError())) return e;
+  }
   return hipSuccess;
 }
 
