@@ -74,6 +74,9 @@ typedef struct xgm_opts {
                             staged lists run one block per item; 0 = 256 */
   int hub_rows;          /* 4-cycles: HBM counter rows shared by hub tasks in one round (each row
                             spans the largest instance); 0 = as many as fit in 8 GiB */
+  int wedge_range;       /* 4-cycles: counters of tasks above 4096 wedges in LDS by id range
+                            (0 = where the range cuts cost at most a quarter of the wedges,
+                            1 = every task above 512 wedges, 2 = never) */
 } xgm_opts;
 
 typedef struct xgm_ctx xgm_ctx;
@@ -165,7 +168,8 @@ void xgm_plan_free(xgm_plan *plan);
  * fastest into *best.  Candidates: triangles -- LDS hash load 1/2 or 1/4 x
  * five wave/block tier splits, and binary search; cliques -- auto, rank LUT,
  * id-ordered LUT with threshold 1024 or 256, edge LUT; 4-cycles -- the wedge
- * aggregation only.  Each candidate runs 1 + reps times. */
+ * aggregation with each wedge_range mode.  Each candidate runs 1 + reps
+ * times. */
 int xgm_autotune(xgm_ctx *ctx, const xgm_dgraph *g, int pattern, int reps, xgm_opts *best, float *best_ms);
 
 /* Measured HBM bandwidth: a plain streaming copy kernel (k_copy_u4) over

@@ -86,6 +86,10 @@ def test_batch_per_instance_counts(xgm, ctx, mixed):
     _say("batch rectangle hub rounds")
     for rows in (1, 2, 7):
         assert np.array_equal(ctx.count_instances(und, 5, hub_rows=rows), _col(want, "rectangle")), rows
+    _say("batch rectangle LDS range counters")
+    for rm in (1, 2):
+        assert np.array_equal(ctx.count_instances(und, 5, wedge_range=rm), _col(want, "rectangle")), rm
+    assert np.array_equal(ctx.count_instances(und, 5, wedge_range=1, hub_rows=3), _col(want, "rectangle"))
     _say("batch rectangle pair strategy")
     assert np.array_equal(ctx.count_instances(und, 5, strategy=4), _col(want, "rectangle"))
     _say("batch clique strategies")
@@ -158,7 +162,24 @@ def test_cycle4_hub_rounds_single_graph(xgm, ctx):
     want = math.comb(520, 2) ** 2
     for rows in (0, 1, 4, 15, 100):
         assert ctx.count(dg, 5, hub_rows=rows) == want, rows
+    for rm in (0, 1, 2):
+        assert ctx.count(dg, 5, wedge_range=rm) == want, rm
     dg.close()
+
+
+def test_cycle4_range_counters_golden(xgm, ctx):
+    """Forced LDS range items on the golden graphs (Chung-Lu 20k has tasks
+    in every wedge tier), and the one-range and many-range cases."""
+    import json
+    from golden.make_golden import SMALL, make_graph
+    gold = json.load(open(os.path.join(GOLDEN, "counts.json")))
+    for name in ("chunglu20k_s9", "rmat16_s1", "rmat12_s3"):
+        _say("range golden", name)
+        g = make_graph(xgm, *SMALL[name][:2])
+        dg = ctx.upload(g)
+        for rm in (1, 2, 0):
+            assert ctx.count(dg, 5, wedge_range=rm) == gold[name]["counts"]["rectangle"], (name, rm)
+        dg.close()
 
 
 def test_tc_tile_options_exact(xgm, orc, ctx):

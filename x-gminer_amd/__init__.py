@@ -58,14 +58,16 @@ class _Stats(C.Structure):
 class _Opts(C.Structure):
     _fields_ = [("strategy", C.c_int), ("lut_max_degree", C.c_int),
                 ("flags", C.c_uint32), ("hash_shift", C.c_int),
-                ("wave_max_small", C.c_int), ("wave_max_large", C.c_int), ("hub_rows", C.c_int)]
+                ("wave_max_small", C.c_int), ("wave_max_large", C.c_int), ("hub_rows", C.c_int),
+                ("wedge_range", C.c_int)]
 
     def as_dict(self):
         return {k: int(getattr(self, k)) for k, _ in self._fields_ if k != "flags"}
 
 
-def _opts(strategy=0, lut_max_degree=0, hash_shift=0, wave_max_small=0, wave_max_large=0, hub_rows=0):
-    return _Opts(strategy, lut_max_degree, 0, hash_shift, wave_max_small, wave_max_large, hub_rows)
+def _opts(strategy=0, lut_max_degree=0, hash_shift=0, wave_max_small=0, wave_max_large=0, hub_rows=0,
+          wedge_range=0):
+    return _Opts(strategy, lut_max_degree, 0, hash_shift, wave_max_small, wave_max_large, hub_rows, wedge_range)
 
 
 class _KTime(C.Structure):

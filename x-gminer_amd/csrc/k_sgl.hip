@@ -389,6 +389,94 @@ __global__ __launch_bounds__(NT) void k_cycle4_wedge(const int64_t *__restrict__
   if (lane == 0 && acc) atomicAdd(out, acc);
 }
 
+// Range items of the wedge strategy: (x, r0, r1, -) counts the wedges of x
+// whose far end w lies in [r0, r1) (r1 - r0 <= R, r1 <= x), with c_x(w) in
+// R LDS counters indexed by w - r0.  The ranges of one task partition the
+// ids below x, so every w is counted whole by exactly one item and the
+// previous values returned by the increments still sum to C(c_x(w), 2);
+// no HBM counters or global atomics are involved.  Each lower neighbour's
+// list is cut to [r0, r1) by two binary searches.
+template <int R, int PCAP, int NT>
+struct RangeStage {
+  int cnt[R];
+  Segs<PCAP, false> seg;
+  int tmp[PCAP];
+  int flagrow[NT / 64][64];
+  int ns, L, task;
+};
+
+template <int R, int PCAP, int NT>
+__global__ __launch_bounds__(NT) void k_cycle4_range(const int64_t *__restrict__ rowptr,
+                                                     const int32_t *__restrict__ colidx,
+                                                     const int4 *__restrict__ items, int nitems,
+                                                     int *__restrict__ queue, u64 *__restrict__ out,
+                                                     const int32_t *__restrict__ slots) {
+  __shared__ RangeStage<R, PCAP, NT> S;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  WaveFlags fl;
+  fl.init(S.flagrow[wave], lane);
+  u64 acc = 0;
+  for (;;) {
+    if (tid == 0) S.task = atomicAdd(queue, 1);
+    __syncthreads();
+    const int t = S.task;
+    if (t >= nitems) break;
+    const int4 it = items[t];
+    const int x = it.x, r0 = it.y, r1 = it.z;
+    for (int q = tid; q < r1 - r0; q += NT) S.cnt[q] = 0;
+    const int64_t bx = rowptr[x];
+    const int32_t *nx = colidx + bx;
+    const int nlow = lower_bound_dev(nx, (int)(rowptr[x + 1] - bx), x);
+    u64 part = 0;
+    for (int k0 = 0; k0 < nlow; k0 += PCAP) {
+      const int n = min(nlow - k0, PCAP);
+      __syncthreads();  // counters zeroed / previous piece walked
+      for (int q = tid; q < n; q += NT) {
+        const int v = nx[k0 + q];
+        const int64_t vb = rowptr[v];
+        const int dv = (int)(rowptr[v + 1] - vb);
+        const int32_t *nv = colidx + vb;
+        const int a = lower_bound_dev(nv, dv, r0);
+        const int b = a + lower_bound_dev(nv + a, dv - a, r1);
+        S.seg.os[q].x = (int)(vb + a);
+        S.tmp[q] = b - a;
+      }
+      __syncthreads();
+      if (wave == 0) {  // compact the non-empty pieces in place
+        int ns = 0, L = 0;
+        for (int k = 0; k < n; k += 64) {
+          const int kk = k + lane;
+          int len = 0, vb = 0;
+          if (kk < n) {
+            len = S.tmp[kk];
+            vb = S.seg.os[kk].x;
+          }
+          const int incl = wave_incl_scan(len, lane) + L;
+          const u64 m = __ballot(len > 0);
+          if (len > 0) S.seg.os[ns + __popcll(m & lanemask_lt(lane))] = make_int2(vb, incl - len);
+          ns += __popcll(m);
+          L = __shfl(incl, 63, 64);
+        }
+        if (lane == 0) {
+          S.seg.os[ns] = make_int2(0, L);
+          S.ns = ns;
+          S.L = L;
+        }
+      }
+      __syncthreads();
+      int lo, hi;
+      wave_share(S.L, wave, NT / 64, lo, hi);
+      wave_walk<4>(S.seg, colidx, S.ns, lo, hi, lane, fl, [&](const WindowLane &r) {
+        if (r.valid) part += (u64)atomicAdd(&S.cnt[r.w - r0], 1);
+      });
+    }
+    task_flush(acc, part, slots, x, out, lane);
+    __syncthreads();
+  }
+  acc = wave_sum_u64(acc);
+  if (lane == 0 && acc) atomicAdd(out, acc);
+}
+
 // Hub tasks of the wedge strategy, split into items (x, k0, k1, slot): the
 // slice [k0, k1) of x's lower neighbours.  All items of one hub add into
 // the same HBM counter row hub + slot * stride (indexed like the dense rows)
@@ -582,9 +670,18 @@ int cycle4_dense_grid(int num_cu, int64_t tasks) {
 }
 
 hipError_t launch_cycle4_wedge(const DevCSR &g, const TaskList tl[3], int *dense, const TaskList *hub_rounds,
-                               int n_rounds, int *hub, size_t hub_round_u64, const CounterRows &rows, Launch &l,
-                               int *d_queues) {
+                               int n_rounds, int *hub, size_t hub_round_u64, const TaskList &range_items,
+                               const CounterRows &rows, Launch &l, int *d_queues) {
   hipError_t e;
+  if (range_items.count) {
+    auto k = k_cycle4_range<kRangeCounters, 1024, 1024>;
+    const int grid = grid_of(k, 1024, l.num_cu, range_items.count);
+    l.begin("cycle4_wedge_range", range_items.count);
+    k<<<grid, 1024, 0, l.stream>>>(g.rowptr, g.colidx, (const int4 *)range_items.tasks, (int)range_items.count,
+                                   d_queues + 6, l.d_out, l.slots);
+    l.end();
+    if ((e = hipGetLastError())) return e;
+  }
   int64_t hub_items = 0;
   for (int r = 0; r < n_rounds; ++r) hub_items += hub_rounds[r].count;
   if (hub_items) {

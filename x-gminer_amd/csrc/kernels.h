@@ -110,10 +110,13 @@ struct CounterRows {
 // cycle4_dense_grid() rows, all zero on entry (left zero on exit).
 // hub_rounds[r] are int4 (x, k0, k1, slot) slices of hub tasks adding into
 // row `slot` of hub; the first hub_round_u64 words of hub are zeroed before
-// every round.  Uses queue counters 0-2 and 4-5 of d_queues.
+// every round.  Uses queue counters 0-2 and 4-6 of d_queues.
+// range_items are int4 (x, r0, r1, -) with r1 - r0 <= kRangeCounters: the
+// wedges of x ending in [r0, r1), counted in LDS.  Queue counter 6.
+constexpr int kRangeCounters = 32768;
 hipError_t launch_cycle4_wedge(const DevCSR &g, const TaskList tl[3], int *dense, const TaskList *hub_rounds,
-                               int n_rounds, int *hub, size_t hub_round_u64, const CounterRows &rows, Launch &l,
-                               int *d_queues);
+                               int n_rounds, int *hub, size_t hub_round_u64, const TaskList &range_items,
+                               const CounterRows &rows, Launch &l, int *d_queues);
 int cycle4_dense_grid(int num_cu, int64_t tasks);
 
 // Edge-induced 4-cycle, bounded pairwise intersections in the reference's

@@ -133,7 +133,7 @@ struct xgm_dgraph {
   std::vector<int64_t> rl_rowptr;  // host copy of the relabelled CSR
   std::vector<int32_t> rl_colidx;
   int *d_hub = nullptr;            // 4-cycle hub counter rows (hub_rows * stride ints)
-  int nhubs = 0;
+  std::map<std::string, int> hub_count;  // hub tasks per cached hub list
   // batched instances (disjoint union): instance of every vertex, first id
   // of every instance (n_inst + 1 bounds), largest instance
   int n_inst = 1;
@@ -888,14 +888,32 @@ static int dispatch_pattern(xgm_ctx *ctx, const xgm_dgraph *g, int pattern, cons
       const int64_t kHubWork = 1 << 18, kSlice = 1 << 16, kHubBudgetInts = (int64_t)1 << 31;
       const int64_t row_cap = std::max<int64_t>(1, kHubBudgetInts / std::max<int64_t>(1, stride));
       const int64_t rows_max = (opts && opts->hub_rows > 0) ? std::min<int64_t>(opts->hub_rows, row_cap) : row_cap;
-      const std::string hkey = "cycle4_hubs_" + std::to_string(rows_max);
-      std::vector<std::vector<int32_t>> parts(3), hubp(1);
-      if (!cached(gm, "cycle4_wedge") || !cached(gm, hkey)) {
+      // tasks above 4096 wedges (above 512 when forced) may run as LDS range
+      // items instead: ceil((x - lo) / kRangeCounters) ranges, each cutting
+      // the nlow lower lists with two binary searches; taken when those
+      // P * nlow cuts are at most a quarter of the wedges
+      const int rmode = opts ? opts->wedge_range : 0;
+      const std::string sfx = "_" + std::to_string(rmode);
+      const std::string hkey = "cycle4_hubs_" + std::to_string(rows_max) + sfx;
+      const std::string wkey = "cycle4_wedge" + sfx, rkey = "cycle4_range" + sfx;
+      std::vector<std::vector<int32_t>> parts(3), hubp(1), rngp(1);
+      if (!cached(gm, wkey) || !cached(gm, hkey) || !cached(gm, rkey)) {
         const auto &rp = gm->rl_rowptr;
         const auto &ci = gm->rl_colidx;
         std::vector<int32_t> hubs;
         for (int32_t x : gm->rl_order) {
           const int64_t w = gm->rl_work[x];
+          if (rmode != 2 && w > (rmode == 1 ? 512 : 4096)) {
+            const int64_t lo = gm->h_slot.empty() ? 0 : gm->inst_base[gm->h_slot[x]];
+            const int32_t *nx = ci.data() + rp[x];
+            const int64_t nlow = std::lower_bound(nx, nx + (rp[x + 1] - rp[x]), x) - nx;
+            const int64_t P = (x - lo + kRangeCounters - 1) / kRangeCounters;
+            if (rmode == 1 || 4 * P * nlow <= w) {
+              for (int64_t r0 = lo; r0 < x; r0 += kRangeCounters)
+                rngp[0].insert(rngp[0].end(), {x, (int32_t)r0, (int32_t)std::min<int64_t>(x, r0 + kRangeCounters), 0});
+              continue;
+            }
+          }
           if (w > kHubWork) hubs.push_back(x);
           else parts[w <= 512 ? 0 : w <= 4096 ? 1 : 2].push_back(x);
         }
@@ -921,15 +939,17 @@ static int dispatch_pattern(xgm_ctx *ctx, const xgm_dgraph *g, int pattern, cons
           }
           if (k0 < nlow) P.insert(P.end(), {x, k0, nlow, slot});
         }
-        gm->nhubs = (int)hubs.size();
+        gm->hub_count[hkey] = (int)hubs.size();
       }
-      std::vector<TaskList> hl;
-      if ((rc = cache_parts(ctx, gm, "cycle4_wedge", parts, 1, tl))) return rc;
+      const int nhubs = gm->hub_count.count(hkey) ? gm->hub_count[hkey] : 0;
+      std::vector<TaskList> hl, rl;
+      if ((rc = cache_parts(ctx, gm, wkey, parts, 1, tl))) return rc;
       if ((rc = cache_parts(ctx, gm, hkey, hubp, 4, hl))) return rc;
+      if ((rc = cache_parts(ctx, gm, rkey, rngp, 4, rl))) return rc;
       TaskList t3[3] = {tl[0], tl[1], tl[2]};
-      const int64_t hub_rows = std::min<int64_t>(gm->nhubs, rows_max);
+      const int64_t hub_rows = std::min<int64_t>(nhubs, rows_max);
       const size_t hub_round_u64 = ((size_t)hub_rows * (size_t)stride + 1) / 2;
-      if (gm->nhubs) {
+      if (nhubs) {
         const size_t need = 2 * hub_round_u64;
         if (gm->hub_ints < need) {
           (void)hipFree(gm->d_hub);
@@ -957,8 +977,8 @@ static int dispatch_pattern(xgm_ctx *ctx, const xgm_dgraph *g, int pattern, cons
       CounterRows rows;
       rows.stride = stride;
       rows.inst_lo = l.slots ? gm->d_inst_lo : nullptr;
-      e = launch_cycle4_wedge(rg, t3, gm->d_dense, hl.data(), (int)hl.size(), gm->d_hub, hub_round_u64, rows, l,
-                              l.d_queue);
+      e = launch_cycle4_wedge(rg, t3, gm->d_dense, hl.data(), (int)hl.size(), gm->d_hub, hub_round_u64, rl[0], rows,
+                              l, l.d_queue);
       break;
     }
     default:
@@ -1468,6 +1488,10 @@ int xgm_autotune(xgm_ctx *ctx, const xgm_dgraph *g, int pattern, int reps, xgm_o
       // intersection per pair of lower neighbours, never less work than the
       // wedge aggregation and O(d^2) intersections on a hub: not a candidate
       cand.push_back(mk(XGM_STRAT_AUTO, 0, 0, 0, 0));
+      for (int rm : {1, 2}) {
+        cand.push_back(mk(XGM_STRAT_AUTO, 0, 0, 0, 0));
+        cand.back().wedge_range = rm;
+      }
       break;
     default:
       return ctx->fail(XGM_EUNSUPPORTED, "pattern not supported");
