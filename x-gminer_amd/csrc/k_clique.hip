@@ -500,6 +500,105 @@ __global__ __launch_bounds__(NT) void k_clique_vlut(const int64_t *__restrict__ 
   if ((tid & 63) == 0 && acc) atomicAdd(out, acc);
 }
 
+// Packed vertex-LUT kernel for s = d+(v0) <= 64 (one 64-bit word per row):
+// a wave claims G tasks with one atomic; the G sets N+(v0) are staged in
+// one flattened pass and the lists of all their members form one segment
+// table whose index is the LUT row (g * 64 + i), so the dependent loads of
+// G tasks are in flight together.  Counting runs lut_count per task.
+template <int G>
+struct LutPack {
+  int x[G * 64];
+  u64 rows[G * 64];
+  Segs<64 * G, true> seg;
+  int v0[G], s[G], off[G + 1];
+  int64_t b[G];
+};
+
+template <int G>
+__global__ __launch_bounds__(128) void k_clique_vlut_packed(const int64_t *__restrict__ rowptr,
+                                                            const int32_t *__restrict__ colidx,
+                                                            const int32_t *__restrict__ tasks, int ntasks, int min_s,
+                                                            int depth, int *__restrict__ queue, u64 *__restrict__ out,
+                                                            const int32_t *__restrict__ slots) {
+  __shared__ LutPack<G> stage[2];
+  __shared__ int flagrow[2][64];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  LutPack<G> &S = stage[wave];
+  WaveFlags fl;
+  fl.init(flagrow[wave], lane);
+  u64 acc = 0;
+  for (;;) {
+    int base = 0;
+    if (lane == 0) base = atomicAdd(queue, G);
+    base = __shfl(base, 0, 64);
+    if (base >= ntasks) break;
+    const int ng = min(G, ntasks - base);
+    int sz = 0;
+    if (lane < ng) {
+      const int v = tasks[base + lane];
+      const int64_t b = rowptr[v];
+      sz = (int)(rowptr[v + 1] - b);
+      if (sz < min_s) sz = 0;  // no clique of this size through v
+      S.v0[lane] = v;
+      S.b[lane] = b;
+      S.s[lane] = sz;
+    }
+    const int inc = wave_incl_scan(sz, lane);
+    if (lane < ng) S.off[lane + 1] = inc;
+    if (lane == 0) S.off[0] = 0;
+    for (int q = lane; q < G * 64; q += 64) S.rows[q] = 0ull;
+    wave_sync();
+    const int tot = __shfl(inc, ng - 1, 64);
+    auto item_of = [&](int q) {
+      int g = 0;
+#pragma unroll
+      for (int h = 1; h < G; ++h) g += (h < ng && q >= S.off[h]) ? 1 : 0;
+      return g;
+    };
+    for (int q = lane; q < tot; q += 64) {
+      const int g = item_of(q);
+      S.x[g * 64 + (q - S.off[g])] = colidx[S.b[g] + (q - S.off[g])];
+    }
+    wave_sync();
+    int ns = 0, L = 0;
+    wave_segments_fn(S.seg, rowptr, 0, tot,
+                     [&](int q) {
+                       const int g = item_of(q);
+                       return S.x[g * 64 + (q - S.off[g])];
+                     },
+                     [](int, int, int) { return true; }, lane, ns, L);
+    if (lane == 0) S.seg.os[ns] = make_int2(0, L);
+    wave_sync();
+    for (int j = lane; j < ns; j += 64) {
+      const int q = S.seg.idx[j], g = item_of(q);
+      S.seg.idx[j] = g * 64 + (q - S.off[g]);  // LUT row of the segment's member
+    }
+    wave_sync();
+    wave_walk<4>(S.seg, colidx, ns, 0, L, lane, fl, [&](const WindowLane &r) {
+      if (r.valid) {
+        const int row = S.seg.idx[r.j], g = row >> 6;
+        const int *xg = S.x + g * 64;
+        const int pos = lower_bound_dev(xg, S.s[g], r.w);
+        if (pos < S.s[g] && xg[pos] == r.w) atomicOr(&S.rows[row], 1ull << pos);
+      }
+    });
+    wave_sync();
+    for (int g = 0; g < ng; ++g) {
+      const int sg = S.s[g];
+      u64 c = sg ? lut_count<1>(S.rows + g * 64, sg, 1, depth, lane, 64) : 0;
+      if (!slots) {
+        acc += c;
+      } else {
+        c = wave_sum_u64(c);
+        if (lane == 0 && c) atomicAdd(out + slots[S.v0[g]], c);
+      }
+    }
+    wave_sync();  // stage is rewritten by the next claim
+  }
+  acc = wave_sum_u64(acc);
+  if (lane == 0 && acc) atomicAdd(out, acc);
+}
+
 // Edge-task kernel (G²Miner edge parallelism): one block per DAG edge
 // (v0, v1).  W1 = N+(v0) ∩ N+(v1) is materialised in order; with LUT it is
 // encoded as a bitmap sub-DAG and K4 adds its edges, K5 its triangles
@@ -700,7 +799,16 @@ hipError_t launch_clique(const DevCSR &g, const CliqueTasks &ct, int k, int *scr
   if ((e = run_vlut_rank<64, 1, 64>(k4 ? "k4_rlut_64" : "k5_rlut_64", g, ct.rlut[0], k, d_queues + 7, ct.log_rlut[0],
                                     l)))
     return e;
-  if ((e = run_vlut<64, 1, 64>(k4 ? "k4_lut_64" : "k5_lut_64", g, ct.lut[0], k, d_queues + 3, ct.log_lut[0], l)))
+  if (!ct.log_lut[0] && ct.lut[0].count) {  // per-task timing logs keep one task per wave
+    constexpr int kPack = 8;
+    auto kern = k_clique_vlut_packed<kPack>;
+    const int grid = grid_for(kern, 128, 0, l.num_cu, (ct.lut[0].count + 2 * kPack - 1) / (2 * kPack));
+    l.begin(k4 ? "k4_lut_64" : "k5_lut_64", ct.lut[0].count);
+    kern<<<grid, 128, 0, l.stream>>>(g.rowptr, g.colidx, ct.lut[0].tasks, (int)ct.lut[0].count, k - 1, k - 1,
+                                     d_queues + 3, l.d_out, l.slots);
+    l.end();
+    if ((e = hipGetLastError())) return e;
+  } else if ((e = run_vlut<64, 1, 64>(k4 ? "k4_lut_64" : "k5_lut_64", g, ct.lut[0], k, d_queues + 3, ct.log_lut[0], l)))
     return e;
   return hipSuccess;
 }
