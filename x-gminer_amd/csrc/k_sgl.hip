@@ -638,8 +638,10 @@ hipError_t launch_diamond_mode(const DevCSR &g, const TaskList tl[4], Launch &l,
   if ((e = run_diamond<1, 1024, 256, false, MOTIF>(MOTIF ? "motif_edges_hbm_set" : "diamond_hbm_set", g, tl[3],
                                                    d_queues + 3, l)))
     return e;
-  if ((e = run_diamond<8192, 1024, 256, true, MOTIF>(MOTIF ? "motif_edges_block_8192" : "diamond_block_8192", g,
-                                                     tl[2], d_queues + 2, l)))
+  // 64 KB of LDS hash (+ a table of 1000 partners) per block leaves room for
+  // two blocks per CU; 1024 threads each keep ~28 waves resident
+  if ((e = run_diamond<8192, 1000, 1024, true, MOTIF>(MOTIF ? "motif_edges_block_8192" : "diamond_block_8192", g,
+                                                      tl[2], d_queues + 2, l)))
     return e;
   if ((e = run_diamond<1024, 1024, 256, true, MOTIF>(MOTIF ? "motif_edges_block_1024" : "diamond_block_1024", g,
                                                      tl[1], d_queues + 1, l)))
@@ -708,7 +710,7 @@ hipError_t launch_cycle4_wedge(const DevCSR &g, const TaskList tl[3], int *dense
     l.end();
     if ((e = hipGetLastError())) return e;
   }
-  if ((e = run_wedge_lds<8192, 1024, 256>("cycle4_wedge_8192", g, tl[1], d_queues + 1, l))) return e;
+  if ((e = run_wedge_lds<8192, 1024, 1024>("cycle4_wedge_8192", g, tl[1], d_queues + 1, l))) return e;
   if ((e = run_wedge_lds<1024, 256, 64>("cycle4_wedge_1024", g, tl[0], d_queues + 0, l))) return e;
   return hipSuccess;
 }
