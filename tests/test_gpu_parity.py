@@ -336,3 +336,20 @@ def test_clique_hubs_hbm_lut(xgm, orc, ctx):
         assert s.count() == want5
         s.close()
     dg.close()
+
+
+def test_sgl_sampled_hub_owner(xgm, ctx, orc):
+    """An owner of degree 39,999 (> the 36,352-entry LDS set): diamonds and
+    the motif census through the 1024-sample index over HBM."""
+    import numpy as np
+    _say("sgl sampled hub")
+    n = 40000
+    ring = np.arange(1, 3001)
+    src = np.concatenate([np.zeros(n - 1, np.int64), ring, ring[::3]])
+    dst = np.concatenate([np.arange(1, n), ring % 3000 + 1, (ring[::3] + 7) % 3000 + 1])
+    g = xgm.from_edges(n, src, dst)
+    assert int(np.diff(g.rowptr).max()) > 36352
+    assert ctx.count(g, 4) == orc.count("diamond", g.rowptr, g.colidx)[0]
+    dg = ctx.upload(g)
+    assert ctx.motifs(dg, 4)[0].tolist() == orc.motif(g.rowptr, g.colidx, 4)
+    dg.close()

@@ -139,6 +139,112 @@ __global__ __launch_bounds__(NT) void k_diamond(const int64_t *__restrict__ rowp
   }
 }
 
+// Owners of degree > 8192 (hub slices): N(t) is staged sorted in LDS when
+// it fits (d <= LCAP) and binary-searched there; above that an LDS index of
+// 1024 samples narrows each search in HBM to one d/1024-element stretch.
+template <int LCAP, int PCAP, int NT>
+struct DiaBigStage {
+  union {
+    int list[LCAP];
+    int sample[1024];
+  };
+  Segs<PCAP, false> seg;
+  int cnt[PCAP];
+  int ns, L, task;
+};
+
+template <int LCAP, int PCAP, int NT, bool MOTIF>
+__global__ __launch_bounds__(NT) void k_diamond_big(const int64_t *__restrict__ rowptr,
+                                                    const int32_t *__restrict__ colidx,
+                                                    const int4 *__restrict__ items, int ntasks,
+                                                    int *__restrict__ queue, u64 *__restrict__ out,
+                                                    const int32_t *__restrict__ slots) {
+  __shared__ DiaBigStage<LCAP, PCAP, NT> S;
+  __shared__ int flagrow[NT / 64][64];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  WaveFlags fl;
+  fl.init(flagrow[wave], lane);
+  u64 acc[4] = {0, 0, 0, 0};
+  for (;;) {
+    if (tid == 0) S.task = atomicAdd(queue, 1);
+    __syncthreads();
+    const int t = S.task;
+    if (t >= ntasks) break;
+    const int4 it = items[t];
+    const int v = it.x;
+    u64 part[4] = {0, 0, 0, 0};
+    const int64_t bt = rowptr[v];
+    const int dt = (int)(rowptr[v + 1] - bt);
+    const int32_t *nt = colidx + bt;
+    const bool in_lds = dt <= LCAP;
+    if (in_lds) {
+      for (int q = tid; q < dt; q += NT) S.list[q] = nt[q];
+    } else {
+      for (int q = tid; q < 1024; q += NT) S.sample[q] = nt[(int64_t)q * dt / 1024];
+    }
+    auto contains = [&](int w) {
+      if (in_lds) return sorted_contains(S.list, dt, w);
+      // last sample <= w, then its stretch [j d / 1024, (j + 1) d / 1024) in HBM
+      const int j = upper_bound_dev(S.sample, 1024, w) - 1;
+      if (j < 0) return false;
+      const int a = (int)((int64_t)j * dt / 1024);
+      const int b = j == 1023 ? dt : (int)((int64_t)(j + 1) * dt / 1024);
+      return sorted_contains(nt + a, b - a, w);
+    };
+    auto keep = [dt, v](int p, int lp) { return lp < dt || (lp == dt && p < v); };
+    for (int k0 = it.y; k0 < it.z; k0 += PCAP) {
+      const int k1 = min(it.z, k0 + PCAP);
+      __syncthreads();  // staged set written / previous piece consumed
+      if (wave == 0) {
+        int ns, L;
+        wave_segments(S.seg, rowptr, colidx, nt, k0, k1, -1, keep, lane, ns, L);
+        if (lane == 0) { S.ns = ns; S.L = L; }
+      }
+      __syncthreads();
+      const int ns = S.ns, L = S.L;
+      for (int q = tid; q < ns; q += NT) S.cnt[q] = 0;
+      __syncthreads();
+      int lo, hi;
+      wave_share(L, wave, NT / 64, lo, hi);
+      wave_walk<4>(S.seg, colidx, ns, lo, hi, lane, fl, [&](const WindowLane &r) {
+        const bool hit = r.valid && contains(r.w);
+        const int h = segment_run_hits(__ballot(hit), r, lane);
+        if (h) atomicAdd(&S.cnt[r.j], h);
+      });
+      __syncthreads();
+      for (int q = tid; q < ns; q += NT) {
+        const u64 c = (u64)S.cnt[q];
+        if (MOTIF) {
+          const u64 dp = (u64)(S.seg.os[q + 1].y - S.seg.os[q].y);
+          const u64 su = (u64)dt - c - 1, sv = dp - c - 1;
+          part[0] += su * (su - 1) + sv * (sv - 1);
+          part[1] += su * sv;
+          part[2] += c * (su + sv);
+          part[3] += c * (c - 1);
+        } else {
+          part[0] += c * (c - 1) / 2;
+        }
+      }
+    }
+    if (!slots) {
+#pragma unroll
+      for (int j = 0; j < 4; ++j) acc[j] += part[j];
+    } else {
+#pragma unroll
+      for (int j = 0; j < (MOTIF ? 4 : 1); ++j) {
+        const u64 w = wave_sum_u64(part[j]);
+        if (lane == 0 && w) atomicAdd(out + (MOTIF ? 4 * (size_t)slots[v] + j : (size_t)slots[v]), w);
+      }
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int j = 0; j < (MOTIF ? 4 : 1); ++j) {
+    const u64 w = wave_sum_u64(acc[j]);
+    if (lane == 0 && w) atomicAdd(out + j, w);
+  }
+}
+
 // Packed variant for owners of degree <= 64: a wave claims G items with one
 // atomic; the staged lists of all G owners go into G hash regions through
 // one flattened insert pass and the partners of all G owners form one
@@ -635,9 +741,15 @@ hipError_t run_wedge_lds(const char *name, const DevCSR &g, const TaskList &tl, 
 template <bool MOTIF>
 hipError_t launch_diamond_mode(const DevCSR &g, const TaskList tl[4], Launch &l, int *d_queues) {
   hipError_t e;
-  if ((e = run_diamond<1, 1024, 256, false, MOTIF>(MOTIF ? "motif_edges_hbm_set" : "diamond_hbm_set", g, tl[3],
-                                                   d_queues + 3, l)))
-    return e;
+  if (tl[3].count) {  // owners above 8192: sorted set in LDS (36,352 ints), else a sample index
+    auto k = k_diamond_big<36352, 1000, 1024, MOTIF>;
+    const int grid = grid_of(k, 1024, l.num_cu, tl[3].count);
+    l.begin(MOTIF ? "motif_edges_hbm_set" : "diamond_hbm_set", tl[3].count);
+    k<<<grid, 1024, 0, l.stream>>>(g.rowptr, g.colidx, (const int4 *)tl[3].tasks, (int)tl[3].count, d_queues + 3,
+                                   l.d_out, l.slots);
+    l.end();
+    if ((e = hipGetLastError())) return e;
+  }
   // 64 KB of LDS hash (+ a table of 1000 partners) per block leaves room for
   // two blocks per CU; 1024 threads each keep ~28 waves resident
   if ((e = run_diamond<8192, 1000, 1024, true, MOTIF>(MOTIF ? "motif_edges_block_8192" : "diamond_block_8192", g,
