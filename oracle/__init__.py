@@ -130,6 +130,16 @@ def cycle4_wedge(rowptr, colidx, threads=0):
     return int(fn(len(rp) - 1, prp, pci, threads))
 
 
+def clique_bitset(rowptr, colidx, k, v_begin=0, v_end=-1, threads=0):
+    """k-cliques (4 or 5) of a DAG CSR by bitset sub-DAGs (a second CPU
+    algorithm for large golden counts; pinned against count("clique4/5"))."""
+    rp, prp = _p64(rowptr); ci, pci = _p32(colidx)
+    fn = lib().orc_clique_bitset
+    fn.restype = C.c_uint64
+    fn.argtypes = [C.c_int64, C.POINTER(C.c_int64), C.POINTER(C.c_int32), C.c_int, C.c_int64, C.c_int64, C.c_int]
+    return int(fn(len(rp) - 1, prp, pci, k, v_begin, v_end, threads))
+
+
 MOTIF_NAMES = {3: ("wedges", "triangles"),
                4: ("3-stars", "4-paths", "tailed-triangles", "4-cycles", "diamonds", "4-cliques")}
 

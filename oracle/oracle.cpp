@@ -451,4 +451,65 @@ int orc_motif(eid n, const eid *rp, const vid *ci, int k, int threads, u64 *out)
   return 0;
 }
 
+
+// k-cliques (k = 4, 5) of a DAG by bitset sub-DAGs -- a second CPU
+// algorithm (not a restatement of automine_4clique / automine_5clique) for
+// golden counts where the reference loop order takes CPU-days.  For each v0
+// the out-neighbours X = N+(v0) (sorted) get rows row_i = {j : X[j] in
+// N+(X[i])} as 64-bit words; K4(v0) = sum_{i, j in row_i} popc(row_i &
+// row_j), K5(v0) adds one more level through l in row_i & row_j.  Pinned
+// against the restatement, the reference's counts, brute force and closed
+// forms (tests/test_oracle.py).  Vertices [v_begin, v_end) only.
+uint64_t orc_clique_bitset(int64_t n, const int64_t *rp, const vid *ci, int k, int64_t v_begin, int64_t v_end,
+                           int threads) {
+  if (k != 4 && k != 5) return 0;
+  G g{n, rp, ci};
+  if (v_end < 0 || v_end > n) v_end = n;
+  set_threads(threads);
+  const eid md = max_degree(g);
+  const eid Wmax = (md + 63) / 64;
+  u64 count = 0;
+#pragma omp parallel reduction(+ : count)
+  {
+    std::vector<u64> rows((size_t)std::max<eid>(1, md) * std::max<eid>(1, Wmax)), t1(Wmax);
+#pragma omp for schedule(dynamic, 1)
+    for (eid v0 = v_begin; v0 < v_end; ++v0) {
+      const vid *X = g.N(v0);
+      const eid s = g.d(v0);
+      if (s < k - 1) continue;
+      const eid W = (s + 63) / 64;
+      std::fill(rows.begin(), rows.begin() + s * W, 0ull);
+      for (eid i = 0; i < s; ++i) {
+        const vid *Y = g.N(X[i]);
+        const eid dy = g.d(X[i]);
+        u64 *ri = rows.data() + i * W;
+        for (eid a = 0, b = 0; a < s && b < dy;) {  // positions of X ∩ N+(X[i]) in X
+          if (X[a] < Y[b]) ++a;
+          else if (Y[b] < X[a]) ++b;
+          else { ri[a >> 6] |= 1ull << (a & 63); ++a; ++b; }
+        }
+      }
+      for (eid i = 0; i < s; ++i) {
+        const u64 *ri = rows.data() + i * W;
+        for (eid wj = 0; wj < W; ++wj)
+          for (u64 bj = ri[wj]; bj; bj &= bj - 1) {
+            const eid j = wj * 64 + __builtin_ctzll(bj);
+            const u64 *rj = rows.data() + j * W;
+            if (k == 4) {
+              for (eid w = 0; w < W; ++w) count += __builtin_popcountll(ri[w] & rj[w]);
+              continue;
+            }
+            for (eid w = 0; w < W; ++w) t1[w] = ri[w] & rj[w];
+            for (eid wl = 0; wl < W; ++wl)
+              for (u64 bl = t1[wl]; bl; bl &= bl - 1) {
+                const u64 *rl = rows.data() + (wl * 64 + __builtin_ctzll(bl)) * W;
+                for (eid w = 0; w < W; ++w) count += __builtin_popcountll(t1[w] & rl[w]);
+              }
+          }
+      }
+    }
+  }
+  return count;
+}
+
 }  // extern "C"

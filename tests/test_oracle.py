@@ -254,6 +254,32 @@ def test_motif_restatement_matches_reference_binary(xgm, orc):
             assert orc.motif(g.rowptr, g.colidx, k) == orc.ref_motif_graph(g, k), (seed, k)
 
 
+def test_clique_bitset_matches_restatement(xgm, orc):
+    """The bitset clique counter (large golden counts) equals the automine
+    restatement, the reference's golden counts, brute force and K_n."""
+    from golden.make_golden import SMALL, make_graph
+    gold = json.load(open(os.path.join(GOLDEN, "counts.json")))
+    for name in ("rmat10_s7", "rmat12_s3", "er2000_d16_s5", "chunglu20k_s9", "rmat16_s1"):
+        d = make_graph(xgm, *SMALL[name][:2]).orient()
+        for k in (4, 5):
+            assert orc.clique_bitset(d.rowptr, d.colidx, k, threads=4) == gold[name]["counts"][f"clique{k}"], (name, k)
+    for seed in (11, 12, 13):
+        g = random_graph(xgm, 16, 60, seed)
+        d = g.orient()
+        b = _brute(g)
+        assert orc.clique_bitset(d.rowptr, d.colidx, 4) == b["clique4"]
+        assert orc.clique_bitset(d.rowptr, d.colidx, 5) == b["clique5"]
+    for n in (5, 9, 70, 130):   # 70 and 130 cross 64-bit words
+        d = complete_graph(xgm, n).orient()
+        assert orc.clique_bitset(d.rowptr, d.colidx, 4) == math.comb(n, 4)
+        assert orc.clique_bitset(d.rowptr, d.colidx, 5) == math.comb(n, 5)
+    d = random_graph(xgm, 3000, 60000, 43).orient()
+    for k in (4, 5):
+        assert orc.clique_bitset(d.rowptr, d.colidx, k) == orc.count(f"clique{k}", d.rowptr, d.colidx)[0]
+    c = xgm.load_csr(CITESEER).orient()
+    assert orc.clique_bitset(c.rowptr, c.colidx, 4) == 255 and orc.clique_bitset(c.rowptr, c.colidx, 5) == 46
+
+
 def test_algorithmic_bytes_tc_closed_form(xgm, orc):
     """B_TC = 4*sum_v d+(v)*(d+(v)+d-(v)) + 40*m on the DAG."""
     g = random_graph(xgm, 500, 5000, seed=5)
