@@ -495,6 +495,113 @@ __global__ __launch_bounds__(NT) void k_cycle4_wedge(const int64_t *__restrict__
   if (lane == 0 && acc) atomicAdd(out, acc);
 }
 
+// Packed wedge tasks (<= 128 wedges, <= 64 lower neighbours): a wave claims G
+// tasks with one atomic; their lower lists N(v) ∩ [0, x_g) form one segment
+// table (each cut at its own task's x) and c_x(w) goes into G LDS hash
+// regions of 256 slots, so the dependent loads of G tasks overlap.
+template <int G>
+struct WedgePack {
+  static constexpr int SLOTS = 256;  // <= 128 wedges at load <= 1/2
+  int key[G * SLOTS];
+  int val[G * SLOTS];
+  Segs<64 * G, true> seg;            // idx[] = task of the segment
+  int x[G], nl[G], off[G + 1];
+  int64_t bx[G];
+  u64 sum[G];
+};
+
+template <int G>
+__global__ __launch_bounds__(128) void k_cycle4_wedge_packed(const int64_t *__restrict__ rowptr,
+                                                             const int32_t *__restrict__ colidx,
+                                                             const int32_t *__restrict__ tasks, int ntasks,
+                                                             int *__restrict__ queue, u64 *__restrict__ out,
+                                                             const int32_t *__restrict__ slots) {
+  using PS = WedgePack<G>;
+  __shared__ PS stage[2];
+  __shared__ int flagrow[2][64];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  PS &S = stage[wave];
+  WaveFlags fl;
+  fl.init(flagrow[wave], lane);
+  u64 acc = 0;
+  for (;;) {
+    int base = 0;
+    if (lane == 0) base = atomicAdd(queue, G);
+    base = __shfl(base, 0, 64);
+    if (base >= ntasks) break;
+    const int ng = min(G, ntasks - base);
+    int nl = 0;
+    if (lane < ng) {
+      const int x = tasks[base + lane];
+      const int64_t b = rowptr[x];
+      nl = lower_bound_dev(colidx + b, (int)(rowptr[x + 1] - b), x);  // |N(x) ∩ [0, x)| <= 64
+      S.x[lane] = x;
+      S.bx[lane] = b;
+      S.nl[lane] = nl;
+      S.sum[lane] = 0;
+    }
+    const int inc = wave_incl_scan(nl, lane);
+    if (lane < ng) S.off[lane + 1] = inc;
+    if (lane == 0) S.off[0] = 0;
+    for (int q = lane; q < ng * PS::SLOTS; q += 64) {
+      S.key[q] = -1;
+      S.val[q] = 0;
+    }
+    wave_sync();
+    const int P = __shfl(inc, ng - 1, 64);
+    auto item_of = [&](int q) {
+      int g = 0;
+#pragma unroll
+      for (int h = 1; h < G; ++h) g += (h < ng && q >= S.off[h]) ? 1 : 0;
+      return g;
+    };
+    // one segment per lower neighbour v of x_g: N(v) ∩ [0, x_g)
+    int ns = 0, L = 0;
+    for (int k = 0; k < P; k += 64) {
+      const int q = k + lane;
+      int len = 0, g = 0;
+      int64_t vb = 0;
+      if (q < P) {
+        g = item_of(q);
+        const int v = colidx[S.bx[g] + (q - S.off[g])];
+        vb = rowptr[v];
+        len = lower_bound_dev(colidx + vb, (int)(rowptr[v + 1] - vb), S.x[g]);
+      }
+      const int incl = wave_incl_scan(len, lane) + L;
+      const u64 m = __ballot(len > 0);
+      if (len > 0) {
+        const int pos = ns + __popcll(m & lanemask_lt(lane));
+        S.seg.os[pos] = make_int2((int)vb, incl - len);
+        S.seg.idx[pos] = g;
+      }
+      ns += __popcll(m);
+      L = __shfl(incl, 63, 64);
+    }
+    if (lane == 0) S.seg.os[ns] = make_int2(0, L);
+    wave_sync();
+    u64 part = 0;
+    wave_walk<4>(S.seg, colidx, ns, 0, L, lane, fl, [&](const WindowLane &r) {
+      if (r.valid) {
+        const int g = S.seg.idx[r.j];
+        const int old = hash_bump(S.key + g * PS::SLOTS, S.val + g * PS::SLOTS, 8, r.w);
+        if (old) {
+          if (slots) atomicAdd(&S.sum[g], (u64)old);
+          else part += (u64)old;
+        }
+      }
+    });
+    if (!slots) {
+      acc += part;
+    } else {
+      wave_sync();
+      if (lane < ng && S.sum[lane]) atomicAdd(out + slots[S.x[lane]], S.sum[lane]);
+    }
+    wave_sync();  // stage is rewritten by the next claim
+  }
+  acc = wave_sum_u64(acc);
+  if (lane == 0 && acc) atomicAdd(out, acc);
+}
+
 // Range items of the wedge strategy: (x, r0, r1, -) counts the wedges of x
 // whose far end w lies in [r0, r1) (r1 - r0 <= R, r1 <= x), with c_x(w) in
 // R LDS counters indexed by w - r0.  The ranges of one task partition the
@@ -783,7 +890,7 @@ int cycle4_dense_grid(int num_cu, int64_t tasks) {
   return grid_of(k_cycle4_wedge<1, 1024, 256, true>, 256, num_cu, tasks, 1);
 }
 
-hipError_t launch_cycle4_wedge(const DevCSR &g, const TaskList tl[3], int *dense, const TaskList *hub_rounds,
+hipError_t launch_cycle4_wedge(const DevCSR &g, const TaskList tl[4], int *dense, const TaskList *hub_rounds,
                                int n_rounds, int *hub, size_t hub_round_u64, const TaskList &range_items,
                                const CounterRows &rows, Launch &l, int *d_queues) {
   hipError_t e;
@@ -824,6 +931,16 @@ hipError_t launch_cycle4_wedge(const DevCSR &g, const TaskList tl[3], int *dense
   }
   if ((e = run_wedge_lds<8192, 1024, 1024>("cycle4_wedge_8192", g, tl[1], d_queues + 1, l))) return e;
   if ((e = run_wedge_lds<1024, 256, 64>("cycle4_wedge_1024", g, tl[0], d_queues + 0, l))) return e;
+  if (tl[3].count) {  // <= 128 wedges and <= 64 lower neighbours: 8 tasks per wave
+    constexpr int kPack = 8;
+    auto k = k_cycle4_wedge_packed<kPack>;
+    const int grid = grid_of(k, 128, l.num_cu, (tl[3].count + 2 * kPack - 1) / (2 * kPack));
+    l.begin("cycle4_wedge_packed", tl[3].count);
+    k<<<grid, 128, 0, l.stream>>>(g.rowptr, g.colidx, tl[3].tasks, (int)tl[3].count, d_queues + 7, l.d_out,
+                                  l.slots);
+    l.end();
+    if ((e = hipGetLastError())) return e;
+  }
   return hipSuccess;
 }
 

@@ -106,15 +106,16 @@ struct CounterRows {
 };
 
 // Edge-induced 4-cycle, wedge aggregation over the (degree, id)-relabelled
-// CSR; vertex tasks by wedge work (<=512, <=4096, larger); dense holds
+// CSR; vertex tasks by wedge work (tl[0] <= 512, tl[1] <= 4096, tl[2]
+// larger, tl[3] <= 128 wedges with <= 64 lower neighbours, packed); dense holds
 // cycle4_dense_grid() rows, all zero on entry (left zero on exit).
 // hub_rounds[r] are int4 (x, k0, k1, slot) slices of hub tasks adding into
 // row `slot` of hub; the first hub_round_u64 words of hub are zeroed before
-// every round.  Uses queue counters 0-2 and 4-6 of d_queues.
+// every round.  Uses queue counters 0-2 and 4-7 of d_queues.
 // range_items are int4 (x, r0, r1, -) with r1 - r0 <= kRangeCounters: the
 // wedges of x ending in [r0, r1), counted in LDS.  Queue counter 6.
 constexpr int kRangeCounters = 32768;
-hipError_t launch_cycle4_wedge(const DevCSR &g, const TaskList tl[3], int *dense, const TaskList *hub_rounds,
+hipError_t launch_cycle4_wedge(const DevCSR &g, const TaskList tl[4], int *dense, const TaskList *hub_rounds,
                                int n_rounds, int *hub, size_t hub_round_u64, const TaskList &range_items,
                                const CounterRows &rows, Launch &l, int *d_queues);
 int cycle4_dense_grid(int num_cu, int64_t tasks);

@@ -896,7 +896,7 @@ static int dispatch_pattern(xgm_ctx *ctx, const xgm_dgraph *g, int pattern, cons
       const std::string sfx = "_" + std::to_string(rmode);
       const std::string hkey = "cycle4_hubs_" + std::to_string(rows_max) + sfx;
       const std::string wkey = "cycle4_wedge" + sfx, rkey = "cycle4_range" + sfx;
-      std::vector<std::vector<int32_t>> parts(3), hubp(1), rngp(1);
+      std::vector<std::vector<int32_t>> parts(4), hubp(1), rngp(1);
       if (!cached(gm, wkey) || !cached(gm, hkey) || !cached(gm, rkey)) {
         const auto &rp = gm->rl_rowptr;
         const auto &ci = gm->rl_colidx;
@@ -914,8 +914,15 @@ static int dispatch_pattern(xgm_ctx *ctx, const xgm_dgraph *g, int pattern, cons
               continue;
             }
           }
-          if (w > kHubWork) hubs.push_back(x);
-          else parts[w <= 512 ? 0 : w <= 4096 ? 1 : 2].push_back(x);
+          if (w > kHubWork) {
+            hubs.push_back(x);
+          } else if (w <= 128) {
+            const int32_t *nx = ci.data() + rp[x];
+            const int64_t nlow = std::lower_bound(nx, nx + (rp[x + 1] - rp[x]), x) - nx;
+            parts[nlow <= 64 ? 3 : 0].push_back(x);
+          } else {
+            parts[w <= 512 ? 0 : w <= 4096 ? 1 : 2].push_back(x);
+          }
         }
         const int64_t R = std::min<int64_t>((int64_t)hubs.size(), rows_max);
         if (R > 0) hubp.assign((hubs.size() + R - 1) / R, {});
@@ -946,7 +953,7 @@ static int dispatch_pattern(xgm_ctx *ctx, const xgm_dgraph *g, int pattern, cons
       if ((rc = cache_parts(ctx, gm, wkey, parts, 1, tl))) return rc;
       if ((rc = cache_parts(ctx, gm, hkey, hubp, 4, hl))) return rc;
       if ((rc = cache_parts(ctx, gm, rkey, rngp, 4, rl))) return rc;
-      TaskList t3[3] = {tl[0], tl[1], tl[2]};
+      TaskList t3[4] = {tl[0], tl[1], tl[2], tl[3]};
       const int64_t hub_rows = std::min<int64_t>(nhubs, rows_max);
       const size_t hub_round_u64 = ((size_t)hub_rows * (size_t)stride + 1) / 2;
       if (nhubs) {
