@@ -156,12 +156,8 @@ __device__ u64 block_lut_count(const int64_t *__restrict__ rowptr, const int32_t
   const int weff = (s + 63) >> 6;
   for (int q = tid; q < s * W; q += NT) S.rows[q] = 0ull;
   __syncthreads();
-  if (wave == 0) {
-    int ns, L;
-    wave_segments(S.seg, rowptr, colidx, S.x, 0, s, -1, KeepAll(), lane, ns, L);
-    if (lane == 0) { S.ns = ns; S.L = L; }
-  }
-  __syncthreads();
+  // every thread loads its members' list bounds; x2 (free here) is scratch
+  block_segments<NT>(S.seg, S.x2, rowptr, colidx, S.x, 0, s, -1, KeepAll(), &S.ns, &S.L);
   int lo, hi;
   wave_share(S.L, wave, NT / 64, lo, hi);
   wave_walk<4>(S.seg, colidx, S.ns, lo, hi, lane, fl, [&](const WindowLane &r) {
@@ -266,6 +262,7 @@ struct RStage {
     };
     Segs<SMAX> seg;
   };
+  int tmp[SMAX];  // scratch of the block-cooperative segment build
   int flagrow[NT / 64][64];
   int ns, L, item;
 };
@@ -307,12 +304,7 @@ __global__ __launch_bounds__(NT) void k_clique_vlut_rank(const int64_t *__restri
       block_bitonic_sort<SMAX, NT>(S.key, S.val, s);
       for (int r = tid; r < s; r += NT) S.perm[S.val[r]] = r;
       __syncthreads();  // keys are overwritten by the segment table
-      if (wave == 0) {
-        int ns, L;
-        wave_segments(S.seg, rowptr, colidx, S.x, 0, s, -1, KeepAll(), lane, ns, L);
-        if (lane == 0) { S.ns = ns; S.L = L; }
-      }
-      __syncthreads();
+      block_segments<NT>(S.seg, S.tmp, rowptr, colidx, S.x, 0, s, -1, KeepAll(), &S.ns, &S.L);
       int lo, hi;
       wave_share(S.L, wave, NT / 64, lo, hi);
       wave_walk<4>(S.seg, colidx, S.ns, lo, hi, lane, fl, [&](const WindowLane &r) {

@@ -77,13 +77,8 @@ __global__ __launch_bounds__(NT) void k_diamond(const int64_t *__restrict__ rowp
     auto keep = [dt, v](int p, int lp) { return lp < dt || (lp == dt && p < v); };
     for (int k0 = it.y; k0 < it.z; k0 += PCAP) {
       const int k1 = min(it.z, k0 + PCAP);
-      __syncthreads();
-      if (wave == 0) {
-        int ns, L;
-        wave_segments(S.seg, rowptr, colidx, nt, k0, k1, -1, keep, lane, ns, L);
-        if (lane == 0) { S.ns = ns; S.L = L; }
-      }
-      __syncthreads();
+      __syncthreads();  // previous piece consumed; S.cnt is the build's scratch
+      block_segments<NT>(S.seg, S.cnt, rowptr, colidx, nt, k0, k1, -1, keep, &S.ns, &S.L);
       const int ns = S.ns, L = S.L;
       for (int q = tid; q < ns; q += NT) S.cnt[q] = 0;
       __syncthreads();
@@ -194,13 +189,8 @@ __global__ __launch_bounds__(NT) void k_diamond_big(const int64_t *__restrict__ 
     auto keep = [dt, v](int p, int lp) { return lp < dt || (lp == dt && p < v); };
     for (int k0 = it.y; k0 < it.z; k0 += PCAP) {
       const int k1 = min(it.z, k0 + PCAP);
-      __syncthreads();  // staged set written / previous piece consumed
-      if (wave == 0) {
-        int ns, L;
-        wave_segments(S.seg, rowptr, colidx, nt, k0, k1, -1, keep, lane, ns, L);
-        if (lane == 0) { S.ns = ns; S.L = L; }
-      }
-      __syncthreads();
+      __syncthreads();  // staged set written / previous piece consumed; S.cnt is scratch
+      block_segments<NT>(S.seg, S.cnt, rowptr, colidx, nt, k0, k1, -1, keep, &S.ns, &S.L);
       const int ns = S.ns, L = S.L;
       for (int q = tid; q < ns; q += NT) S.cnt[q] = 0;
       __syncthreads();
@@ -393,6 +383,7 @@ struct WedgeStage {
   int key[HSLOTS];
   int val[HSLOTS];
   Segs<PCAP, false> seg;
+  int tmp[PCAP];  // scratch of the block-cooperative segment build
   int ns, L, task;
 };
 
@@ -451,12 +442,7 @@ __global__ __launch_bounds__(NT) void k_cycle4_wedge(const int64_t *__restrict__
     for (int k0 = 0; k0 < nlow; k0 += PCAP) {
       const int k1 = min(nlow, k0 + PCAP);
       __syncthreads();
-      if (wave == 0) {
-        int ns, L;
-        wave_segments(S.seg, rowptr, colidx, nx, k0, k1, x, KeepAll(), lane, ns, L);
-        if (lane == 0) { S.ns = ns; S.L = L; }
-      }
-      __syncthreads();
+      block_segments<NT>(S.seg, S.tmp, rowptr, colidx, nx, k0, k1, x, KeepAll(), &S.ns, &S.L);
       const int ns = S.ns, L = S.L;
       int lo, hi;
       wave_share(L, wave, NT / 64, lo, hi);
@@ -473,12 +459,7 @@ __global__ __launch_bounds__(NT) void k_cycle4_wedge(const int64_t *__restrict__
       for (int k0 = 0; k0 < nlow; k0 += PCAP) {
         const int k1 = min(nlow, k0 + PCAP);
         __syncthreads();
-        if (wave == 0) {
-          int ns, L;
-          wave_segments(S.seg, rowptr, colidx, nx, k0, k1, x, KeepAll(), lane, ns, L);
-          if (lane == 0) { S.ns = ns; S.L = L; }
-        }
-        __syncthreads();
+        block_segments<NT>(S.seg, S.tmp, rowptr, colidx, nx, k0, k1, x, KeepAll(), &S.ns, &S.L);
         const int ns = S.ns, L = S.L;
         int lo, hi;
         wave_share(L, wave, NT / 64, lo, hi);
@@ -929,7 +910,9 @@ hipError_t launch_cycle4_wedge(const DevCSR &g, const TaskList tl[4], int *dense
     l.end();
     if ((e = hipGetLastError())) return e;
   }
-  if ((e = run_wedge_lds<8192, 1024, 1024>("cycle4_wedge_8192", g, tl[1], d_queues + 1, l))) return e;
+  // 1000-partner pieces keep the 8192-slot stage (with its build scratch)
+  // at two blocks per CU
+  if ((e = run_wedge_lds<8192, 1000, 1024>("cycle4_wedge_8192", g, tl[1], d_queues + 1, l))) return e;
   if ((e = run_wedge_lds<1024, 256, 64>("cycle4_wedge_1024", g, tl[0], d_queues + 0, l))) return e;
   if (tl[3].count) {  // <= 128 wedges and <= 64 lower neighbours: 8 tasks per wave
     constexpr int kPack = 8;
