@@ -31,15 +31,28 @@ namespace xgm {
 namespace {
 
 // ------------------------------------------------------------- LUT engine
-// Count cliques of `depth` vertices inside the sub-DAG encoded by rows
-// (s rows of W 64-bit words; only the first weff words are non-zero).
+// LDS row layout: row r starts at r * W + r / 8 (W >= 16), so eight rows
+// share one padding word.  The counting loops have every lane of a wave
+// read the same word index w of a different row; with rows of 8W bytes
+// back to back those reads all land in one LDS bank.  The per-row shift
+// spreads them over the banks and keeps w a constant offset from the row
+// base.  lut_words(s) is the footprint of s rows.
+template <int W>
+__device__ __forceinline__ int lut_word(int r, int w) {
+  return r * W + (W >= 16 ? (r >> 3) : 0) + w;
+}
+template <int W>
+__host__ __device__ constexpr int lut_words(int s) {
+  return s * W + (W >= 16 ? (s + 7) >> 3 : 0);
+}
+
 template <int W>
 __device__ __forceinline__ u64 lut_count(const u64 *rows, int s, int weff, int depth, int tid, int nthreads) {
   u64 acc = 0;
   const int items = s * weff;
   for (int t = tid; t < items; t += nthreads) {
     const int i = t / weff, q = t - i * weff;
-    u64 bits = rows[i * W + q];
+    u64 bits = rows[lut_word<W>(i, q)];
     if (depth == 2) {
       acc += __popcll(bits);
       continue;
@@ -50,11 +63,11 @@ __device__ __forceinline__ u64 lut_count(const u64 *rows, int s, int weff, int d
       if (depth == 3) {
 #pragma unroll
         for (int w = 0; w < W; ++w)
-          if (w < weff) acc += __popcll(rows[i * W + w] & rows[j * W + w]);
+          if (w < weff) acc += __popcll(rows[lut_word<W>(i, w)] & rows[lut_word<W>(j, w)]);
       } else {
         u64 tw[W];
 #pragma unroll
-        for (int w = 0; w < W; ++w) tw[w] = (w < weff) ? (rows[i * W + w] & rows[j * W + w]) : 0ull;
+        for (int w = 0; w < W; ++w) tw[w] = (w < weff) ? (rows[lut_word<W>(i, w)] & rows[lut_word<W>(j, w)]) : 0ull;
 #pragma unroll
         for (int w2 = 0; w2 < W; ++w2) {
           u64 tb = tw[w2];
@@ -63,7 +76,7 @@ __device__ __forceinline__ u64 lut_count(const u64 *rows, int s, int weff, int d
             tb &= tb - 1;
 #pragma unroll
             for (int w = 0; w < W; ++w)
-              if (w < weff) acc += __popcll(tw[w] & rows[k * W + w]);
+              if (w < weff) acc += __popcll(tw[w] & rows[lut_word<W>(k, w)]);
           }
         }
       }
@@ -79,7 +92,7 @@ template <int SMAX, int W, int NT>
 struct CStage {
   int x[SMAX];
   int x2[SMAX];
-  u64 rows[W ? SMAX * W : 1];
+  u64 rows[W ? lut_words<W>(SMAX) : 1];
   Segs<SMAX> seg;
   int cnt[NT / 64];
   int flagrow[NT / 64][64];
@@ -154,7 +167,7 @@ __device__ u64 block_lut_count(const int64_t *__restrict__ rowptr, const int32_t
                                CStage<SMAX, W, NT> &S, int s, int depth, WaveFlags &fl) {
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int weff = (s + 63) >> 6;
-  for (int q = tid; q < s * W; q += NT) S.rows[q] = 0ull;
+  for (int q = tid; q < lut_words<W>(s); q += NT) S.rows[q] = 0ull;
   __syncthreads();
   // every thread loads its members' list bounds; x2 (free here) is scratch
   block_segments<NT>(S.seg, S.x2, rowptr, colidx, S.x, 0, s, -1, KeepAll(), &S.ns, &S.L);
@@ -163,7 +176,7 @@ __device__ u64 block_lut_count(const int64_t *__restrict__ rowptr, const int32_t
   wave_walk<4>(S.seg, colidx, S.ns, lo, hi, lane, fl, [&](const WindowLane &r) {
     if (r.valid) {
       const int pos = lower_bound_dev(S.x, s, r.w);
-      if (pos < s && S.x[pos] == r.w) atomicOr(&S.rows[S.seg.idx[r.j] * W + (pos >> 6)], 1ull << (pos & 63));
+      if (pos < s && S.x[pos] == r.w) atomicOr(&S.rows[lut_word<W>(S.seg.idx[r.j], pos >> 6)], 1ull << (pos & 63));
     }
   });
   __syncthreads();
@@ -183,7 +196,7 @@ __device__ __forceinline__ u64 lut_count_tri(const u64 *rows, int s, int weff, i
   for (int t = tid; t < items; t += nthreads) {
     const int a = t / weff, q = t - a * weff;
     if (q < (a >> 6)) continue;
-    u64 bits = rows[a * W + q];
+    u64 bits = rows[lut_word<W>(a, q)];
     if (depth == 2) {
       acc += __popcll(bits);
       continue;
@@ -195,11 +208,11 @@ __device__ __forceinline__ u64 lut_count_tri(const u64 *rows, int s, int weff, i
       if (depth == 3) {
 #pragma unroll
         for (int w = 0; w < W; ++w)
-          if (w >= ws && w < weff) acc += __popcll(rows[a * W + w] & rows[j * W + w]);
+          if (w >= ws && w < weff) acc += __popcll(rows[lut_word<W>(a, w)] & rows[lut_word<W>(j, w)]);
       } else {
         u64 tw[W];
 #pragma unroll
-        for (int w = 0; w < W; ++w) tw[w] = (w >= ws && w < weff) ? (rows[a * W + w] & rows[j * W + w]) : 0ull;
+        for (int w = 0; w < W; ++w) tw[w] = (w >= ws && w < weff) ? (rows[lut_word<W>(a, w)] & rows[lut_word<W>(j, w)]) : 0ull;
 #pragma unroll
         for (int w2 = 0; w2 < W; ++w2) {
           u64 tb = tw[w2];
@@ -209,7 +222,7 @@ __device__ __forceinline__ u64 lut_count_tri(const u64 *rows, int s, int weff, i
             const int ks = (k + 1) >> 6;
 #pragma unroll
             for (int w = 0; w < W; ++w)
-              if (w >= ks && w < weff) acc += __popcll(tw[w] & rows[k * W + w]);
+              if (w >= ks && w < weff) acc += __popcll(tw[w] & rows[lut_word<W>(k, w)]);
           }
         }
       }
@@ -254,7 +267,7 @@ template <int SMAX, int W, int NT>
 struct RStage {
   int x[SMAX];
   int perm[SMAX];
-  u64 rows[SMAX * W];
+  u64 rows[lut_words<W>(SMAX)];
   union {  // the sort keys are dead once perm is built, before the segments
     struct {
       u64 key[SMAX];
@@ -299,7 +312,7 @@ __global__ __launch_bounds__(NT) void k_clique_vlut_rank(const int64_t *__restri
         S.key[q] = (deg << 32) | (u64)(uint32_t)x;  // (undirected degree, id): the orientation order
         S.val[q] = q;
       }
-      for (int q = tid; q < s * W; q += NT) S.rows[q] = 0ull;
+      for (int q = tid; q < lut_words<W>(s); q += NT) S.rows[q] = 0ull;
       __syncthreads();
       block_bitonic_sort<SMAX, NT>(S.key, S.val, s);
       for (int r = tid; r < s; r += NT) S.perm[S.val[r]] = r;
@@ -313,7 +326,7 @@ __global__ __launch_bounds__(NT) void k_clique_vlut_rank(const int64_t *__restri
           if (pos < s && S.x[pos] == r.w) {
             const int col = S.perm[pos], row = S.perm[S.seg.idx[r.j]];
             if (col <= row) atomicOr(err, 1ull);  // DAG not in (degree, id) orientation
-            atomicOr(&S.rows[row * W + (col >> 6)], 1ull << (col & 63));
+            atomicOr(&S.rows[lut_word<W>(row, col >> 6)], 1ull << (col & 63));
           }
         }
       });
@@ -359,6 +372,46 @@ __device__ __forceinline__ u64 lut_count_tri_wide(const u64 *__restrict__ rows, 
             const u64 *rk = rows + (size_t)k * weff;
             for (int w = (k + 1) >> 6; w < weff; ++w) acc += __popcll(ra[w] & rj[w] & rk[w]);
           }
+        }
+      }
+    }
+  }
+  return acc;
+}
+
+// 4-clique count over HBM rows of weff <= W words: the pair mask
+// rows[a] & rows[j] is held in registers (W words, statically indexed), so
+// the innermost loop loads only rows[k].  Same item order as
+// lut_count_tri_wide.
+template <int W>
+__device__ __forceinline__ u64 lut_count_tri4_reg(const u64 *__restrict__ rows, int s, int weff, int tid,
+                                                  int nthreads) {
+  u64 acc = 0;
+  const int items = s * weff;
+  for (int t = tid; t < items; t += nthreads) {
+    const int a = t / weff, q = t - a * weff;
+    if (q < (a >> 6)) continue;
+    const u64 *ra = rows + (size_t)a * weff;
+    u64 bits = ra[q];
+    while (bits) {
+      const int j = q * 64 + __builtin_ctzll(bits);
+      bits &= bits - 1;
+      const u64 *rj = rows + (size_t)j * weff;
+      const int ws = (j + 1) >> 6;
+      u64 tw[W];
+#pragma unroll
+      for (int w = 0; w < W; ++w) tw[w] = (w >= ws && w < weff) ? (ra[w] & rj[w]) : 0ull;
+#pragma unroll
+      for (int w2 = 0; w2 < W; ++w2) {
+        u64 tb = tw[w2];
+        while (tb) {
+          const int k = w2 * 64 + __builtin_ctzll(tb);
+          tb &= tb - 1;
+          const u64 *rk = rows + (size_t)k * weff;
+          const int ks = (k + 1) >> 6;
+#pragma unroll
+          for (int w = 0; w < W; ++w)
+            if (w >= ks && w < weff) acc += __popcll(tw[w] & rk[w]);
         }
       }
     }
@@ -447,7 +500,10 @@ __global__ __launch_bounds__(NT) void k_clique_vlut_rank_hbm(const int64_t *__re
       // lines cached by the zero fill or an earlier task read stale.
       __threadfence();
       __syncthreads();
-      part += lut_count_tri_wide(rows, s, weff, depth, tid, NT);
+      if (depth == 4 && weff <= 32)
+        part += lut_count_tri4_reg<32>(rows, s, weff, tid, NT);
+      else
+        part += lut_count_tri_wide(rows, s, weff, depth, tid, NT);
     }
     task_flush(acc, part, slots, v0, out, tid & 63);
     __syncthreads();
