@@ -292,6 +292,50 @@ def measure_traffic(args, dom):
     return int(out["FETCH_SIZE"]["bytes"] + out["WRITE_SIZE"]["bytes"]), out
 
 
+def bench_prep(args, xgm, g, m, desc):
+    """Undirected host CSR -> device DAG ready to count, two ways: host
+    orientation (xgm_orient_dag) then xgm_graph_upload (host task order,
+    reverse lists, H2D), against xgm_graph_upload of the undirected CSR then
+    xgm_graph_orient (kernels in HBM, host mirrors copied back).  Wall time
+    per leg, median over --steps after --warmup; the two DAGs are checked
+    equal (CSR hash and task order)."""
+    ctx = xgm.Context(0)
+    legs = {"host_orient_upload": [], "upload_undirected": [], "device_orient": []}
+    for i in range(args.warmup + args.steps):
+        t0 = time.perf_counter()
+        d = g.orient()
+        dg = ctx.upload(d)
+        t1 = time.perf_counter()
+        du = ctx.upload(g)
+        t2 = time.perf_counter()
+        dd = ctx.orient(du)
+        t3 = time.perf_counter()
+        if i == 0:
+            if dd.download().hash() != d.hash() or not np.array_equal(dd.task_order(), dg.task_order()):
+                raise SystemExit("device orientation differs from the host orientation")
+        if i >= args.warmup:
+            legs["host_orient_upload"].append((t1 - t0) * 1e3)
+            legs["upload_undirected"].append((t2 - t1) * 1e3)
+            legs["device_orient"].append((t3 - t2) * 1e3)
+        log(f"step {i}: host {1e3 * (t1 - t0):.1f} ms, upload {1e3 * (t2 - t1):.1f} ms, "
+            f"device orient {1e3 * (t3 - t2):.1f} ms")
+        for x in (dd, du, dg):
+            x.close()
+        del d
+    med = {k: round(float(np.median(v)), 3) for k, v in legs.items()}
+    dev_total = med["upload_undirected"] + med["device_orient"]
+    print(json.dumps({
+        "metric": f"host CSR to device DAG on {args.workload}, ms (orientation, reverse lists, task order)",
+        "value": round(dev_total, 3), "unit": "ms", "higher_is_better": False, "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup,
+        "config": {"workload": desc, "n": g.n, "undirected_edges": m},
+        "legs_ms": med, "host_path_ms": med["host_orient_upload"],
+        "speedup_vs_host_path": round(med["host_orient_upload"] / dev_total, 3),
+        "host_threads": int(os.environ.get("XGM_NUM_THREADS", 0)) or (os.cpu_count() or 1),
+    }), flush=True)
+    ctx.close()
+
+
 def bench_sched(args, xgm, d, m, desc, golden):
     """Learned vs heuristic clique schedules on one graph: the learned
     schedule is timed over the K steps; the heuristic (and the all-rank-LUT
@@ -379,6 +423,8 @@ def main():
     ap.add_argument("--sched", default="", choices=["", "compare"],
                     help="clique patterns: time the learned schedule against the heuristic and rank-LUT ones")
     ap.add_argument("--baseline-steps", type=int, default=1)
+    ap.add_argument("--prep", default="", choices=["", "compare"],
+                    help="time host vs on-device orientation of the workload graph into a device DAG")
     ap.add_argument("--copy-calibrate", action="store_true",
                     help="only run the plain copy kernel over 1 GiB (counter calibration)")
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
@@ -417,6 +463,8 @@ def main():
 
     if args.sched == "compare":
         return bench_sched(args, xgm, d, m, desc, golden)
+    if args.prep == "compare":
+        return bench_prep(args, xgm, g, m, desc)
 
     # xgm_count is synchronous on the library's own stream (it waits for its
     # kernels and the D2H copy of the count), so no extra device barrier is

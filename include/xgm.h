@@ -98,6 +98,23 @@ const char *xgm_ctx_last_error(const xgm_ctx *ctx);
 int xgm_graph_upload(xgm_ctx *ctx, const xgm_csr *g, xgm_dgraph **out);
 void xgm_graph_free(xgm_dgraph *g);
 
+/* Degree-orient an uploaded undirected graph (sorted, symmetric, not a
+ * batch) in HBM: the DAG, its reverse (in-neighbour) lists and its vertex
+ * task order are built by kernels, with no host pass over the edges; the
+ * host mirrors used for task preparation are copied back.  *out equals
+ * xgm_graph_upload() of xgm_orient_dag() of the same CSR.  Replaces
+ * Graph::orientation (codes/GraphMiner/src/common/graph.cc:233-279) and
+ * Graph_V2::orientation (xgminer/src/graph_v2.cpp:243-289). */
+int xgm_graph_orient(xgm_ctx *ctx, const xgm_dgraph *und, xgm_dgraph **out);
+
+/* Copy a device graph's CSR from HBM into a new host graph (flags kept). */
+int xgm_graph_download(xgm_ctx *ctx, const xgm_dgraph *g, xgm_hgraph **out);
+
+/* Vertex task order of a DAG device graph: vertices with d+ >= 2 by
+ * decreasing sum over N+(u) of d+, ties by increasing id.  Writes
+ * min(cap, len) ids to out; *len receives the full length. */
+int xgm_graph_task_order(const xgm_dgraph *g, int32_t *out, int64_t cap, int64_t *len);
+
 /* Count `pattern` on a device graph; *out receives the uint64 count.
  * opts may be NULL (auto).  Synchronous. */
 int xgm_count(xgm_ctx *ctx, const xgm_dgraph *g, int pattern, const xgm_opts *opts,

@@ -34,6 +34,9 @@ def test_strerror(xgm):
 def test_null_arguments_rejected(xgm):
     assert xgm.lib.xgm_count(None, None, 1, None, None) == xgm.XGM_EINVAL
     assert xgm.lib.xgm_graph_upload(None, None, None) == xgm.XGM_EINVAL
+    assert xgm.lib.xgm_graph_orient(None, None, None) == xgm.XGM_EINVAL
+    assert xgm.lib.xgm_graph_download(None, None, None) == xgm.XGM_EINVAL
+    assert xgm.lib.xgm_graph_task_order(None, None, 0, None) == xgm.XGM_EINVAL
     assert xgm.lib.xgm_gen_rmat(0, 16, 0.57, 0.19, 0.19, 1, None) == xgm.XGM_EINVAL
     with pytest.raises(xgm.XGMError):
         xgm.gen_rmat(12, 16, 0.9, 0.9, 0.9, 1)  # probabilities sum > 1

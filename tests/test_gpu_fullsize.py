@@ -37,7 +37,16 @@ def test_rmat22_full(xgm, ctx):
     _say("rmat22 generate")
     g = workloads.rmat(xgm, 22, 2)
     assert f"{g.hash():016x}" == gold["hash"]
-    dg = ctx.upload(g.orient())
+    d = g.orient()
+    dg = ctx.upload(d)
+    _say("rmat22 device orientation")
+    du = ctx.upload(g)
+    dd = ctx.orient(du)
+    assert dd.download().hash() == d.hash()
+    assert (dd.task_order() == dg.task_order()).all()
+    assert ctx.count(dd, 1) == gold["counts"]["triangle"]
+    dd.close()
+    du.close()
     _say("rmat22 triangle")
     assert ctx.count(dg, 1) == gold["counts"]["triangle"]
     _say("rmat22 clique4")

@@ -1,0 +1,95 @@
+"""On-device orientation (xgm_graph_orient): the DAG built in HBM equals the
+host orientation (GraphMiner Graph::orientation,
+codes/GraphMiner/src/common/graph.cc:233-279) array for array, its task
+order equals the one built for the uploaded host DAG, and every DAG count
+on it equals the oracle's."""
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import CITESEER, complete_bipartite, complete_graph, random_graph
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(*a):
+    print("[case]", *a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def ctx(xgm):
+    c = xgm.Context(0)
+    yield c
+    c.close()
+
+
+def _graphs(xgm):
+    star_src = np.zeros(300, np.int32)
+    star_dst = np.arange(1, 301, dtype=np.int32)
+    return {
+        "citeseer": xgm.load_csr(CITESEER),
+        "empty": xgm.from_edges(10, np.zeros(0, np.int32), np.zeros(0, np.int32)),
+        "path": xgm.from_edges(4, [0, 1, 2], [1, 2, 3]),
+        "k9": complete_graph(xgm, 9),           # equal degrees: id tie-break everywhere
+        "k5_7": complete_bipartite(xgm, 5, 7),
+        "star300": xgm.from_edges(301, star_src, star_dst),  # one hub, 300 leaves (list > 64)
+        "random": random_graph(xgm, 3000, 40000, 5),
+        "rmat12": xgm.gen_rmat(12, 16, 0.57, 0.19, 0.19, 3),
+        "chunglu": xgm.gen_chung_lu(20000, 12.0, 2.1, 2000.0, 7),
+    }
+
+
+def test_orient_equals_host(xgm, orc, ctx):
+    for name, g in _graphs(xgm).items():
+        _say("orient", name)
+        want = g.orient()
+        dd = ctx.orient(ctx.upload(g))
+        got = dd.download()
+        assert got.flags == want.flags, name
+        assert np.array_equal(got.rowptr, want.rowptr), name
+        assert np.array_equal(got.colidx, want.colidx), name
+        ref = ctx.upload(want)
+        assert np.array_equal(dd.task_order(), ref.task_order()), name
+        for pid, pname in ((xgm.TRIANGLE, "triangle"), (xgm.CLIQUE4, "clique4"), (xgm.CLIQUE5, "clique5")):
+            c = ctx.count(dd, pid)
+            assert c == orc.count(pname, want.rowptr, want.colidx)[0], (name, pname)
+            if pid != xgm.TRIANGLE:  # the rank-ordered LUT reads the reverse lists
+                assert ctx.count(dd, pid, xgm.STRAT_LUT_RANK) == c, (name, pname)
+        dd.close()
+        ref.close()
+
+
+def test_orient_rejects(xgm, ctx):
+    _say("orient rejects")
+    g = xgm.gen_rmat(10, 8, 0.57, 0.19, 0.19, 1)
+    dag = ctx.upload(g.orient())
+    with pytest.raises(xgm.XGMError):
+        ctx.orient(dag)  # already a DAG
+    with pytest.raises(xgm.XGMError):
+        ctx.orient(ctx.upload_batch([g, g]))
+
+
+def test_orient_rmat18_hash(xgm, ctx):
+    _say("orient rmat18")
+    g = xgm.gen_rmat(18, 16, 0.57, 0.19, 0.19, 2)
+    want = g.orient()
+    dd = ctx.orient(ctx.upload(g))
+    assert dd.download().hash() == want.hash()
+    assert ctx.count(dd, xgm.TRIANGLE) == ctx.count(want, xgm.TRIANGLE)
+
+
+def test_upload_rejects_asymmetric(xgm, ctx):
+    """The symmetry of an undirected upload is checked on the device: an
+    entry above the diagonal without its mirror, and one below it."""
+    _say("upload asymmetric")
+    flags = xgm.CSR_SORTED | xgm.CSR_SYMMETRIC
+    for rowptr, colidx in (([0, 1, 1], [1]), ([0, 0, 1], [0]), ([0, 2, 3, 4], [1, 2, 0, 1])):
+        g = xgm.Graph.from_arrays(np.array(rowptr), np.array(colidx), flags)
+        assert not g.validate()
+        with pytest.raises(xgm.XGMError):
+            ctx.upload(g)
+    ok = xgm.Graph.from_arrays(np.array([0, 2, 3, 4]), np.array([1, 2, 0, 0]), flags)
+    assert ok.validate()
+    assert ctx.count(ctx.upload(ok), xgm.DIAMOND_EDGE) == 0

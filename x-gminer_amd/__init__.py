@@ -103,6 +103,9 @@ def _load():
         "xgm_ctx_last_error": (C.c_char_p, [VP]),
         "xgm_graph_upload": (C.c_int, [VP, P(_Csr), P(VP)]),
         "xgm_graph_free": (None, [VP]),
+        "xgm_graph_orient": (C.c_int, [VP, VP, P(VP)]),
+        "xgm_graph_download": (C.c_int, [VP, VP, P(VP)]),
+        "xgm_graph_task_order": (C.c_int, [VP, P(I32), I64, P(I64)]),
         "xgm_count": (C.c_int, [VP, VP, C.c_int, P(_Opts), P(U64)]),
         "xgm_count_csr": (C.c_int, [VP, P(_Csr), C.c_int, P(_Opts), P(U64)]),
         "xgm_last_profile": (C.c_int, [VP, P(_Profile)]),
@@ -286,6 +289,15 @@ class Context:
     def upload(self, g: Graph) -> "DeviceGraph":
         return DeviceGraph(self, g)
 
+    def orient(self, dg: "DeviceGraph") -> "DeviceGraph":
+        """Degree-oriented DAG of an uploaded undirected graph, built in HBM
+        (equal to ``upload(g.orient())``)."""
+        h = C.c_void_p()
+        _check(lib.xgm_graph_orient(self._h, dg._h, C.byref(h)), self._h)
+        d = DeviceGraph(self, None, handle=h)
+        d.n = dg.n
+        return d
+
     def upload_batch(self, graphs) -> "DeviceGraph":
         """Disjoint union of graphs sharing one kind (DAG or undirected)."""
         return DeviceGraph(self, list(graphs))
@@ -452,22 +464,48 @@ class DeviceGraph:
                                      codes.ctypes.data_as(C.POINTER(C.c_int8))), self.ctx._h)
         return codes
 
-    def __init__(self, ctx: Context, g):
+    def __init__(self, ctx: Context, g, handle=None):
         h = C.c_void_p()
-        if isinstance(g, list):  # batch of instances (disjoint union)
+        if handle is not None:  # built on the device (Context.orient)
+            h = handle
+            self.instances = None
+            self._host = None
+            self.n = 0
+        elif isinstance(g, list):  # batch of instances (disjoint union)
             arr = (_Csr * len(g))(*[x._csr for x in g])
             _check(lib.xgm_graph_upload_batch(ctx._h, arr, len(g), C.byref(h)), ctx._h)
             self.instances = g
-            self.host = None
+            self._host = None
             self.n = sum(x.n for x in g)
         else:
             _check(lib.xgm_graph_upload(ctx._h, g.csr, C.byref(h)), ctx._h)
             self.instances = None
-            self.host = g
+            self._host = g
             self.n = g.n
         self._h, self.ctx = h, ctx
         self.n_inst = int(lib.xgm_graph_instances(h))
         ctx._graphs.add(self)
+
+    @property
+    def host(self):
+        """Host CSR of a single graph (downloaded once for a device-built one)."""
+        if self._host is None and self.instances is None:
+            self._host = self.download()
+        return self._host
+
+    def download(self) -> Graph:
+        """The CSR as stored in HBM, copied to a new host graph."""
+        h = C.c_void_p()
+        _check(lib.xgm_graph_download(self.ctx._h, self._h, C.byref(h)), self.ctx._h)
+        return Graph(h)
+
+    def task_order(self) -> np.ndarray:
+        """DAG vertex tasks (d+ >= 2) by decreasing work estimate."""
+        n = C.c_int64()
+        _check(lib.xgm_graph_task_order(self._h, None, 0, C.byref(n)))
+        out = np.zeros(n.value, np.int32)
+        _check(lib.xgm_graph_task_order(self._h, out.ctypes.data_as(C.POINTER(C.c_int32)), n.value, C.byref(n)))
+        return out
 
     def close(self):
         if getattr(self, "_h", None) is not None:

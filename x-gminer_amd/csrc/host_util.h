@@ -77,4 +77,8 @@ inline uint64_t bounded_draw(uint64_t r, uint64_t range) {
 // Exclusive prefix sum of counts[0..n) into out[0..n]; returns total.
 int64_t exclusive_scan(const std::vector<int64_t> &counts, std::vector<int64_t> &out);
 
+// xgm_csr_validate, optionally without the O(m log d) mirror check of a
+// symmetric CSR (the upload path runs that check on the device instead).
+int validate_csr(const xgm_csr *g, bool check_mirrors);
+
 }  // namespace xgm

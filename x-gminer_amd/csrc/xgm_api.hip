@@ -635,6 +635,117 @@ int check_device(xgm_ctx *ctx) {
 
 }  // namespace
 
+namespace {
+
+// A device allocation released at scope exit.
+struct DevBuf {
+  void *p = nullptr;
+  ~DevBuf() {
+    if (p) (void)hipFree(p);
+  }
+  template <class T>
+  T *as() const {
+    return static_cast<T *>(p);
+  }
+};
+
+// Degree orientation of the undirected device graph u, in HBM (k_prep.hip):
+// degrees; out/in sizes; scans into rowptr / rrowptr; out/in lists; task
+// sort keys and a stable descending radix sort.  The host mirrors that the
+// task preparation reads (CSR, reverse CSR, d+, order) are copied back.
+// Equal to uploading xgm_orient_dag() of the host CSR.
+int orient_device(xgm_ctx *ctx, const xgm_dgraph *u, std::unique_ptr<xgm_dgraph> &out) {
+  const int64_t n = u->n;
+  const hipStream_t s = ctx->stream;
+  const int cu = ctx->num_cu;
+  std::unique_ptr<xgm_dgraph> d(new (std::nothrow) xgm_dgraph);
+  if (!d) return XGM_ENOMEM;
+  d->ctx = ctx;
+  d->device = ctx->device;
+  d->n = n;
+  d->flags = XGM_CSR_SORTED | XGM_CSR_DAG;
+  const size_t n1 = (size_t)std::max<int64_t>(1, n);
+  DevBuf deg, cnt, keys, ids, dplus, tmp;
+  HIP_TRY(ctx, hipMalloc(&deg.p, n1 * sizeof(int32_t)));
+  HIP_TRY(ctx, hipMalloc(&cnt.p, 2 * n1 * sizeof(int64_t)));
+  HIP_TRY(ctx, hipMalloc(&d->d_rowptr, (n + 1) * sizeof(int64_t)));
+  HIP_TRY(ctx, hipMalloc(&d->d_rrowptr, (n + 1) * sizeof(int64_t)));
+  int64_t *c_out = cnt.as<int64_t>(), *c_in = c_out + n1;
+  HIP_TRY(ctx, launch_prep_degree(u->d_rowptr, n, deg.as<int32_t>(), cu, s));
+  HIP_TRY(ctx, launch_prep_split(u->d_rowptr, u->d_colidx, deg.as<int32_t>(), n, c_out, c_in, nullptr, nullptr,
+                                 nullptr, nullptr, cu, s));
+  HIP_TRY(ctx, launch_zero(d->d_rowptr, 1, s));
+  HIP_TRY(ctx, launch_zero(d->d_rrowptr, 1, s));
+  size_t scan_bytes = 0, sort_bytes = 0;
+  HIP_TRY(ctx, prep_scan(nullptr, scan_bytes, c_out, d->d_rowptr + 1, n, s));
+  HIP_TRY(ctx, prep_sort(nullptr, sort_bytes, nullptr, nullptr, nullptr, nullptr, n, s));
+  size_t tmp_bytes = std::max<size_t>(16, std::max(scan_bytes, sort_bytes));
+  HIP_TRY(ctx, hipMalloc(&tmp.p, tmp_bytes));
+  if (n) {
+    size_t b = tmp_bytes;
+    HIP_TRY(ctx, prep_scan(tmp.p, b, c_out, d->d_rowptr + 1, n, s));
+    b = tmp_bytes;
+    HIP_TRY(ctx, prep_scan(tmp.p, b, c_in, d->d_rrowptr + 1, n, s));
+  }
+  int64_t tot[2] = {0, 0};
+  HIP_TRY(ctx, hipMemcpyAsync(&tot[0], d->d_rowptr + n, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+  HIP_TRY(ctx, hipMemcpyAsync(&tot[1], d->d_rrowptr + n, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+  HIP_TRY(ctx, hipStreamSynchronize(s));
+  // In a symmetric graph every kept edge {u, w} is one out- and one in-entry.
+  if (tot[0] != tot[1]) return ctx->fail(XGM_EINVAL, "graph is not symmetric (out and in totals differ)");
+  d->nnz = tot[0];
+  const size_t m1 = (size_t)std::max<int64_t>(1, d->nnz);
+  HIP_TRY(ctx, hipMalloc(&d->d_colidx, m1 * sizeof(int32_t)));
+  HIP_TRY(ctx, hipMalloc(&d->d_rcolidx, m1 * sizeof(int32_t)));
+  HIP_TRY(ctx, launch_prep_split(u->d_rowptr, u->d_colidx, deg.as<int32_t>(), n, c_out, c_in, d->d_rowptr,
+                                 d->d_colidx, d->d_rrowptr, d->d_rcolidx, cu, s));
+  HIP_TRY(ctx, hipMalloc(&keys.p, 2 * n1 * sizeof(uint32_t)));
+  HIP_TRY(ctx, hipMalloc(&ids.p, 2 * n1 * sizeof(int32_t)));
+  HIP_TRY(ctx, hipMalloc(&dplus.p, n1 * sizeof(int32_t)));
+  uint32_t *k_in = keys.as<uint32_t>(), *k_out = k_in + n1;
+  int32_t *i_in = ids.as<int32_t>(), *i_out = i_in + n1;
+  HIP_TRY(ctx, launch_prep_work(d->d_rowptr, d->d_colidx, n, k_in, i_in, dplus.as<int32_t>(), cu, s));
+  if (n) {
+    size_t b = tmp_bytes;
+    HIP_TRY(ctx, prep_sort(tmp.p, b, k_in, k_out, i_in, i_out, n, s));
+  }
+  std::vector<int32_t> sorted;
+  try {
+    d->h_rowptr.resize(n + 1);
+    d->h_rrowptr.resize(n + 1);
+    d->h_colidx.resize(d->nnz);
+    d->h_rcolidx.resize(d->nnz);
+    d->dplus.resize(n);
+    sorted.resize(n);
+  } catch (const std::bad_alloc &) {
+    return XGM_ENOMEM;
+  }
+  HIP_TRY(ctx, hipMemcpyAsync(d->h_rowptr.data(), d->d_rowptr, (n + 1) * sizeof(int64_t), hipMemcpyDeviceToHost, s));
+  HIP_TRY(ctx, hipMemcpyAsync(d->h_rrowptr.data(), d->d_rrowptr, (n + 1) * sizeof(int64_t), hipMemcpyDeviceToHost,
+                              s));
+  if (d->nnz) {
+    HIP_TRY(ctx, hipMemcpyAsync(d->h_colidx.data(), d->d_colidx, d->nnz * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    HIP_TRY(ctx,
+            hipMemcpyAsync(d->h_rcolidx.data(), d->d_rcolidx, d->nnz * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+  }
+  if (n) {
+    HIP_TRY(ctx, hipMemcpyAsync(d->dplus.data(), dplus.p, n * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    HIP_TRY(ctx, hipMemcpyAsync(sorted.data(), i_out, n * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+  }
+  HIP_TRY(ctx, hipStreamSynchronize(s));
+  int64_t n_tasks = 0;
+  d->max_out = 0;
+  for (int64_t v = 0; v < n; ++v) {
+    d->max_out = std::max(d->max_out, d->dplus[v]);
+    n_tasks += d->dplus[v] >= 2;
+  }
+  d->order.assign(sorted.begin(), sorted.begin() + n_tasks);
+  out = std::move(d);
+  return XGM_OK;
+}
+
+}  // namespace
+
 extern "C" {
 
 int xgm_ctx_create(int device, xgm_ctx **out) {
@@ -674,7 +785,8 @@ const char *xgm_ctx_last_error(const xgm_ctx *ctx) { return ctx ? ctx->err.c_str
 int xgm_graph_upload(xgm_ctx *ctx, const xgm_csr *g, xgm_dgraph **out) {
   if (!ctx || !g || !out) return XGM_EINVAL;
   *out = nullptr;
-  if (xgm_csr_validate(g) != XGM_OK) return ctx->fail(XGM_EINVAL, "CSR failed validation");
+  // the O(m log d) symmetry check runs on the device after the copy
+  if (validate_csr(g, false) != XGM_OK) return ctx->fail(XGM_EINVAL, "CSR failed validation");
   if (!(g->flags & XGM_CSR_SORTED)) return ctx->fail(XGM_EINVAL, "neighbour lists must be sorted");
   if (g->nnz >= (int64_t)INT32_MAX) return ctx->fail(XGM_EUNSUPPORTED, "nnz >= 2^31 not supported");
   HIP_TRY(ctx, hipSetDevice(ctx->device));
@@ -697,11 +809,25 @@ int xgm_graph_upload(xgm_ctx *ctx, const xgm_csr *g, xgm_dgraph **out) {
   if (g->nnz && (e = hipMemcpyAsync(d->d_colidx, g->colidx, g->nnz * sizeof(int32_t),
                                     hipMemcpyHostToDevice, ctx->stream)) != hipSuccess)
     return cleanup(ctx->hip_fail(e, "H2D colidx"));
-  try {
-    d->h_rowptr.assign(g->rowptr, g->rowptr + g->n + 1);
-    d->h_colidx.assign(g->colidx, g->colidx + g->nnz);
-  } catch (const std::bad_alloc &) {
-    return XGM_ENOMEM;
+  if (g->flags & XGM_CSR_SYMMETRIC) {
+    DevBuf bad;
+    u64 h_bad[3] = {0, 0, 0};
+    HIP_TRY(ctx, hipMalloc(&bad.p, 4 * sizeof(u64)));
+    HIP_TRY(ctx, launch_zero(bad.p, 4, ctx->stream));
+    HIP_TRY(ctx, launch_prep_mirror(d->d_rowptr, d->d_colidx, g->n, bad.as<u64>(), ctx->num_cu, ctx->stream));
+    HIP_TRY(ctx, hipMemcpyAsync(h_bad, bad.p, 3 * sizeof(u64), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    if (h_bad[0] || h_bad[1] != h_bad[2]) return ctx->fail(XGM_EINVAL, "CSR failed validation (not symmetric)");
+  }
+  // The host copy feeds task preparation: DAG task lists need it at once;
+  // an undirected graph copies it back from HBM on first use.
+  if (g->flags & XGM_CSR_DAG) {
+    try {
+      d->h_rowptr.assign(g->rowptr, g->rowptr + g->n + 1);
+      d->h_colidx.assign(g->colidx, g->colidx + g->nnz);
+    } catch (const std::bad_alloc &) {
+      return XGM_ENOMEM;
+    }
   }
   if (g->flags & XGM_CSR_DAG) {
     build_dag_order(*g, d->order, d->dplus, d->max_out);
@@ -721,9 +847,54 @@ int xgm_graph_upload(xgm_ctx *ctx, const xgm_csr *g, xgm_dgraph **out) {
                                       hipMemcpyHostToDevice, ctx->stream)) != hipSuccess)
       return cleanup(ctx->hip_fail(e, "H2D rcolidx"));
   }
-  else if (g->flags & XGM_CSR_SYMMETRIC) build_undirected_order(*g, d->order, d->dplus, d->work);
   if ((e = hipStreamSynchronize(ctx->stream)) != hipSuccess) return cleanup(ctx->hip_fail(e, "upload sync"));
   *out = d.release();
+  return XGM_OK;
+}
+
+int xgm_graph_orient(xgm_ctx *ctx, const xgm_dgraph *und, xgm_dgraph **out) {
+  if (!ctx || !und || !out) return XGM_EINVAL;
+  *out = nullptr;
+  if (und->ctx != ctx) return ctx->fail(XGM_EINVAL, "graph belongs to another context");
+  if ((und->flags & (XGM_CSR_SORTED | XGM_CSR_SYMMETRIC)) != (XGM_CSR_SORTED | XGM_CSR_SYMMETRIC) ||
+      (und->flags & XGM_CSR_DAG))
+    return ctx->fail(XGM_EINVAL, "orientation needs a sorted, symmetric (undirected) graph");
+  if (und->d_slot) return ctx->fail(XGM_EINVAL, "orient a batch through its counts, not xgm_graph_orient");
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  std::unique_ptr<xgm_dgraph> d;
+  const int rc = orient_device(ctx, und, d);
+  if (rc) return rc;
+  *out = d.release();
+  return XGM_OK;
+}
+
+int xgm_graph_download(xgm_ctx *ctx, const xgm_dgraph *g, xgm_hgraph **out) {
+  if (!ctx || !g || !out) return XGM_EINVAL;
+  *out = nullptr;
+  if (g->ctx != ctx) return ctx->fail(XGM_EINVAL, "graph belongs to another context");
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  std::unique_ptr<xgm_hgraph> h(new (std::nothrow) xgm_hgraph);
+  if (!h) return XGM_ENOMEM;
+  try {
+    h->rowptr.resize(g->n + 1);
+    h->colidx.resize(g->nnz);
+  } catch (const std::bad_alloc &) {
+    return XGM_ENOMEM;
+  }
+  h->n = g->n;
+  h->nnz = g->nnz;
+  h->flags = g->flags;
+  HIP_TRY(ctx, hipMemcpy(h->rowptr.data(), g->d_rowptr, (g->n + 1) * sizeof(int64_t), hipMemcpyDeviceToHost));
+  if (g->nnz) HIP_TRY(ctx, hipMemcpy(h->colidx.data(), g->d_colidx, g->nnz * sizeof(int32_t), hipMemcpyDeviceToHost));
+  *out = h.release();
+  return XGM_OK;
+}
+
+int xgm_graph_task_order(const xgm_dgraph *g, int32_t *out, int64_t cap, int64_t *len) {
+  if (!g || !len || cap < 0 || (cap > 0 && !out)) return XGM_EINVAL;
+  if (!(g->flags & XGM_CSR_DAG)) return XGM_EINVAL;
+  *len = (int64_t)g->order.size();
+  std::copy(g->order.begin(), g->order.begin() + std::min<int64_t>(cap, *len), out);
   return XGM_OK;
 }
 
@@ -737,6 +908,25 @@ void xgm_graph_free(xgm_dgraph *g) {
 
 static DevCSR dev_view(const xgm_dgraph *g);
 
+// Host copy of a device graph's CSR (task preparation of the diamond,
+// 4-cycle and motif paths), copied from HBM on first use.
+static int ensure_host_mirror(xgm_ctx *ctx, xgm_dgraph *g) {
+  if ((int64_t)g->h_rowptr.size() == g->n + 1) return XGM_OK;
+  std::vector<int64_t> rp;
+  std::vector<int32_t> ci;
+  try {
+    rp.resize(g->n + 1);
+    ci.resize(g->nnz);
+  } catch (const std::bad_alloc &) {
+    return XGM_ENOMEM;
+  }
+  HIP_TRY(ctx, hipMemcpy(rp.data(), g->d_rowptr, (g->n + 1) * sizeof(int64_t), hipMemcpyDeviceToHost));
+  if (g->nnz) HIP_TRY(ctx, hipMemcpy(ci.data(), g->d_colidx, g->nnz * sizeof(int32_t), hipMemcpyDeviceToHost));
+  g->h_rowptr = std::move(rp);
+  g->h_colidx = std::move(ci);
+  return XGM_OK;
+}
+
 // Diamond / motif edge items: every undirected edge owned by its (degree,
 // id)-larger endpoint, owners in degree tiers (<= 64, <= 1024, <= 8192,
 // larger), owners above 1024 split into slices of ~64K probed elements so
@@ -747,6 +937,15 @@ static int diamond_lists(xgm_ctx *ctx, xgm_dgraph *gm, std::vector<TaskList> &tl
   std::vector<std::vector<int32_t>> parts(4);
   const char *key = with_deg2 ? "motif_edges" : "diamond";
   if (!cached(gm, key)) {
+    if (int rc = ensure_host_mirror(ctx, gm)) return rc;
+    if ((int64_t)gm->dplus.size() != gm->n) {  // owner order, built on first use
+      const xgm_csr hv{gm->n, gm->nnz, gm->h_rowptr.data(), gm->h_colidx.data(), gm->flags};
+      try {
+        build_undirected_order(hv, gm->order, gm->dplus, gm->work);
+      } catch (const std::bad_alloc &) {
+        return XGM_ENOMEM;
+      }
+    }
     const auto &deg = gm->dplus;
     const int64_t kChunk = 1 << 16;
     std::vector<int32_t> owners = gm->order;  // degree >= 3, by decreasing work
@@ -852,6 +1051,7 @@ static int dispatch_pattern(xgm_ctx *ctx, const xgm_dgraph *g, int pattern, cons
         // reference loop order: tasks v0 with >= 2 lower neighbours (original ids)
         std::vector<int32_t> ord;
         if (!cached(gm, "cycle4_pair")) {
+          if ((rc = ensure_host_mirror(ctx, gm))) return rc;
           const auto &rp = gm->h_rowptr;
           const auto &ci = gm->h_colidx;
           for (int64_t v = 0; v < g->n; ++v) {
@@ -867,6 +1067,7 @@ static int dispatch_pattern(xgm_ctx *ctx, const xgm_dgraph *g, int pattern, cons
       if (strategy != XGM_STRAT_AUTO && strategy != XGM_STRAT_HASH)
         return ctx->fail(XGM_EUNSUPPORTED, "strategy not available for CYCLE4_EDGE");
       if (!gm->relabelled) {
+        if ((rc = ensure_host_mirror(ctx, gm))) return rc;
         xgm_csr hv{g->n, g->nnz, gm->h_rowptr.data(), gm->h_colidx.data(), g->flags};
         // instances keep their id ranges, so the vertex -> instance slots
         // hold for the relabelled ids too
@@ -1218,16 +1419,9 @@ int xgm_count_batch(xgm_ctx *ctx, const xgm_csr *gs, const int *patterns, int n_
 // slots), built on first use and owned by g.
 static int ensure_dag(xgm_ctx *ctx, xgm_dgraph *g) {
   if (g->dag) return XGM_OK;
-  xgm_csr u{g->n, g->nnz, g->h_rowptr.data(), g->h_colidx.data(), g->flags};
-  xgm_hgraph *h = nullptr;
-  int rc = xgm_orient_dag(&u, &h);
-  if (rc) return ctx->fail(rc, "orientation failed");
-  std::unique_ptr<xgm_hgraph, void (*)(xgm_hgraph *)> hold(h, xgm_hgraph_free);
-  xgm_csr v;
-  xgm_hgraph_view(h, &v);
-  xgm_dgraph *d = nullptr;
-  if ((rc = xgm_graph_upload(ctx, &v, &d))) return rc;
-  std::unique_ptr<xgm_dgraph> dd(d);
+  std::unique_ptr<xgm_dgraph> dd;
+  const int rc = orient_device(ctx, g, dd);
+  if (rc) return rc;
   if (g->d_slot) {
     dd->n_inst = g->n_inst;
     dd->h_slot = g->h_slot;
@@ -1254,6 +1448,7 @@ int xgm_motifs(xgm_ctx *ctx, const xgm_dgraph *g, int k, uint64_t *out) {
   int rc = ensure_dag(ctx, gm);
   if (rc) return rc;
   std::vector<uint64_t> tri(ni), k4(ni), c4(ni);
+  if ((rc = ensure_host_mirror(ctx, gm))) return rc;
   if (k == 3) {
     // wedges = sum_v d(d-1) / 2 - 3 * triangles (src/motif/omp_formula.cc:38-39)
     std::vector<u64> dd(ni, 0);
