@@ -97,6 +97,16 @@ int xgm_orient_dag(const xgm_csr *g, xgm_hgraph **out);
 int xgm_load_csr(const char *prefix, uint32_t flags, xgm_hgraph **out);
 int xgm_save_csr(const char *prefix, const xgm_csr *g);
 
+/* Edge-list text (the SNAP form GraphPi reads, xgminer/src/dataloader.cpp:
+ * 29-175): a header "n m", then m lines "u v" of arbitrary integer ids.
+ * Ids are renumbered 0..n-1 in order of first appearance, self-loops are
+ * dropped, duplicates merged, and the result is the sorted symmetric CSR.
+ * XGM_EIO if the file is unreadable, the line count differs from m, or the
+ * number of distinct endpoints differs from n (the reference's own
+ * consistency checks).  The format is detected as in
+ * xgminer/src/common.cpp:66-103 by the xgminer front end. */
+int xgm_load_edgelist(const char *path, xgm_hgraph **out);
+
 /* Check the CSR invariants implied by g->flags; XGM_OK or XGM_EINVAL. */
 int xgm_csr_validate(const xgm_csr *g);
 

@@ -74,3 +74,27 @@ def test_chunglu4m_full(xgm, ctx):
     assert ctx.count(dg, 5) == gold["counts"]["rectangle"]
     assert ctx.count(dg, 5, hub_rows=64) == gold["counts"]["rectangle"]
     dg.close()
+
+
+@pytest.mark.timeout(600)
+def test_rmat23_clique5_full(xgm, ctx):
+    """The held-out learned-scheduler graph: 5-cliques on R-MAT-23 (seed 4)
+    under the rank LUT and the learned schedule, against the oracle's
+    bitset count; the DAG is oriented on the device."""
+    from xgminer_amd import scheduler, workloads
+    gold = _gold("rmat23_s4")
+    _say("rmat23 generate")
+    g = workloads.rmat(xgm, 23, 4)
+    assert f"{g.hash():016x}" == gold["hash"]
+    du = ctx.upload(g)
+    dg = ctx.orient(du)
+    du.close()
+    want = gold["counts"]["clique5"]
+    _say("rmat23 clique5 rank LUT")
+    assert ctx.count(dg, 3, 6) == want
+    _say("rmat23 clique5 learned")
+    codes = scheduler.schedule(dg, scheduler.load(), xgm.CLIQUE5)
+    s = xgm.Schedule(dg, xgm.CLIQUE5, codes)
+    assert s.count() == want
+    s.close()
+    dg.close()

@@ -95,6 +95,7 @@ def _load():
         "xgm_orient_dag": (C.c_int, [P(_Csr), P(VP)]),
         "xgm_load_csr": (C.c_int, [C.c_char_p, C.c_uint32, P(VP)]),
         "xgm_save_csr": (C.c_int, [C.c_char_p, P(_Csr)]),
+        "xgm_load_edgelist": (C.c_int, [C.c_char_p, P(VP)]),
         "xgm_csr_validate": (C.c_int, [P(_Csr)]),
         "xgm_csr_hash": (U64, [P(_Csr)]),
         "xgm_csr_get_stats": (C.c_int, [P(_Csr), P(_Stats)]),
@@ -256,6 +257,15 @@ def from_edges(n, src, dst) -> Graph:
 
 def load_csr(prefix: str, flags=CSR_SORTED | CSR_SYMMETRIC) -> Graph:
     return _gen(lib.xgm_load_csr, prefix.encode(), flags)
+
+
+def load_edgelist(path: str) -> Graph:
+    """Edge-list text ("n m" header, then "u v" lines), ids renumbered in
+    order of first appearance; the undirected CSR."""
+    return _gen(lib.xgm_load_edgelist, path.encode())
+
+
+CLI_PATH = os.path.join(_HERE, "xgminer")
 
 
 # --------------------------------------------------------------------------

@@ -1,5 +1,6 @@
 // Host graph substrate: generators, CSR cleaning, degree orientation,
 // on-disk format and content hashing.  Declared in include/xgm_graph.h.
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -387,6 +388,61 @@ int xgm_load_csr(const char *prefix, uint32_t flags, xgm_hgraph **out) {
   if (xgm_csr_validate(&v) != XGM_OK) return XGM_EINVAL;
   *out = g.release();
   return XGM_OK;
+}
+
+int xgm_load_edgelist(const char *path, xgm_hgraph **out) {
+  if (!path || !out) return XGM_EINVAL;
+  *out = nullptr;
+  FILE *f = std::fopen(path, "r");
+  if (!f) return XGM_EIO;
+  std::unique_ptr<FILE, int (*)(FILE *)> hold(f, std::fclose);
+  long long n = -1, m = -1;
+  if (std::fscanf(f, "%lld %lld", &n, &m) != 2 || n < 0 || m < 0 || n > INT32_MAX) return XGM_EIO;
+  std::vector<int32_t> src, dst;
+  try {
+    src.reserve((size_t)m);
+    dst.reserve((size_t)m);
+    long long lines = 0, x, y;
+    std::vector<long long> rx, ry;
+    rx.reserve((size_t)m);
+    ry.reserve((size_t)m);
+    while (std::fscanf(f, "%lld %lld", &x, &y) == 2) {
+      ++lines;
+      if (x == y) continue;  // self-loops are dropped
+      rx.push_back(x);
+      ry.push_back(y);
+    }
+    if (!std::feof(f) || lines != m) return XGM_EIO;
+    // ids in order of first appearance (x before y on each line)
+    std::vector<long long> seq;
+    seq.reserve(2 * rx.size());
+    for (size_t i = 0; i < rx.size(); ++i) {
+      seq.push_back(rx[i]);
+      seq.push_back(ry[i]);
+    }
+    std::vector<size_t> idx(seq.size());
+    for (size_t i = 0; i < idx.size(); ++i) idx[i] = i;
+    std::stable_sort(idx.begin(), idx.end(), [&](size_t a, size_t b) { return seq[a] < seq[b]; });
+    std::vector<size_t> first_pos;  // per distinct raw id, its first position
+    std::vector<int32_t> group(seq.size());
+    for (size_t k = 0; k < idx.size(); ++k) {
+      if (k == 0 || seq[idx[k]] != seq[idx[k - 1]]) first_pos.push_back(idx[k]);
+      group[idx[k]] = (int32_t)(first_pos.size() - 1);
+    }
+    if ((long long)first_pos.size() != n) return XGM_EIO;
+    std::vector<size_t> by_first(first_pos.size());
+    for (size_t i = 0; i < by_first.size(); ++i) by_first[i] = i;
+    std::sort(by_first.begin(), by_first.end(), [&](size_t a, size_t b) { return first_pos[a] < first_pos[b]; });
+    std::vector<int32_t> new_id(first_pos.size());
+    for (size_t r = 0; r < by_first.size(); ++r) new_id[by_first[r]] = (int32_t)r;
+    for (size_t i = 0; i < rx.size(); ++i) {
+      src.push_back(new_id[group[2 * i]]);
+      dst.push_back(new_id[group[2 * i + 1]]);
+    }
+  } catch (const std::bad_alloc &) {
+    return XGM_ENOMEM;
+  }
+  return build_clean(n, (int64_t)src.size(), src.data(), dst.data(), out);
 }
 
 uint64_t xgm_csr_hash(const xgm_csr *g) {
