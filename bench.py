@@ -509,6 +509,8 @@ def main():
     if args.pattern == "triangle":
         per_tier, btot = tc_alg_bytes_per_tier(d)
         dom = max(avg, key=avg.get)
+        if dom not in per_tier:  # a strategy with one kernel for every edge (bucket bitmap)
+            per_tier = {dom: btot}
         ach = per_tier[dom] / (avg[dom] * 1e-3) / 1e9
         roof = dict(bound="hbm", achieved=round(ach, 1), peak=HBM_PEAK_GBS, unit="GB/s",
                     frac=round(ach / HBM_PEAK_GBS, 4), traffic=None, kernel=dom,

@@ -60,7 +60,10 @@ enum xgm_strategy {
   XGM_STRAT_LIST = 4,   /* cliques: edge tasks, nested list intersections (G2Miner plain);
                            4-cycle: bounded pairwise intersections in the reference loop order */
   XGM_STRAT_ELUT = 5,   /* cliques: edge tasks, bitmap LUT over N+(v0) ∩ N+(v1) */
-  XGM_STRAT_LUT_RANK = 6 /* cliques: vertex LUT with rows in DAG order (upper-triangular) */
+  XGM_STRAT_LUT_RANK = 6,/* cliques: vertex LUT with rows in DAG order (upper-triangular) */
+  XGM_STRAT_BUCKET = 7  /* triangles: X-GMiner bucket-bitmap prefilter -- lists split into 64
+                           buckets by id & 63, only buckets non-empty on both sides intersected
+                           (xgminer/include/xgminer_bitmap.cuh:161-320, 661-822) */
 };
 
 typedef struct xgm_opts {

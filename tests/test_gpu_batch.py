@@ -92,6 +92,8 @@ def test_batch_per_instance_counts(xgm, ctx, mixed):
     assert np.array_equal(ctx.count_instances(und, 5, wedge_range=1, hub_rows=3), _col(want, "rectangle"))
     _say("batch rectangle pair strategy")
     assert np.array_equal(ctx.count_instances(und, 5, strategy=4), _col(want, "rectangle"))
+    _say("batch triangle bucket strategy")
+    assert np.array_equal(ctx.count_instances(dag, 1, strategy=7), _col(want, "triangle"))
     _say("batch clique strategies")
     for strat in (3, 4, 5, 6):
         assert np.array_equal(ctx.count_instances(dag, 3, strategy=strat), _col(want, "clique5")), strat

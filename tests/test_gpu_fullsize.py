@@ -49,6 +49,8 @@ def test_rmat22_full(xgm, ctx):
     du.close()
     _say("rmat22 triangle")
     assert ctx.count(dg, 1) == gold["counts"]["triangle"]
+    _say("rmat22 triangle bucket strategy")
+    assert ctx.count(dg, 1, 7) == gold["counts"]["triangle"]
     _say("rmat22 clique4")
     for strat in (0, 6):
         assert ctx.count(dg, 2, strat) == gold["counts"]["clique4"], strat

@@ -34,7 +34,7 @@ TRIANGLE, CLIQUE4, CLIQUE5, DIAMOND_EDGE, CYCLE4_EDGE = 1, 2, 3, 4, 5
 PATTERN_NAMES = {TRIANGLE: "triangle", CLIQUE4: "clique4", CLIQUE5: "clique5",
                  DIAMOND_EDGE: "diamond", CYCLE4_EDGE: "rectangle"}
 # strategies
-STRAT_AUTO, STRAT_HASH, STRAT_BSEARCH, STRAT_LUT, STRAT_LIST, STRAT_ELUT, STRAT_LUT_RANK = 0, 1, 2, 3, 4, 5, 6
+STRAT_AUTO, STRAT_HASH, STRAT_BSEARCH, STRAT_LUT, STRAT_LIST, STRAT_ELUT, STRAT_LUT_RANK, STRAT_BUCKET = range(8)
 SETOP_INTERSECT, SETOP_DIFF, SETOP_DIFF_EXCL = 1, 2, 3
 
 

@@ -58,6 +58,15 @@ constexpr int kTcWaveCap[2] = {64, 256};
 hipError_t launch_tc(const DevCSR &g, const TaskList tiers[NUM_TIERS], int strategy, int hash_shift, Launch &l,
                      int *d_queues /* NUM_TIERS counters */);
 
+// Bucket-bitmap triangle strategy (k_bucket.hip): bucket-major copy of the
+// DAG's lists (64 buckets by w & 63), per-vertex uint16 bucket offsets and
+// non-empty masks; then one wave per task vertex (d+ >= 2) intersecting
+// only the buckets set in both masks.  Adds into l.d_out.
+hipError_t launch_bucket_build(const int64_t *rowptr, const int32_t *colidx, int64_t n, int32_t *bcol, uint16_t *boff,
+                               u64 *bmap, int num_cu, hipStream_t s);
+hipError_t launch_tc_bucket(const DevCSR &g, const int32_t *bcol, const uint16_t *boff, const u64 *bmap,
+                            const TaskList &tasks, Launch &l, int *queue);
+
 // Edge tasks (v0, v1) of a k-clique count, as int32 pairs in HBM.
 struct EdgeItems {
   const int32_t *items = nullptr;  // 2 * count ints

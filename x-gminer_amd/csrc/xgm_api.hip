@@ -133,6 +133,11 @@ struct xgm_dgraph {
   std::vector<int64_t> rl_rowptr;  // host copy of the relabelled CSR
   std::vector<int32_t> rl_colidx;
   int *d_hub = nullptr;            // 4-cycle hub counter rows (hub_rows * stride ints)
+  // bucket-bitmap triangle strategy: bucket-major colidx, bucket offsets
+  // (64 per vertex), non-empty bucket masks; built on first use
+  int32_t *d_bcol = nullptr;
+  uint16_t *d_boff = nullptr;
+  u64 *d_bmap = nullptr;
   std::map<std::string, int> hub_count;  // hub tasks per cached hub list
   // batched instances (disjoint union): instance of every vertex, first id
   // of every instance (n_inst + 1 bounds), largest instance
@@ -149,6 +154,9 @@ struct xgm_dgraph {
   ~xgm_dgraph() {
     (void)hipFree(d_dense);
     (void)hipFree(d_hub);
+    (void)hipFree(d_bcol);
+    (void)hipFree(d_boff);
+    (void)hipFree(d_bmap);
     (void)hipFree(d_hub_lut);
     (void)hipFree(d_slot);
     (void)hipFree(d_inst_lo);
@@ -998,6 +1006,19 @@ static int dispatch_pattern(xgm_ctx *ctx, const xgm_dgraph *g, int pattern, cons
   switch (pattern) {
     case XGM_TRIANGLE: {
       if (!(g->flags & XGM_CSR_DAG)) return ctx->fail(XGM_EINVAL, "TRIANGLE needs a DAG-oriented CSR");
+      if (strategy == XGM_STRAT_BUCKET) {
+        if (gm->max_out >= 65536) return ctx->fail(XGM_EUNSUPPORTED, "bucket strategy needs out-degrees < 65536");
+        if (!gm->d_bcol) {
+          HIP_TRY(ctx, hipMalloc(&gm->d_bcol, std::max<int64_t>(1, g->nnz) * sizeof(int32_t)));
+          HIP_TRY(ctx, hipMalloc(&gm->d_boff, std::max<int64_t>(1, g->n) * 64 * sizeof(uint16_t)));
+          HIP_TRY(ctx, hipMalloc(&gm->d_bmap, std::max<int64_t>(1, g->n) * sizeof(u64)));
+          HIP_TRY(ctx, launch_bucket_build(gm->d_rowptr, gm->d_colidx, g->n, gm->d_bcol, gm->d_boff, gm->d_bmap,
+                                           ctx->num_cu, l.stream));
+        }
+        if ((rc = tasks_by_ranges(ctx, gm, "tc_bucket", 2, {INT32_MAX}, tl))) return rc;
+        e = launch_tc_bucket(dg, gm->d_bcol, gm->d_boff, gm->d_bmap, tl[0], l, l.d_queue);
+        break;
+      }
       if (strategy != XGM_STRAT_AUTO && strategy != XGM_STRAT_HASH && strategy != XGM_STRAT_BSEARCH)
         return ctx->fail(XGM_EUNSUPPORTED, "strategy not available for TRIANGLE");
       int s = (strategy == XGM_STRAT_BSEARCH) ? 2 : 1;
@@ -1673,6 +1694,7 @@ int xgm_autotune(xgm_ctx *ctx, const xgm_dgraph *g, int pattern, int reps, xgm_o
                        std::make_pair(32, 128), std::make_pair(16, 64)})
           cand.push_back(mk(XGM_STRAT_HASH, 0, hs, t.first, t.second));
       cand.push_back(mk(XGM_STRAT_BSEARCH, 0, 0, 0, 0));
+      if (g->max_out < 65536) cand.push_back(mk(XGM_STRAT_BUCKET, 0, 0, 0, 0));
       break;
     case XGM_CLIQUE4:
     case XGM_CLIQUE5:
