@@ -27,7 +27,7 @@ def test_cli_options(xgm):
     r = _run(xgm, "--help")
     assert r.returncode == 0 and "--patternID" in r.stdout
     for args, msg in ((("-g", CITESEER + "_missing", "--patternID", "20"), "cannot load graph"),
-                      (("-g", os.path.dirname(CITESEER), "--patternID", "19"), "unsupported --patternID"),
+                      (("-g", os.path.dirname(CITESEER), "--patternID", "22"), "unsupported --patternID"),
                       (("-g", os.path.dirname(CITESEER), "--bogus", "1"), "unknown option"),
                       (("-g", os.path.dirname(CITESEER), "--algorithm", "cpu_baseline", "--patternID", "20"),
                        "no CPU path"),

@@ -55,6 +55,10 @@ def test_cli_citeseer(xgm, tmp_path):
         assert _counts(xgm, "-g", graph, "--pattern-size", "4", "--pattern-adj-mat", K4).popitem()[1] == 255
         assert _counts(xgm, "-g", graph, "--pattern-size", "4", "--pattern-adj-mat", DIAMOND).popitem()[1] == 3730
         assert _counts(xgm, "-g", graph, "--pattern-size", "4", "--pattern-adj-mat", CYCLE4).popitem()[1] == 6059
+    # GLUMIN P1-P3 in unique-embedding semantics (src/motif/README.md:53-56)
+    assert _counts(xgm, "-g", d, "--patternID", "17") == {"P1": 222630}
+    assert _counts(xgm, "-g", d, "--patternID", "18") == {"P2": 2200}
+    assert _counts(xgm, "-g", d, "--patternID", "19") == {"P3": 22900}
     m = _counts(xgm, "-g", d, "--pattern-name", "motif4")
     assert m == {"3-star": 222630, "4-path": 111153, "tailed-triangle": 22900, "4-cycle": 3094,
                  "diamond": 2200, "4-clique": 255}

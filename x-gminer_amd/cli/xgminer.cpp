@@ -33,7 +33,8 @@ const char *kUsage =
     "  -g, --graph PATH        binary CSR directory or prefix, or an edge-list text file\n"
     "  --algorithm NAME        glumin_g2miner | glumin_g2miner_lut | glumin_cliques |\n"
     "                          glumin_cliques_lut | graphminer (default glumin_g2miner_lut)\n"
-    "  --patternID N           GLUMIN pattern id (20: 4-clique, 21: 5-clique)\n"
+    "  --patternID N           GLUMIN pattern id (17-19: vertex-induced 3-star / diamond /\n"
+    "                          tailed triangle, unique embeddings; 20: 4-clique, 21: 5-clique)\n"
     "  --pattern-name NAME     triangle | clique4 | clique5 | diamond | rectangle | motif3 | motif4\n"
     "  --pattern-size S --pattern-adj-mat M   pattern by adjacency matrix (row-major 0/1 string)\n"
     "  --repeat R              count R times (default 1)\n"
@@ -94,6 +95,7 @@ int load_graph(const std::string &path, xgm_hgraph **g, std::string &how) {
 struct Pattern {
   int id = 0;          // xgm_pattern, or 0 for a motif census
   int motif_k = 0;     // 3 or 4 for a census
+  int motif_pick = -1; // one entry of the 4-motif census (GLUMIN P1-P3), or -1 for all
   std::string label;   // as printed
 };
 
@@ -190,9 +192,18 @@ int main(int argc, char **argv) {
     // GLUMIN numbering: local id = patternID - 16 (xgminer/include/glumin.h:36)
     const int pid = opt.count("patternID") ? std::atoi(opt["patternID"].c_str()) : 0;
     const int local = pid - 16;
+    // P1-P3 are the vertex-induced 3-star, diamond and tailed triangle, read
+    // from the 4-motif census in unique-embedding (GraphMiner) semantics;
+    // GLUMIN's P2 kernel walks both directions of every edge and reports
+    // twice the induced-diamond count (glumin_GM_solver.cu:267).
     if (local == 4) pat.id = XGM_CLIQUE4;
     else if (local == 5) pat.id = XGM_CLIQUE5;
-    else return fail("unsupported --patternID " + std::to_string(pid) + " (20: 4-clique, 21: 5-clique)");
+    else if (local >= 1 && local <= 3) {
+      pat.motif_k = 4;
+      pat.motif_pick = local == 1 ? 0 : local == 2 ? 4 : 2;
+    } else
+      return fail("unsupported --patternID " + std::to_string(pid) +
+                  " (17-19: vertex-induced 3-star / diamond / tailed triangle, 20: 4-clique, 21: 5-clique)");
     pat.label = "P" + std::to_string(local);
   }
 
@@ -264,8 +275,11 @@ int main(int argc, char **argv) {
       const char *n3[] = {"wedge", "triangle"};
       const char *n4[] = {"3-star", "4-path", "tailed-triangle", "4-cycle", "diamond", "4-clique"};
       const int cnt = pat.motif_k == 3 ? 2 : 6;
-      for (int i = 0; i < cnt; ++i)
-        std::printf("Pattern %s count: %llu\n", pat.motif_k == 3 ? n3[i] : n4[i], (unsigned long long)m[i]);
+      if (pat.motif_pick >= 0)
+        std::printf("Pattern %s count: %llu\n", pat.label.c_str(), (unsigned long long)m[pat.motif_pick]);
+      else
+        for (int i = 0; i < cnt; ++i)
+          std::printf("Pattern %s count: %llu\n", pat.motif_k == 3 ? n3[i] : n4[i], (unsigned long long)m[i]);
     } else {
       uint64_t c = 0;
       if ((rc = xgm_count(ctx, needs_dag ? dag : und, pat.id, &o, &c))) {

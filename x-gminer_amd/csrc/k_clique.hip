@@ -85,6 +85,82 @@ __device__ __forceinline__ u64 lut_count(const u64 *rows, int s, int weff, int d
   return acc;
 }
 
+// Row access for the wave-cooperative counters: LDS rows in the padded
+// lut_word layout, or HBM rows at a runtime stride of weff words.
+template <int W>
+struct LdsRows {
+  const u64 *rows;
+  __device__ __forceinline__ u64 operator()(int r, int w) const { return rows[lut_word<W>(r, w)]; }
+};
+struct HbmRows {
+  const u64 *rows;
+  int stride;
+  __device__ __forceinline__ u64 operator()(int r, int w) const { return rows[(size_t)r * stride + w]; }
+};
+
+// Wave-cooperative LUT count for depth 3 and 4 (W = the register bound on
+// the words per row, weff <= W): waves claim rows a through an LDS counter
+// (lowest first: in the triangular layout those rows hold the most bits);
+// the set bits j of row a are compacted into a per-wave queue of 128 ids
+// (ballot rank) and handed out one per lane, so each lane owns one pair
+// (a, j) instead of a 64-bit word of row a with 0..64 pairs in it, and the
+// row-a words are the same address across the wave.  TRI: rows hold only
+// later ranks (rank-ordered LUT), so words below (j+1)/64 are skipped.
+// jq: 128 ints of LDS for this wave; rowq: the block's row counter, zero
+// on entry.  All lanes of every wave call.
+template <int W, bool TRI, class Rows>
+__device__ __forceinline__ u64 lut_count_wq(Rows row, int s, int weff, int depth, int *jq, int *rowq, int lane) {
+  u64 acc = 0;
+  for (;;) {
+    int a = 0;
+    if (lane == 0) a = atomicAdd(rowq, 1);
+    a = __shfl(a, 0, 64);
+    if (a >= s) break;
+    int pending = 0;
+    for (int q = TRI ? (a >> 6) : 0; q <= weff; ++q) {
+      if (q < weff) {
+        const u64 bits = row(a, q);
+        if ((bits >> lane) & 1) jq[pending + __popcll(bits & lanemask_lt(lane))] = q * 64 + lane;
+        pending += __popcll(bits);
+        wave_sync();
+      }
+      while (pending >= 64 || (q == weff && pending > 0)) {
+        const int take = min(pending, 64);
+        if (lane < take) {
+          const int j = jq[lane];
+          const int ws = TRI ? (j + 1) >> 6 : 0;
+          if (depth == 3) {
+#pragma unroll
+            for (int w = 0; w < W; ++w)
+              if (w >= ws && w < weff) acc += __popcll(row(a, w) & row(j, w));
+          } else {
+            u64 tw[W];
+#pragma unroll
+            for (int w = 0; w < W; ++w) tw[w] = (w >= ws && w < weff) ? (row(a, w) & row(j, w)) : 0ull;
+#pragma unroll
+            for (int w2 = 0; w2 < W; ++w2) {
+              u64 tb = tw[w2];
+              while (tb) {
+                const int k = w2 * 64 + __builtin_ctzll(tb);
+                tb &= tb - 1;
+                const int ks = TRI ? (k + 1) >> 6 : 0;
+#pragma unroll
+                for (int w = 0; w < W; ++w)
+                  if (w >= ks && w < weff) acc += __popcll(tw[w] & row(k, w));
+              }
+            }
+          }
+        }
+        wave_sync();
+        if (lane + 64 < pending) jq[lane] = jq[lane + 64];
+        wave_sync();
+        pending -= take;
+      }
+    }
+  }
+  return acc;
+}
+
 // Stage shared by the clique kernels: a sorted vertex set x (N+(v0) for a
 // vertex task, W1 for an edge task), a second set x2, the LUT rows over x,
 // the segment table and block-reduction slots.
@@ -96,7 +172,7 @@ struct CStage {
   Segs<SMAX> seg;
   int cnt[NT / 64];
   int flagrow[NT / 64][64];
-  int ns, L, item;
+  int ns, L, item, rowq;
 };
 
 struct Member {
@@ -179,8 +255,18 @@ __device__ u64 block_lut_count(const int64_t *__restrict__ rowptr, const int32_t
       if (pos < s && S.x[pos] == r.w) atomicOr(&S.rows[lut_word<W>(S.seg.idx[r.j], pos >> 6)], 1ull << (pos & 63));
     }
   });
+  if (tid == 0) S.rowq = 0;
   __syncthreads();
-  const u64 c = lut_count<W>(S.rows, s, weff, depth, tid, NT);
+  u64 c;
+  if (W >= 4 && (depth == 4 || (depth == 3 && W >= 16))) {
+    // the segment table (dead after the build) holds the per-wave queues
+    // (depth 3 on rows of <= 8 words measured faster per thread-word)
+    static_assert(sizeof(S.seg) >= (NT / 64) * 128 * sizeof(int), "queue space");
+    c = lut_count_wq<W, false>(LdsRows<W>{S.rows}, s, weff, depth, reinterpret_cast<int *>(&S.seg) + wave * 128,
+                               &S.rowq, lane);
+  } else {
+    c = lut_count<W>(S.rows, s, weff, depth, tid, NT);
+  }
   __syncthreads();
   return c;
 }
@@ -277,7 +363,7 @@ struct RStage {
   };
   int tmp[SMAX];  // scratch of the block-cooperative segment build
   int flagrow[NT / 64][64];
-  int ns, L, item;
+  int ns, L, item, rowq;
 };
 
 template <int SMAX, int W, int NT>
@@ -330,8 +416,17 @@ __global__ __launch_bounds__(NT) void k_clique_vlut_rank(const int64_t *__restri
           }
         }
       });
+      if (tid == 0) S.rowq = 0;
       __syncthreads();
-      part += lut_count_tri<W>(S.rows, s, weff, depth, tid, NT);
+      if (W >= 4 && (depth == 4 || (depth == 3 && W >= 16))) {
+        // the sort keys (dead after the LUT build) hold the per-wave queues
+        // (depth 3 on rows of <= 8 words measured faster per thread-word)
+        static_assert(sizeof(S.key) + sizeof(S.val) >= (NT / 64) * 128 * sizeof(int), "queue space");
+        part += lut_count_wq<W, true>(LdsRows<W>{S.rows}, s, weff, depth, reinterpret_cast<int *>(S.key) + wave * 128,
+                                      &S.rowq, lane);
+      } else {
+        part += lut_count_tri<W>(S.rows, s, weff, depth, tid, NT);
+      }
     }
     task_flush(acc, part, slots, v0, out, tid & 63);
     __syncthreads();
@@ -379,46 +474,6 @@ __device__ __forceinline__ u64 lut_count_tri_wide(const u64 *__restrict__ rows, 
   return acc;
 }
 
-// 4-clique count over HBM rows of weff <= W words: the pair mask
-// rows[a] & rows[j] is held in registers (W words, statically indexed), so
-// the innermost loop loads only rows[k].  Same item order as
-// lut_count_tri_wide.
-template <int W>
-__device__ __forceinline__ u64 lut_count_tri4_reg(const u64 *__restrict__ rows, int s, int weff, int tid,
-                                                  int nthreads) {
-  u64 acc = 0;
-  const int items = s * weff;
-  for (int t = tid; t < items; t += nthreads) {
-    const int a = t / weff, q = t - a * weff;
-    if (q < (a >> 6)) continue;
-    const u64 *ra = rows + (size_t)a * weff;
-    u64 bits = ra[q];
-    while (bits) {
-      const int j = q * 64 + __builtin_ctzll(bits);
-      bits &= bits - 1;
-      const u64 *rj = rows + (size_t)j * weff;
-      const int ws = (j + 1) >> 6;
-      u64 tw[W];
-#pragma unroll
-      for (int w = 0; w < W; ++w) tw[w] = (w >= ws && w < weff) ? (ra[w] & rj[w]) : 0ull;
-#pragma unroll
-      for (int w2 = 0; w2 < W; ++w2) {
-        u64 tb = tw[w2];
-        while (tb) {
-          const int k = w2 * 64 + __builtin_ctzll(tb);
-          tb &= tb - 1;
-          const u64 *rk = rows + (size_t)k * weff;
-          const int ks = (k + 1) >> 6;
-#pragma unroll
-          for (int w = 0; w < W; ++w)
-            if (w >= ks && w < weff) acc += __popcll(tw[w] & rk[w]);
-        }
-      }
-    }
-  }
-  return acc;
-}
-
 // Rank-ordered vertex LUT for hubs (1024 < d+(v0) <= GMAX): the staging
 // (ids, ranks, sort keys, segment table) stays in LDS, the s x s-bit rows
 // live in this block's HBM slot (GMAX * GMAX / 8 bytes).
@@ -434,7 +489,7 @@ struct GStage {
     Segs<GMAX> seg;
   };
   int flagrow[NT / 64][64];
-  int ns, L, item;
+  int ns, L, item, rowq;
 };
 
 template <int GMAX, int NT>
@@ -499,11 +554,16 @@ __global__ __launch_bounds__(NT) void k_clique_vlut_rank_hbm(const int64_t *__re
       // (acquire invalidates this CU's L1) is needed before plain loads, or
       // lines cached by the zero fill or an earlier task read stale.
       __threadfence();
+      if (tid == 0) S.rowq = 0;
       __syncthreads();
-      if (depth == 4 && weff <= 32)
-        part += lut_count_tri4_reg<32>(rows, s, weff, tid, NT);
-      else
+      if (depth >= 3 && weff <= 32) {
+        // the sort keys (dead after the LUT build) hold the per-wave queues
+        static_assert(sizeof(S.key) >= (NT / 64) * 128 * sizeof(int), "queue space");
+        part += lut_count_wq<32, true>(HbmRows{rows, weff}, s, weff, depth, reinterpret_cast<int *>(S.key) + wave * 128,
+                                       &S.rowq, lane);
+      } else {
         part += lut_count_tri_wide(rows, s, weff, depth, tid, NT);
+      }
     }
     task_flush(acc, part, slots, v0, out, tid & 63);
     __syncthreads();
