@@ -319,11 +319,14 @@ def _hub_graph(xgm, nb=1100, nl=1200, inner=10, seed=3):
     return xgm.from_edges(1 + nb + nl, np.concatenate(src), np.concatenate(dst))
 
 
-def test_clique_hubs_hbm_lut(xgm, orc, ctx):
+@pytest.mark.parametrize("nb,nl", [(1100, 1200), (1500, 1500)])
+def test_clique_hubs_hbm_lut(xgm, orc, ctx, nb, nl):
+    """Hub tasks above the 1024-row LDS LUT: out-degree 1100 runs the packed
+    triangular LDS LUT (<= 1344 rows), 1500 the LUT with rows in HBM."""
     import numpy as np
-    _say("clique hubs hbm lut")
-    d = _hub_graph(xgm).orient()
-    assert int(np.diff(d.rowptr).max()) > 1024
+    _say("clique hubs lut", nb, nl)
+    d = _hub_graph(xgm, nb=nb, nl=nl).orient()
+    assert int(np.diff(d.rowptr).max()) == nb
     want4 = orc.count("clique4", d.rowptr, d.colidx)[0]
     want5 = orc.count("clique5", d.rowptr, d.colidx)[0]
     dg = ctx.upload(d)

@@ -436,6 +436,114 @@ __global__ __launch_bounds__(NT) void k_clique_vlut_rank(const int64_t *__restri
   if (lane == 0 && acc) atomicAdd(out, acc);
 }
 
+// Packed upper-triangular rows for the rank LUT: row a stores only its
+// words w >= a/64 (the rank order puts every bit of row a above a), so s
+// rows of weff words take about half of s * weff.  tri_base(a) + w is the
+// address of word w of row a.
+__host__ __device__ constexpr int tri_words(int s, int weff) {
+  return 64 * ((s >> 6) * weff - (s >> 6) * ((s >> 6) - 1) / 2) + (s & 63) * (weff - (s >> 6));
+}
+__device__ __forceinline__ int tri_base(int a, int weff) { return tri_words(a, weff) - (a >> 6); }
+
+struct TriRows {
+  const u64 *rows;
+  int weff;
+  __device__ __forceinline__ u64 operator()(int r, int w) const { return rows[tri_base(r, weff) + w]; }
+};
+
+// Rank-ordered LUT for the smaller hubs (1024 < d+(v0) <= SMAX) with the
+// rows packed upper-triangular in LDS instead of in HBM: at SMAX = 1344
+// (21 words per row) the rows take 118 KB.  The segment-build scratch lives
+// in the rows' space before they are zeroed; the sort keys hold the count
+// queues afterwards.  Tasks outside [min_s, max_s] are left to other kernels.
+template <int SMAX, int W, int NT>
+struct TStage {
+  int x[SMAX];
+  int perm[SMAX];
+  union {
+    u64 rows[tri_words(SMAX, W)];
+    int tmp[SMAX];
+  };
+  union {
+    struct {
+      u64 key[2 * 1024];  // bitonic sort pads 1024 < s <= SMAX to 2048
+      int val[2 * 1024];
+    };
+    Segs<SMAX> seg;
+  };
+  int flagrow[NT / 64][64];
+  int ns, L, item, rowq;
+};
+
+template <int SMAX, int W, int NT>
+__global__ __launch_bounds__(NT) void k_clique_vlut_rank_tri(const int64_t *__restrict__ rowptr,
+                                                             const int32_t *__restrict__ colidx,
+                                                             const int64_t *__restrict__ rrowptr,
+                                                             const int32_t *__restrict__ tasks, int ntasks, int min_s,
+                                                             int max_s, int depth, int *__restrict__ queue,
+                                                             u64 *__restrict__ out, u64 *__restrict__ err,
+                                                             u64 *__restrict__ tlog, const int32_t *__restrict__ slots) {
+  static_assert(SMAX <= 2 * 1024 && W * 64 >= SMAX, "stage bounds");
+  __shared__ TStage<SMAX, W, NT> S;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  WaveFlags fl;
+  fl.init(S.flagrow[wave], lane);
+  u64 acc = 0;
+  for (;;) {
+    if (tid == 0) S.item = atomicAdd(queue, 1);
+    __syncthreads();
+    const int t = S.item;
+    if (t >= ntasks) break;
+    const u64 t_start = tlog ? (u64)wall_clock64() : 0;
+    u64 part = 0;
+    const int v0 = tasks[t];
+    const int64_t b = rowptr[v0];
+    const int s = (int)(rowptr[v0 + 1] - b);
+    const bool mine = s >= min_s && s <= max_s;
+    if (mine) {
+      const int weff = (s + 63) >> 6;
+      for (int q = tid; q < s; q += NT) {
+        const int x = colidx[b + q];
+        S.x[q] = x;
+        const u64 deg = (u64)((rowptr[x + 1] - rowptr[x]) + (rrowptr[x + 1] - rrowptr[x]));
+        S.key[q] = (deg << 32) | (u64)(uint32_t)x;  // (undirected degree, id): the orientation order
+        S.val[q] = q;
+      }
+      __syncthreads();
+      block_bitonic_sort<2 * 1024, NT>(S.key, S.val, s);
+      for (int r = tid; r < s; r += NT) S.perm[S.val[r]] = r;
+      __syncthreads();  // keys are overwritten by the segment table
+      block_segments<NT>(S.seg, S.tmp, rowptr, colidx, S.x, 0, s, -1, KeepAll(), &S.ns, &S.L);
+      for (int q = tid; q < tri_words(s, weff); q += NT) S.rows[q] = 0ull;  // tmp is dead
+      __syncthreads();
+      int lo, hi;
+      wave_share(S.L, wave, NT / 64, lo, hi);
+      wave_walk<4>(S.seg, colidx, S.ns, lo, hi, lane, fl, [&](const WindowLane &r) {
+        if (r.valid) {
+          const int pos = lower_bound_dev(S.x, s, r.w);
+          if (pos < s && S.x[pos] == r.w) {
+            const int col = S.perm[pos], row = S.perm[S.seg.idx[r.j]];
+            if (col <= row)
+              atomicOr(err, 1ull);  // DAG not in (degree, id) orientation
+            else
+              atomicOr(&S.rows[tri_base(row, weff) + (col >> 6)], 1ull << (col & 63));
+          }
+        }
+      });
+      if (tid == 0) S.rowq = 0;
+      __syncthreads();
+      static_assert(sizeof(S.key) >= (NT / 64) * 128 * sizeof(int), "queue space");
+      part += lut_count_wq<W, true>(TriRows{S.rows, weff}, s, weff, depth, reinterpret_cast<int *>(S.key) + wave * 128,
+                                    &S.rowq, lane);
+    }
+    task_flush(acc, part, slots, v0, out, tid & 63);
+    __syncthreads();
+    if (mine && tlog && tid == 0) tlog[t] = (u64)wall_clock64() - t_start;
+  }
+  acc = wave_sum_u64(acc);
+  if (lane == 0 && acc) atomicAdd(out, acc);
+}
+
 // Triangular counting over rows in HBM with a runtime word count (no
 // per-thread row copies): K5 terms recompute row_a & row_j per word.
 __device__ __forceinline__ u64 lut_count_tri_wide(const u64 *__restrict__ rows, int s, int weff, int depth, int tid,
@@ -517,7 +625,8 @@ __global__ __launch_bounds__(NT) void k_clique_vlut_rank_hbm(const int64_t *__re
     const int v0 = tasks[t];
     const int64_t b = rowptr[v0];
     const int s = (int)(rowptr[v0 + 1] - b);
-    if (s >= min_s && s <= GMAX) {
+    const bool mine = s >= min_s && s <= GMAX;
+    if (mine) {
       const int weff = (s + 63) >> 6;
       for (int q = tid; q < s; q += NT) {
         const int x = colidx[b + q];
@@ -567,7 +676,7 @@ __global__ __launch_bounds__(NT) void k_clique_vlut_rank_hbm(const int64_t *__re
     }
     task_flush(acc, part, slots, v0, out, tid & 63);
     __syncthreads();
-    if (tlog && tid == 0) tlog[t] = (u64)wall_clock64() - t_start;
+    if (mine && tlog && tid == 0) tlog[t] = (u64)wall_clock64() - t_start;
   }
   acc = wave_sum_u64(acc);
   if (lane == 0 && acc) atomicAdd(out, acc);
@@ -807,14 +916,25 @@ hipError_t run_vlut_rank(const char *name, const DevCSR &g, const TaskList &tl, 
   return hipGetLastError();
 }
 
-hipError_t run_vlut_rank_hbm(const char *name, const DevCSR &g, const TaskList &tl, int k, u64 *slots, int *queue,
-                             u64 *tlog, Launch &l) {
+// Hub tasks (1024 < d+ <= kHubLutMax): those up to kTriLutMax with packed
+// triangular rows in LDS, the rest with rows in HBM.  Both kernels walk the
+// whole list and skip the other's tasks (queue and queue + 1).
+hipError_t run_vlut_rank_hubs(const char *name_tri, const char *name_hbm, const DevCSR &g, const TaskList &tl, int k,
+                              u64 *slots, int *queue, u64 *tlog, Launch &l) {
   if (!tl.count) return hipSuccess;
+  auto tri = k_clique_vlut_rank_tri<kTriLutMax, kTriLutMax / 64, 1024>;
+  int grid = grid_for(tri, 1024, 0, l.num_cu, tl.count);
+  l.begin(name_tri, tl.count);
+  tri<<<grid, 1024, 0, l.stream>>>(g.rowptr, g.colidx, g.rrowptr, tl.tasks, (int)tl.count, 1024 + 1,
+                                   kTriLutMax, k - 1, queue, l.d_out, l.d_err, tlog, l.slots);
+  l.end();
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
   auto kern = k_clique_vlut_rank_hbm<kHubLutMax, 1024>;
-  int grid = std::min(grid_for(kern, 1024, 0, l.num_cu, tl.count), clique_hub_slots(l.num_cu));
-  l.begin(name, tl.count);
-  kern<<<grid, 1024, 0, l.stream>>>(g.rowptr, g.colidx, g.rrowptr, tl.tasks, (int)tl.count, k - 1, k - 1, slots,
-                                    queue, l.d_out, l.d_err, tlog, l.slots);
+  grid = std::min(grid_for(kern, 1024, 0, l.num_cu, tl.count), clique_hub_slots(l.num_cu));
+  l.begin(name_hbm, tl.count);
+  kern<<<grid, 1024, 0, l.stream>>>(g.rowptr, g.colidx, g.rrowptr, tl.tasks, (int)tl.count, kTriLutMax + 1, k - 1,
+                                    slots, queue + 1, l.d_out, l.d_err, tlog, l.slots);
   l.end();
   return hipGetLastError();
 }
@@ -885,8 +1005,8 @@ hipError_t launch_clique(const DevCSR &g, const CliqueTasks &ct, int k, int *scr
   if ((e = run_edge<128, 0, 64>(k4 ? "k4_edge_list_128" : "k5_edge_list_128", g, ct.list[0], k, scratch, md,
                                 d_queues + 11, ct.log_list[0], l)))
     return e;
-  if ((e = run_vlut_rank_hbm(k4 ? "k4_rlut_hbm" : "k5_rlut_hbm", g, ct.rlut_hbm, k, ct.hub_slots, d_queues + 12,
-                             ct.log_rlut_hbm, l)))
+  if ((e = run_vlut_rank_hubs(k4 ? "k4_rlut_tri" : "k5_rlut_tri", k4 ? "k4_rlut_hbm" : "k5_rlut_hbm", g, ct.rlut_hbm,
+                              k, ct.hub_slots, d_queues + 12, ct.log_rlut_hbm, l)))
     return e;
   if ((e = run_vlut_rank<1024, 16, 1024>(k4 ? "k4_rlut_1024" : "k5_rlut_1024", g, ct.rlut[3], k, d_queues + 4,
                                          ct.log_rlut[3], l)))

@@ -86,6 +86,9 @@ struct CliqueTasks {
   u64 *log_lut[4] = {}, *log_rlut[4] = {}, *log_elut[2] = {}, *log_list[2] = {}, *log_rlut_hbm = nullptr;
 };
 constexpr int kHubLutMax = 4096;
+// Hubs up to kTriLutMax (21 row words) keep the rank LUT in LDS as packed
+// upper-triangular rows; larger ones up to kHubLutMax use rows in HBM.
+constexpr int kTriLutMax = 1344;
 
 // k-clique counting (k = 4 or 5) over a DAG.  scratch holds
 // clique_edge_grid() * 2 * md ints (md = max out-degree).  Adds into l.d_out.
