@@ -451,13 +451,17 @@ struct TriRows {
   __device__ __forceinline__ u64 operator()(int r, int w) const { return rows[tri_base(r, weff) + w]; }
 };
 
-// Rank-ordered LUT for the smaller hubs (1024 < d+(v0) <= SMAX) with the
-// rows packed upper-triangular in LDS instead of in HBM: at SMAX = 1344
-// (21 words per row) the rows take 118 KB.  The segment-build scratch lives
+// Rank-ordered LUT with the rows packed upper-triangular in LDS: for the
+// smaller hubs (1024 < d+(v0) <= 1344, 21 words per row, 118 KB instead of
+// rows in HBM) and for the 512-row bin (18 KB instead of 33 KB, so more
+// blocks fit a CU).  The segment-build scratch lives
 // in the rows' space before they are zeroed; the sort keys hold the count
 // queues afterwards.  Tasks outside [min_s, max_s] are left to other kernels.
+constexpr int pow2_ceil(int n) { return n <= 1 ? 1 : 2 * pow2_ceil((n + 1) / 2); }
+
 template <int SMAX, int W, int NT>
 struct TStage {
+  static constexpr int KP = pow2_ceil(SMAX);  // the bitonic sort pads to a power of two
   int x[SMAX];
   int perm[SMAX];
   union {
@@ -466,8 +470,8 @@ struct TStage {
   };
   union {
     struct {
-      u64 key[2 * 1024];  // bitonic sort pads 1024 < s <= SMAX to 2048
-      int val[2 * 1024];
+      u64 key[KP];
+      int val[KP];
     };
     Segs<SMAX> seg;
   };
@@ -483,7 +487,7 @@ __global__ __launch_bounds__(NT) void k_clique_vlut_rank_tri(const int64_t *__re
                                                              int max_s, int depth, int *__restrict__ queue,
                                                              u64 *__restrict__ out, u64 *__restrict__ err,
                                                              u64 *__restrict__ tlog, const int32_t *__restrict__ slots) {
-  static_assert(SMAX <= 2 * 1024 && W * 64 >= SMAX, "stage bounds");
+  static_assert(W * 64 >= SMAX, "stage bounds");
   __shared__ TStage<SMAX, W, NT> S;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   WaveFlags fl;
@@ -510,7 +514,7 @@ __global__ __launch_bounds__(NT) void k_clique_vlut_rank_tri(const int64_t *__re
         S.val[q] = q;
       }
       __syncthreads();
-      block_bitonic_sort<2 * 1024, NT>(S.key, S.val, s);
+      block_bitonic_sort<TStage<SMAX, W, NT>::KP, NT>(S.key, S.val, s);
       for (int r = tid; r < s; r += NT) S.perm[S.val[r]] = r;
       __syncthreads();  // keys are overwritten by the segment table
       block_segments<NT>(S.seg, S.tmp, rowptr, colidx, S.x, 0, s, -1, KeepAll(), &S.ns, &S.L);
@@ -916,6 +920,20 @@ hipError_t run_vlut_rank(const char *name, const DevCSR &g, const TaskList &tl, 
   return hipGetLastError();
 }
 
+// Rank-LUT bin with packed triangular rows (every task of the list).
+template <int SMAX, int W, int NT>
+hipError_t run_vlut_rank_tri(const char *name, const DevCSR &g, const TaskList &tl, int k, int *queue, u64 *tlog,
+                             Launch &l) {
+  if (!tl.count) return hipSuccess;
+  auto kern = k_clique_vlut_rank_tri<SMAX, W, NT>;
+  int grid = grid_for(kern, NT, 0, l.num_cu, tl.count);
+  l.begin(name, tl.count);
+  kern<<<grid, NT, 0, l.stream>>>(g.rowptr, g.colidx, g.rrowptr, tl.tasks, (int)tl.count, k - 1, SMAX, k - 1, queue,
+                                  l.d_out, l.d_err, tlog, l.slots);
+  l.end();
+  return hipGetLastError();
+}
+
 // Hub tasks (1024 < d+ <= kHubLutMax): those up to kTriLutMax with packed
 // triangular rows in LDS, the rest with rows in HBM.  Both kernels walk the
 // whole list and skip the other's tasks (queue and queue + 1).
@@ -966,7 +984,7 @@ int clique_list_capacity(int num_cu, int li) {
     case 3: return grid_for(k_clique_vlut<1024, 16, 1024>, 1024, 0, num_cu, inf);
     case 4: return grid_for(k_clique_vlut_rank<64, 1, 64>, 64, 0, num_cu, inf);
     case 5: return grid_for(k_clique_vlut_rank<256, 4, 256>, 256, 0, num_cu, inf);
-    case 6: return grid_for(k_clique_vlut_rank<512, 8, 256>, 256, 0, num_cu, inf);
+    case 6: return grid_for(k_clique_vlut_rank_tri<512, 8, 256>, 256, 0, num_cu, inf);
     case 7: return grid_for(k_clique_vlut_rank<1024, 16, 1024>, 1024, 0, num_cu, inf);
     case 8: return grid_for(k_clique_edge<128, 2, 64>, 64, 0, num_cu, inf);
     case 9: return grid_for(k_clique_edge<1024, 16, 256>, 256, 0, num_cu, inf);
@@ -1014,7 +1032,7 @@ hipError_t launch_clique(const DevCSR &g, const CliqueTasks &ct, int k, int *scr
   if ((e = run_vlut<1024, 16, 1024>(k4 ? "k4_lut_1024" : "k5_lut_1024", g, ct.lut[3], k, d_queues + 0, ct.log_lut[3],
                                     l)))
     return e;
-  if ((e = run_vlut_rank<512, 8, 256>(k4 ? "k4_rlut_512" : "k5_rlut_512", g, ct.rlut[2], k, d_queues + 5,
+  if ((e = run_vlut_rank_tri<512, 8, 256>(k4 ? "k4_rlut_512" : "k5_rlut_512", g, ct.rlut[2], k, d_queues + 5,
                                       ct.log_rlut[2], l)))
     return e;
   if ((e = run_vlut<512, 8, 256>(k4 ? "k4_lut_512" : "k5_lut_512", g, ct.lut[2], k, d_queues + 1, ct.log_lut[2], l)))
