@@ -150,21 +150,24 @@ enum SetOp { SETOP_INTERSECT = 1, SETOP_DIFF = 2, SETOP_DIFF_EXCL = 3 };
 hipError_t launch_setop_pairs(const DevCSR &g, int op, const int32_t *a, const int32_t *b, const int32_t *bounds,
                               int64_t npairs, uint32_t *counts, const int64_t *offsets, int32_t *values, Launch &l);
 
-// On-device degree orientation (k_prep.hip): deg[v]; per-vertex out/in
-// sizes (ocol == nullptr) or the out/in lists at orow/irow offsets; the task
-// sort keys; and the hipCUB scan / descending radix sort behind them
-// (temp == nullptr sizes the temporary storage).
+// On-device degree orientation (k_prep.hip): deg[v]; out/in flag per entry
+// of the undirected CSR (flag[nnz] = 0); after an exclusive scan pos of the
+// nnz + 1 flags, the out/in lists and row pointers; the task sort keys; and
+// the hipCUB scan / descending radix sort behind them (temp == nullptr sizes
+// the temporary storage).
 hipError_t launch_prep_degree(const int64_t *rowptr, int64_t n, int32_t *deg, int num_cu, hipStream_t s);
-hipError_t launch_prep_split(const int64_t *rowptr, const int32_t *colidx, const int32_t *deg, int64_t n,
-                             int64_t *cnt_out, int64_t *cnt_in, const int64_t *orow, int32_t *ocol,
-                             const int64_t *irow, int32_t *icol, int num_cu, hipStream_t s);
+hipError_t launch_prep_split(const int64_t *rowptr, const int32_t *colidx, const int32_t *deg, int64_t n, int64_t nnz,
+                             int32_t *flag, int num_cu, hipStream_t s);
+hipError_t launch_prep_scatter(const int64_t *rowptr, const int32_t *colidx, const int32_t *flag, const int32_t *pos,
+                               int64_t n, int64_t nnz, int64_t *orow, int32_t *ocol, int64_t *irow, int32_t *icol,
+                               int num_cu, hipStream_t s);
 hipError_t launch_prep_work(const int64_t *orow, const int32_t *ocol, int64_t n, uint32_t *key, int32_t *ids,
                             int32_t *dplus, int num_cu, hipStream_t s);
 // bad[0] |= 1 if some entry v > u of N(u) lacks its mirror; bad[1] / bad[2]
 // += entries above / below the diagonal (bad zeroed by the caller).
-hipError_t launch_prep_mirror(const int64_t *rowptr, const int32_t *colidx, int64_t n, u64 *bad, int num_cu,
-                              hipStream_t s);
-hipError_t prep_scan(void *temp, size_t &bytes, const int64_t *in, int64_t *out, int64_t n, hipStream_t s);
+hipError_t launch_prep_mirror(const int64_t *rowptr, const int32_t *colidx, int64_t n, int64_t nnz, u64 *bad,
+                              int num_cu, hipStream_t s);
+hipError_t prep_scan(void *temp, size_t &bytes, const int32_t *in, int32_t *out, int64_t n, hipStream_t s);
 hipError_t prep_sort(void *temp, size_t &bytes, const uint32_t *key_in, uint32_t *key_out, const int32_t *ids_in,
                      int32_t *ids_out, int64_t n, hipStream_t s);
 

@@ -658,8 +658,9 @@ struct DevBuf {
 };
 
 // Degree orientation of the undirected device graph u, in HBM (k_prep.hip):
-// degrees; out/in sizes; scans into rowptr / rrowptr; out/in lists; task
-// sort keys and a stable descending radix sort.  The host mirrors that the
+// degrees; an out/in flag per entry and its exclusive scan, which place the
+// out/in lists and give both row pointers; task sort keys and a stable
+// descending radix sort.  The host mirrors that the
 // task preparation reads (CSR, reverse CSR, d+, order) are copied back.
 // Equal to uploading xgm_orient_dag() of the host CSR.
 int orient_device(xgm_ctx *ctx, const xgm_dgraph *u, std::unique_ptr<xgm_dgraph> &out) {
@@ -673,40 +674,35 @@ int orient_device(xgm_ctx *ctx, const xgm_dgraph *u, std::unique_ptr<xgm_dgraph>
   d->n = n;
   d->flags = XGM_CSR_SORTED | XGM_CSR_DAG;
   const size_t n1 = (size_t)std::max<int64_t>(1, n);
-  DevBuf deg, cnt, keys, ids, dplus, tmp;
+  const int64_t m = u->nnz;
+  DevBuf deg, flag, pos, keys, ids, dplus, tmp;
   HIP_TRY(ctx, hipMalloc(&deg.p, n1 * sizeof(int32_t)));
-  HIP_TRY(ctx, hipMalloc(&cnt.p, 2 * n1 * sizeof(int64_t)));
+  HIP_TRY(ctx, hipMalloc(&flag.p, (m + 1) * sizeof(int32_t)));
+  HIP_TRY(ctx, hipMalloc(&pos.p, (m + 1) * sizeof(int32_t)));
   HIP_TRY(ctx, hipMalloc(&d->d_rowptr, (n + 1) * sizeof(int64_t)));
   HIP_TRY(ctx, hipMalloc(&d->d_rrowptr, (n + 1) * sizeof(int64_t)));
-  int64_t *c_out = cnt.as<int64_t>(), *c_in = c_out + n1;
   HIP_TRY(ctx, launch_prep_degree(u->d_rowptr, n, deg.as<int32_t>(), cu, s));
-  HIP_TRY(ctx, launch_prep_split(u->d_rowptr, u->d_colidx, deg.as<int32_t>(), n, c_out, c_in, nullptr, nullptr,
-                                 nullptr, nullptr, cu, s));
-  HIP_TRY(ctx, launch_zero(d->d_rowptr, 1, s));
-  HIP_TRY(ctx, launch_zero(d->d_rrowptr, 1, s));
+  HIP_TRY(ctx, launch_prep_split(u->d_rowptr, u->d_colidx, deg.as<int32_t>(), n, m, flag.as<int32_t>(), cu, s));
   size_t scan_bytes = 0, sort_bytes = 0;
-  HIP_TRY(ctx, prep_scan(nullptr, scan_bytes, c_out, d->d_rowptr + 1, n, s));
+  HIP_TRY(ctx, prep_scan(nullptr, scan_bytes, flag.as<int32_t>(), pos.as<int32_t>(), m + 1, s));
   HIP_TRY(ctx, prep_sort(nullptr, sort_bytes, nullptr, nullptr, nullptr, nullptr, n, s));
   size_t tmp_bytes = std::max<size_t>(16, std::max(scan_bytes, sort_bytes));
   HIP_TRY(ctx, hipMalloc(&tmp.p, tmp_bytes));
-  if (n) {
+  {
     size_t b = tmp_bytes;
-    HIP_TRY(ctx, prep_scan(tmp.p, b, c_out, d->d_rowptr + 1, n, s));
-    b = tmp_bytes;
-    HIP_TRY(ctx, prep_scan(tmp.p, b, c_in, d->d_rrowptr + 1, n, s));
+    HIP_TRY(ctx, prep_scan(tmp.p, b, flag.as<int32_t>(), pos.as<int32_t>(), m + 1, s));
   }
-  int64_t tot[2] = {0, 0};
-  HIP_TRY(ctx, hipMemcpyAsync(&tot[0], d->d_rowptr + n, sizeof(int64_t), hipMemcpyDeviceToHost, s));
-  HIP_TRY(ctx, hipMemcpyAsync(&tot[1], d->d_rrowptr + n, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+  int32_t n_out = 0;
+  HIP_TRY(ctx, hipMemcpyAsync(&n_out, pos.as<int32_t>() + m, sizeof(int32_t), hipMemcpyDeviceToHost, s));
   HIP_TRY(ctx, hipStreamSynchronize(s));
-  // In a symmetric graph every kept edge {u, w} is one out- and one in-entry.
-  if (tot[0] != tot[1]) return ctx->fail(XGM_EINVAL, "graph is not symmetric (out and in totals differ)");
-  d->nnz = tot[0];
+  // In a symmetric graph every edge {u, w} is one out- and one in-entry.
+  if (2 * (int64_t)n_out != m) return ctx->fail(XGM_EINVAL, "graph is not symmetric (out and in totals differ)");
+  d->nnz = n_out;
   const size_t m1 = (size_t)std::max<int64_t>(1, d->nnz);
   HIP_TRY(ctx, hipMalloc(&d->d_colidx, m1 * sizeof(int32_t)));
   HIP_TRY(ctx, hipMalloc(&d->d_rcolidx, m1 * sizeof(int32_t)));
-  HIP_TRY(ctx, launch_prep_split(u->d_rowptr, u->d_colidx, deg.as<int32_t>(), n, c_out, c_in, d->d_rowptr,
-                                 d->d_colidx, d->d_rrowptr, d->d_rcolidx, cu, s));
+  HIP_TRY(ctx, launch_prep_scatter(u->d_rowptr, u->d_colidx, flag.as<int32_t>(), pos.as<int32_t>(), n, m, d->d_rowptr,
+                                   d->d_colidx, d->d_rrowptr, d->d_rcolidx, cu, s));
   HIP_TRY(ctx, hipMalloc(&keys.p, 2 * n1 * sizeof(uint32_t)));
   HIP_TRY(ctx, hipMalloc(&ids.p, 2 * n1 * sizeof(int32_t)));
   HIP_TRY(ctx, hipMalloc(&dplus.p, n1 * sizeof(int32_t)));
@@ -822,7 +818,7 @@ int xgm_graph_upload(xgm_ctx *ctx, const xgm_csr *g, xgm_dgraph **out) {
     u64 h_bad[3] = {0, 0, 0};
     HIP_TRY(ctx, hipMalloc(&bad.p, 4 * sizeof(u64)));
     HIP_TRY(ctx, launch_zero(bad.p, 4, ctx->stream));
-    HIP_TRY(ctx, launch_prep_mirror(d->d_rowptr, d->d_colidx, g->n, bad.as<u64>(), ctx->num_cu, ctx->stream));
+    HIP_TRY(ctx, launch_prep_mirror(d->d_rowptr, d->d_colidx, g->n, g->nnz, bad.as<u64>(), ctx->num_cu, ctx->stream));
     HIP_TRY(ctx, hipMemcpyAsync(h_bad, bad.p, 3 * sizeof(u64), hipMemcpyDeviceToHost, ctx->stream));
     HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
     if (h_bad[0] || h_bad[1] != h_bad[2]) return ctx->fail(XGM_EINVAL, "CSR failed validation (not symmetric)");
