@@ -341,17 +341,23 @@ def test_clique_hubs_hbm_lut(xgm, orc, ctx, nb, nl):
     dg.close()
 
 
-def test_sgl_sampled_hub_owner(xgm, ctx, orc):
-    """An owner of degree 39,999 (> the 36,352-entry LDS set): diamonds and
-    the motif census through the 1024-sample index over HBM."""
+@pytest.mark.parametrize("n,spread", [(40000, 1), (20000, 1), (20000, 97)])
+def test_sgl_sampled_hub_owner(xgm, ctx, orc, n, spread):
+    """An owner of degree n - 1: above the 36,352-entry LDS set the probes go
+    through the 1024-sample index over HBM, below it to the sorted set in
+    LDS (spread 97 puts the ring on ids spaced 97 apart).  Diamonds and the
+    motif census."""
     import numpy as np
-    _say("sgl sampled hub")
-    n = 40000
+    _say("sgl hub owner", n, spread)
     ring = np.arange(1, 3001)
     src = np.concatenate([np.zeros(n - 1, np.int64), ring, ring[::3]])
     dst = np.concatenate([np.arange(1, n), ring % 3000 + 1, (ring[::3] + 7) % 3000 + 1])
+    if spread > 1:  # ring vertex r -> id r * spread (mod n - 1) + 1, a bijection on 1..n-1
+        perm = np.arange(n)
+        perm[1:] = (np.arange(n - 1) * spread) % (n - 1) + 1
+        src, dst = perm[src], perm[dst]
     g = xgm.from_edges(n, src, dst)
-    assert int(np.diff(g.rowptr).max()) > 36352
+    assert int(np.diff(g.rowptr).max()) == n - 1
     assert ctx.count(g, 4) == orc.count("diamond", g.rowptr, g.colidx)[0]
     dg = ctx.upload(g)
     assert ctx.motifs(dg, 4)[0].tolist() == orc.motif(g.rowptr, g.colidx, 4)
