@@ -244,8 +244,9 @@ int xgm_mlp_grads(xgm_mlp *m, const float *X, const float *T, const float *M, in
 /* Clique schedules: one code per vertex -- 0 id-ordered LUT, 1 rank-ordered
  * LUT, 2 edge LUT, 3 edge nested lists, 4 rank-ordered LUT one capacity bin
  * up; -1 for vertices that are not tasks (d+ < k-1).  On vertices with
- * 1024 < d+ <= 4096 the rank-ordered codes (1, 4) keep the LUT rows in HBM;
- * other LUT codes above 1024 run as edge LUT.  Every schedule gives the same
+ * 1024 < d+ <= 4096 the rank-ordered codes (1, 4) run the hub LUT (rows
+ * packed upper-triangular in LDS up to 1344, in HBM above); other LUT codes
+ * above 1024 run as edge LUT.  Every schedule gives the same
  * count. */
 typedef struct xgm_sched xgm_sched;
 /* Degree-threshold codes of a strategy (XGM_STRAT_*; AUTO = LUT up to

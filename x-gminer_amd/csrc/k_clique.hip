@@ -1134,7 +1134,7 @@ __global__ void k_sched_argmin(const int64_t *__restrict__ rowptr, int64_t n, co
   int best = 2;
   float bv = y[2];
   for (int c = 0; c < 5; ++c) {
-    // id-ordered LUT up to 1024 rows; rank LUT up to kHubLutMax (HBM rows above 1024)
+    // id-ordered LUT up to 1024 rows; rank LUT up to kHubLutMax (hub kernels above 1024)
     const bool ok = (c == 2 || c == 3) || (c == 0 ? d <= 1024 : d <= kHubLutMax);
     if (ok && y[c] < bv) { bv = y[c]; best = c; }
   }

@@ -81,7 +81,7 @@ struct EdgeItems {
 struct CliqueTasks {
   TaskList lut[4], rlut[4];
   EdgeItems elut[2], list[2];
-  TaskList rlut_hbm;        // rank-ordered LUT with rows in HBM (1024 < d+ <= kHubLutMax)
+  TaskList rlut_hbm;        // rank-ordered LUT for hubs (1024 < d+ <= kHubLutMax): LDS up to kTriLutMax, HBM above
   u64 *hub_slots = nullptr; // clique_hub_slots() * kHubLutMax^2 / 64 words
   u64 *log_lut[4] = {}, *log_rlut[4] = {}, *log_elut[2] = {}, *log_list[2] = {}, *log_rlut_hbm = nullptr;
 };

@@ -54,7 +54,8 @@ constexpr int kBinCap[4] = {64, 256, 512, 1024};
 
 // Lists of a clique schedule: 0-3 id-ordered LUT bins, 4-7 rank-ordered
 // LUT bins, 8-9 edge LUT (min out-degree <= 128 / larger), 10-11 edge list,
-// 12 rank-ordered LUT with rows in HBM (1024 < d+ <= kHubLutMax).
+// 12 rank-ordered LUT for hubs (1024 < d+ <= kHubLutMax; rows in LDS up to
+// kTriLutMax, in HBM above).
 
 struct xgm_sched {
   int pattern = 0, k = 0;
@@ -546,7 +547,7 @@ void build_tc_items(const xgm_dgraph *g, int max_small, int max_large, std::vect
 
 // codes[v]: 0 LUT, 1 rank LUT, 2 edge LUT, 3 edge list, 4 rank LUT one bin
 // up; vertices with d+ < k-1 are not tasks.  Rank codes on hubs
-// (1024 < d+ <= kHubLutMax) use the HBM-row LUT; other LUT codes on hubs and
+// (1024 < d+ <= kHubLutMax) use the hub rank LUT; other LUT codes on hubs and
 // unknown codes fall back to the edge LUT.
 int build_clique_sched(xgm_ctx *ctx, const xgm_dgraph *g, int pattern, const int8_t *codes, xgm_sched *sc) {
   const int k = pattern == XGM_CLIQUE4 ? 4 : 5;
@@ -606,7 +607,7 @@ void heuristic_codes(const xgm_dgraph *g, int k, int strategy, int lut_max, int8
       switch (strategy) {
         case XGM_STRAT_LIST: c = 3; break;
         case XGM_STRAT_ELUT: c = 2; break;
-        case XGM_STRAT_LUT_RANK: c = (d <= lut_max || d > 1024) ? 1 : 2; break;  // hubs: HBM-row LUT
+        case XGM_STRAT_LUT_RANK: c = (d <= lut_max || d > 1024) ? 1 : 2; break;  // hubs: hub rank LUT
         default: c = d <= lut_max ? 0 : 2; break;
       }
     }

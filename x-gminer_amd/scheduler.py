@@ -3,7 +3,8 @@
 3 edge nested lists, 4 rank-ordered LUT one capacity bin up).
 
 Codes 1 and 4 on vertices with 1024 < d+ <= 4096 run the rank-ordered LUT
-with its rows in HBM; the plain heuristic sends those vertices to the edge LUT.
+for hubs (rows packed upper-triangular in LDS up to 1344 out-neighbours, in
+HBM above); the plain heuristic sends those vertices to the edge LUT.
 
 * ``collect``: per-vertex features (computed on the GPU) and per-vertex
   cost under every code, from instrumented runs of the plain kernels: the
