@@ -336,6 +336,60 @@ def bench_prep(args, xgm, g, m, desc):
     ctx.close()
 
 
+def bench_motif(args, xgm, g, m, desc, gold_counts):
+    """Vertex-induced 3- or 4-motif census of the undirected workload graph
+    (xgm_motifs: the reference's formula path).  Wall time per census over
+    --steps after --warmup (the oriented copy and task lists are built in the
+    warm-up).  Checks: the census' triangle (k = 3) or 4-clique (k = 4) entry
+    against the golden count of the graph.  CPU baseline: the reference's
+    motif_omp_formula binary on R-MAT-16 (seed 1), a bounded sample."""
+    k = int(args.pattern[-1])
+    ctx = xgm.Context(0)
+    dg = ctx.upload(g)
+    census = None
+    for i in range(max(1, args.warmup)):
+        census = ctx.motifs(dg, k)[0].tolist()
+        log(f"warmup {i}: {census}")
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        c = ctx.motifs(dg, k)[0].tolist()
+        if c != census:
+            raise SystemExit(f"non-deterministic census: {c} vs {census}")
+    ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    check_key, check_idx = ("triangle", 1) if k == 3 else ("clique4", 5)
+    want = gold_counts.get(check_key)
+    if want is not None and census[check_idx] != want:
+        raise SystemExit(f"COUNT MISMATCH vs golden {check_key}: {census[check_idx]} != {want}")
+    cpu = None
+    if not args.no_cpu:
+        import oracle
+        if oracle.ref_available("triangle") and os.access(os.path.join(oracle.REF_DIR, "motif_omp_formula"), os.X_OK):
+            s = xgm.gen_rmat(16, 16, 0.57, 0.19, 0.19, 1)
+            log(f"cpu baseline: reference motif_omp_formula on R-MAT-16, {args.cpu_threads} threads")
+            ref, sec = oracle.ref_motif_timed(s, k, threads=args.cpu_threads, timeout=300)
+            sd = ctx.upload(s)
+            ok = ctx.motifs(sd, k)[0].tolist() == ref
+            sd.close()
+            cpu = dict(value=(s.nnz // 2) / sec, unit="|E|/s", cores=args.cpu_threads, kind="reference",
+                       sample=f"R-MAT-16 seed 1 ({s.nnz // 2} edges), full {k}-motif census (counting loop only)",
+                       seconds=sec, census_matches_gpu=ok)
+    print(json.dumps({
+        "metric": f"|E|/s on {args.workload} ({k}-motif census)", "value": m / (ms * 1e-3), "unit": "|E|/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "int64", "data": "synthetic",
+        "config": {"workload": f"vertex-induced {k}-motif census on {desc}", "n": g.n, "undirected_edges": m},
+        "census": dict(zip(oracle_motif_names(k), census)),
+        "golden_check": {check_key: want, "matches": (want is None) or census[check_idx] == want},
+        "roofline": None, "cpu_baseline": cpu,
+    }), flush=True)
+    ctx.close()
+
+
+def oracle_motif_names(k):
+    return ["wedge", "triangle"] if k == 3 else ["3-star", "4-path", "tailed-triangle", "4-cycle", "diamond",
+                                                  "4-clique"]
+
+
 def bench_sched(args, xgm, d, m, desc, golden):
     """Learned vs heuristic clique schedules on one graph: the learned
     schedule is timed over the K steps; the heuristic (and the all-rank-LUT
@@ -415,7 +469,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--pattern", default="triangle", choices=list(PATTERN_IDS))
+    ap.add_argument("--pattern", default="triangle", choices=list(PATTERN_IDS) + ["motif3", "motif4"])
     ap.add_argument("--workload", default="rmat22", choices=list(WORKLOADS))
     ap.add_argument("--strategy", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
@@ -458,13 +512,16 @@ def main():
     log(f"graph {args.workload}: n={g.n} m={m} hash={g.hash():016x} ({time.time()-t0:.1f}s)")
     gold = json.load(open(os.path.join(ROOT, "tests", "golden", "counts.json")))
     golden = None
-    if gname in gold and gold[gname]["hash"] == f"{g.hash():016x}":
+    golden_hash_ok = gname in gold and gold[gname]["hash"] == f"{g.hash():016x}"
+    if golden_hash_ok:
         golden = gold[gname]["counts"].get(args.pattern)
 
     if args.sched == "compare":
         return bench_sched(args, xgm, d, m, desc, golden)
     if args.prep == "compare":
         return bench_prep(args, xgm, g, m, desc)
+    if args.pattern.startswith("motif"):
+        return bench_motif(args, xgm, g, m, desc, gold.get(gname, {}).get("counts", {}) if golden_hash_ok else {})
 
     # xgm_count is synchronous on the library's own stream (it waits for its
     # kernels and the D2H copy of the count), so no extra device barrier is

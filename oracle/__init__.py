@@ -157,8 +157,9 @@ def motif(rowptr, colidx, k, threads=0):
     return [int(out[i]) for i in range(len(MOTIF_NAMES[k]))]
 
 
-def ref_motif_graph(g, k, threads=0, timeout=3600):
-    """The reference's motif_omp_formula binary on an undirected graph."""
+def ref_motif_timed(g, k, threads=0, timeout=3600):
+    """The reference's motif_omp_formula binary on an undirected graph:
+    (census, seconds of its counting loop)."""
     env = dict(os.environ)
     if threads > 0:
         env["OMP_NUM_THREADS"] = str(threads)
@@ -168,7 +169,13 @@ def ref_motif_graph(g, k, threads=0, timeout=3600):
         out = subprocess.run([os.path.join(REF_DIR, "motif_omp_formula"), prefix, str(k)], capture_output=True,
                              text=True, timeout=timeout, env=env, check=True).stdout
     got = dict((int(a), int(b)) for a, b in re.findall(r"pattern (\d+): (\d+)", out))
-    return [got[i] for i in range(len(MOTIF_NAMES[k]))]
+    t = re.search(r"runtime[^=]*= ([0-9.eE+-]+)", out)
+    return [got[i] for i in range(len(MOTIF_NAMES[k]))], (float(t.group(1)) if t else float("nan"))
+
+
+def ref_motif_graph(g, k, threads=0, timeout=3600):
+    """The reference's motif_omp_formula binary on an undirected graph."""
+    return ref_motif_timed(g, k, threads, timeout)[0]
 
 
 # ---- the reference's own binaries (oracle/_ref) ----
