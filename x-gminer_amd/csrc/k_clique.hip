@@ -161,6 +161,20 @@ __device__ __forceinline__ u64 lut_count_wq(Rows row, int s, int weff, int depth
   return acc;
 }
 
+// lut_count_wq with the row width fixed at compile time when weff lies in
+// [WMIN, WMAX], so the unrolled word loops spend no issue slots on words
+// past the task's last one (wave-uniform branch per task); other widths take
+// the WMAX-wide loop with per-word predicates.
+template <int WMIN, int WMAX, bool TRI, class Rows>
+__device__ __forceinline__ u64 lut_count_wq_fit(Rows row, int s, int weff, int depth, int *jq, int *rowq, int lane) {
+  if constexpr (WMIN < WMAX) {
+    if (weff == WMIN) return lut_count_wq<WMIN, TRI>(row, s, WMIN, depth, jq, rowq, lane);
+    return lut_count_wq_fit<WMIN + 1, WMAX, TRI>(row, s, weff, depth, jq, rowq, lane);
+  } else {
+    return lut_count_wq<WMAX, TRI>(row, s, weff, depth, jq, rowq, lane);
+  }
+}
+
 // Stage shared by the clique kernels: a sorted vertex set x (N+(v0) for a
 // vertex task, W1 for an edge task), a second set x2, the LUT rows over x,
 // the segment table and block-reduction slots.
@@ -238,7 +252,7 @@ __device__ u64 block_sum_intersections(const int64_t *__restrict__ rowptr, const
 // Build the LUT of the induced sub-DAG on S.x[0..s) (s <= SMAX) and count
 // its cliques of `depth` vertices.  All threads call; returns this thread's
 // partial count.
-template <int SMAX, int W, int NT>
+template <int SMAX, int W, int NT, int WMIN = 1>
 __device__ u64 block_lut_count(const int64_t *__restrict__ rowptr, const int32_t *__restrict__ colidx,
                                CStage<SMAX, W, NT> &S, int s, int depth, WaveFlags &fl) {
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -262,8 +276,8 @@ __device__ u64 block_lut_count(const int64_t *__restrict__ rowptr, const int32_t
     // the segment table (dead after the build) holds the per-wave queues
     // (depth 3 on rows of <= 8 words measured faster per thread-word)
     static_assert(sizeof(S.seg) >= (NT / 64) * 128 * sizeof(int), "queue space");
-    c = lut_count_wq<W, false>(LdsRows<W>{S.rows}, s, weff, depth, reinterpret_cast<int *>(&S.seg) + wave * 128,
-                               &S.rowq, lane);
+    c = lut_count_wq_fit<WMIN, W, false>(LdsRows<W>{S.rows}, s, weff, depth,
+                                         reinterpret_cast<int *>(&S.seg) + wave * 128, &S.rowq, lane);
   } else {
     c = lut_count<W>(S.rows, s, weff, depth, tid, NT);
   }
@@ -422,8 +436,9 @@ __global__ __launch_bounds__(NT) void k_clique_vlut_rank(const int64_t *__restri
         // the sort keys (dead after the LUT build) hold the per-wave queues
         // (depth 3 on rows of <= 8 words measured faster per thread-word)
         static_assert(sizeof(S.key) + sizeof(S.val) >= (NT / 64) * 128 * sizeof(int), "queue space");
-        part += lut_count_wq<W, true>(LdsRows<W>{S.rows}, s, weff, depth, reinterpret_cast<int *>(S.key) + wave * 128,
-                                      &S.rowq, lane);
+        // tasks one bin up also land here: widths from half the previous bin
+        part += lut_count_wq_fit<(W >= 8 ? W / 4 + 1 : W), W, true>(
+            LdsRows<W>{S.rows}, s, weff, depth, reinterpret_cast<int *>(S.key) + wave * 128, &S.rowq, lane);
       } else {
         part += lut_count_tri<W>(S.rows, s, weff, depth, tid, NT);
       }
@@ -537,8 +552,8 @@ __global__ __launch_bounds__(NT) void k_clique_vlut_rank_tri(const int64_t *__re
       if (tid == 0) S.rowq = 0;
       __syncthreads();
       static_assert(sizeof(S.key) >= (NT / 64) * 128 * sizeof(int), "queue space");
-      part += lut_count_wq<W, true>(TriRows{S.rows, weff}, s, weff, depth, reinterpret_cast<int *>(S.key) + wave * 128,
-                                    &S.rowq, lane);
+      part += lut_count_wq_fit<(W > 16 ? 17 : W / 4 + 1), W, true>(
+          TriRows{S.rows, weff}, s, weff, depth, reinterpret_cast<int *>(S.key) + wave * 128, &S.rowq, lane);
     }
     task_flush(acc, part, slots, v0, out, tid & 63);
     __syncthreads();
@@ -711,7 +726,7 @@ __global__ __launch_bounds__(NT) void k_clique_vlut(const int64_t *__restrict__ 
     const int s = (int)(rowptr[v0 + 1] - b);
     if (s >= min_s) {
       for (int q = tid; q < s; q += NT) S.x[q] = colidx[b + q];
-      part += block_lut_count<SMAX, W, NT>(rowptr, colidx, S, s, depth, fl);
+      part += block_lut_count<SMAX, W, NT, (W >= 8 ? SMAX / 128 + 1 : 1)>(rowptr, colidx, S, s, depth, fl);
     }
     task_flush(acc, part, slots, v0, out, tid & 63);
     __syncthreads();
