@@ -129,10 +129,13 @@ __device__ __forceinline__ u64 lut_count_wq(Rows row, int s, int weff, int depth
         if (lane < take) {
           const int j = jq[lane];
           const int ws = TRI ? (j + 1) >> 6 : 0;
+          // a pair's sum stays below 2^22 (at most 2048 k times 2048 bits):
+          // 32-bit accumulation lets each popcount add into its counter
+          uint32_t c = 0;
           if (depth == 3) {
 #pragma unroll
             for (int w = 0; w < W; ++w)
-              if (w >= ws && w < weff) acc += __popcll(row(a, w) & row(j, w));
+              if (w >= ws && w < weff) c += (uint32_t)__popcll(row(a, w) & row(j, w));
           } else {
             u64 tw[W];
 #pragma unroll
@@ -146,10 +149,11 @@ __device__ __forceinline__ u64 lut_count_wq(Rows row, int s, int weff, int depth
                 const int ks = TRI ? (k + 1) >> 6 : 0;
 #pragma unroll
                 for (int w = 0; w < W; ++w)
-                  if (w >= ks && w < weff) acc += __popcll(tw[w] & row(k, w));
+                  if (w >= ks && w < weff) c += (uint32_t)__popcll(tw[w] & row(k, w));
               }
             }
           }
+          acc += c;
         }
         wave_sync();
         if (lane + 64 < pending) jq[lane] = jq[lane + 64];
