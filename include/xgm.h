@@ -78,8 +78,10 @@ typedef struct xgm_opts {
   int hub_rows;          /* 4-cycles: HBM counter rows shared by hub tasks in one round (each row
                             spans the largest instance); 0 = as many as fit in 8 GiB */
   int wedge_range;       /* 4-cycles: counters of tasks above 4096 wedges in LDS by id range
-                            (0 = where the range cuts cost at most a quarter of the wedges,
-                            1 = every task above 512 wedges, 2 = never) */
+                            (0 = where the range cuts are at most as many as the wedges,
+                            1 = every task above 512 wedges, 2 = never, 3 = as 1 with 32-bit
+                            counters only; otherwise ranges of tasks with < 65536 lower
+                            neighbours use 16-bit counters over twice the ids) */
 } xgm_opts;
 
 typedef struct xgm_ctx xgm_ctx;

@@ -87,7 +87,7 @@ def test_batch_per_instance_counts(xgm, ctx, mixed):
     for rows in (1, 2, 7):
         assert np.array_equal(ctx.count_instances(und, 5, hub_rows=rows), _col(want, "rectangle")), rows
     _say("batch rectangle LDS range counters")
-    for rm in (1, 2):
+    for rm in (1, 2, 3):
         assert np.array_equal(ctx.count_instances(und, 5, wedge_range=rm), _col(want, "rectangle")), rm
     assert np.array_equal(ctx.count_instances(und, 5, wedge_range=1, hub_rows=3), _col(want, "rectangle"))
     _say("batch rectangle pair strategy")
@@ -164,7 +164,7 @@ def test_cycle4_hub_rounds_single_graph(xgm, ctx):
     want = math.comb(520, 2) ** 2
     for rows in (0, 1, 4, 15, 100):
         assert ctx.count(dg, 5, hub_rows=rows) == want, rows
-    for rm in (0, 1, 2):
+    for rm in (0, 1, 2, 3):
         assert ctx.count(dg, 5, wedge_range=rm) == want, rm
     dg.close()
 
@@ -179,9 +179,22 @@ def test_cycle4_range_counters_golden(xgm, ctx):
         _say("range golden", name)
         g = make_graph(xgm, *SMALL[name][:2])
         dg = ctx.upload(g)
-        for rm in (1, 2, 0):
+        for rm in (1, 2, 0, 3):
             assert ctx.count(dg, 5, wedge_range=rm) == gold[name]["counts"]["rectangle"], (name, rm)
         dg.close()
+
+
+def test_cycle4_range_spans(xgm, orc, ctx):
+    """Tasks whose ids below x span several ranges (n = 300,000): 16-bit
+    counters over 65,536 ids (wedge_range 1) and 32-bit counters over
+    32,768 ids (wedge_range 3) give the oracle's wedge count."""
+    _say("range spans chung-lu 300k")
+    g = xgm.gen_chung_lu(300000, 12.0, 2.1, 3000.0, 11)
+    want = orc.cycle4_wedge(g.rowptr, g.colidx)
+    dg = ctx.upload(g)
+    for rm in (0, 1, 2, 3):
+        assert ctx.count(dg, 5, wedge_range=rm) == want, rm
+    dg.close()
 
 
 def test_tc_tile_options_exact(xgm, orc, ctx):

@@ -583,16 +583,21 @@ __global__ __launch_bounds__(128) void k_cycle4_wedge_packed(const int64_t *__re
   if (lane == 0 && acc) atomicAdd(out, acc);
 }
 
-// Range items of the wedge strategy: (x, r0, r1, -) counts the wedges of x
-// whose far end w lies in [r0, r1) (r1 - r0 <= R, r1 <= x), with c_x(w) in
-// R LDS counters indexed by w - r0.  The ranges of one task partition the
+// Range items of the wedge strategy: (x, r0, r1, half) counts the wedges of
+// x whose far end w lies in [r0, r1) (r1 <= x), with c_x(w) in LDS counters
+// indexed by w - r0: R 32-bit counters (r1 - r0 <= R), or with half = 1, 2R
+// 16-bit counters packed two per word (r1 - r0 <= 2R), which the host picks
+// only when x has fewer than 65536 lower neighbours: c_x(w) never exceeds
+// that number, so a half-word never carries into its neighbour.  Halving
+// the ranges per task halves the list cuts, which dominate this kernel's
+// time.  The ranges of one task partition the
 // ids below x, so every w is counted whole by exactly one item and the
 // previous values returned by the increments still sum to C(c_x(w), 2);
 // no HBM counters or global atomics are involved.  Each lower neighbour's
 // list is cut to [r0, r1) by two binary searches.
 template <int R, int PCAP, int NT>
 struct RangeStage {
-  int cnt[R];
+  uint32_t cnt[R];
   Segs<PCAP, false> seg;
   int tmp[PCAP];
   int flagrow[NT / 64][64];
@@ -617,7 +622,9 @@ __global__ __launch_bounds__(NT) void k_cycle4_range(const int64_t *__restrict__
     if (t >= nitems) break;
     const int4 it = items[t];
     const int x = it.x, r0 = it.y, r1 = it.z;
-    for (int q = tid; q < r1 - r0; q += NT) S.cnt[q] = 0;
+    const bool half = it.w != 0;  // 16-bit counters: w - r0 in word (w - r0) / 2
+    const int words = half ? (r1 - r0 + 1) >> 1 : r1 - r0;
+    for (int q = tid; q < words; q += NT) S.cnt[q] = 0u;
     const int64_t bx = rowptr[x];
     const int32_t *nx = colidx + bx;
     const int nlow = lower_bound_dev(nx, (int)(rowptr[x + 1] - bx), x);
@@ -661,7 +668,15 @@ __global__ __launch_bounds__(NT) void k_cycle4_range(const int64_t *__restrict__
       int lo, hi;
       wave_share(S.L, wave, NT / 64, lo, hi);
       wave_walk<4>(S.seg, colidx, S.ns, lo, hi, lane, fl, [&](const WindowLane &r) {
-        if (r.valid) part += (u64)atomicAdd(&S.cnt[r.w - r0], 1);
+        if (r.valid) {
+          const int i = r.w - r0;
+          if (half) {
+            const int sh = (i & 1) << 4;
+            part += (u64)((atomicAdd(&S.cnt[i >> 1], 1u << sh) >> sh) & 0xFFFFu);
+          } else {
+            part += (u64)atomicAdd(&S.cnt[i], 1u);
+          }
+        }
       });
     }
     task_flush(acc, part, slots, x, out, lane);

@@ -124,8 +124,11 @@ struct CounterRows {
 // hub_rounds[r] are int4 (x, k0, k1, slot) slices of hub tasks adding into
 // row `slot` of hub; the first hub_round_u64 words of hub are zeroed before
 // every round.  Uses queue counters 0-2 and 4-7 of d_queues.
-// range_items are int4 (x, r0, r1, -) with r1 - r0 <= kRangeCounters: the
-// wedges of x ending in [r0, r1), counted in LDS.  Queue counter 6.
+// range_items are int4 (x, r0, r1, half): the wedges of x ending in
+// [r0, r1), counted in LDS, with r1 - r0 <= kRangeCounters (32-bit
+// counters, half = 0) or <= 2 * kRangeCounters (half = 1: 16-bit counters,
+// two per word; only for tasks with fewer than 65536 lower neighbours, which
+// bound every count).  Queue counter 6.
 constexpr int kRangeCounters = 32768;
 hipError_t launch_cycle4_wedge(const DevCSR &g, const TaskList tl[4], int *dense, const TaskList *hub_rounds,
                                int n_rounds, int *hub, size_t hub_round_u64, const TaskList &range_items,

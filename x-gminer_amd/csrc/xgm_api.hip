@@ -1107,10 +1107,12 @@ static int dispatch_pattern(xgm_ctx *ctx, const xgm_dgraph *g, int pattern, cons
       const int64_t kHubWork = 1 << 18, kSlice = 1 << 16, kHubBudgetInts = (int64_t)1 << 31;
       const int64_t row_cap = std::max<int64_t>(1, kHubBudgetInts / std::max<int64_t>(1, stride));
       const int64_t rows_max = (opts && opts->hub_rows > 0) ? std::min<int64_t>(opts->hub_rows, row_cap) : row_cap;
-      // tasks above 4096 wedges (above 512 when forced) may run as LDS range
-      // items instead: ceil((x - lo) / kRangeCounters) ranges, each cutting
-      // the nlow lower lists with two binary searches; taken when those
-      // P * nlow cuts are at most a quarter of the wedges
+      // tasks above 4096 wedges (above 512 when forced: modes 1, 3) may run as LDS range
+      // items instead: P = ceil((x - lo) / span) ranges, each cutting the
+      // nlow lower lists with two binary searches; taken when those P * nlow
+      // cuts are at most as many as the wedges.  span = 2 * kRangeCounters
+      // (16-bit counters) when nlow < 65536 bounds every count, else (and
+      // always in mode 3) kRangeCounters
       const int rmode = opts ? opts->wedge_range : 0;
       const std::string sfx = "_" + std::to_string(rmode);
       const std::string hkey = "cycle4_hubs_" + std::to_string(rows_max) + sfx;
@@ -1122,14 +1124,18 @@ static int dispatch_pattern(xgm_ctx *ctx, const xgm_dgraph *g, int pattern, cons
         std::vector<int32_t> hubs;
         for (int32_t x : gm->rl_order) {
           const int64_t w = gm->rl_work[x];
-          if (rmode != 2 && w > (rmode == 1 ? 512 : 4096)) {
+          const bool forced = rmode == 1 || rmode == 3;
+          if (rmode != 2 && w > (forced ? 512 : 4096)) {
             const int64_t lo = gm->h_slot.empty() ? 0 : gm->inst_base[gm->h_slot[x]];
             const int32_t *nx = ci.data() + rp[x];
             const int64_t nlow = std::lower_bound(nx, nx + (rp[x + 1] - rp[x]), x) - nx;
-            const int64_t P = (x - lo + kRangeCounters - 1) / kRangeCounters;
-            if (rmode == 1 || 4 * P * nlow <= w) {
-              for (int64_t r0 = lo; r0 < x; r0 += kRangeCounters)
-                rngp[0].insert(rngp[0].end(), {x, (int32_t)r0, (int32_t)std::min<int64_t>(x, r0 + kRangeCounters), 0});
+            const bool half = rmode != 3 && nlow < 65536;
+            const int64_t span = half ? 2 * (int64_t)kRangeCounters : kRangeCounters;
+            const int64_t P = (x - lo + span - 1) / span;
+            if (forced || P * nlow <= w) {
+              for (int64_t r0 = lo; r0 < x; r0 += span)
+                rngp[0].insert(rngp[0].end(),
+                               {x, (int32_t)r0, (int32_t)std::min<int64_t>(x, r0 + span), half ? 1 : 0});
               continue;
             }
           }
