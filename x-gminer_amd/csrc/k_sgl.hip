@@ -584,17 +584,27 @@ __global__ __launch_bounds__(128) void k_cycle4_wedge_packed(const int64_t *__re
 }
 
 // Range items of the wedge strategy: (x, r0, r1, half) counts the wedges of
-// x whose far end w lies in [r0, r1) (r1 <= x), with c_x(w) in LDS counters
-// indexed by w - r0: R 32-bit counters (r1 - r0 <= R), or with half = 1, 2R
-// 16-bit counters packed two per word (r1 - r0 <= 2R), which the host picks
-// only when x has fewer than 65536 lower neighbours: c_x(w) never exceeds
-// that number, so a half-word never carries into its neighbour.  Halving
-// the ranges per task halves the list cuts, which dominate this kernel's
-// time.  The ranges of one task partition the
-// ids below x, so every w is counted whole by exactly one item and the
-// previous values returned by the increments still sum to C(c_x(w), 2);
-// no HBM counters or global atomics are involved.  Each lower neighbour's
-// list is cut to [r0, r1) by two binary searches.
+// x whose far end w lies in [r0, r1) (r1 <= x).  [r0, r1) is one or more
+// consecutive spans of `span` ids, taken one after the other, each with
+// c_x(w) in LDS counters indexed by w - s0 (s0 = the span's first id):
+// span = R with R 32-bit counters, or with half = 1, span = 2R with 16-bit
+// counters packed two per word, which the host picks only when x has fewer
+// than 65536 lower neighbours: c_x(w) never exceeds that number, so a
+// half-word never carries into its neighbour.  The spans of one task
+// partition the ids below x, so every w is counted whole in exactly one
+// span and the previous values returned by the increments still sum to
+// C(c_x(w), 2); no HBM counters or global atomics are involved.
+//
+// Cutting every lower neighbour's list to a span (scattered probes into the
+// lists) dominates this kernel's time.  The first span of an item finds its
+// start in each list by binary search; every span's end is found by a
+// galloping search from its start and kept in the block's cursor row
+// (cursor + blockIdx.x * cursor_stride; the host sizes cursor_stride to the
+// largest lower-neighbour count) as the next span's start, next to the
+// list's end, both as colidx positions: later spans read one cursor instead
+// of the neighbour id and its rowptr entries.  Each cursor is written and
+// read back by the same thread.  Positions are int32, as in the segment
+// tables.
 template <int R, int PCAP, int NT>
 struct RangeStage {
   uint32_t cnt[R];
@@ -604,16 +614,31 @@ struct RangeStage {
   int ns, L, task;
 };
 
+// First index i in [lo, n) with a[i] >= key, where every a[j], j < lo, is
+// below key: probes lo, lo + 1, lo + 3, lo + 7, ... until one reaches key,
+// then binary-searches the last gap (about 2 log2 of the distance moved).
+__device__ __forceinline__ int gallop_lower_bound(const int32_t *__restrict__ a, int lo, int n, int key) {
+  int hi = lo, step = 1;
+  while (hi < n && a[hi] < key) {
+    lo = hi + 1;
+    hi = (n - hi > step) ? hi + step : n;
+    step <<= 1;
+  }
+  return lo + lower_bound_dev(a + lo, hi - lo, key);
+}
+
 template <int R, int PCAP, int NT>
 __global__ __launch_bounds__(NT) void k_cycle4_range(const int64_t *__restrict__ rowptr,
                                                      const int32_t *__restrict__ colidx,
                                                      const int4 *__restrict__ items, int nitems,
+                                                     int2 *__restrict__ cursor, int64_t cursor_stride,
                                                      int *__restrict__ queue, u64 *__restrict__ out,
                                                      const int32_t *__restrict__ slots) {
   __shared__ RangeStage<R, PCAP, NT> S;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   WaveFlags fl;
   fl.init(S.flagrow[wave], lane);
+  int2 *const cur = cursor + (size_t)blockIdx.x * (size_t)cursor_stride;
   u64 acc = 0;
   for (;;) {
     if (tid == 0) S.task = atomicAdd(queue, 1);
@@ -622,62 +647,76 @@ __global__ __launch_bounds__(NT) void k_cycle4_range(const int64_t *__restrict__
     if (t >= nitems) break;
     const int4 it = items[t];
     const int x = it.x, r0 = it.y, r1 = it.z;
-    const bool half = it.w != 0;  // 16-bit counters: w - r0 in word (w - r0) / 2
-    const int words = half ? (r1 - r0 + 1) >> 1 : r1 - r0;
-    for (int q = tid; q < words; q += NT) S.cnt[q] = 0u;
+    const bool half = it.w != 0;  // 16-bit counters: w - s0 in word (w - s0) / 2
+    const int span = half ? 2 * R : R;
     const int64_t bx = rowptr[x];
     const int32_t *nx = colidx + bx;
     const int nlow = lower_bound_dev(nx, (int)(rowptr[x + 1] - bx), x);
     u64 part = 0;
-    for (int k0 = 0; k0 < nlow; k0 += PCAP) {
-      const int n = min(nlow - k0, PCAP);
-      __syncthreads();  // counters zeroed / previous piece walked
-      for (int q = tid; q < n; q += NT) {
-        const int v = nx[k0 + q];
-        const int64_t vb = rowptr[v];
-        const int dv = (int)(rowptr[v + 1] - vb);
-        const int32_t *nv = colidx + vb;
-        const int a = lower_bound_dev(nv, dv, r0);
-        const int b = a + lower_bound_dev(nv + a, dv - a, r1);
-        S.seg.os[q].x = (int)(vb + a);
-        S.tmp[q] = b - a;
-      }
-      __syncthreads();
-      if (wave == 0) {  // compact the non-empty pieces in place
-        int ns = 0, L = 0;
-        for (int k = 0; k < n; k += 64) {
-          const int kk = k + lane;
-          int len = 0, vb = 0;
-          if (kk < n) {
-            len = S.tmp[kk];
-            vb = S.seg.os[kk].x;
-          }
-          const int incl = wave_incl_scan(len, lane) + L;
-          const u64 m = __ballot(len > 0);
-          if (len > 0) S.seg.os[ns + __popcll(m & lanemask_lt(lane))] = make_int2(vb, incl - len);
-          ns += __popcll(m);
-          L = __shfl(incl, 63, 64);
-        }
-        if (lane == 0) {
-          S.seg.os[ns] = make_int2(0, L);
-          S.ns = ns;
-          S.L = L;
-        }
-      }
-      __syncthreads();
-      int lo, hi;
-      wave_share(S.L, wave, NT / 64, lo, hi);
-      wave_walk<4>(S.seg, colidx, S.ns, lo, hi, lane, fl, [&](const WindowLane &r) {
-        if (r.valid) {
-          const int i = r.w - r0;
-          if (half) {
-            const int sh = (i & 1) << 4;
-            part += (u64)((atomicAdd(&S.cnt[i >> 1], 1u << sh) >> sh) & 0xFFFFu);
+    for (int s0 = r0; s0 < r1; s0 += span) {
+      const int s1 = r1 - s0 > span ? s0 + span : r1;
+      const bool first = s0 == r0, last = s1 == r1;
+      const int words = half ? (s1 - s0 + 1) >> 1 : s1 - s0;
+      __syncthreads();  // the previous span's walk is done with the counters
+      for (int q = tid; q < words; q += NT) S.cnt[q] = 0u;
+      for (int k0 = 0; k0 < nlow; k0 += PCAP) {
+        const int n = min(nlow - k0, PCAP);
+        __syncthreads();  // counters zeroed / previous piece walked
+        for (int q = tid; q < n; q += NT) {
+          int2 c;  // (span start, list end) as colidx positions
+          if (first) {
+            const int v = nx[k0 + q];
+            const int64_t vb = rowptr[v];
+            const int dv = (int)(rowptr[v + 1] - vb);
+            const int32_t *nv = colidx + vb;
+            // an item that starts at the task's first id starts every list at 0
+            const int a = (dv == 0 || nv[0] >= s0) ? 0 : lower_bound_dev(nv, dv, s0);
+            c = make_int2((int)vb + a, (int)vb + dv);
           } else {
-            part += (u64)atomicAdd(&S.cnt[i], 1u);
+            c = cur[k0 + q];
+          }
+          const int len = gallop_lower_bound(colidx + c.x, 0, c.y - c.x, s1);
+          if (!last) cur[k0 + q] = make_int2(c.x + len, c.y);
+          S.seg.os[q].x = c.x;
+          S.tmp[q] = len;
+        }
+        __syncthreads();
+        if (wave == 0) {  // compact the non-empty pieces in place
+          int ns = 0, L = 0;
+          for (int k = 0; k < n; k += 64) {
+            const int kk = k + lane;
+            int len = 0, vb = 0;
+            if (kk < n) {
+              len = S.tmp[kk];
+              vb = S.seg.os[kk].x;
+            }
+            const int incl = wave_incl_scan(len, lane) + L;
+            const u64 m = __ballot(len > 0);
+            if (len > 0) S.seg.os[ns + __popcll(m & lanemask_lt(lane))] = make_int2(vb, incl - len);
+            ns += __popcll(m);
+            L = __shfl(incl, 63, 64);
+          }
+          if (lane == 0) {
+            S.seg.os[ns] = make_int2(0, L);
+            S.ns = ns;
+            S.L = L;
           }
         }
-      });
+        __syncthreads();
+        int lo, hi;
+        wave_share(S.L, wave, NT / 64, lo, hi);
+        wave_walk<4>(S.seg, colidx, S.ns, lo, hi, lane, fl, [&](const WindowLane &r) {
+          if (r.valid) {
+            const int i = r.w - s0;
+            if (half) {
+              const int sh = (i & 1) << 4;
+              part += (u64)((atomicAdd(&S.cnt[i >> 1], 1u << sh) >> sh) & 0xFFFFu);
+            } else {
+              part += (u64)atomicAdd(&S.cnt[i], 1u);
+            }
+          }
+        });
+      }
     }
     task_flush(acc, part, slots, x, out, lane);
     __syncthreads();
@@ -886,16 +925,20 @@ int cycle4_dense_grid(int num_cu, int64_t tasks) {
   return grid_of(k_cycle4_wedge<1, 1024, 256, true>, 256, num_cu, tasks, 1);
 }
 
+int cycle4_range_grid(int num_cu, int64_t items) {
+  return grid_of(k_cycle4_range<kRangeCounters, 1024, 1024>, 1024, num_cu, items);
+}
+
 hipError_t launch_cycle4_wedge(const DevCSR &g, const TaskList tl[4], int *dense, const TaskList *hub_rounds,
                                int n_rounds, int *hub, size_t hub_round_u64, const TaskList &range_items,
                                const CounterRows &rows, Launch &l, int *d_queues) {
   hipError_t e;
   if (range_items.count) {
     auto k = k_cycle4_range<kRangeCounters, 1024, 1024>;
-    const int grid = grid_of(k, 1024, l.num_cu, range_items.count);
+    const int grid = cycle4_range_grid(l.num_cu, range_items.count);
     l.begin("cycle4_wedge_range", range_items.count);
     k<<<grid, 1024, 0, l.stream>>>(g.rowptr, g.colidx, (const int4 *)range_items.tasks, (int)range_items.count,
-                                   d_queues + 6, l.d_out, l.slots);
+                                   rows.cursor, rows.cursor_stride, d_queues + 6, l.d_out, l.slots);
     l.end();
     if ((e = hipGetLastError())) return e;
   }

@@ -115,6 +115,11 @@ hipError_t launch_motif_edges(const DevCSR &g, const TaskList tl[4], Launch &l, 
 struct CounterRows {
   int64_t stride = 0;
   const int64_t *inst_lo = nullptr;
+  // 4-cycle range items: cycle4_range_grid() rows of cursor_stride int2,
+  // cursor_stride >= the lower-neighbour count of every range task; no
+  // initial contents needed
+  int2 *cursor = nullptr;
+  int64_t cursor_stride = 0;
 };
 
 // Edge-induced 4-cycle, wedge aggregation over the (degree, id)-relabelled
@@ -125,15 +130,16 @@ struct CounterRows {
 // row `slot` of hub; the first hub_round_u64 words of hub are zeroed before
 // every round.  Uses queue counters 0-2 and 4-7 of d_queues.
 // range_items are int4 (x, r0, r1, half): the wedges of x ending in
-// [r0, r1), counted in LDS, with r1 - r0 <= kRangeCounters (32-bit
-// counters, half = 0) or <= 2 * kRangeCounters (half = 1: 16-bit counters,
-// two per word; only for tasks with fewer than 65536 lower neighbours, which
-// bound every count).  Queue counter 6.
+// [r0, r1), counted in LDS one span after another, a span being
+// kRangeCounters ids (32-bit counters, half = 0) or 2 * kRangeCounters ids
+// (half = 1: 16-bit counters, two per word; only for tasks with fewer than
+// 65536 lower neighbours, which bound every count).  Queue counter 6.
 constexpr int kRangeCounters = 32768;
 hipError_t launch_cycle4_wedge(const DevCSR &g, const TaskList tl[4], int *dense, const TaskList *hub_rounds,
                                int n_rounds, int *hub, size_t hub_round_u64, const TaskList &range_items,
                                const CounterRows &rows, Launch &l, int *d_queues);
 int cycle4_dense_grid(int num_cu, int64_t tasks);
+int cycle4_range_grid(int num_cu, int64_t items);
 
 // Edge-induced 4-cycle, bounded pairwise intersections in the reference's
 // loop order (original ids).

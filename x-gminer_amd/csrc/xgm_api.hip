@@ -140,6 +140,9 @@ struct xgm_dgraph {
   uint16_t *d_boff = nullptr;
   u64 *d_bmap = nullptr;
   std::map<std::string, int> hub_count;  // hub tasks per cached hub list
+  std::map<std::string, int64_t> range_nlow;  // largest lower-neighbour count per cached range list
+  int2 *d_cursor = nullptr;        // 4-cycle range items: per-block list cursors
+  size_t cursor_n = 0;
   // batched instances (disjoint union): instance of every vertex, first id
   // of every instance (n_inst + 1 bounds), largest instance
   int n_inst = 1;
@@ -154,6 +157,7 @@ struct xgm_dgraph {
   size_t hub_ints = 0;
   ~xgm_dgraph() {
     (void)hipFree(d_dense);
+    (void)hipFree(d_cursor);
     (void)hipFree(d_hub);
     (void)hipFree(d_bcol);
     (void)hipFree(d_boff);
@@ -1105,6 +1109,7 @@ static int dispatch_pattern(xgm_ctx *ctx, const xgm_dgraph *g, int pattern, cons
       // counter rows span one instance (the largest) or the whole graph
       const int64_t stride = gm->n_inst > 1 ? gm->max_inst_n : g->n;
       const int64_t kHubWork = 1 << 18, kSlice = 1 << 16, kHubBudgetInts = (int64_t)1 << 31;
+      const int64_t kRangeGroupWork = 1 << 22;
       const int64_t row_cap = std::max<int64_t>(1, kHubBudgetInts / std::max<int64_t>(1, stride));
       const int64_t rows_max = (opts && opts->hub_rows > 0) ? std::min<int64_t>(opts->hub_rows, row_cap) : row_cap;
       // tasks above 4096 wedges (above 512 when forced: modes 1, 3) may run as LDS range
@@ -1112,7 +1117,10 @@ static int dispatch_pattern(xgm_ctx *ctx, const xgm_dgraph *g, int pattern, cons
       // nlow lower lists with two binary searches; taken when those P * nlow
       // cuts are at most as many as the wedges.  span = 2 * kRangeCounters
       // (16-bit counters) when nlow < 65536 bounds every count, else (and
-      // always in mode 3) kRangeCounters
+      // always in mode 3) kRangeCounters.  Consecutive spans of one task
+      // share an item up to about kRangeGroupWork wedges: inside an item
+      // each list cut is carried into the next span, so only the item's
+      // first span binary-searches
       const int rmode = opts ? opts->wedge_range : 0;
       const std::string sfx = "_" + std::to_string(rmode);
       const std::string hkey = "cycle4_hubs_" + std::to_string(rows_max) + sfx;
@@ -1122,6 +1130,7 @@ static int dispatch_pattern(xgm_ctx *ctx, const xgm_dgraph *g, int pattern, cons
         const auto &rp = gm->rl_rowptr;
         const auto &ci = gm->rl_colidx;
         std::vector<int32_t> hubs;
+        int64_t range_nlow = 0;
         for (int32_t x : gm->rl_order) {
           const int64_t w = gm->rl_work[x];
           const bool forced = rmode == 1 || rmode == 3;
@@ -1133,9 +1142,12 @@ static int dispatch_pattern(xgm_ctx *ctx, const xgm_dgraph *g, int pattern, cons
             const int64_t span = half ? 2 * (int64_t)kRangeCounters : kRangeCounters;
             const int64_t P = (x - lo + span - 1) / span;
             if (forced || P * nlow <= w) {
-              for (int64_t r0 = lo; r0 < x; r0 += span)
+              const int64_t per_span = std::max<int64_t>(1, w / P);
+              const int64_t group = std::max<int64_t>(1, std::min<int64_t>(P, kRangeGroupWork / per_span));
+              for (int64_t r0 = lo; r0 < x; r0 += group * span)
                 rngp[0].insert(rngp[0].end(),
-                               {x, (int32_t)r0, (int32_t)std::min<int64_t>(x, r0 + span), half ? 1 : 0});
+                               {x, (int32_t)r0, (int32_t)std::min<int64_t>(x, r0 + group * span), half ? 1 : 0});
+              range_nlow = std::max(range_nlow, nlow);
               continue;
             }
           }
@@ -1172,6 +1184,7 @@ static int dispatch_pattern(xgm_ctx *ctx, const xgm_dgraph *g, int pattern, cons
           if (k0 < nlow) P.insert(P.end(), {x, k0, nlow, slot});
         }
         gm->hub_count[hkey] = (int)hubs.size();
+        gm->range_nlow[rkey] = range_nlow;
       }
       const int nhubs = gm->hub_count.count(hkey) ? gm->hub_count[hkey] : 0;
       std::vector<TaskList> hl, rl;
@@ -1189,6 +1202,17 @@ static int dispatch_pattern(xgm_ctx *ctx, const xgm_dgraph *g, int pattern, cons
           gm->hub_ints = 0;
           HIP_TRY(ctx, hipMalloc(&gm->d_hub, need * sizeof(int)));
           gm->hub_ints = need;
+        }
+      }
+      const int64_t cursor_stride = gm->range_nlow.count(rkey) ? gm->range_nlow[rkey] : 0;
+      if (rl[0].count) {
+        const size_t need = (size_t)cycle4_range_grid(ctx->num_cu, rl[0].count) * (size_t)cursor_stride;
+        if (gm->cursor_n < need) {
+          (void)hipFree(gm->d_cursor);
+          gm->d_cursor = nullptr;
+          gm->cursor_n = 0;
+          HIP_TRY(ctx, hipMalloc(&gm->d_cursor, need * sizeof(int2)));
+          gm->cursor_n = need;
         }
       }
       if (t3[2].count) {
@@ -1209,6 +1233,8 @@ static int dispatch_pattern(xgm_ctx *ctx, const xgm_dgraph *g, int pattern, cons
       CounterRows rows;
       rows.stride = stride;
       rows.inst_lo = l.slots ? gm->d_inst_lo : nullptr;
+      rows.cursor = gm->d_cursor;
+      rows.cursor_stride = cursor_stride;
       e = launch_cycle4_wedge(rg, t3, gm->d_dense, hl.data(), (int)hl.size(), gm->d_hub, hub_round_u64, rl[0], rows,
                               l, l.d_queue);
       break;
