@@ -926,7 +926,7 @@ int cycle4_dense_grid(int num_cu, int64_t tasks) {
 }
 
 int cycle4_range_grid(int num_cu, int64_t items) {
-  return grid_of(k_cycle4_range<kRangeCounters, 1024, 1024>, 1024, num_cu, items);
+  return grid_of(k_cycle4_range<kRangeCounters, 2048, 1024>, 1024, num_cu, items);
 }
 
 hipError_t launch_cycle4_wedge(const DevCSR &g, const TaskList tl[4], int *dense, const TaskList *hub_rounds,
@@ -934,7 +934,9 @@ hipError_t launch_cycle4_wedge(const DevCSR &g, const TaskList tl[4], int *dense
                                const CounterRows &rows, Launch &l, int *d_queues) {
   hipError_t e;
   if (range_items.count) {
-    auto k = k_cycle4_range<kRangeCounters, 1024, 1024>;
+    // 2048-neighbour pieces: 128 KB of counters + 24 KB of segment tables
+    // still fit one CU's LDS, with half the barriers of 1024-pieces
+    auto k = k_cycle4_range<kRangeCounters, 2048, 1024>;
     const int grid = cycle4_range_grid(l.num_cu, range_items.count);
     l.begin("cycle4_wedge_range", range_items.count);
     k<<<grid, 1024, 0, l.stream>>>(g.rowptr, g.colidx, (const int4 *)range_items.tasks, (int)range_items.count,
