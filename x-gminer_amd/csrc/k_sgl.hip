@@ -601,10 +601,11 @@ __global__ __launch_bounds__(128) void k_cycle4_wedge_packed(const int64_t *__re
 // galloping search from its start and kept in the block's cursor row
 // (cursor + blockIdx.x * cursor_stride; the host sizes cursor_stride to the
 // largest lower-neighbour count) as the next span's start, next to the
-// list's end, both as colidx positions: later spans read one cursor instead
-// of the neighbour id and its rowptr entries.  Each cursor is written and
-// read back by the same thread.  Positions are int32, as in the segment
-// tables.
+// list's end, both as colidx positions, and the id stored there: later
+// spans read one cursor instead of the neighbour id and its rowptr entries,
+// and a list whose next id lies beyond the span is skipped without a
+// probe.  Each cursor is written and read back by the same thread.
+// Positions are int32, as in the segment tables.
 template <int R, int PCAP, int NT>
 struct RangeStage {
   uint32_t cnt[R];
@@ -631,14 +632,14 @@ template <int R, int PCAP, int NT>
 __global__ __launch_bounds__(NT) void k_cycle4_range(const int64_t *__restrict__ rowptr,
                                                      const int32_t *__restrict__ colidx,
                                                      const int4 *__restrict__ items, int nitems,
-                                                     int2 *__restrict__ cursor, int64_t cursor_stride,
+                                                     int4 *__restrict__ cursor, int64_t cursor_stride,
                                                      int *__restrict__ queue, u64 *__restrict__ out,
                                                      const int32_t *__restrict__ slots) {
   __shared__ RangeStage<R, PCAP, NT> S;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   WaveFlags fl;
   fl.init(S.flagrow[wave], lane);
-  int2 *const cur = cursor + (size_t)blockIdx.x * (size_t)cursor_stride;
+  int4 *const cur = cursor + (size_t)blockIdx.x * (size_t)cursor_stride;
   u64 acc = 0;
   for (;;) {
     if (tid == 0) S.task = atomicAdd(queue, 1);
@@ -663,7 +664,9 @@ __global__ __launch_bounds__(NT) void k_cycle4_range(const int64_t *__restrict__
         const int n = min(nlow - k0, PCAP);
         __syncthreads();  // counters zeroed / previous piece walked
         for (int q = tid; q < n; q += NT) {
-          int2 c;  // (span start, list end) as colidx positions
+          // (span start, list end) as colidx positions, and the id at the
+          // span start (INT32_MAX past the end; unknown = -1 on a first span)
+          int4 c;
           if (first) {
             const int v = nx[k0 + q];
             const int64_t vb = rowptr[v];
@@ -671,12 +674,16 @@ __global__ __launch_bounds__(NT) void k_cycle4_range(const int64_t *__restrict__
             const int32_t *nv = colidx + vb;
             // an item that starts at the task's first id starts every list at 0
             const int a = (dv == 0 || nv[0] >= s0) ? 0 : lower_bound_dev(nv, dv, s0);
-            c = make_int2((int)vb + a, (int)vb + dv);
+            c = make_int4((int)vb + a, (int)vb + dv, -1, 0);
           } else {
             c = cur[k0 + q];
           }
-          const int len = gallop_lower_bound(colidx + c.x, 0, c.y - c.x, s1);
-          if (!last) cur[k0 + q] = make_int2(c.x + len, c.y);
+          int len = 0;
+          if (c.z < s1) {  // else the list has nothing in this span: no probe, cursor unchanged
+            len = gallop_lower_bound(colidx + c.x, 0, c.y - c.x, s1);
+            const int p = c.x + len;
+            if (!last) cur[k0 + q] = make_int4(p, c.y, p < c.y ? colidx[p] : INT32_MAX, 0);
+          }
           S.seg.os[q].x = c.x;
           S.tmp[q] = len;
         }

@@ -115,10 +115,10 @@ hipError_t launch_motif_edges(const DevCSR &g, const TaskList tl[4], Launch &l, 
 struct CounterRows {
   int64_t stride = 0;
   const int64_t *inst_lo = nullptr;
-  // 4-cycle range items: cycle4_range_grid() rows of cursor_stride int2,
+  // 4-cycle range items: cycle4_range_grid() rows of cursor_stride int4,
   // cursor_stride >= the lower-neighbour count of every range task; no
   // initial contents needed
-  int2 *cursor = nullptr;
+  int4 *cursor = nullptr;
   int64_t cursor_stride = 0;
 };
 

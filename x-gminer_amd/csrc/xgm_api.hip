@@ -141,7 +141,7 @@ struct xgm_dgraph {
   u64 *d_bmap = nullptr;
   std::map<std::string, int> hub_count;  // hub tasks per cached hub list
   std::map<std::string, int64_t> range_nlow;  // largest lower-neighbour count per cached range list
-  int2 *d_cursor = nullptr;        // 4-cycle range items: per-block list cursors
+  int4 *d_cursor = nullptr;        // 4-cycle range items: per-block list cursors
   size_t cursor_n = 0;
   // batched instances (disjoint union): instance of every vertex, first id
   // of every instance (n_inst + 1 bounds), largest instance
@@ -1211,7 +1211,7 @@ static int dispatch_pattern(xgm_ctx *ctx, const xgm_dgraph *g, int pattern, cons
           (void)hipFree(gm->d_cursor);
           gm->d_cursor = nullptr;
           gm->cursor_n = 0;
-          HIP_TRY(ctx, hipMalloc(&gm->d_cursor, need * sizeof(int2)));
+          HIP_TRY(ctx, hipMalloc(&gm->d_cursor, need * sizeof(int4)));
           gm->cursor_n = need;
         }
       }
