@@ -178,7 +178,9 @@ int xgm_count_batch(xgm_ctx *ctx, const xgm_csr *gs, const int *patterns, int n_
 /* A plan captures the counts of several patterns -- triangles / cliques on a
  * DAG batch, diamonds / 4-cycles on an undirected batch (either may be NULL
  * if unused) -- into one hipGraph of persistent work-queue kernels that is
- * replayed by xgm_plan_run.  opts: one per pattern (NULL = auto). */
+ * replayed by xgm_plan_run.  opts: one per pattern (NULL = auto).  A plan
+ * must be freed before its graphs; later counts on those graphs leave it
+ * valid (scratch buffers they outgrow are kept until the graph is freed). */
 typedef struct xgm_plan xgm_plan;
 int xgm_plan_create(xgm_ctx *ctx, const xgm_dgraph *dag, const xgm_dgraph *und, const int *patterns, int n_pat,
                     const xgm_opts *opts, xgm_plan **out);

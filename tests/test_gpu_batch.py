@@ -127,6 +127,19 @@ def test_batch_plan_replays(xgm, ctx, mixed):
     plan.close()
     dag.close()
     und.close()
+    # eager counts whose task splits need more counter / cursor rows than the
+    # plan's replace the graph's scratch buffers; the plan keeps replaying on
+    # the ones it captured
+    _say("plan replay after scratch growth")
+    und = ctx.upload_batch(inst)
+    plan = ctx.plan([4, 5], und=und)
+    for kw in (dict(wedge_range=1), dict(wedge_range=3), dict(wedge_range=2), dict(wedge_range=2, hub_rows=1)):
+        assert np.array_equal(ctx.count_instances(und, 5, **kw), expect[3]), kw
+    for rep in range(2):
+        got, _ = plan.run()
+        assert np.array_equal(got, expect[2:]), rep
+    plan.close()
+    und.close()
 
 
 def test_plan_single_graph(xgm, orc, ctx):

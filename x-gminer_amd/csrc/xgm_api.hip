@@ -143,6 +143,9 @@ struct xgm_dgraph {
   std::map<std::string, int64_t> range_nlow;  // largest lower-neighbour count per cached range list
   int4 *d_cursor = nullptr;        // 4-cycle range items: per-block list cursors
   size_t cursor_n = 0;
+  // scratch buffers replaced by larger ones: a plan captured earlier still
+  // launches kernels on them, so they live as long as the graph
+  std::vector<void *> retired;
   // batched instances (disjoint union): instance of every vertex, first id
   // of every instance (n_inst + 1 bounds), largest instance
   int n_inst = 1;
@@ -175,6 +178,7 @@ struct xgm_dgraph {
     (void)hipFree(d_rrowptr);
     (void)hipFree(d_rcolidx);
     (void)hipFree(d_scratch);
+    for (void *x : retired) (void)hipFree(x);
   }
 };
 
@@ -281,14 +285,24 @@ int tasks_by_ranges(xgm_ctx *ctx, xgm_dgraph *g, const std::string &key, int min
   return XGM_OK;
 }
 
-int ensure_scratch(xgm_ctx *ctx, xgm_dgraph *g, size_t ints) {
-  if (g->scratch_ints >= ints) return XGM_OK;
-  (void)hipFree(g->d_scratch);
-  g->d_scratch = nullptr;
-  g->scratch_ints = 0;
-  HIP_TRY(ctx, hipMalloc(&g->d_scratch, std::max<size_t>(1, ints) * sizeof(int)));
-  g->scratch_ints = ints;
+// Grow a per-graph scratch buffer to `need` elements of `elem` bytes; sets
+// *grown when it allocated.  The old buffer is retired, not freed: a plan
+// captured earlier still launches kernels on it.
+int grow_scratch(xgm_ctx *ctx, xgm_dgraph *g, void **buf, size_t &have, size_t need, size_t elem,
+                 bool *grown = nullptr) {
+  if (grown) *grown = false;
+  if (*buf && have >= need) return XGM_OK;
+  if (*buf) g->retired.push_back(*buf);
+  *buf = nullptr;
+  have = 0;
+  HIP_TRY(ctx, hipMalloc(buf, std::max<size_t>(1, need) * elem));
+  have = need;
+  if (grown) *grown = true;
   return XGM_OK;
+}
+
+int ensure_scratch(xgm_ctx *ctx, xgm_dgraph *g, size_t ints) {
+  return grow_scratch(ctx, g, (void **)&g->d_scratch, g->scratch_ints, ints, sizeof(int));
 }
 
 // Undirected task order: vertices with degree >= 3 by decreasing owned
@@ -1196,36 +1210,19 @@ static int dispatch_pattern(xgm_ctx *ctx, const xgm_dgraph *g, int pattern, cons
       const size_t hub_round_u64 = ((size_t)hub_rows * (size_t)stride + 1) / 2;
       if (nhubs) {
         const size_t need = 2 * hub_round_u64;
-        if (gm->hub_ints < need) {
-          (void)hipFree(gm->d_hub);
-          gm->d_hub = nullptr;
-          gm->hub_ints = 0;
-          HIP_TRY(ctx, hipMalloc(&gm->d_hub, need * sizeof(int)));
-          gm->hub_ints = need;
-        }
+        if ((rc = grow_scratch(ctx, gm, (void **)&gm->d_hub, gm->hub_ints, need, sizeof(int)))) return rc;
       }
       const int64_t cursor_stride = gm->range_nlow.count(rkey) ? gm->range_nlow[rkey] : 0;
       if (rl[0].count) {
         const size_t need = (size_t)cycle4_range_grid(ctx->num_cu, rl[0].count) * (size_t)cursor_stride;
-        if (gm->cursor_n < need) {
-          (void)hipFree(gm->d_cursor);
-          gm->d_cursor = nullptr;
-          gm->cursor_n = 0;
-          HIP_TRY(ctx, hipMalloc(&gm->d_cursor, need * sizeof(int4)));
-          gm->cursor_n = need;
-        }
+        if ((rc = grow_scratch(ctx, gm, (void **)&gm->d_cursor, gm->cursor_n, need, sizeof(int4)))) return rc;
       }
       if (t3[2].count) {
         size_t need = (size_t)cycle4_dense_grid(ctx->num_cu, t3[2].count) * (size_t)stride;
         need = (need + 1) & ~(size_t)1;  // whole 64-bit words for launch_zero
-        if (gm->dense_ints < need) {
-          (void)hipFree(gm->d_dense);
-          gm->d_dense = nullptr;
-          gm->dense_ints = 0;
-          HIP_TRY(ctx, hipMalloc(&gm->d_dense, need * sizeof(int)));
-          HIP_TRY(ctx, launch_zero(gm->d_dense, need / 2, ctx->stream));
-          gm->dense_ints = need;
-        }
+        bool grown = false;
+        if ((rc = grow_scratch(ctx, gm, (void **)&gm->d_dense, gm->dense_ints, need, sizeof(int), &grown))) return rc;
+        if (grown) HIP_TRY(ctx, launch_zero(gm->d_dense, need / 2, ctx->stream));
       }
       DevCSR rg = dg;
       rg.rowptr = gm->d_rl_rowptr;
