@@ -146,10 +146,18 @@ __device__ __forceinline__ u64 lut_count_wq(Rows row, int s, int weff, int depth
               while (tb) {
                 const int k = w2 * 64 + __builtin_ctzll(tb);
                 tb &= tb - 1;
-                const int ks = TRI ? (k + 1) >> 6 : 0;
+                if (TRI) {
+                  // row k holds ranks above k only: its words start at
+                  // (k + 1) / 64, which is w2, or w2 + 1 for a word's last bit
+                  const bool last_bit = (k & 63) == 63;
 #pragma unroll
-                for (int w = 0; w < W; ++w)
-                  if (w >= ks && w < weff) c += (uint32_t)__popcll(tw[w] & row(k, w));
+                  for (int w = w2; w < W; ++w)
+                    if ((w > w2 || !last_bit) && w < weff) c += (uint32_t)__popcll(tw[w] & row(k, w));
+                } else {
+#pragma unroll
+                  for (int w = 0; w < W; ++w)
+                    if (w < weff) c += (uint32_t)__popcll(tw[w] & row(k, w));
+                }
               }
             }
           }
