@@ -635,6 +635,7 @@ __global__ __launch_bounds__(NT) void k_cycle4_range(const int64_t *__restrict__
                                                      int4 *__restrict__ cursor, int64_t cursor_stride,
                                                      int *__restrict__ queue, u64 *__restrict__ out,
                                                      const int32_t *__restrict__ slots) {
+  static_assert(PCAP % NT == 0, "whole lists per thread");
   __shared__ RangeStage<R, PCAP, NT> S;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   WaveFlags fl;
@@ -663,28 +664,45 @@ __global__ __launch_bounds__(NT) void k_cycle4_range(const int64_t *__restrict__
       for (int k0 = 0; k0 < nlow; k0 += PCAP) {
         const int n = min(nlow - k0, PCAP);
         __syncthreads();  // counters zeroed / previous piece walked
-        for (int q = tid; q < n; q += NT) {
-          // (span start, list end) as colidx positions, and the id at the
-          // span start (INT32_MAX past the end; unknown = -1 on a first span)
-          int4 c;
+        // each thread cuts lists tid + h * NT, h < PCAP / NT; all of its
+        // cursors (or list bounds) are loaded before the first is used, so
+        // their latencies overlap
+        // (span start, list end) as colidx positions, and the id at the
+        // span start (INT32_MAX past the end; unknown = -1 on a first span)
+        int4 c[PCAP / NT];
+#pragma unroll
+        for (int h = 0; h < PCAP / NT; ++h) {
+          const int q = tid + h * NT;
+          c[h] = make_int4(0, 0, INT32_MAX, 0);
+          if (q < n) {
+            if (first) {
+              const int v = nx[k0 + q];
+              const int64_t vb = rowptr[v];
+              c[h] = make_int4((int)vb, (int)rowptr[v + 1], -1, 0);
+            } else {
+              c[h] = cur[k0 + q];
+            }
+          }
+        }
+#pragma unroll
+        for (int h = 0; h < PCAP / NT; ++h) {
+          const int q = tid + h * NT;
+          if (q >= n) continue;
           if (first) {
-            const int v = nx[k0 + q];
-            const int64_t vb = rowptr[v];
-            const int dv = (int)(rowptr[v + 1] - vb);
-            const int32_t *nv = colidx + vb;
+            const int32_t *nv = colidx + c[h].x;
+            const int dv = c[h].y - c[h].x;
             // an item that starts at the task's first id starts every list at 0
-            const int a = (dv == 0 || nv[0] >= s0) ? 0 : lower_bound_dev(nv, dv, s0);
-            c = make_int4((int)vb + a, (int)vb + dv, -1, 0);
-          } else {
-            c = cur[k0 + q];
+            c[h].x += (dv == 0 || nv[0] >= s0) ? 0 : lower_bound_dev(nv, dv, s0);
           }
           int len = 0;
-          if (c.z < s1) {  // else the list has nothing in this span: no probe, cursor unchanged
-            len = gallop_lower_bound(colidx + c.x, 0, c.y - c.x, s1);
-            const int p = c.x + len;
-            if (!last) cur[k0 + q] = make_int4(p, c.y, p < c.y ? colidx[p] : INT32_MAX, 0);
+          if (c[h].z < s1) {  // else the list has nothing in this span: no probe, cursor unchanged
+            // a known id at the span start is below s1: its probe is skipped
+            len = c[h].z >= 0 ? 1 + gallop_lower_bound(colidx + c[h].x + 1, 0, c[h].y - c[h].x - 1, s1)
+                              : gallop_lower_bound(colidx + c[h].x, 0, c[h].y - c[h].x, s1);
+            const int p = c[h].x + len;
+            if (!last) cur[k0 + q] = make_int4(p, c[h].y, p < c[h].y ? colidx[p] : INT32_MAX, 0);
           }
-          S.seg.os[q].x = c.x;
+          S.seg.os[q].x = c[h].x;
           S.tmp[q] = len;
         }
         __syncthreads();
