@@ -655,8 +655,8 @@ __global__ __launch_bounds__(NT) void k_cycle4_range(const int64_t *__restrict__
     const int32_t *nx = colidx + bx;
     const int nlow = lower_bound_dev(nx, (int)(rowptr[x + 1] - bx), x);
     u64 part = 0;
-    for (int s0 = r0; s0 < r1; s0 += span) {
-      const int s1 = r1 - s0 > span ? s0 + span : r1;
+    for (int s0 = r0, s1 = r0; s0 < r1; s0 = s1) {  // s1 never passes r1: no int overflow near 2^31
+      s1 = r1 - s0 > span ? s0 + span : r1;
       const bool first = s0 == r0, last = s1 == r1;
       const int words = half ? (s1 - s0 + 1) >> 1 : s1 - s0;
       __syncthreads();  // the previous span's walk is done with the counters
