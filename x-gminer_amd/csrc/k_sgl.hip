@@ -30,6 +30,7 @@ namespace {
 // ------------------------------------------------------------------ diamond
 template <int HCAP, int PCAP>
 struct DiaStage {
+  static_assert((HCAP & (HCAP - 1)) == 0, "power-of-two hash table");
   int key[2 * HCAP];
   Segs<PCAP, false> seg;
   int cnt[PCAP];
@@ -68,7 +69,9 @@ __global__ __launch_bounds__(NT) void k_diamond(const int64_t *__restrict__ rowp
     const int32_t *nt = colidx + bt;
     int hb = 0;
     if (LDS_SET) {
-      hb = hash_bits_for(dt);
+      // the whole table (2 * HCAP slots, allocated anyway): the smaller the
+      // owner, the lower the load and the shorter a missing probe's run
+      hb = 31 - __clz(2 * HCAP);
       for (int q = tid; q < (1 << hb); q += NT) S.key[q] = -1;
       __syncthreads();
       for (int q = tid; q < dt; q += NT) hash_insert(S.key, hb, nt[q]);
