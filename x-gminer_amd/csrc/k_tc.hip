@@ -91,7 +91,9 @@ __global__ __launch_bounds__(256) void k_tc_wave(const int64_t *__restrict__ row
     const int32_t *nu = colidx + b;
     int hbits = 0;
     if (STRAT == STRAT_HASH) {
-      hbits = hash_bits_for(du, HSHIFT);
+      // the whole table (CAP << HSHIFT slots, allocated anyway): load
+      // du / slots, so missing probes run shorter than at the 2^-HSHIFT cap
+      hbits = 31 - __clz(CAP << HSHIFT);
       for (int k = lane; k < (1 << hbits); k += 64) s.key[k] = -1;
       wave_sync();
       for (int k = lane; k < du; k += 64) hash_insert(s.key, hbits, nu[k]);
@@ -268,7 +270,7 @@ __global__ __launch_bounds__(NT) void k_tc_block(const int64_t *__restrict__ row
     int hbits = 0;
     if (lds_member) {
       if (STRAT == STRAT_HASH) {
-        hbits = hash_bits_for(du, HSHIFT);
+        hbits = 31 - __clz(CAP << HSHIFT);  // the whole table, as in k_tc_wave
         for (int k = threadIdx.x; k < (1 << hbits); k += NT) s.key[k] = -1;
         __syncthreads();
         for (int k = threadIdx.x; k < du; k += NT) hash_insert(s.key, hbits, nu[k]);
