@@ -39,9 +39,10 @@ WORKLOADS = {
 BATCH_PATTERNS = ("triangle", "clique4", "diamond", "rectangle")
 DIAMOND_TIERS = [("diamond_wave_64", 1, 64), ("diamond_block_1024", 65, 1024), ("diamond_block_8192", 1025, 8192),
                  ("diamond_hbm_set", 8193, 1 << 40)]
-# tier bounds of the triangle kernels (out-degree), mirrored from kernels.h
-TC_TIERS = [("tc_wave_64", 1, 64), ("tc_wave_256", 65, 256), ("tc_block_2048", 257, 2048),
-            ("tc_block_big", 2049, 1 << 40)]
+# tier bounds of the triangle kernels (out-degree) under the library's default
+# split (kernels.h: kTcWaveCap, kTcWaveLargeDefault = 64, so the 256-slot wave
+# tier is empty and the block kernel takes 65..2048)
+TC_TIERS = [("tc_wave_64", 1, 64), ("tc_block_2048", 65, 2048), ("tc_block_big", 2049, 1 << 40)]
 
 
 def log(*a):

@@ -74,7 +74,7 @@ typedef struct xgm_opts {
   int wave_max_small;    /* triangles: staged out-degree up to this runs one wave per item in the
                             64-slot kernel (1..64); 0 = 64 */
   int wave_max_large;    /* triangles: ... in the 256-slot wave kernel (wave_max_small..256), larger
-                            staged lists run one block per item; 0 = 256 */
+                            staged lists run one block per item; 0 = 64 (no 256-slot tier) */
   int hub_rows;          /* 4-cycles: HBM counter rows shared by hub tasks in one round (each row
                             spans the largest instance); 0 = as many as fit in 8 GiB */
   int wedge_range;       /* 4-cycles: counters of tasks above 4096 wedges in LDS by id range

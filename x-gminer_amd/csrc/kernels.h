@@ -51,6 +51,10 @@ enum TcTier { TIER_W64 = 0, TIER_W256 = 1, TIER_B2048 = 2, TIER_BIG = 3, NUM_TIE
 constexpr int kTierMaxDeg[NUM_TIERS] = {64, 256, 2048, 1 << 30};
 // wave tiers take at most this many partners per item (one segment table)
 constexpr int kTcWaveCap[2] = {64, 256};
+// Default wave_max_large: staged lists above 64 go to the block kernel,
+// whose LDS hash spans a 2048 << shift table (load <= d+/8192); on R-MAT-22
+// that beats the 256-slot wave tier (28.2 vs 29.3 ms per count).
+constexpr int kTcWaveLargeDefault = 64;
 
 // Triangle counting over a degree-oriented DAG with its reverse lists.
 // tiers[] hold int4 items (t, k0, k1, 0) (TaskList.count = items).

@@ -1040,7 +1040,7 @@ static int dispatch_pattern(xgm_ctx *ctx, const xgm_dgraph *g, int pattern, cons
       const int hs = (opts && opts->hash_shift == 1) ? 1 : 2;
       const int ws = (opts && opts->wave_max_small > 0) ? std::min(opts->wave_max_small, kTcWaveCap[0]) : kTcWaveCap[0];
       const int wl = (opts && opts->wave_max_large > 0) ? std::max(ws, std::min(opts->wave_max_large, kTcWaveCap[1]))
-                                                        : kTcWaveCap[1];
+                                                        : std::max(ws, kTcWaveLargeDefault);
       const std::string key = "tc_items_" + std::to_string(ws) + "_" + std::to_string(wl);
       std::vector<std::vector<int32_t>> parts;
       if (!cached(gm, key)) build_tc_items(gm, ws, wl, parts);
