@@ -53,7 +53,9 @@ enum xgm_pattern {
 /* Intersection / counting strategies (xgm_opts.strategy).  All produce
  * identical counts; they differ only in speed. */
 enum xgm_strategy {
-  XGM_STRAT_AUTO = 0,   /* library heuristic (degree thresholds) */
+  XGM_STRAT_AUTO = 0,   /* library default: triangles -- LDS hash; cliques -- rank-ordered LUT
+                           (as XGM_STRAT_LUT_RANK; XGM_STRAT_LUT on a DAG not in (degree, id)
+                           orientation); diamonds / 4-cycles -- the tiered kernels */
   XGM_STRAT_HASH = 1,   /* vertex task, LDS hash of the staged list, flattened probes */
   XGM_STRAT_BSEARCH = 2,/* vertex task, binary search in the staged sorted list */
   XGM_STRAT_LUT = 3,    /* cliques: vertex tasks with a bitmap LUT in LDS, edge LUT above lut_max */
@@ -179,8 +181,10 @@ int xgm_count_batch(xgm_ctx *ctx, const xgm_csr *gs, const int *patterns, int n_
  * DAG batch, diamonds / 4-cycles on an undirected batch (either may be NULL
  * if unused) -- into one hipGraph of persistent work-queue kernels that is
  * replayed by xgm_plan_run.  opts: one per pattern (NULL = auto).  A plan
- * must be freed before its graphs; later counts on those graphs leave it
- * valid (scratch buffers they outgrow are kept until the graph is freed). */
+ * holds its graphs: xgm_graph_free() on a graph a live plan uses only marks
+ * it, and the graph's device memory is released by the last xgm_plan_free()
+ * of a plan using it.  Later counts on those graphs leave the plan valid
+ * (scratch buffers they outgrow are kept until the graph is released). */
 typedef struct xgm_plan xgm_plan;
 int xgm_plan_create(xgm_ctx *ctx, const xgm_dgraph *dag, const xgm_dgraph *und, const int *patterns, int n_pat,
                     const xgm_opts *opts, xgm_plan **out);
@@ -253,8 +257,10 @@ int xgm_mlp_grads(xgm_mlp *m, const float *X, const float *T, const float *M, in
  * above 1024 run as edge LUT.  Every schedule gives the same
  * count. */
 typedef struct xgm_sched xgm_sched;
-/* Degree-threshold codes of a strategy (XGM_STRAT_*; AUTO = LUT up to
- * lut_max_degree, edge LUT above). */
+/* Degree-threshold codes of a strategy (XGM_STRAT_*), no model: AUTO and
+ * LUT_RANK = rank-ordered LUT up to lut_max_degree and for hubs (1024 < d+ <=
+ * 4096), edge LUT between lut_max_degree and 1024 and above 4096; LUT =
+ * id-ordered LUT up to lut_max_degree, edge LUT above. */
 int xgm_sched_heuristic(const xgm_dgraph *g, int pattern, int strategy, int lut_max_degree, int8_t *codes);
 /* Per-vertex scheduler features (n x XGM_MLP_IN floats, computed on the GPU):
  * log out-degree, log sum / max of 2-hop out-degrees, sampled neighbour

@@ -156,6 +156,29 @@ def test_plan_single_graph(xgm, orc, ctx):
     plan.close()
 
 
+def test_plan_outlives_dropped_graphs(xgm, orc, ctx):
+    """A plan keeps the graphs its kernels read: a temporary upload dropped
+    before run(), and a graph closed while the plan is alive, still replay
+    the exact counts; the memory goes with the plan."""
+    import gc
+    _say("plan outlives its graphs")
+    g = random_graph(xgm, 1500, 20000, 34)
+    d = g.orient()
+    want_tc = orc.count("triangle", d.rowptr, d.colidx)[0]
+    want_dm = orc.count("diamond", g.rowptr, g.colidx)[0]
+    plan = ctx.plan([1], dag=ctx.upload(d))
+    gc.collect()
+    for _ in range(2):
+        assert int(plan.run()[0][0, 0]) == want_tc
+    und = ctx.upload(g)
+    plan2 = ctx.plan([4], und=und)
+    und.close()
+    ctx.count(ctx.upload(g), 4)  # other device work between replays
+    assert int(plan2.run()[0][0, 0]) == want_dm
+    plan.close()
+    plan2.close()
+
+
 def test_autotune_picks_an_exact_candidate(xgm, ctx, mixed):
     inst, want = mixed
     dag = ctx.upload_batch([g.orient() for g in inst])

@@ -286,9 +286,12 @@ class Context:
         _check(lib.xgm_ctx_create(device, C.byref(h)))
         self._h = h
         self._graphs = weakref.WeakSet()
+        self._plans = weakref.WeakSet()
 
     def close(self):
         if getattr(self, "_h", None) is not None:
+            for p in list(self._plans):
+                p.close()
             for g in list(self._graphs):
                 g.close()
             lib.xgm_ctx_destroy(self._h)
@@ -437,7 +440,11 @@ class Plan:
         _check(lib.xgm_plan_create(ctx._h, dag._h if dag else None, und._h if und else None, pats, len(patterns),
                                    oa, C.byref(h)), ctx._h)
         self._h, self.ctx, self.patterns = h, ctx, list(patterns)
+        # the captured kernels read these graphs' buffers: keep them alive
+        # as long as the plan (the library also defers their release)
+        self._graphs = (dag, und)
         self.n_inst = (dag or und).n_inst
+        ctx._plans.add(self)
 
     def run(self):
         """(counts[n_patterns, n_instances], replay ms)."""
@@ -451,6 +458,7 @@ class Plan:
         if getattr(self, "_h", None) is not None:
             lib.xgm_plan_free(self._h)
             self._h = None
+            self._graphs = None
 
     __del__ = close
 

@@ -158,6 +158,13 @@ struct xgm_dgraph {
   u64 *d_motif = nullptr;          // per-instance 4-motif edge sums (4 per instance)
   std::unique_ptr<xgm_dgraph> dag; // undirected graphs: degree-oriented copy for motif censuses
   size_t hub_ints = 0;
+  // plans whose captured kernels read this graph's buffers: xgm_graph_free
+  // defers the release until the last of them is freed
+  int plan_refs = 0;
+  bool free_pending = false;
+  // cliques under XGM_STRAT_AUTO use the rank-ordered LUT until a count
+  // finds the DAG outside (degree, id) orientation; then the id-ordered LUT
+  bool rank_ok = true;
   ~xgm_dgraph() {
     (void)hipFree(d_dense);
     (void)hipFree(d_cursor);
@@ -615,8 +622,12 @@ int build_clique_sched(xgm_ctx *ctx, const xgm_dgraph *g, int pattern, const int
   return XGM_OK;
 }
 
-// Heuristic codes: id-ordered LUT up to lut_max out-degree, edge LUT above
-// (plain degree thresholds, the reference's fixed-threshold scheme).
+// Degree-threshold codes of a strategy (no model).  AUTO and LUT_RANK: the
+// rank-ordered LUT for every task up to lut_max out-degree and for hubs
+// (1024 < d+ <= kHubLutMax, the hub rank LUT), the edge LUT between lut_max
+// and 1024 and above kHubLutMax.  LUT: the id-ordered LUT up to lut_max, the
+// edge LUT above (the reference's fixed-threshold scheme, GLUMIN's LUT tasks
+// up to 1024 out-neighbours, xgminer/include/common.h:61).
 void heuristic_codes(const xgm_dgraph *g, int k, int strategy, int lut_max, int8_t *codes) {
   for (int64_t v = 0; v < g->n; ++v) {
     const int d = g->dplus[v];
@@ -625,8 +636,8 @@ void heuristic_codes(const xgm_dgraph *g, int k, int strategy, int lut_max, int8
       switch (strategy) {
         case XGM_STRAT_LIST: c = 3; break;
         case XGM_STRAT_ELUT: c = 2; break;
-        case XGM_STRAT_LUT_RANK: c = (d <= lut_max || d > 1024) ? 1 : 2; break;  // hubs: hub rank LUT
-        default: c = d <= lut_max ? 0 : 2; break;
+        case XGM_STRAT_LUT: c = d <= lut_max ? 0 : 2; break;
+        default: c = (d <= lut_max || d > 1024) ? 1 : 2; break;  // rank LUT; hubs: hub rank LUT
       }
     }
     codes[v] = c;
@@ -923,6 +934,12 @@ int xgm_graph_task_order(const xgm_dgraph *g, int32_t *out, int64_t cap, int64_t
 
 void xgm_graph_free(xgm_dgraph *g) {
   if (!g) return;
+  // a plan still replays kernels on this graph's buffers: release them with
+  // the last such plan (xgm_plan_free)
+  if (g->plan_refs > 0) {
+    g->free_pending = true;
+    return;
+  }
   // hipFree waits for outstanding work on the device; the context is not
   // touched, so a graph may be freed after its context.
   (void)hipSetDevice(g->device);
@@ -1058,12 +1075,15 @@ static int dispatch_pattern(xgm_ctx *ctx, const xgm_dgraph *g, int pattern, cons
         return ctx->fail(XGM_EUNSUPPORTED, "strategy not available for CLIQUE");
       const int k = (pattern == XGM_CLIQUE4) ? 4 : 5;
       const int lut_max = (opts && opts->lut_max_degree > 0) ? std::min(opts->lut_max_degree, 1024) : 1024;
-      const std::string key = "csched" + std::to_string(k) + "_" + std::to_string(strategy) + "_" +
+      // AUTO: the rank-ordered LUT, or the id-ordered one on a DAG that a
+      // rank-LUT count found outside (degree, id) orientation
+      const int strat = (strategy == XGM_STRAT_AUTO) ? (gm->rank_ok ? XGM_STRAT_LUT_RANK : XGM_STRAT_LUT) : strategy;
+      const std::string key = "csched" + std::to_string(k) + "_" + std::to_string(strat) + "_" +
                               std::to_string(lut_max);
       auto it = gm->scheds.find(key);
       if (it == gm->scheds.end()) {
         std::vector<int8_t> codes(g->n);
-        heuristic_codes(gm, k, strategy, lut_max, codes.data());
+        heuristic_codes(gm, k, strat, lut_max, codes.data());
         std::unique_ptr<xgm_sched> sc(new (std::nothrow) xgm_sched);
         if (!sc) return XGM_ENOMEM;
         if ((rc = build_clique_sched(ctx, gm, pattern, codes.data(), sc.get()))) return rc;
@@ -1381,7 +1401,16 @@ static int count_into(xgm_ctx *ctx, const xgm_dgraph *g, int pattern, const xgm_
   HIP_TRY(ctx, hipMemcpyAsync(h.data(), l.d_out, ni * sizeof(u64), hipMemcpyDeviceToHost, ctx->stream));
   HIP_TRY(ctx, hipMemcpyAsync(&err, l.d_err, sizeof(u64), hipMemcpyDeviceToHost, ctx->stream));
   HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
-  if (err) return ctx->fail(XGM_EINVAL, "DAG is not in (degree, id) orientation; use XGM_STRAT_LUT");
+  if (err) {
+    // the rank-ordered LUT needs a (degree, id)-oriented DAG; AUTO falls
+    // back to the id-ordered LUT for this graph from now on
+    const bool clique = pattern == XGM_CLIQUE4 || pattern == XGM_CLIQUE5;
+    if (clique && (!opts || opts->strategy == XGM_STRAT_AUTO) && g->rank_ok) {
+      const_cast<xgm_dgraph *>(g)->rank_ok = false;
+      return count_into(ctx, g, pattern, opts, out, per_inst);
+    }
+    return ctx->fail(XGM_EINVAL, "DAG is not in (degree, id) orientation; use XGM_STRAT_LUT");
+  }
   float total = 0.f;
   for (int i = 0; i < ctx->prof.n_kernels; ++i) {
     float ms = 0.f;
@@ -1406,6 +1435,8 @@ struct xgm_plan {
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   hipEvent_t ev[2] = {nullptr, nullptr};
+  // graphs the captured kernels read (each holds one plan reference)
+  xgm_dgraph *uses[2] = {nullptr, nullptr};
   ~xgm_plan() {
     (void)hipSetDevice(device);
     if (exec) (void)hipGraphExecDestroy(exec);
@@ -1414,6 +1445,13 @@ struct xgm_plan {
       if (e) (void)hipEventDestroy(e);
     (void)hipFree(d_out);
     (void)hipFree(d_queues);
+    for (xgm_dgraph *g : uses) {
+      if (!g) continue;
+      if (--g->plan_refs == 0 && g->free_pending) {
+        g->free_pending = false;
+        xgm_graph_free(g);
+      }
+    }
   }
 };
 
@@ -1678,6 +1716,16 @@ int xgm_plan_create(xgm_ctx *ctx, const xgm_dgraph *dag, const xgm_dgraph *und, 
   if (e != hipSuccess) return ctx->hip_fail(e, "plan capture");
   p->graph = graph;
   HIP_TRY(ctx, hipGraphInstantiate(&p->exec, p->graph, nullptr, nullptr, 0));
+  // pin the graphs the capture reads until the plan is freed
+  const xgm_dgraph *used[2] = {dag, und};
+  for (int j = 0; j < 2; ++j) {
+    bool needed = false;
+    for (int i = 0; i < n_pat; ++i) needed |= (dag_pattern(patterns[i]) ? dag : und) == used[j];
+    if (needed && used[j]) {
+      p->uses[j] = const_cast<xgm_dgraph *>(used[j]);
+      ++p->uses[j]->plan_refs;
+    }
+  }
   *out = p.release();
   return XGM_OK;
 }
@@ -1724,8 +1772,8 @@ int xgm_autotune(xgm_ctx *ctx, const xgm_dgraph *g, int pattern, int reps, xgm_o
       break;
     case XGM_CLIQUE4:
     case XGM_CLIQUE5:
-      cand.push_back(mk(XGM_STRAT_AUTO, 0, 0, 0, 0));
-      cand.push_back(mk(XGM_STRAT_LUT_RANK, 0, 0, 0, 0));
+      cand.push_back(mk(XGM_STRAT_AUTO, 0, 0, 0, 0));  // rank LUT (id LUT off degree orientation)
+      cand.push_back(mk(XGM_STRAT_LUT_RANK, 256, 0, 0, 0));
       cand.push_back(mk(XGM_STRAT_LUT, 1024, 0, 0, 0));
       cand.push_back(mk(XGM_STRAT_LUT, 256, 0, 0, 0));
       cand.push_back(mk(XGM_STRAT_ELUT, 0, 0, 0, 0));
