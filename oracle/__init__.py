@@ -119,6 +119,23 @@ def count(pattern: str, rowptr, colidx, v_begin=0, v_end=-1, threads=0):
     return int(c), int(b.value)
 
 
+def count_tasks(pattern: str, rowptr, colidx, tasks, threads=0):
+    """count() over an explicit list of vertex tasks (the vertex loop visits
+    exactly these vertices, e.g. a sample of them); returns (count,
+    algorithmic_bytes) of those tasks."""
+    t = np.ascontiguousarray(tasks, dtype=np.int32)
+    L = lib()
+    L.orc_set_tasks.restype = None
+    L.orc_set_tasks.argtypes = [C.POINTER(C.c_int32), C.c_int64]
+    if len(t) == 0:
+        return 0, 0
+    L.orc_set_tasks(t.ctypes.data_as(C.POINTER(C.c_int32)), len(t))
+    try:
+        return count(pattern, rowptr, colidx, 0, -1, threads)
+    finally:
+        L.orc_set_tasks(None, 0)
+
+
 def cycle4_wedge(rowptr, colidx, threads=0):
     """Edge-induced 4-cycles of an undirected CSR by wedge aggregation (a
     second CPU algorithm for large golden counts; pinned against

@@ -157,8 +157,24 @@ eid orc_orient(eid n, const eid *rp, const vid *ci, eid *out_rp, vid *out_ci) {
 }
 
 // ---- patterns.  Vertex loop over [v_begin, v_end) (the whole graph when
-// v_end < 0); `threads` <= 0 keeps the OpenMP default; *bytes (optional)
-// receives the algorithmic byte total. ----
+// v_end < 0), or over the explicit task list set by orc_set_tasks();
+// `threads` <= 0 keeps the OpenMP default; *bytes (optional) receives the
+// algorithmic byte total. ----
+
+namespace {
+const vid *g_tasks = nullptr;  // explicit vertex tasks (orc_set_tasks), or null
+eid g_ntasks = 0;
+inline eid loop_len(eid b, eid e) { return g_tasks ? g_ntasks : e - b; }
+inline eid loop_vertex(eid b, eid i) { return g_tasks ? (eid)g_tasks[i] : b + i; }
+}  // namespace
+
+// Restrict the pattern loops to tasks[0..ntasks) (a sample of the vertex
+// tasks, e.g. every S-th vertex in decreasing-work order) until called with
+// tasks = null.  The array must outlive the counts that use it.
+void orc_set_tasks(const vid *tasks, eid ntasks) {
+  g_tasks = ntasks > 0 ? tasks : nullptr;
+  g_ntasks = ntasks > 0 ? ntasks : 0;
+}
 
 // TCSolver (src/triangle/omp_base.cc:15-21) on a DAG.
 u64 orc_tc(eid n, const eid *rp, const vid *ci, eid v_begin, eid v_end, int threads, u64 *bytes) {
@@ -167,7 +183,8 @@ u64 orc_tc(eid n, const eid *rp, const vid *ci, eid v_begin, eid v_end, int thre
   set_threads(threads);
   u64 count = 0, B = 0;
 #pragma omp parallel for schedule(dynamic, 1) reduction(+ : count, B)
-  for (eid u = v_begin; u < v_end; ++u) {
+  for (eid i = 0; i < loop_len(v_begin, v_end); ++i) {
+    const eid u = loop_vertex(v_begin, i);
     const vid *yu = g.N(u);
     const eid du = g.d(u);
     for (eid k = 0; k < du; ++k) {
@@ -191,7 +208,8 @@ u64 orc_clique4(eid n, const eid *rp, const vid *ci, eid v_begin, eid v_end, int
   {
     std::vector<vid> y01(md + 1);
 #pragma omp for schedule(dynamic, 1)
-    for (eid v0 = v_begin; v0 < v_end; ++v0) {
+    for (eid i = 0; i < loop_len(v_begin, v_end); ++i) {
+      const eid v0 = loop_vertex(v_begin, i);
       const vid *y0 = g.N(v0);
       const eid d0 = g.d(v0);
       for (eid k = 0; k < d0; ++k) {
@@ -221,7 +239,8 @@ u64 orc_clique5(eid n, const eid *rp, const vid *ci, eid v_begin, eid v_end, int
   {
     std::vector<vid> y12(md + 1), y123(md + 1);
 #pragma omp for schedule(dynamic, 1)
-    for (eid v1 = v_begin; v1 < v_end; ++v1) {
+    for (eid i = 0; i < loop_len(v_begin, v_end); ++i) {
+      const eid v1 = loop_vertex(v_begin, i);
       const vid *y1 = g.N(v1);
       const eid d1 = g.d(v1);
       for (eid k = 0; k < d1; ++k) {
@@ -253,7 +272,8 @@ u64 orc_diamond(eid n, const eid *rp, const vid *ci, eid v_begin, eid v_end, int
   set_threads(threads);
   u64 count = 0, B = 0;
 #pragma omp parallel for schedule(dynamic, 1) reduction(+ : count, B)
-  for (eid v0 = v_begin; v0 < v_end; ++v0) {
+  for (eid i = 0; i < loop_len(v_begin, v_end); ++i) {
+    const eid v0 = loop_vertex(v_begin, i);
     const vid *y0 = g.N(v0);
     const eid d0 = g.d(v0);
     for (eid k = 0; k < d0; ++k) {
@@ -277,7 +297,8 @@ u64 orc_cycle4(eid n, const eid *rp, const vid *ci, eid v_begin, eid v_end, int 
   set_threads(threads);
   u64 count = 0, B = 0;
 #pragma omp parallel for schedule(dynamic, 1) reduction(+ : count, B)
-  for (eid v0 = v_begin; v0 < v_end; ++v0) {
+  for (eid i = 0; i < loop_len(v_begin, v_end); ++i) {
+    const eid v0 = loop_vertex(v_begin, i);
     const vid *y0 = g.N(v0);
     const eid d0 = g.d(v0);
     for (eid k1 = 0; k1 < d0; ++k1) {

@@ -55,3 +55,80 @@ def test_tc_bytes_equal_oracle_and_closed_form(xgm, orc):
         m = int(dplus.sum())
         assert total == 4 * int((dplus * deg).sum()) + 40 * m
         assert total == orc.count("triangle", d.rowptr, d.colidx)[1]
+
+
+def _brute_tc_kernel_bytes(d, tiers):
+    rp, ci = np.asarray(d.rowptr), np.asarray(d.colidx)
+    dout = np.diff(rp)
+    out = {name: 0 for name, _, _ in tiers}
+    tier = lambda x: next(name for name, lo, hi in tiers if lo <= x <= hi)
+    staged = set()
+    for u in range(d.n):
+        for v in ci[rp[u]:rp[u + 1]]:
+            t, p = (u, v) if dout[u] > dout[v] else (v, u)
+            out[tier(dout[t])] += 20 + 4 * int(dout[p])
+            staged.add(int(t))
+    for t in staged:
+        out[tier(dout[t])] += 32 + 4 * int(dout[t])
+    return out
+
+
+def test_kernel_byte_models_match_loops(xgm):
+    """The per-kernel byte models (read-once bytes of each kernel's own
+    algorithm) equal straightforward loops over the graph."""
+    import bench
+    g = random_graph(xgm, 400, 3000, 11)
+    d = g.orient()
+    assert bench.tc_kernel_bytes(d) == _brute_tc_kernel_bytes(d, bench.TC_TIERS)
+    # cliques: rank-LUT codes everywhere; vertex v0 reads N+(v0) and every
+    # member's row pointers, in-list bounds and out-list
+    rp, ci = np.asarray(d.rowptr), np.asarray(d.colidx)
+    dout = np.diff(rp)
+    codes = np.where(dout >= 3, 1, -1).astype(np.int8)
+    got = bench.clique_kernel_bytes(d, codes, 4)
+    want = 0
+    for v in range(d.n):
+        if dout[v] < 3:
+            continue
+        want += 16 + 4 * int(dout[v]) + sum(32 + 4 * int(dout[u]) for u in ci[rp[v]:rp[v + 1]])
+    assert sum(got.values()) == want
+    assert set(got) <= {"k4_rlut_64", "k4_rlut_256"}
+    # diamonds: owner = (degree, id)-larger endpoint of degree >= 3
+    grp, gci = np.asarray(g.rowptr), np.asarray(g.colidx)
+    deg = np.diff(grp)
+    want, owners = 0, set()
+    for u in range(g.n):
+        for v in gci[grp[u]:grp[u + 1]]:
+            if (deg[u], u) > (deg[v], v) and deg[u] >= 3:
+                want += 20 + 4 * int(deg[v])
+                owners.add(u)
+    want += sum(16 + 4 * int(deg[t]) for t in owners)
+    assert sum(bench.diamond_kernel_bytes(g).values()) == want
+    # 4-cycles: relabel by (degree, id); task x reads its lower list and the
+    # part of each lower neighbour's list below x
+    key = sorted(range(g.n), key=lambda v: (deg[v], v))
+    rank = np.empty(g.n, np.int64)
+    rank[key] = np.arange(g.n)
+    nl = [sorted(rank[gci[grp[v]:grp[v + 1]]]) for v in key]  # lists in new ids
+    want = 0
+    for x in range(g.n):
+        low = [v for v in nl[x] if v < x]
+        if low:
+            want += 16 + 4 * len(low)
+        for v in low:
+            want += 16 + 4 * sum(1 for w in nl[v] if w < x)
+    assert bench.cycle4_bytes(g) == want
+
+
+def test_oracle_task_sample_partitions_counts(xgm, orc):
+    """The oracle's explicit task lists (the CPU baseline's samples) visit
+    exactly the given vertices: two halves add up to the full count and op
+    stream."""
+    g = xgm.gen_rmat(11, 16, 0.57, 0.19, 0.19, 5)
+    d = g.orient()
+    for p, gr in (("triangle", d), ("clique4", d), ("clique5", d), ("diamond", g), ("rectangle", g)):
+        full = orc.count(p, gr.rowptr, gr.colidx)
+        a = orc.count_tasks(p, gr.rowptr, gr.colidx, np.arange(0, gr.n, 2))
+        b = orc.count_tasks(p, gr.rowptr, gr.colidx, np.arange(1, gr.n, 2))
+        assert (a[0] + b[0], a[1] + b[1]) == full, p
+        assert orc.count(p, gr.rowptr, gr.colidx) == full  # the list is cleared afterwards
