@@ -152,6 +152,20 @@ int xgm_setop_pairs(xgm_ctx *ctx, const xgm_dgraph *g, int op, const int32_t *a,
                     const int32_t *bounds, int64_t n_pairs, uint32_t *counts, int64_t *offsets, int32_t *values,
                     int64_t values_cap);
 
+/* Allocation-free variant of xgm_setop_pairs (counts only, values are not
+ * materialised).  No counterpart in the reference: it is for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  The pair arrays are
+ * staged in a static device array of the library (one per GPU, 16 MiB,
+ * held until the process ends) instead of buffers allocated per call; no
+ * hipMalloc / hipFree is called.  n_pairs above XGM_NOALLOC_MAX_PAIRS ->
+ * XGM_EINVAL.  Calls that overlap on one GPU (several threads or contexts)
+ * share that array and race: serialise them.  Same counts as
+ * xgm_setop_pairs for the same arguments. */
+#define XGM_NOALLOC_MAX_PAIRS (1 << 20)
+int xgm_setop_pairs_noalloc(xgm_ctx *ctx, const xgm_dgraph *g, int op, const int32_t *a, const int32_t *b,
+                            const int32_t *bounds, int64_t n_pairs, uint32_t *counts);
+
 /* ---- vertex-induced motif census -------------------------------------
  * g undirected (or a batch of undirected instances).  k = 3: out[2 * i ..] =
  * {wedges, triangles}; k = 4: out[6 * i ..] = {3-stars, 4-paths, tailed

@@ -15,6 +15,8 @@
 // in HBM to |L|/64 elements.  The streamed list is read 64 elements per
 // window; results are written in order with a ballot prefix.
 #include <algorithm>
+#include <map>
+#include <mutex>
 
 #include "device_common.h"
 #include "kernels.h"
@@ -105,6 +107,32 @@ __global__ __launch_bounds__(64 * kSetopWaves) void k_setop_pairs(
 }
 
 }  // namespace
+
+// Staging of the allocation-free entry point: pair ids a, b, bounds and
+// the counts, kNoAllocPairs each, in one file-scope device array (every GPU
+// the library runs on holds its own copy until the process ends).  Host code
+// reaches it only through the address hipGetSymbolAddress returns on that
+// device, never through the name.
+static __device__ __align__(256) int32_t scratch[4 * kNoAllocPairs];
+
+hipError_t setop_scratch_address(int device, int32_t **ptr) {
+  // looked up once per device, with that device current, by the caller's
+  // context set-up; later lookups return the cached address
+  static std::mutex mu;
+  static std::map<int, int32_t *> addr;
+  std::lock_guard<std::mutex> lock(mu);
+  auto it = addr.find(device);
+  if (it != addr.end()) {
+    *ptr = it->second;
+    return hipSuccess;
+  }
+  void *p = nullptr;
+  hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess) return e;
+  addr[device] = static_cast<int32_t *>(p);
+  *ptr = static_cast<int32_t *>(p);
+  return hipSuccess;
+}
 
 hipError_t launch_setop_pairs(const DevCSR &g, int op, const int32_t *a, const int32_t *b, const int32_t *bounds,
                               int64_t npairs, uint32_t *counts, const int64_t *offsets, int32_t *values, Launch &l) {

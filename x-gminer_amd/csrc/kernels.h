@@ -160,6 +160,11 @@ hipError_t launch_sched_argmin(const DevCSR &g, const float *F, const float *Y, 
 // of SETOP_*; counts[i] always written; values (at offsets[i]) in order when
 // non-null.  Pair ids must be < g.n (checked by the caller).
 enum SetOp { SETOP_INTERSECT = 1, SETOP_DIFF = 2, SETOP_DIFF_EXCL = 3 };
+// Allocation-free set operations: largest pair count of one call, and the
+// device address (on `device`, which must be current) of the library's
+// static staging array of 4 * kNoAllocPairs int32: a, b, bounds, counts.
+constexpr int64_t kNoAllocPairs = 1 << 20;
+hipError_t setop_scratch_address(int device, int32_t **ptr);
 hipError_t launch_setop_pairs(const DevCSR &g, int op, const int32_t *a, const int32_t *b, const int32_t *bounds,
                               int64_t npairs, uint32_t *counts, const int64_t *offsets, int32_t *values, Launch &l);
 

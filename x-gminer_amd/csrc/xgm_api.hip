@@ -33,6 +33,7 @@ struct xgm_ctx {
   std::string err;
   u64 *d_out = nullptr;
   int *d_queues = nullptr;
+  int32_t *d_noalloc = nullptr;  // this device's static staging array (xgm_setop_pairs_noalloc)
   std::vector<hipEvent_t> ev;  // 2 per profiled launch
   xgm_profile prof{};
   int n_marks = 0;
@@ -799,6 +800,7 @@ int xgm_ctx_create(int device, xgm_ctx **out) {
   HIP_TRY(ctx.get(), hipMalloc(&ctx->d_queues, kScratchQueues * sizeof(int)));
   ctx->ev.resize(2 * XGM_MAX_PROFILE_KERNELS);
   for (auto &e : ctx->ev) HIP_TRY(ctx.get(), hipEventCreate(&e));
+  HIP_TRY(ctx.get(), setop_scratch_address(device, &ctx->d_noalloc));
   *out = ctx.release();
   return XGM_OK;
 }
@@ -1649,6 +1651,46 @@ int xgm_setop_pairs(xgm_ctx *ctx, const xgm_dgraph *g, int op, const int32_t *a,
     if (total) HIP_TRY(ctx, hipMemcpyAsync(values, dval, (size_t)total * 4, hipMemcpyDeviceToHost, ctx->stream));
     HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
   }
+  for (int i = 0; i < ctx->prof.n_kernels; ++i) {
+    float ms = 0.f;
+    HIP_TRY(ctx, hipEventElapsedTime(&ms, ctx->ev[2 * i], ctx->ev[2 * i + 1]));
+    ctx->prof.k[i].ms = ms;
+    ctx->prof.total_ms += ms;
+  }
+  return XGM_OK;
+}
+
+static_assert(XGM_NOALLOC_MAX_PAIRS == kNoAllocPairs, "allocation-free pair limit");
+
+int xgm_setop_pairs_noalloc(xgm_ctx *ctx, const xgm_dgraph *g, int op, const int32_t *a, const int32_t *b,
+                            const int32_t *bounds, int64_t n_pairs, uint32_t *counts) {
+  if (!ctx || !g || n_pairs < 0 || (n_pairs && (!a || !b || !counts))) return XGM_EINVAL;
+  if (g->ctx != ctx) return ctx->fail(XGM_EINVAL, "graph belongs to another context");
+  if (op != XGM_SETOP_INTERSECT && op != XGM_SETOP_DIFF && op != XGM_SETOP_DIFF_EXCL)
+    return ctx->fail(XGM_EINVAL, "unknown set operation");
+  if (n_pairs > XGM_NOALLOC_MAX_PAIRS)
+    return ctx->fail(XGM_EINVAL, std::to_string(n_pairs) + " pairs exceed the allocation-free maximum of " +
+                                     std::to_string((int64_t)XGM_NOALLOC_MAX_PAIRS));
+  for (int64_t i = 0; i < n_pairs; ++i)
+    if (a[i] < 0 || a[i] >= g->n || b[i] < 0 || b[i] >= g->n)
+      return ctx->fail(XGM_EINVAL, "pair " + std::to_string(i) + " names a vertex outside the graph");
+  if (n_pairs == 0) return XGM_OK;
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  // the staging regions of this device's static array; every input region
+  // is overwritten before the launch and the kernel writes every count, so
+  // nothing left by an earlier call is read
+  int32_t *da = ctx->d_noalloc, *db = da + kNoAllocPairs, *dbound = db + kNoAllocPairs;
+  uint32_t *dcnt = reinterpret_cast<uint32_t *>(dbound + kNoAllocPairs);
+  const size_t n = (size_t)n_pairs;
+  HIP_TRY(ctx, hipMemcpyAsync(da, a, n * 4, hipMemcpyHostToDevice, ctx->stream));
+  HIP_TRY(ctx, hipMemcpyAsync(db, b, n * 4, hipMemcpyHostToDevice, ctx->stream));
+  if (bounds) HIP_TRY(ctx, hipMemcpyAsync(dbound, bounds, n * 4, hipMemcpyHostToDevice, ctx->stream));
+  ctx->prof = xgm_profile{};
+  Launch l = make_launch(ctx, 0);
+  HIP_TRY(ctx, launch_setop_pairs(dev_view(g), op, da, db, bounds ? dbound : nullptr, n_pairs, dcnt, nullptr,
+                                  nullptr, l));
+  HIP_TRY(ctx, hipMemcpyAsync(counts, dcnt, n * 4, hipMemcpyDeviceToHost, ctx->stream));
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
   for (int i = 0; i < ctx->prof.n_kernels; ++i) {
     float ms = 0.f;
     HIP_TRY(ctx, hipEventElapsedTime(&ms, ctx->ev[2 * i], ctx->ev[2 * i + 1]));
