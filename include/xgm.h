@@ -148,9 +148,30 @@ int xgm_count_csr(xgm_ctx *ctx, const xgm_csr *g, int pattern, const xgm_opts *o
  * values is not written and XGM_EINVAL is returned (offsets still filled).
  * a, b, bounds, counts, offsets and values are host arrays. */
 enum { XGM_SETOP_INTERSECT = 1, XGM_SETOP_DIFF = 2, XGM_SETOP_DIFF_EXCL = 3 };
+/* Strategy, OR-ed into op (same results; default = binary search of the
+ * streamed list's elements in the other list, staged in LDS up to 2,048
+ * elements, else narrowed by a 64-sample LDS index):
+ *   XGM_SETOP_MERGE   wave64 merge path over both lists (tiles of 512 path
+ *                     elements, 8 per lane) -- intersect_merge /
+ *                     intersect_num_merge, xgminer/include/glumin/
+ *                     operations.cuh:151-204, set_intersect.cuh:468-515
+ *   XGM_SETOP_HYBRID  merge path when both lists exceed 1024 elements, else
+ *                     binary search -- intersect_num_hybrid,
+ *                     operations.cuh:208-300 (ADJ_SIZE_THREASHOLD, common.h:61) */
+enum { XGM_SETOP_MERGE = 0x100, XGM_SETOP_HYBRID = 0x200 };
 int xgm_setop_pairs(xgm_ctx *ctx, const xgm_dgraph *g, int op, const int32_t *a, const int32_t *b,
                     const int32_t *bounds, int64_t n_pairs, uint32_t *counts, int64_t *offsets, int32_t *values,
                     int64_t values_cap);
+
+/* Counts of op (with its strategy flag) excluding ancestor vertices: for
+ * pair i, the elements x of the result with x != ancestors[i * n_anc + q]
+ * for q < n_anc (n_anc <= XGM_SETOP_MAX_ANCESTORS; -1 entries are unused).
+ * For XGM_SETOP_INTERSECT this is intersect_num(a, b, [upper_bound,]
+ * ancestor) / intersect_num(a, b, ancestor[], n)
+ * (xgminer/include/glumin/set_intersect.cuh:640-707).  Host arrays. */
+#define XGM_SETOP_MAX_ANCESTORS 4
+int xgm_setop_pairs_anc(xgm_ctx *ctx, const xgm_dgraph *g, int op, const int32_t *a, const int32_t *b,
+                        const int32_t *bounds, const int32_t *ancestors, int n_anc, int64_t n_pairs, uint32_t *counts);
 
 /* Allocation-free variant of xgm_setop_pairs (counts only, values are not
  * materialised).  No counterpart in the reference: it is for callers that

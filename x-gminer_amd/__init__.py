@@ -36,6 +36,9 @@ PATTERN_NAMES = {TRIANGLE: "triangle", CLIQUE4: "clique4", CLIQUE5: "clique5",
 # strategies
 STRAT_AUTO, STRAT_HASH, STRAT_BSEARCH, STRAT_LUT, STRAT_LIST, STRAT_ELUT, STRAT_LUT_RANK, STRAT_BUCKET = range(8)
 SETOP_INTERSECT, SETOP_DIFF, SETOP_DIFF_EXCL = 1, 2, 3
+# strategy flags OR-ed into a set operation (include/xgm.h)
+SETOP_MERGE, SETOP_HYBRID = 0x100, 0x200
+SETOP_MAX_ANCESTORS = 4
 NOALLOC_MAX_PAIRS = 1 << 20
 
 
@@ -120,6 +123,8 @@ def _load():
         "xgm_autotune": (C.c_int, [VP, VP, C.c_int, C.c_int, P(_Opts), P(C.c_float)]),
         "xgm_count_batch": (C.c_int, [VP, P(_Csr), P(C.c_int), C.c_int, P(U64)]),
         "xgm_setop_pairs_noalloc": (C.c_int, [VP, VP, C.c_int, P(I32), P(I32), P(I32), I64, P(C.c_uint32)]),
+        "xgm_setop_pairs_anc": (C.c_int, [VP, VP, C.c_int, P(I32), P(I32), P(I32), P(I32), C.c_int, I64,
+                                          P(C.c_uint32)]),
         "xgm_copy_bandwidth": (C.c_int, [VP, C.c_size_t, C.c_int, P(C.c_float)]),
         "xgm_motifs": (C.c_int, [VP, VP, C.c_int, P(U64)]),
         "xgm_setop_pairs": (C.c_int, [VP, VP, C.c_int, P(I32), P(I32), P(I32), I64, P(C.c_uint32), P(I64),
@@ -379,6 +384,20 @@ class Context:
         _check(lib.xgm_setop_pairs(self._h, dg._h, op, pi32(a), pi32(b), pbd, n, pc, po, pi32(vals), len(vals)),
                self._h)
         return counts, offsets, vals[:int(offsets[-1])]
+
+    def setop_pairs_anc(self, dg: "DeviceGraph", op: int, a, b, ancestors, bounds=None):
+        """Counts of op on N(a[i]), N(b[i]) below bounds[i], excluding the
+        vertices ancestors[i, :] (an n x k int array, k <= 4, -1 unused)."""
+        a = np.ascontiguousarray(a, np.int32)
+        b = np.ascontiguousarray(b, np.int32)
+        anc = np.ascontiguousarray(ancestors, np.int32).reshape(len(a), -1)
+        pi32 = lambda x: x.ctypes.data_as(C.POINTER(C.c_int32))
+        bd = None if bounds is None else np.ascontiguousarray(bounds, np.int32)
+        counts = np.zeros(len(a), np.uint32)
+        _check(lib.xgm_setop_pairs_anc(self._h, dg._h, op, pi32(a), pi32(b), pi32(bd) if bd is not None else None,
+                                       pi32(anc), anc.shape[1], len(a), counts.ctypes.data_as(C.POINTER(C.c_uint32))),
+               self._h)
+        return counts
 
     def setop_pairs_noalloc(self, dg: "DeviceGraph", op: int, a, b, bounds=None):
         """Counts of setop_pairs through the allocation-free entry point

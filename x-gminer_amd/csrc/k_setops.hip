@@ -1,19 +1,34 @@
 // Batched set operations on the neighbour lists of a resident graph: for
 // every pair (a, b) with A = N(a), B = N(b), truncated below an optional
 // bound, the intersection count, the difference count with or without the
-// owner exclusion, or the ordered result set.
+// owner exclusion, or the ordered result set; optionally excluding given
+// ancestor vertices.
 //
 // Reference semantics: intersect_num_bs_cache (+ upper bound) and
 // intersect_bs_cache (xgminer/include/glumin/set_intersect.cuh:99-132,
-// 440-465, 596-637); difference_num_bs_cache without owner exclusion
-// (set_difference.cuh:20-35, 110-129) and difference_set_bs_cache
-// (:218-241, 308-333); GraphMiner's VertexSet::difference_num, which also
-// drops the owner vid(B) = b (codes/GraphMiner/src/common/VertexSet.cc:69-88).
+// 440-465, 596-637); intersect_num with ancestors (:640-707);
+// difference_num_bs_cache without owner exclusion (set_difference.cuh:20-35,
+// 110-129) and difference_set_bs_cache (:218-241, 308-333); GraphMiner's
+// VertexSet::difference_num, which also drops the owner vid(B) = b
+// (codes/GraphMiner/src/common/VertexSet.cc:69-88); intersect_merge /
+// intersect_num_merge / intersect_num_hybrid (operations.cuh:151-204,
+// 208-300; set_intersect.cuh:468-515).
 //
-// A wave per pair.  The list searched into (L) sits in LDS when it has at
-// most kStage elements, else a 64-sample LDS index narrows a binary search
-// in HBM to |L|/64 elements.  The streamed list is read 64 elements per
-// window; results are written in order with a ballot prefix.
+// A wave per pair, three strategies with identical results:
+//  * binary search: the list searched into (L) sits in LDS when it has at
+//    most kStage elements, else a 64-sample LDS index narrows a binary
+//    search in HBM to |L|/64 elements; the other list is streamed 64
+//    elements per window;
+//  * merge path: both lists are merged in tiles of 512 path elements; the
+//    tile boundaries of 64 tiles at a time are found by one merge-path
+//    search per lane in HBM, a tile's two list pieces are staged in LDS with
+//    coalesced loads, and every lane merges its own 8 path elements after a
+//    merge-path search in LDS (work per lane fixed, whatever the list
+//    lengths);
+//  * hybrid: merge path when both lists exceed 1024 elements, else binary
+//    search (the reference's ADJ_SIZE_THREASHOLD rule, common.h:61).
+// Results are written in order with a ballot prefix (binary search) or a
+// wave scan of per-lane counts (merge path).
 #include <algorithm>
 #include <map>
 #include <mutex>
@@ -26,11 +41,106 @@ namespace {
 
 constexpr int kStage = 2048;
 constexpr int kSetopWaves = 4;  // waves per block
+constexpr int kMergeE = 8;                  // merge-path elements per lane per tile
+constexpr int kMergeTile = 64 * kMergeE;    // path elements per tile
+constexpr int kHybridMin = 1024;            // hybrid: merge when both lists are longer
+constexpr int kMaxAnc = 4;
 
 struct WaveStage {
   int list[kStage];
   int sample[64];
+  int split[64];
 };
+static_assert(2 * kMergeTile + 1 <= kStage, "merge tile fits the stage");
+
+// Number of A elements among the first d elements of the merge of A and B
+// (A before B on equal keys): the first i in [max(0, d - lb), min(d, la)]
+// with A[i] > B[d - 1 - i].
+template <class PA, class PB>
+__device__ __forceinline__ int merge_split(PA A, int la, PB B, int lb, int d) {
+  int lo = max(0, d - lb), hi = min(d, la);
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (A[mid] <= B[d - 1 - mid]) lo = mid + 1; else hi = mid;
+  }
+  return lo;
+}
+
+// Merge-path walk of A[0..la) and B[0..lb) (HBM): every element x of A is
+// visited once with found = (x in B); the lanes' counts of keep(x, found)
+// are summed over the wave, and with out != null the kept elements are
+// written in increasing order to out[0..).  All 64 lanes call.
+template <class Keep>
+__device__ uint32_t merge_walk(WaveStage &S, const int32_t *__restrict__ A, int la, const int32_t *__restrict__ B,
+                               int lb, int lane, Keep keep, int32_t *__restrict__ out) {
+  const int L = la + lb;
+  uint32_t total = 0;
+  int ia0 = 0, ib0 = 0;  // split at the start of the current tile
+  int *As = S.list, *Bs = S.list + kMergeTile;
+  for (int t0 = 0; t0 < L; t0 += 64 * kMergeTile) {
+    // tile ends of the next 64 tiles, one merge-path search per lane
+    wave_sync();
+    {
+      const int d = (int)min((int64_t)L, (int64_t)t0 + (int64_t)(lane + 1) * kMergeTile);
+      S.split[lane] = merge_split(A, la, B, lb, d);
+    }
+    wave_sync();
+    const int ntiles = min(64, (L - t0 + kMergeTile - 1) / kMergeTile);
+    for (int q = 0; q < ntiles; ++q) {
+      const int d1 = (int)min((int64_t)L, (int64_t)t0 + (int64_t)(q + 1) * kMergeTile);
+      const int ia1 = S.split[q], ib1 = d1 - ia1;
+      const int na = ia1 - ia0, nb = ib1 - ib0;
+      wave_sync();  // the previous tile's reads of As / Bs are done
+      for (int i = lane; i < na; i += 64) As[i] = A[ia0 + i];
+      // B one past the tile: the partner of the tile's last A element may open the next tile
+      for (int i = lane; i <= nb; i += 64) Bs[i] = (ib0 + i < lb) ? B[ib0 + i] : INT32_MAX;
+      wave_sync();
+      const int dl = min(lane * kMergeE, na + nb);
+      const int ia_s = merge_split(As, na, Bs, nb, dl);
+      const int steps = min(kMergeE, na + nb - dl);
+      // pass 1: count this lane's kept elements
+      uint32_t c = 0;
+      {
+        int ia = ia_s, ib = dl - ia_s;
+#pragma unroll
+        for (int st = 0; st < kMergeE; ++st) {
+          if (st < steps) {
+            if (ia < na && (ib >= nb || As[ia] <= Bs[ib])) {
+              const int x = As[ia];
+              c += keep(x, Bs[ib] == x) ? 1u : 0u;
+              ++ia;
+            } else {
+              ++ib;
+            }
+          }
+        }
+      }
+      const uint32_t incl = (uint32_t)wave_incl_scan((int)c, lane);
+      if (out && c) {
+        // pass 2: write them at this lane's place in the wave's order
+        uint32_t pos = total + incl - c;
+        int ia = ia_s, ib = dl - ia_s;
+#pragma unroll
+        for (int st = 0; st < kMergeE; ++st) {
+          if (st < steps) {
+            if (ia < na && (ib >= nb || As[ia] <= Bs[ib])) {
+              const int x = As[ia];
+              if (keep(x, Bs[ib] == x)) out[pos++] = x;
+              ++ia;
+            } else {
+              ++ib;
+            }
+          }
+        }
+      }
+      total += (uint32_t)__shfl((int)incl, 63, 64);
+      ia0 = ia1;
+      ib0 = ib1;
+    }
+  }
+  wave_sync();
+  return total;
+}
 
 // Membership of key in L (length ll, sorted): LDS copy when staged, else
 // the sample index then HBM.
@@ -63,8 +173,9 @@ __device__ __forceinline__ void stage(WaveStage &S, const int32_t *__restrict__ 
 
 __global__ __launch_bounds__(64 * kSetopWaves) void k_setop_pairs(
     const int64_t *__restrict__ rowptr, const int32_t *__restrict__ colidx, const int32_t *__restrict__ pa,
-    const int32_t *__restrict__ pb, const int32_t *__restrict__ bounds, int64_t npairs, int op,
-    uint32_t *__restrict__ counts, const int64_t *__restrict__ offsets, int32_t *__restrict__ values) {
+    const int32_t *__restrict__ pb, const int32_t *__restrict__ bounds, const int32_t *__restrict__ anc, int n_anc,
+    int64_t npairs, int op, int strategy, uint32_t *__restrict__ counts, const int64_t *__restrict__ offsets,
+    int32_t *__restrict__ values) {
   __shared__ WaveStage stages[kSetopWaves];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   WaveStage &S = stages[wave];
@@ -72,12 +183,31 @@ __global__ __launch_bounds__(64 * kSetopWaves) void k_setop_pairs(
   for (int64_t p = (int64_t)blockIdx.x * kSetopWaves + wave; p < npairs; p += nw) {
     const int a = pa[p], b = pb[p];
     const int ub = bounds ? bounds[p] : INT32_MAX;
+    // ancestors of this pair (at most kMaxAnc; -1 = unused), wave-uniform
+    int an[kMaxAnc];
+#pragma unroll
+    for (int q = 0; q < kMaxAnc; ++q) an[q] = (anc && q < n_anc) ? anc[p * n_anc + q] : -1;
     const int32_t *A = colidx + rowptr[a];
     const int32_t *B = colidx + rowptr[b];
     // truncate both lists below the bound
     const int la = lower_bound_dev(A, (int)(rowptr[a + 1] - rowptr[a]), ub);
     const int lb = lower_bound_dev(B, (int)(rowptr[b + 1] - rowptr[b]), ub);
     const bool intersect = op == SETOP_INTERSECT;
+    const bool excl = op == SETOP_DIFF_EXCL;
+    auto keep = [&](int x, bool found) {
+      bool k = intersect ? found : (!found && !(excl && x == b));
+#pragma unroll
+      for (int q = 0; q < kMaxAnc; ++q) k = k && x != an[q];
+      return k;
+    };
+    const bool merge = strategy == SETOP_STRAT_MERGE ||
+                       (strategy == SETOP_STRAT_HYBRID && la > kHybridMin && lb > kHybridMin);
+    if (merge) {
+      // A is the list whose elements are kept (either list for an intersection)
+      const uint32_t hits = merge_walk(S, A, la, B, lb, lane, keep, values ? values + offsets[p] : nullptr);
+      if (lane == 0) counts[p] = hits;
+      continue;
+    }
     // intersections stream the shorter list; differences stream A
     const bool swap = intersect && la > lb;
     const int32_t *T = swap ? B : A;   // streamed
@@ -91,14 +221,13 @@ __global__ __launch_bounds__(64 * kSetopWaves) void k_setop_pairs(
     int64_t out = values ? offsets[p] : 0;
     for (int k0 = 0; k0 < lt; k0 += 64) {
       const int k = k0 + lane;
-      bool keep = false;
+      bool kept = false;
       if (k < lt) {
         const int x = T[k];
-        const bool found = in_list(S, staged, L, ll, x);
-        keep = intersect ? found : (!found && !(op == SETOP_DIFF_EXCL && x == b));
+        kept = keep(x, in_list(S, staged, L, ll, x));
       }
-      const u64 m = __ballot(keep);
-      if (values && keep) values[out + __popcll(m & lanemask_lt(lane))] = T[k];
+      const u64 m = __ballot(kept);
+      if (values && kept) values[out + __popcll(m & lanemask_lt(lane))] = T[k];
       out += __popcll(m);
       hits += (uint32_t)__popcll(m);
     }
@@ -134,8 +263,10 @@ hipError_t setop_scratch_address(int device, int32_t **ptr) {
   return hipSuccess;
 }
 
-hipError_t launch_setop_pairs(const DevCSR &g, int op, const int32_t *a, const int32_t *b, const int32_t *bounds,
-                              int64_t npairs, uint32_t *counts, const int64_t *offsets, int32_t *values, Launch &l) {
+hipError_t launch_setop_pairs(const DevCSR &g, int op, int strategy, const int32_t *a, const int32_t *b,
+                              const int32_t *bounds, const int32_t *anc, int n_anc, int64_t npairs, uint32_t *counts,
+                              const int64_t *offsets, int32_t *values, Launch &l) {
+  if (n_anc < 0 || n_anc > kMaxAnc) return hipErrorInvalidValue;
   if (npairs <= 0) return hipSuccess;
   int per_cu = 0;
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_setop_pairs, 64 * kSetopWaves, 0) != hipSuccess ||
@@ -143,9 +274,12 @@ hipError_t launch_setop_pairs(const DevCSR &g, int op, const int32_t *a, const i
     per_cu = 1;
   const int64_t want = (npairs + kSetopWaves - 1) / kSetopWaves;
   const int grid = (int)std::max<int64_t>(1, std::min<int64_t>(want, (int64_t)per_cu * l.num_cu));
-  l.begin(values ? "setop_pairs_materialise" : "setop_pairs_count", npairs);
-  k_setop_pairs<<<grid, 64 * kSetopWaves, 0, l.stream>>>(g.rowptr, g.colidx, a, b, bounds, npairs, op, counts,
-                                                         offsets, values);
+  static const char *names[3][2] = {{"setop_pairs_count", "setop_pairs_materialise"},
+                                    {"setop_merge_count", "setop_merge_materialise"},
+                                    {"setop_hybrid_count", "setop_hybrid_materialise"}};
+  l.begin(names[strategy][values ? 1 : 0], npairs);
+  k_setop_pairs<<<grid, 64 * kSetopWaves, 0, l.stream>>>(g.rowptr, g.colidx, a, b, bounds, anc, n_anc, npairs, op,
+                                                         strategy, counts, offsets, values);
   l.end();
   return hipGetLastError();
 }

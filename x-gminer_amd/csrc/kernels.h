@@ -165,8 +165,12 @@ enum SetOp { SETOP_INTERSECT = 1, SETOP_DIFF = 2, SETOP_DIFF_EXCL = 3 };
 // static staging array of 4 * kNoAllocPairs int32: a, b, bounds, counts.
 constexpr int64_t kNoAllocPairs = 1 << 20;
 hipError_t setop_scratch_address(int device, int32_t **ptr);
-hipError_t launch_setop_pairs(const DevCSR &g, int op, const int32_t *a, const int32_t *b, const int32_t *bounds,
-                              int64_t npairs, uint32_t *counts, const int64_t *offsets, int32_t *values, Launch &l);
+// strategy: SETOP_STRAT_*; anc: n_anc (<= 4) ancestor ids per pair excluded
+// from the result (-1 = none), or null.
+enum SetOpStrategy { SETOP_STRAT_BSEARCH = 0, SETOP_STRAT_MERGE = 1, SETOP_STRAT_HYBRID = 2 };
+hipError_t launch_setop_pairs(const DevCSR &g, int op, int strategy, const int32_t *a, const int32_t *b,
+                              const int32_t *bounds, const int32_t *anc, int n_anc, int64_t npairs, uint32_t *counts,
+                              const int64_t *offsets, int32_t *values, Launch &l);
 
 // On-device degree orientation (k_prep.hip): deg[v]; out/in flag per entry
 // of the undirected CSR (flag[nnz] = 0); after an exclusive scan pos of the

@@ -1591,13 +1591,27 @@ int xgm_motifs(xgm_ctx *ctx, const xgm_dgraph *g, int k, uint64_t *out) {
   return XGM_OK;
 }
 
-int xgm_setop_pairs(xgm_ctx *ctx, const xgm_dgraph *g, int op, const int32_t *a, const int32_t *b,
-                    const int32_t *bounds, int64_t n_pairs, uint32_t *counts, int64_t *offsets, int32_t *values,
-                    int64_t values_cap) {
+}  // extern "C"
+
+// op: XGM_SETOP_{INTERSECT, DIFF, DIFF_EXCL} | XGM_SETOP_{MERGE, HYBRID}; the
+// kernel operation and strategy, or false for anything else.
+static bool split_setop(int op, int &kop, int &strategy) {
+  kop = op & 0xff;
+  const int flags = op & ~0xff;
+  strategy = flags == 0 ? SETOP_STRAT_BSEARCH : flags == XGM_SETOP_MERGE ? SETOP_STRAT_MERGE
+           : flags == XGM_SETOP_HYBRID ? SETOP_STRAT_HYBRID : -1;
+  return strategy >= 0 && (kop == XGM_SETOP_INTERSECT || kop == XGM_SETOP_DIFF || kop == XGM_SETOP_DIFF_EXCL);
+}
+
+static int setop_pairs_impl(xgm_ctx *ctx, const xgm_dgraph *g, int op, const int32_t *a, const int32_t *b,
+                            const int32_t *bounds, const int32_t *anc, int n_anc, int64_t n_pairs, uint32_t *counts,
+                            int64_t *offsets, int32_t *values, int64_t values_cap) {
   if (!ctx || !g || n_pairs < 0 || (n_pairs && (!a || !b || !counts))) return XGM_EINVAL;
   if (g->ctx != ctx) return ctx->fail(XGM_EINVAL, "graph belongs to another context");
-  if (op != XGM_SETOP_INTERSECT && op != XGM_SETOP_DIFF && op != XGM_SETOP_DIFF_EXCL)
-    return ctx->fail(XGM_EINVAL, "unknown set operation");
+  int kop = 0, strategy = 0;
+  if (!split_setop(op, kop, strategy)) return ctx->fail(XGM_EINVAL, "unknown set operation");
+  if (n_anc < 0 || n_anc > XGM_SETOP_MAX_ANCESTORS || (n_anc && !anc))
+    return ctx->fail(XGM_EINVAL, "ancestors: 0 to 4 per pair");
   if (values && !offsets) return ctx->fail(XGM_EINVAL, "values need offsets");
   for (int64_t i = 0; i < n_pairs; ++i)
     if (a[i] < 0 || a[i] >= g->n || b[i] < 0 || b[i] >= g->n)
@@ -1619,7 +1633,7 @@ int xgm_setop_pairs(xgm_ctx *ctx, const xgm_dgraph *g, int op, const int32_t *a,
     if (e == hipSuccess) bufs.p.push_back(*d);
     return e;
   };
-  int32_t *da = nullptr, *db = nullptr, *dbound = nullptr, *dval = nullptr;
+  int32_t *da = nullptr, *db = nullptr, *dbound = nullptr, *dval = nullptr, *danc = nullptr;
   uint32_t *dcnt = nullptr;
   int64_t *doff = nullptr;
   const size_t n = (size_t)n_pairs;
@@ -1632,10 +1646,14 @@ int xgm_setop_pairs(xgm_ctx *ctx, const xgm_dgraph *g, int op, const int32_t *a,
     HIP_TRY(ctx, dev((void **)&dbound, n * 4));
     HIP_TRY(ctx, hipMemcpyAsync(dbound, bounds, n * 4, hipMemcpyHostToDevice, ctx->stream));
   }
+  if (n_anc) {
+    HIP_TRY(ctx, dev((void **)&danc, n * n_anc * 4));
+    HIP_TRY(ctx, hipMemcpyAsync(danc, anc, n * n_anc * 4, hipMemcpyHostToDevice, ctx->stream));
+  }
   ctx->prof = xgm_profile{};
   Launch l = make_launch(ctx, 0);
   const DevCSR dg = dev_view(g);
-  HIP_TRY(ctx, launch_setop_pairs(dg, op, da, db, dbound, n_pairs, dcnt, nullptr, nullptr, l));
+  HIP_TRY(ctx, launch_setop_pairs(dg, kop, strategy, da, db, dbound, danc, n_anc, n_pairs, dcnt, nullptr, nullptr, l));
   HIP_TRY(ctx, hipMemcpyAsync(counts, dcnt, n * 4, hipMemcpyDeviceToHost, ctx->stream));
   HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
   if (values) {
@@ -1647,7 +1665,7 @@ int xgm_setop_pairs(xgm_ctx *ctx, const xgm_dgraph *g, int op, const int32_t *a,
     HIP_TRY(ctx, dev((void **)&doff, (n + 1) * 8));
     HIP_TRY(ctx, dev((void **)&dval, (size_t)total * 4));
     HIP_TRY(ctx, hipMemcpyAsync(doff, offsets, (n + 1) * 8, hipMemcpyHostToDevice, ctx->stream));
-    HIP_TRY(ctx, launch_setop_pairs(dg, op, da, db, dbound, n_pairs, dcnt, doff, dval, l));
+    HIP_TRY(ctx, launch_setop_pairs(dg, kop, strategy, da, db, dbound, danc, n_anc, n_pairs, dcnt, doff, dval, l));
     if (total) HIP_TRY(ctx, hipMemcpyAsync(values, dval, (size_t)total * 4, hipMemcpyDeviceToHost, ctx->stream));
     HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
   }
@@ -1660,14 +1678,27 @@ int xgm_setop_pairs(xgm_ctx *ctx, const xgm_dgraph *g, int op, const int32_t *a,
   return XGM_OK;
 }
 
+extern "C" {
+
+int xgm_setop_pairs(xgm_ctx *ctx, const xgm_dgraph *g, int op, const int32_t *a, const int32_t *b,
+                    const int32_t *bounds, int64_t n_pairs, uint32_t *counts, int64_t *offsets, int32_t *values,
+                    int64_t values_cap) {
+  return setop_pairs_impl(ctx, g, op, a, b, bounds, nullptr, 0, n_pairs, counts, offsets, values, values_cap);
+}
+
+int xgm_setop_pairs_anc(xgm_ctx *ctx, const xgm_dgraph *g, int op, const int32_t *a, const int32_t *b,
+                        const int32_t *bounds, const int32_t *ancestors, int n_anc, int64_t n_pairs, uint32_t *counts) {
+  return setop_pairs_impl(ctx, g, op, a, b, bounds, ancestors, n_anc, n_pairs, counts, nullptr, nullptr, 0);
+}
+
 static_assert(XGM_NOALLOC_MAX_PAIRS == kNoAllocPairs, "allocation-free pair limit");
 
 int xgm_setop_pairs_noalloc(xgm_ctx *ctx, const xgm_dgraph *g, int op, const int32_t *a, const int32_t *b,
                             const int32_t *bounds, int64_t n_pairs, uint32_t *counts) {
   if (!ctx || !g || n_pairs < 0 || (n_pairs && (!a || !b || !counts))) return XGM_EINVAL;
   if (g->ctx != ctx) return ctx->fail(XGM_EINVAL, "graph belongs to another context");
-  if (op != XGM_SETOP_INTERSECT && op != XGM_SETOP_DIFF && op != XGM_SETOP_DIFF_EXCL)
-    return ctx->fail(XGM_EINVAL, "unknown set operation");
+  int kop = 0, strategy = 0;
+  if (!split_setop(op, kop, strategy)) return ctx->fail(XGM_EINVAL, "unknown set operation");
   if (n_pairs > XGM_NOALLOC_MAX_PAIRS)
     return ctx->fail(XGM_EINVAL, std::to_string(n_pairs) + " pairs exceed the allocation-free maximum of " +
                                      std::to_string((int64_t)XGM_NOALLOC_MAX_PAIRS));
@@ -1687,8 +1718,8 @@ int xgm_setop_pairs_noalloc(xgm_ctx *ctx, const xgm_dgraph *g, int op, const int
   if (bounds) HIP_TRY(ctx, hipMemcpyAsync(dbound, bounds, n * 4, hipMemcpyHostToDevice, ctx->stream));
   ctx->prof = xgm_profile{};
   Launch l = make_launch(ctx, 0);
-  HIP_TRY(ctx, launch_setop_pairs(dev_view(g), op, da, db, bounds ? dbound : nullptr, n_pairs, dcnt, nullptr,
-                                  nullptr, l));
+  HIP_TRY(ctx, launch_setop_pairs(dev_view(g), kop, strategy, da, db, bounds ? dbound : nullptr, nullptr, 0, n_pairs,
+                                  dcnt, nullptr, nullptr, l));
   HIP_TRY(ctx, hipMemcpyAsync(counts, dcnt, n * 4, hipMemcpyDeviceToHost, ctx->stream));
   HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
   for (int i = 0; i < ctx->prof.n_kernels; ++i) {
