@@ -864,7 +864,49 @@ class Suite:
                 self.pass_rows.append(dict(row="k5_rmat22_learned", graph=pre, flags=d.flags, pattern=pid,
                                            kind="sched", codes=os.path.join(self.cache, "rmat22_k5_learned.npy")))
         dg.close()
+        self.setops(g)
         return g
+
+    def setops(self, g):
+        """Set intersection counts on the undirected R-MAT-22 lists under
+        each strategy, for three classes of pairs: hub x hub (all pairs of
+        the 64 highest-degree vertices, both lists above 1024), hub x small
+        and the graph's own edges.  Bytes: 4 (|A| + |B|) + 8 per pair, each
+        list read once."""
+        xgm = self.xgm
+        ug = self.ctx.upload(g)
+        deg = np.diff(g.rowptr).astype(np.int64)
+        rng = np.random.default_rng(7)
+        top = np.argsort(-deg, kind="stable")[:64].astype(np.int32)
+        cls = {"hub_x_hub": (np.repeat(top, 64), np.tile(top, 64)),
+               "hub_x_small": (np.repeat(top, 64), rng.choice(np.nonzero((deg > 0) & (deg <= 64))[0], 4096)
+                               .astype(np.int32))}
+        e = rng.integers(0, g.nnz, 1 << 18)
+        src = (np.searchsorted(g.rowptr, e, side="right") - 1).astype(np.int32)
+        cls["edges"] = (src, g.colidx[e].astype(np.int32))
+        out = {}
+        for name, (a, b) in cls.items():
+            nbytes = int(4 * (deg[a] + deg[b]).sum() + 8 * len(a))
+            res, ref = {}, None
+            for sname, flag in (("bsearch", 0), ("merge", xgm.SETOP_MERGE), ("hybrid", xgm.SETOP_HYBRID)):
+                ms = []
+                for _ in range(3):
+                    c = self.ctx.setop_pairs(ug, xgm.SETOP_INTERSECT | flag, a, b)
+                    ms.append(self.ctx.last_profile()[1][0].ms)
+                    if ref is None:
+                        ref = c
+                    elif not np.array_equal(c, ref):
+                        raise SystemExit(f"setops {name}: {sname} disagrees with binary search")
+                t = float(np.median(ms))
+                res[sname] = dict(kernel_ms=round(t, 4), achieved_gbs=round(nbytes / (t * 1e-3) / 1e9, 1),
+                                  frac=round(nbytes / (t * 1e-3) / 1e9 / HBM_PEAK_GBS, 4))
+            out[name] = dict(pairs=len(a), list_bytes=nbytes, total_intersection=int(ref.sum()), **res)
+            log(f"setops {name}: " + ", ".join(f"{k} {v['kernel_ms']:.3f} ms" for k, v in res.items()))
+        ug.close()
+        self.rows.append(dict(row="setops_rmat22", workload="set intersection counts on the undirected R-MAT-22 "
+                              "lists, per pair class and strategy (binary search / wave64 merge path / hybrid)",
+                              pattern="intersect", classes=out, bytes_model="4*(|A|+|B|)+8 per pair",
+                              cpu_baseline=None))
 
     def rmat16(self):
         a = self.args

@@ -47,7 +47,22 @@ enum xgm_pattern {
   XGM_CLIQUE4 = 2,
   XGM_CLIQUE5 = 3,
   XGM_DIAMOND_EDGE = 4,
-  XGM_CYCLE4_EDGE = 5
+  XGM_CYCLE4_EDGE = 5,
+  /* Vertex-induced 4-vertex motifs on the undirected graph, each induced
+   * embedding counted once (GraphMiner's unique-embedding semantics,
+   * src/motif/cpu_kernels/automine_base.h:23-74):
+   *   XGM_STAR3_V      <- GLUMIN --patternID 17 (local id 1, P1; GM_kernels/P1-GM.yml.cuh)
+   *   XGM_DIAMOND_V    <- GLUMIN --patternID 18 (local id 2, P2; P2-GM.yml.cuh).  GLUMIN's P2
+   *                       walks both directions of every edge and reports twice this count.
+   *   XGM_TAILEDTRI_V  <- GLUMIN --patternID 19 (local id 3, P3; P3-GM.yml.cuh)
+   * Strategies: AUTO / LUT -- LDS LUT over N(v0) with AND-NOT row popcounts for
+   * degree <= 1024 (GM_LUT_kernels/P1.yml.cuh, codegen_bitmap.cuh:392-407), set
+   * difference and hybrid merge / binary-search intersections above; LIST --
+   * the set-difference path for every vertex.  The enumeration is cubic in the
+   * degree: xgm_motifs (the formula path) is the census for graphs with hubs. */
+  XGM_STAR3_V = 6,
+  XGM_DIAMOND_V = 7,
+  XGM_TAILEDTRI_V = 8
 };
 
 /* Intersection / counting strategies (xgm_opts.strategy).  All produce
@@ -63,9 +78,13 @@ enum xgm_strategy {
                            4-cycle: bounded pairwise intersections in the reference loop order */
   XGM_STRAT_ELUT = 5,   /* cliques: edge tasks, bitmap LUT over N+(v0) ∩ N+(v1) */
   XGM_STRAT_LUT_RANK = 6,/* cliques: vertex LUT with rows in DAG order (upper-triangular) */
-  XGM_STRAT_BUCKET = 7  /* triangles: X-GMiner bucket-bitmap prefilter -- lists split into 64
+  XGM_STRAT_BUCKET = 7, /* triangles: X-GMiner bucket-bitmap prefilter -- lists split into 64
                            buckets by id & 63, only buckets non-empty on both sides intersected
                            (xgminer/include/xgminer_bitmap.cuh:161-320, 661-822) */
+  XGM_STRAT_MERGE = 8   /* triangles: merge-based intersection -- one lane merges N+(u) with N+(v)
+                           per DAG edge, the wave64 merge path when both lists exceed 1024
+                           (intersect_num_merge / intersect_num_hybrid,
+                           xgminer/include/glumin/set_intersect.cuh:468-515, operations.cuh:208-300) */
 };
 
 typedef struct xgm_opts {
@@ -229,7 +248,8 @@ void xgm_plan_free(xgm_plan *plan);
 /* Autotuner: times the candidate option sets of `pattern` on g (every
  * candidate gives the same count; a disagreement is an error) and writes the
  * fastest into *best.  Candidates: triangles -- LDS hash load 1/2 or 1/4 x
- * five wave/block tier splits, and binary search; cliques -- auto, rank LUT,
+ * five wave/block tier splits, binary search, the merge strategy and (out-
+ * degrees below 65536) the bucket bitmap; cliques -- auto, rank LUT,
  * id-ordered LUT with threshold 1024 or 256, edge LUT; 4-cycles -- the wedge
  * aggregation with each wedge_range mode.  Each candidate runs 1 + reps
  * times. */

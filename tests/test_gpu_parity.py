@@ -12,7 +12,7 @@ from conftest import CITESEER, GOLDEN, complete_graph, random_graph
 pytestmark = pytest.mark.gpu
 
 DAG_PATTERNS = {"triangle": 1}
-STRATS = {"hash": 1, "bsearch": 2, "bucket": 7}
+STRATS = {"hash": 1, "bsearch": 2, "bucket": 7, "merge": 8}
 
 
 def _say(*a):

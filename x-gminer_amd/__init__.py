@@ -31,10 +31,14 @@ XGM_OK, XGM_EINVAL, XGM_ENOMEM, XGM_EUNSUPPORTED, XGM_EHIP, XGM_ENODEV, XGM_EIO 
 CSR_SORTED, CSR_SYMMETRIC, CSR_DAG = 0x1, 0x2, 0x4
 # patterns (include/xgm.h)
 TRIANGLE, CLIQUE4, CLIQUE5, DIAMOND_EDGE, CYCLE4_EDGE = 1, 2, 3, 4, 5
+# vertex-induced 4-motifs (GLUMIN P1 / P2 / P3), each embedding once
+STAR3_V, DIAMOND_V, TAILEDTRI_V = 6, 7, 8
 PATTERN_NAMES = {TRIANGLE: "triangle", CLIQUE4: "clique4", CLIQUE5: "clique5",
-                 DIAMOND_EDGE: "diamond", CYCLE4_EDGE: "rectangle"}
+                 DIAMOND_EDGE: "diamond", CYCLE4_EDGE: "rectangle", STAR3_V: "3-star",
+                 DIAMOND_V: "diamond-induced", TAILEDTRI_V: "tailed-triangle"}
 # strategies
-STRAT_AUTO, STRAT_HASH, STRAT_BSEARCH, STRAT_LUT, STRAT_LIST, STRAT_ELUT, STRAT_LUT_RANK, STRAT_BUCKET = range(8)
+(STRAT_AUTO, STRAT_HASH, STRAT_BSEARCH, STRAT_LUT, STRAT_LIST, STRAT_ELUT, STRAT_LUT_RANK, STRAT_BUCKET,
+ STRAT_MERGE) = range(9)
 SETOP_INTERSECT, SETOP_DIFF, SETOP_DIFF_EXCL = 1, 2, 3
 # strategy flags OR-ed into a set operation (include/xgm.h)
 SETOP_MERGE, SETOP_HYBRID = 0x100, 0x200
@@ -597,6 +601,18 @@ def SglSolver(g: Graph, pattern: str) -> int:
     if pid is None:
         raise XGMError(XGM_EUNSUPPORTED, f"pattern {pattern!r} not supported")
     return _ctx().count(g, pid)
+
+
+def PatternSolver(g: Graph, k: int) -> int:
+    """GLUMIN's pattern entry (GLUMIN::run with --patternID 16 + k,
+    xgminer/include/glumin.h:36-86): k = 1 3-star, 2 diamond, 3 tailed
+    triangle (vertex-induced, on the undirected graph), 4 / 5 = 4- / 5-clique
+    (on its degree-oriented DAG).  Counts are unique embeddings; GLUMIN's own
+    P2 kernel reports twice the diamonds."""
+    pid = {1: STAR3_V, 2: DIAMOND_V, 3: TAILEDTRI_V, 4: CLIQUE4, 5: CLIQUE5}.get(k)
+    if pid is None:
+        raise XGMError(XGM_EUNSUPPORTED, f"pattern {k} not supported")
+    return _ctx().count(g.orient() if k >= 4 else g, pid)
 
 
 # --------------------------------------------------------------------------

@@ -38,7 +38,8 @@ const char *kUsage =
     "  --pattern-name NAME     triangle | clique4 | clique5 | diamond | rectangle | motif3 | motif4\n"
     "  --pattern-size S --pattern-adj-mat M   pattern by adjacency matrix (row-major 0/1 string)\n"
     "  --repeat R              count R times (default 1)\n"
-    "  --strategy NAME         auto | hash | bsearch | lut | list | elut | rank (overrides --algorithm)\n"
+    "  --strategy NAME         auto | hash | bsearch | lut | list | elut | rank | bucket | merge\n"
+    "                          (overrides --algorithm)\n"
     "  --device D              HIP device (default 0)\n"
     "  -h, --help              this text\n"
     "Accepted and ignored (no effect on counts): --dataname --do-validation --use-graphpi-sched\n"
@@ -224,7 +225,8 @@ int main(int argc, char **argv) {
     const std::map<std::string, int> s = {{"auto", XGM_STRAT_AUTO}, {"hash", XGM_STRAT_HASH},
                                           {"bsearch", XGM_STRAT_BSEARCH}, {"lut", XGM_STRAT_LUT},
                                           {"list", XGM_STRAT_LIST}, {"elut", XGM_STRAT_ELUT},
-                                          {"rank", XGM_STRAT_LUT_RANK}};
+                                          {"rank", XGM_STRAT_LUT_RANK}, {"bucket", XGM_STRAT_BUCKET},
+                                          {"merge", XGM_STRAT_MERGE}};
     if (!s.count(opt["strategy"])) return fail("unknown strategy '" + opt["strategy"] + "'");
     strategy = s.at(opt["strategy"]);
   }

@@ -25,6 +25,7 @@
 
 #include "device_common.h"
 #include "kernels.h"
+#include "merge_path.h"
 #include "segments.h"
 
 namespace xgm {
@@ -264,11 +265,14 @@ __device__ u64 block_sum_intersections(const int64_t *__restrict__ rowptr, const
 // Build the LUT of the induced sub-DAG on S.x[0..s) (s <= SMAX) and count
 // its cliques of `depth` vertices.  All threads call; returns this thread's
 // partial count.
-template <int SMAX, int W, int NT, int WMIN = 1>
-__device__ u64 block_lut_count(const int64_t *__restrict__ rowptr, const int32_t *__restrict__ colidx,
-                               CStage<SMAX, W, NT> &S, int s, int depth, WaveFlags &fl) {
+// Build the LUT of the subgraph induced on S.x[0..s) (s <= SMAX): row i
+// gets bit j iff S.x[j] is in the list of S.x[i] (the sub-DAG for a DAG,
+// the symmetric adjacency for an undirected graph).  S.rowq is zero on
+// exit.  All threads call.
+template <int SMAX, int W, int NT>
+__device__ void block_lut_build(const int64_t *__restrict__ rowptr, const int32_t *__restrict__ colidx,
+                                CStage<SMAX, W, NT> &S, int s, WaveFlags &fl) {
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int weff = (s + 63) >> 6;
   for (int q = tid; q < lut_words<W>(s); q += NT) S.rows[q] = 0ull;
   __syncthreads();
   // every thread loads its members' list bounds; x2 (free here) is scratch
@@ -283,6 +287,14 @@ __device__ u64 block_lut_count(const int64_t *__restrict__ rowptr, const int32_t
   });
   if (tid == 0) S.rowq = 0;
   __syncthreads();
+}
+
+template <int SMAX, int W, int NT, int WMIN = 1>
+__device__ u64 block_lut_count(const int64_t *__restrict__ rowptr, const int32_t *__restrict__ colidx,
+                               CStage<SMAX, W, NT> &S, int s, int depth, WaveFlags &fl) {
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int weff = (s + 63) >> 6;
+  block_lut_build<SMAX, W, NT>(rowptr, colidx, S, s, fl);
   u64 c;
   if (W >= 4 && (depth == 4 || (depth == 3 && W >= 16))) {
     // the segment table (dead after the build) holds the per-wave queues
@@ -914,6 +926,201 @@ __global__ __launch_bounds__(NT) void k_clique_edge(const int64_t *__restrict__ 
   if ((tid & 63) == 0 && acc) atomicAdd(out, acc);
 }
 
+// ------------------------------------------------- vertex-induced 4-motifs
+// GLUMIN P1 (3-star), P2 (diamond), P3 (tailed triangle) on an undirected
+// graph, each induced embedding counted once (GraphMiner's unique-embedding
+// semantics; GLUMIN's P2 walks both directions of every edge and reports
+// twice the diamonds).  Reference kernels: GM_kernels/P1-GM.yml.cuh,
+// P2-GM.yml.cuh, P3-GM.yml.cuh (set difference / intersection over edge
+// tasks) and GM_LUT_kernels/P1.yml.cuh (LUT rows with AND-NOT popcounts,
+// include/glumin/codegen_bitmap.cuh:357-407).  Vertex task v0, S = N(v0),
+// R[i] = {j : S[j] in N(S[i])} (symmetric; S sorted by id):
+//   3-star centred at v0:        sum_{a} sum_{j<a, j not in R[a]} |~R[a] & ~R[j] & [0, j)|
+//   diamond, diagonal (v0, S[a]) with S[a] < v0:
+//                                sum_{j in R[a]} |R[a] & ~R[j] & [0, j)|
+//   tailed triangle, v0 the vertex of degree 3:
+//                                sum_{a} sum_{j<a, j in R[a]} |~R[a] & ~R[j]|
+// (a leaf k of the star is adjacent to neither a nor j; the two vertices
+// off the diagonal are adjacent to S[a] but not to each other; the tail
+// is adjacent to v0 only -- ~R[j] drops a and ~R[a] drops j).
+enum { VM_STAR = 0, VM_DIAMOND = 1, VM_TAILED = 2 };
+
+__device__ __forceinline__ u64 low_bits(int j, int w) {  // bits [0, j) of word w
+  const int jw = j >> 6;
+  return w < jw ? ~0ull : (w == jw ? ((1ull << (j & 63)) - 1ull) : 0ull);
+}
+
+// Waves claim rows a from *rowq; lanes take the partners j.  All lanes of
+// every wave call.
+template <int W, int WHICH>
+__device__ u64 vmotif_lut_count(const u64 *rows, const int *x, int s, int v0, int *rowq, int lane) {
+  const int weff = (s + 63) >> 6;
+  const u64 last = (s & 63) ? ((1ull << (s & 63)) - 1ull) : ~0ull;  // valid bits of word weff - 1
+  u64 acc = 0;
+  for (;;) {
+    int a = 0;
+    if (lane == 0) a = atomicAdd(rowq, 1);
+    a = __shfl(a, 0, 64);
+    if (a >= s) break;
+    if (WHICH == VM_DIAMOND && x[a] >= v0) break;  // x sorted: no later row has S[a] < v0
+    u64 ra[W];
+#pragma unroll
+    for (int w = 0; w < W; ++w) ra[w] = w < weff ? rows[lut_word<W>(a, w)] : 0ull;
+    const int jmax = WHICH == VM_DIAMOND ? s : a;
+    uint32_t c = 0;
+    for (int j = lane; j < jmax; j += 64) {
+      const bool adj = (rows[lut_word<W>(a, j >> 6)] >> (j & 63)) & 1ull;
+      if (adj != (WHICH != VM_STAR)) continue;
+#pragma unroll
+      for (int w = 0; w < W; ++w) {
+        if (w >= weff) continue;
+        const u64 rj = rows[lut_word<W>(j, w)];
+        u64 m;
+        if (WHICH == VM_STAR) m = ~ra[w] & ~rj & low_bits(j, w);
+        else if (WHICH == VM_DIAMOND) m = ra[w] & ~rj & low_bits(j, w);
+        else m = ~ra[w] & ~rj & (w == weff - 1 ? last : ~0ull);
+        c += (uint32_t)__popcll(m);
+      }
+    }
+    acc += c;
+  }
+  return acc;
+}
+
+// Vertex tasks v0 with 3 <= d(v0) <= SMAX: the LUT over N(v0) in LDS.
+template <int SMAX, int W, int NT, int WHICH>
+__global__ __launch_bounds__(NT) void k_vmotif_lut(const int64_t *__restrict__ rowptr,
+                                                   const int32_t *__restrict__ colidx,
+                                                   const int32_t *__restrict__ tasks, int ntasks,
+                                                   int *__restrict__ queue, u64 *__restrict__ out,
+                                                   const int32_t *__restrict__ slots) {
+  __shared__ CStage<SMAX, W, NT> S;
+  const int tid = threadIdx.x;
+  WaveFlags fl;
+  fl.init(S.flagrow[tid >> 6], tid & 63);
+  u64 acc = 0;
+  for (;;) {
+    if (tid == 0) S.item = atomicAdd(queue, 1);
+    __syncthreads();
+    const int t = S.item;
+    if (t >= ntasks) break;
+    const int v0 = tasks[t];
+    const int64_t b = rowptr[v0];
+    const int s = (int)(rowptr[v0 + 1] - b);
+    for (int q = tid; q < s; q += NT) S.x[q] = colidx[b + q];
+    block_lut_build<SMAX, W, NT>(rowptr, colidx, S, s, fl);
+    const u64 part = vmotif_lut_count<W, WHICH>(S.rows, S.x, s, v0, &S.rowq, tid & 63);
+    task_flush(acc, part, slots, v0, out, tid & 63);
+    __syncthreads();
+  }
+  acc = wave_sum_u64(acc);
+  if ((tid & 63) == 0 && acc) atomicAdd(out, acc);
+}
+
+// Wave-cooperative |A ∩ B| (A: HBM scratch or graph list, B: graph list):
+// the merge path when both exceed kHybridMin, else every element of the
+// shorter list binary-searched in the longer (the hybrid rule).
+struct VmWave {
+  int As[kMergeTile];
+  int Bs[kMergeTile + 1];
+  int split[64];
+};
+__device__ __forceinline__ uint32_t wave_isect(const int32_t *A, int la, const int32_t *B, int lb, VmWave &M,
+                                               int lane) {
+  if (la > kHybridMin && lb > kHybridMin)
+    return merge_walk(A, la, B, lb, M.As, M.Bs, M.split, lane, [](int, bool f) { return f; }, nullptr);
+  const int32_t *T = la <= lb ? A : B, *L = la <= lb ? B : A;
+  const int lt = min(la, lb), ll = max(la, lb);
+  uint32_t c = 0;
+  for (int k = lane; k < lt; k += 64) c += sorted_contains(L, ll, T[k]) ? 1u : 0u;
+  return (uint32_t)wave_sum_i32((int)c);
+}
+
+// out[0..) <- {x in A : (x in B) == want_in, x < ub}, in order; returns the
+// size (same on every lane).
+__device__ __forceinline__ int wave_filter(const int32_t *A, int la, const int32_t *B, int lb, bool want_in, int ub,
+                                           int32_t *out, int lane) {
+  int n = 0;
+  for (int k0 = 0; k0 < la; k0 += 64) {
+    const int k = k0 + lane;
+    bool keep = false;
+    int x = 0;
+    if (k < la) {
+      x = A[k];
+      keep = x < ub && sorted_contains(B, lb, x) == want_in;
+    }
+    const u64 m = __ballot(keep);
+    if (keep) out[n + __popcll(m & lanemask_lt(lane))] = x;
+    n += __popcll(m);
+  }
+  return n;
+}
+
+// Any degree, by set difference and intersection over the edge tasks of
+// v0 (GLUMIN's GM kernels; a block per vertex task, its waves take the
+// neighbours v1 in turn).  For v1 in N(v0), with GLUMIN's difference (no
+// owner exclusion) and W0, W1 written to this wave's HBM slots:
+//   3-star:  W0 = N(v0) \ N(v1) below v1; for v2 = W0[t]: t - |W0[0, t) ∩ N(v2)|
+//   diamond (v1 < v0 only): W0 = N(v0) ∩ N(v1); for v2 = W0[t]: t - |W0[0, t) ∩ N(v2)|
+//   tailed:  W0 = N(v0) ∩ N(v1) below v1, W1 = N(v0) \ N(v1) (v1 itself stays in
+//            W1 and drops out as a neighbour of v2); for v2 in W0: |W1| - |W1 ∩ N(v2)|
+// scratch: gridDim.x * 4 waves * 2 slots * md ints.
+template <int WHICH>
+__global__ __launch_bounds__(256) void k_vmotif_list(const int64_t *__restrict__ rowptr,
+                                                     const int32_t *__restrict__ colidx,
+                                                     const int32_t *__restrict__ tasks, int ntasks,
+                                                     int32_t *__restrict__ scratch, int md,
+                                                     int *__restrict__ queue, u64 *__restrict__ out,
+                                                     const int32_t *__restrict__ slots) {
+  __shared__ VmWave M[4];
+  __shared__ int sh_task, sh_next;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  int32_t *W0 = scratch + ((size_t)blockIdx.x * 4 + wave) * 2 * (size_t)md;
+  int32_t *W1 = W0 + md;
+  u64 acc = 0;
+  for (;;) {
+    if (tid == 0) {
+      sh_task = atomicAdd(queue, 1);
+      sh_next = 0;
+    }
+    __syncthreads();
+    const int t = sh_task;
+    if (t >= ntasks) break;
+    const int v0 = tasks[t];
+    const int32_t *N0 = colidx + rowptr[v0];
+    const int d0 = (int)(rowptr[v0 + 1] - rowptr[v0]);
+    u64 part = 0;
+    for (;;) {
+      int i = 0;
+      if (lane == 0) i = atomicAdd(&sh_next, 1);
+      i = __shfl(i, 0, 64);
+      if (i >= d0) break;
+      const int v1 = N0[i];
+      if (WHICH == VM_DIAMOND && v1 >= v0) continue;
+      const int32_t *N1 = colidx + rowptr[v1];
+      const int d1 = (int)(rowptr[v1 + 1] - rowptr[v1]);
+      const int n0 = wave_filter(N0, d0, N1, d1, WHICH != VM_STAR, WHICH == VM_DIAMOND ? INT32_MAX : v1, W0, lane);
+      const int n1 = WHICH == VM_TAILED ? wave_filter(N0, d0, N1, d1, false, INT32_MAX, W1, lane) : 0;
+      __threadfence_block();  // W0 / W1 written by other lanes of this wave
+      wave_sync();
+      uint32_t c = 0;
+      for (int q = 0; q < n0; ++q) {
+        const int v2 = W0[q];
+        const int32_t *N2 = colidx + rowptr[v2];
+        const int d2 = (int)(rowptr[v2 + 1] - rowptr[v2]);
+        if (WHICH == VM_TAILED) c += (uint32_t)n1 - wave_isect(W1, n1, N2, d2, M[wave], lane);
+        else c += (uint32_t)q - wave_isect(W0, q, N2, d2, M[wave], lane);
+      }
+      if (lane == 0) part += c;
+      wave_sync();  // W0 / W1 reads done before the next v1 overwrites them
+    }
+    task_flush(acc, part, slots, v0, out, lane);
+    __syncthreads();
+  }
+  acc = wave_sum_u64(acc);
+  if (lane == 0 && acc) atomicAdd(out, acc);
+}
+
 template <class K>
 int grid_for(K kernel, int nt, size_t shmem, int num_cu, int64_t tasks) {
   int per_cu = 0;
@@ -1084,6 +1291,49 @@ hipError_t launch_clique(const DevCSR &g, const CliqueTasks &ct, int k, int *scr
   } else if ((e = run_vlut<64, 1, 64>(k4 ? "k4_lut_64" : "k5_lut_64", g, ct.lut[0], k, d_queues + 3, ct.log_lut[0], l)))
     return e;
   return hipSuccess;
+}
+
+namespace {
+template <int WHICH>
+hipError_t launch_vmotif_which(const DevCSR &g, const TaskList tl[4], int32_t *scratch, int md, Launch &l,
+                               int *q) {
+  const char *names[3][4] = {{"vm_star_lut_64", "vm_star_lut_256", "vm_star_lut_1024", "vm_star_list"},
+                             {"vm_diamond_lut_64", "vm_diamond_lut_256", "vm_diamond_lut_1024", "vm_diamond_list"},
+                             {"vm_tailed_lut_64", "vm_tailed_lut_256", "vm_tailed_lut_1024", "vm_tailed_list"}};
+  if (tl[3].count) {
+    auto k = k_vmotif_list<WHICH>;
+    const int grid = std::min(grid_for(k, 256, 0, l.num_cu, tl[3].count), vmotif_list_grid(l.num_cu));
+    l.begin(names[WHICH][3], tl[3].count);
+    k<<<grid, 256, 0, l.stream>>>(g.rowptr, g.colidx, tl[3].tasks, (int)tl[3].count, scratch, md, q + 3, l.d_out,
+                                  l.slots);
+    l.end();
+    if (hipError_t e = hipGetLastError()) return e;
+  }
+  auto run = [&](auto kern, int nt, int i) -> hipError_t {
+    if (!tl[i].count) return hipSuccess;
+    const int grid = grid_for(kern, nt, 0, l.num_cu, tl[i].count);
+    l.begin(names[WHICH][i], tl[i].count);
+    kern<<<grid, nt, 0, l.stream>>>(g.rowptr, g.colidx, tl[i].tasks, (int)tl[i].count, q + i, l.d_out, l.slots);
+    l.end();
+    return hipGetLastError();
+  };
+  hipError_t e;
+  if ((e = run(k_vmotif_lut<1024, 16, 1024, WHICH>, 1024, 2))) return e;
+  if ((e = run(k_vmotif_lut<256, 4, 256, WHICH>, 256, 1))) return e;
+  return run(k_vmotif_lut<64, 1, 64, WHICH>, 64, 0);
+}
+}  // namespace
+
+int vmotif_list_grid(int num_cu) { return num_cu; }
+
+hipError_t launch_vmotif(const DevCSR &g, int which, const TaskList tl[4], int32_t *scratch, int md, Launch &l,
+                         int *d_queues) {
+  switch (which) {
+    case 0: return launch_vmotif_which<VM_STAR>(g, tl, scratch, md, l, d_queues);
+    case 1: return launch_vmotif_which<VM_DIAMOND>(g, tl, scratch, md, l, d_queues);
+    case 2: return launch_vmotif_which<VM_TAILED>(g, tl, scratch, md, l, d_queues);
+    default: return hipErrorInvalidValue;
+  }
 }
 
 // Per-vertex scheduler features (XGM_MLP_IN floats per vertex; zeros for

@@ -35,15 +35,13 @@
 
 #include "device_common.h"
 #include "kernels.h"
+#include "merge_path.h"
 
 namespace xgm {
 namespace {
 
 constexpr int kStage = 2048;
 constexpr int kSetopWaves = 4;  // waves per block
-constexpr int kMergeE = 8;                  // merge-path elements per lane per tile
-constexpr int kMergeTile = 64 * kMergeE;    // path elements per tile
-constexpr int kHybridMin = 1024;            // hybrid: merge when both lists are longer
 constexpr int kMaxAnc = 4;
 
 struct WaveStage {
@@ -52,95 +50,6 @@ struct WaveStage {
   int split[64];
 };
 static_assert(2 * kMergeTile + 1 <= kStage, "merge tile fits the stage");
-
-// Number of A elements among the first d elements of the merge of A and B
-// (A before B on equal keys): the first i in [max(0, d - lb), min(d, la)]
-// with A[i] > B[d - 1 - i].
-template <class PA, class PB>
-__device__ __forceinline__ int merge_split(PA A, int la, PB B, int lb, int d) {
-  int lo = max(0, d - lb), hi = min(d, la);
-  while (lo < hi) {
-    const int mid = (lo + hi) >> 1;
-    if (A[mid] <= B[d - 1 - mid]) lo = mid + 1; else hi = mid;
-  }
-  return lo;
-}
-
-// Merge-path walk of A[0..la) and B[0..lb) (HBM): every element x of A is
-// visited once with found = (x in B); the lanes' counts of keep(x, found)
-// are summed over the wave, and with out != null the kept elements are
-// written in increasing order to out[0..).  All 64 lanes call.
-template <class Keep>
-__device__ uint32_t merge_walk(WaveStage &S, const int32_t *__restrict__ A, int la, const int32_t *__restrict__ B,
-                               int lb, int lane, Keep keep, int32_t *__restrict__ out) {
-  const int L = la + lb;
-  uint32_t total = 0;
-  int ia0 = 0, ib0 = 0;  // split at the start of the current tile
-  int *As = S.list, *Bs = S.list + kMergeTile;
-  for (int t0 = 0; t0 < L; t0 += 64 * kMergeTile) {
-    // tile ends of the next 64 tiles, one merge-path search per lane
-    wave_sync();
-    {
-      const int d = (int)min((int64_t)L, (int64_t)t0 + (int64_t)(lane + 1) * kMergeTile);
-      S.split[lane] = merge_split(A, la, B, lb, d);
-    }
-    wave_sync();
-    const int ntiles = min(64, (L - t0 + kMergeTile - 1) / kMergeTile);
-    for (int q = 0; q < ntiles; ++q) {
-      const int d1 = (int)min((int64_t)L, (int64_t)t0 + (int64_t)(q + 1) * kMergeTile);
-      const int ia1 = S.split[q], ib1 = d1 - ia1;
-      const int na = ia1 - ia0, nb = ib1 - ib0;
-      wave_sync();  // the previous tile's reads of As / Bs are done
-      for (int i = lane; i < na; i += 64) As[i] = A[ia0 + i];
-      // B one past the tile: the partner of the tile's last A element may open the next tile
-      for (int i = lane; i <= nb; i += 64) Bs[i] = (ib0 + i < lb) ? B[ib0 + i] : INT32_MAX;
-      wave_sync();
-      const int dl = min(lane * kMergeE, na + nb);
-      const int ia_s = merge_split(As, na, Bs, nb, dl);
-      const int steps = min(kMergeE, na + nb - dl);
-      // pass 1: count this lane's kept elements
-      uint32_t c = 0;
-      {
-        int ia = ia_s, ib = dl - ia_s;
-#pragma unroll
-        for (int st = 0; st < kMergeE; ++st) {
-          if (st < steps) {
-            if (ia < na && (ib >= nb || As[ia] <= Bs[ib])) {
-              const int x = As[ia];
-              c += keep(x, Bs[ib] == x) ? 1u : 0u;
-              ++ia;
-            } else {
-              ++ib;
-            }
-          }
-        }
-      }
-      const uint32_t incl = (uint32_t)wave_incl_scan((int)c, lane);
-      if (out && c) {
-        // pass 2: write them at this lane's place in the wave's order
-        uint32_t pos = total + incl - c;
-        int ia = ia_s, ib = dl - ia_s;
-#pragma unroll
-        for (int st = 0; st < kMergeE; ++st) {
-          if (st < steps) {
-            if (ia < na && (ib >= nb || As[ia] <= Bs[ib])) {
-              const int x = As[ia];
-              if (keep(x, Bs[ib] == x)) out[pos++] = x;
-              ++ia;
-            } else {
-              ++ib;
-            }
-          }
-        }
-      }
-      total += (uint32_t)__shfl((int)incl, 63, 64);
-      ia0 = ia1;
-      ib0 = ib1;
-    }
-  }
-  wave_sync();
-  return total;
-}
 
 // Membership of key in L (length ll, sorted): LDS copy when staged, else
 // the sample index then HBM.
@@ -204,7 +113,9 @@ __global__ __launch_bounds__(64 * kSetopWaves) void k_setop_pairs(
                        (strategy == SETOP_STRAT_HYBRID && la > kHybridMin && lb > kHybridMin);
     if (merge) {
       // A is the list whose elements are kept (either list for an intersection)
-      const uint32_t hits = merge_walk(S, A, la, B, lb, lane, keep, values ? values + offsets[p] : nullptr);
+      wave_sync();  // the previous pair's reads of S are done
+      const uint32_t hits = merge_walk(A, la, B, lb, S.list, S.list + kMergeTile, S.split, lane, keep,
+                                       values ? values + offsets[p] : nullptr);
       if (lane == 0) counts[p] = hits;
       continue;
     }

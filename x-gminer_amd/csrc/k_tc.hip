@@ -20,6 +20,7 @@
 
 #include "device_common.h"
 #include "kernels.h"
+#include "merge_path.h"
 #include "segments.h"
 
 namespace xgm {
@@ -297,6 +298,82 @@ __global__ __launch_bounds__(NT) void k_tc_block(const int64_t *__restrict__ row
   if (lane == 0 && acc) atomicAdd(out, acc);
 }
 
+// Merge strategy: a wave per task vertex u (d+(u) >= 2), N+(u) staged in
+// LDS up to kMergeStage elements; each lane takes one partner v in N+(u) and
+// merges N+(u) with N+(v) element by element (GraphMiner's merge count,
+// codes/GraphMiner/include/VertexSet.h:65-76, one lane per DAG edge); a pair
+// with both lists above kHybridMin is merged by the whole wave along the
+// merge path instead (the reference's hybrid rule, operations.cuh:208-300).
+constexpr int kMergeStage = 1024;
+constexpr int kMergeWaves = 4;
+struct MergeStage {
+  int nu[kMergeStage];
+  int As[kMergeTile];
+  int Bs[kMergeTile + 1];
+  int split[64];
+};
+
+__global__ __launch_bounds__(64 * kMergeWaves) void k_tc_merge(const int64_t *__restrict__ rowptr,
+                                                                const int32_t *__restrict__ colidx,
+                                                                const int32_t *__restrict__ tasks, int ntasks,
+                                                                int *__restrict__ queue, u64 *__restrict__ out,
+                                                                const int32_t *__restrict__ slots) {
+  __shared__ MergeStage stage[kMergeWaves];
+  const int lane = threadIdx.x & 63;
+  MergeStage &S = stage[threadIdx.x >> 6];
+  u64 acc = 0;
+  for (;;) {
+    int t = 0;
+    if (lane == 0) t = atomicAdd(queue, 1);
+    t = __shfl(t, 0, 64);
+    if (t >= ntasks) break;
+    const int u = tasks[t];
+    const int64_t ub = rowptr[u];
+    const int du = (int)(rowptr[u + 1] - ub);
+    const bool lds = du <= kMergeStage;
+    wave_sync();  // the previous task's reads of S.nu are done
+    if (lds)
+      for (int i = lane; i < du; i += 64) S.nu[i] = colidx[ub + i];
+    wave_sync();
+    const int *Nu = lds ? S.nu : colidx + ub;
+    uint32_t cnt = 0;
+    for (int k0 = 0; k0 < du; k0 += 64) {
+      const int k = k0 + lane;
+      int v = 0;
+      bool wave_pair = false;
+      if (k < du) {
+        v = Nu[k];
+        const int64_t vb = rowptr[v];
+        const int dv = (int)(rowptr[v + 1] - vb);
+        wave_pair = du > kHybridMin && dv > kHybridMin;
+        if (!wave_pair) {
+          const int32_t *Nv = colidx + vb;
+          int i = 0, j = 0;
+          while (i < du && j < dv) {
+            const int x = Nu[i], y = Nv[j];
+            cnt += (x == y);
+            i += (x <= y);
+            j += (y <= x);
+          }
+        }
+      }
+      u64 m = __ballot(wave_pair);
+      while (m) {
+        const int src = __builtin_ctzll(m);
+        m &= m - 1;
+        const int vv = __shfl(v, src, 64);
+        const int64_t vb = rowptr[vv];
+        const uint32_t c = merge_walk(Nu, du, colidx + vb, (int)(rowptr[vv + 1] - vb), S.As, S.Bs, S.split, lane,
+                                      [](int, bool found) { return found; }, nullptr);
+        if (lane == 0) cnt += c;
+      }
+    }
+    task_flush(acc, (u64)cnt, slots, u, out, lane);
+  }
+  acc = wave_sum_u64(acc);
+  if (lane == 0 && acc) atomicAdd(out, acc);
+}
+
 __global__ void k_zero_u64(u64 *p, size_t n) {
   size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = 0;
@@ -375,6 +452,16 @@ __global__ __launch_bounds__(256) void k_copy_u4(int4 *__restrict__ dst, const i
 hipError_t launch_copy(void *dst, const void *src, size_t n16, int num_cu, hipStream_t s) {
   const int grid = num_cu * 8;
   k_copy_u4<<<grid, 256, 0, s>>>((int4 *)dst, (const int4 *)src, n16);
+  return hipGetLastError();
+}
+
+hipError_t launch_tc_merge(const DevCSR &g, const TaskList &tasks, Launch &l, int *queue) {
+  if (!tasks.count) return hipSuccess;
+  const int grid = persistent_grid(k_tc_merge, l.num_cu, tasks.count, kMergeWaves, 64 * kMergeWaves);
+  l.begin("tc_merge", tasks.count);
+  k_tc_merge<<<grid, 64 * kMergeWaves, 0, l.stream>>>(g.rowptr, g.colidx, tasks.tasks, (int)tasks.count, queue,
+                                                      l.d_out, l.slots);
+  l.end();
   return hipGetLastError();
 }
 

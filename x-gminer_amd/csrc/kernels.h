@@ -62,6 +62,11 @@ constexpr int kTcWaveLargeDefault = 64;
 hipError_t launch_tc(const DevCSR &g, const TaskList tiers[NUM_TIERS], int strategy, int hash_shift, Launch &l,
                      int *d_queues /* NUM_TIERS counters */);
 
+// Merge triangle strategy: tasks = vertices with d+ >= 2; one lane merges
+// N+(u) with N+(v) per DAG edge u->v, the wave merge path for pairs with
+// both lists above 1024.  Adds into l.d_out.  One queue counter.
+hipError_t launch_tc_merge(const DevCSR &g, const TaskList &tasks, Launch &l, int *queue);
+
 // Bucket-bitmap triangle strategy (k_bucket.hip): bucket-major copy of the
 // DAG's lists (64 buckets by w & 63), per-vertex uint16 bucket offsets and
 // non-empty masks; then one wave per task vertex (d+ >= 2) intersecting
@@ -103,6 +108,16 @@ hipError_t launch_clique_features(const DevCSR &g, int k, float *F, hipStream_t 
 int clique_edge_grid(int num_cu);
 int clique_list_capacity(int num_cu, int list);  // list 12 = rank LUT in HBM
 int clique_hub_slots(int num_cu);
+
+// Vertex-induced 4-motifs (which: 0 3-star, 1 diamond, 2 tailed triangle)
+// over an undirected CSR, each embedding once.  Vertex tasks of degree >= 3:
+// tl[0..2] LUT in LDS (degree <= 64 / 256 / 1024), tl[3] the list path by set
+// difference (any degree); scratch holds vmotif_list_grid() * 8 * md ints
+// (md = max degree) when tl[3] is not empty.  Adds into l.d_out.  Queue
+// counters 0-3.
+hipError_t launch_vmotif(const DevCSR &g, int which, const TaskList tl[4], int32_t *scratch, int md, Launch &l,
+                         int *d_queues);
+int vmotif_list_grid(int num_cu);
 
 // Edge-induced diamond over an undirected CSR; items are int4 (owner t,
 // k0, k1, 0) (TaskList.tasks points at 4*count ints, count = items) in

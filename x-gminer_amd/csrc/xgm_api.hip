@@ -142,6 +142,7 @@ struct xgm_dgraph {
   u64 *d_bmap = nullptr;
   std::map<std::string, int> hub_count;  // hub tasks per cached hub list
   std::map<std::string, int64_t> range_nlow;  // largest lower-neighbour count per cached range list
+  std::map<std::string, int64_t> list_md;     // largest degree of the list tasks per cached motif list
   int4 *d_cursor = nullptr;        // 4-cycle range items: per-block list cursors
   size_t cursor_n = 0;
   // scratch buffers replaced by larger ones: a plan captured earlier still
@@ -1053,6 +1054,11 @@ static int dispatch_pattern(xgm_ctx *ctx, const xgm_dgraph *g, int pattern, cons
         e = launch_tc_bucket(dg, gm->d_bcol, gm->d_boff, gm->d_bmap, tl[0], l, l.d_queue);
         break;
       }
+      if (strategy == XGM_STRAT_MERGE) {
+        if ((rc = tasks_by_ranges(ctx, gm, "tc_merge", 2, {INT32_MAX}, tl))) return rc;
+        e = launch_tc_merge(dg, tl[0], l, l.d_queue);
+        break;
+      }
       if (strategy != XGM_STRAT_AUTO && strategy != XGM_STRAT_HASH && strategy != XGM_STRAT_BSEARCH)
         return ctx->fail(XGM_EUNSUPPORTED, "strategy not available for TRIANGLE");
       int s = (strategy == XGM_STRAT_BSEARCH) ? 2 : 1;
@@ -1092,6 +1098,49 @@ static int dispatch_pattern(xgm_ctx *ctx, const xgm_dgraph *g, int pattern, cons
         it = gm->scheds.emplace(key, std::move(sc)).first;
       }
       if ((rc = run_clique_sched(ctx, gm, it->second.get(), dg, l, nullptr))) return rc;
+      break;
+    }
+    case XGM_STAR3_V:
+    case XGM_DIAMOND_V:
+    case XGM_TAILEDTRI_V: {
+      if (!(g->flags & XGM_CSR_SYMMETRIC)) return ctx->fail(XGM_EINVAL, "vertex-induced motifs need an undirected CSR");
+      if (strategy != XGM_STRAT_AUTO && strategy != XGM_STRAT_LUT && strategy != XGM_STRAT_LIST)
+        return ctx->fail(XGM_EUNSUPPORTED, "strategy not available for vertex-induced motifs");
+      // vertex tasks of degree >= 3 by decreasing degree: LUT tiers by
+      // degree (<= 64, <= 256, <= 1024) and the set-difference list path
+      // above 1024 (every task under XGM_STRAT_LIST)
+      const bool list_all = strategy == XGM_STRAT_LIST;
+      const std::string key = list_all ? "vmotif_list" : "vmotif";
+      std::vector<int32_t> ord;
+      int md = 0;
+      if (!cached(gm, key)) {
+        if ((rc = ensure_host_mirror(ctx, gm))) return rc;
+        for (int64_t v = 0; v < g->n; ++v)
+          if (gm->h_rowptr[v + 1] - gm->h_rowptr[v] >= 3) ord.push_back((int32_t)v);
+        std::stable_sort(ord.begin(), ord.end(), [&](int32_t a, int32_t b) {
+          return gm->h_rowptr[a + 1] - gm->h_rowptr[a] > gm->h_rowptr[b + 1] - gm->h_rowptr[b];
+        });
+      }
+      const int64_t *hrp = gm->h_rowptr.data();
+      auto deg_of = [hrp](int32_t v) { return hrp[v + 1] - hrp[v]; };
+      const std::vector<int64_t> bounds = list_all ? std::vector<int64_t>{-1, -1, -1, INT64_MAX}
+                                                   : std::vector<int64_t>{64, 256, 1024, INT64_MAX};
+      if ((rc = tasks_split(ctx, gm, key, ord, deg_of, bounds, tl))) return rc;
+      TaskList t4[4] = {tl[0], tl[1], tl[2], tl[3]};
+      if (!ord.empty()) {
+        // largest degree of a list task bounds its difference / intersection slots
+        int64_t m = 0;
+        for (int32_t v : ord)
+          if (deg_of(v) > (list_all ? 2 : 1024)) m = std::max<int64_t>(m, deg_of(v));
+        gm->list_md[key] = m;
+      }
+      if (t4[3].count) {
+        md = (int)gm->list_md[key];
+        const size_t need = (size_t)vmotif_list_grid(ctx->num_cu) * 8 * (size_t)std::max(1, md);
+        if ((rc = ensure_scratch(ctx, gm, need))) return rc;
+      }
+      const int which = pattern == XGM_STAR3_V ? 0 : pattern == XGM_DIAMOND_V ? 1 : 2;
+      e = launch_vmotif(dg, which, t4, gm->d_scratch, std::max(1, md), l, l.d_queue);
       break;
     }
     case XGM_DIAMOND_EDGE: {
@@ -1841,6 +1890,7 @@ int xgm_autotune(xgm_ctx *ctx, const xgm_dgraph *g, int pattern, int reps, xgm_o
                        std::make_pair(32, 128), std::make_pair(16, 64)})
           cand.push_back(mk(XGM_STRAT_HASH, 0, hs, t.first, t.second));
       cand.push_back(mk(XGM_STRAT_BSEARCH, 0, 0, 0, 0));
+      cand.push_back(mk(XGM_STRAT_MERGE, 0, 0, 0, 0));
       if (g->max_out < 65536) cand.push_back(mk(XGM_STRAT_BUCKET, 0, 0, 0, 0));
       break;
     case XGM_CLIQUE4:
