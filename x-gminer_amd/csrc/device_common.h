@@ -110,6 +110,48 @@ __device__ __forceinline__ bool hash_contains(const int *tab, int bits, int key)
   }
 }
 
+// Bucketed variant: 2^bbits buckets of 4 slots (int4, 16-byte aligned),
+// empty slot = -1.  A key goes into the first empty slot of its home bucket
+// or, when that bucket is full, of the next buckets in turn; slots of a
+// bucket fill in order, so a bucket whose last slot is empty ends a lookup.
+// A lookup reads one 16-byte bucket (one LDS access) unless the bucket is
+// full: at load <= 1/4 that is rare, so the probe loop is almost never
+// taken and a wave's lanes stay converged.
+__device__ __forceinline__ void bhash_insert(int *tab, int bbits, int key) {
+  const uint32_t mask = (1u << bbits) - 1u;
+  uint32_t h = hash_slot(key, bbits);
+  for (;;) {
+    int *b = tab + 4 * h;
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+      if (atomicCAS(&b[q], -1, key) == -1) return;
+    h = (h + 1) & mask;
+  }
+}
+
+// key in bucket q: the minimum of the four XORs is zero (straight-line
+// integer ops, no per-slot branches)
+__device__ __forceinline__ bool bucket_has(const int4 &q, int key) {
+  const uint32_t k = (uint32_t)key;
+  const uint32_t d = min(min((uint32_t)q.x ^ k, (uint32_t)q.y ^ k), min((uint32_t)q.z ^ k, (uint32_t)q.w ^ k));
+  return d == 0u;
+}
+
+__device__ __forceinline__ bool bhash_contains(const int *tab, int bbits, int key) {
+  const uint32_t mask = (1u << bbits) - 1u;
+  uint32_t h = hash_slot(key, bbits);
+  int4 q = reinterpret_cast<const int4 *>(tab)[h];
+  bool hit = bucket_has(q, key);
+  bool more = !hit && q.w >= 0;
+  while (__builtin_expect(more, 0)) {
+    h = (h + 1) & mask;
+    q = reinterpret_cast<const int4 *>(tab)[h];
+    hit = bucket_has(q, key);
+    more = !hit && q.w >= 0;
+  }
+  return hit;
+}
+
 // Table bits for n keys: smallest b >= 6 with 2^b >= n << shift (load
 // factor <= 2^-shift; misses probe ~1.4 slots at 1/4, ~2.5 at 1/2).
 __device__ __forceinline__ int hash_bits_for(int n, int shift = 1) {

@@ -39,14 +39,15 @@ constexpr int kBlockPieces = 480;
 template <int HCAP, int PCAP, int STRAT, int HSHIFT>
 struct Stage {
   static constexpr int KEYS = (STRAT == STRAT_HASH) ? (HCAP << HSHIFT) : HCAP;
-  int key[KEYS];
+  alignas(16) int key[KEYS];  // hash: KEYS / 4 buckets of 4 slots
   Segs<PCAP, false> seg;
 };
 
+// hbits: log2 of the hash table's buckets (4 slots each)
 template <int STRAT>
 __device__ __forceinline__ int member(const int *key, int hbits, const int32_t *nu_global, int du, bool lds, int w) {
   if (!lds) return sorted_contains(nu_global, du, w);
-  if (STRAT == STRAT_HASH) return hash_contains(key, hbits, w);
+  if (STRAT == STRAT_HASH) return bhash_contains(key, hbits, w);
   return sorted_contains(key, du, w);
 }
 
@@ -94,10 +95,10 @@ __global__ __launch_bounds__(256) void k_tc_wave(const int64_t *__restrict__ row
     if (STRAT == STRAT_HASH) {
       // the whole table (CAP << HSHIFT slots, allocated anyway): load
       // du / slots, so missing probes run shorter than at the 2^-HSHIFT cap
-      hbits = 31 - __clz(CAP << HSHIFT);
-      for (int k = lane; k < (1 << hbits); k += 64) s.key[k] = -1;
+      hbits = 31 - __clz(CAP << HSHIFT) - 2;  // buckets of 4 slots
+      for (int k = lane; k < (4 << hbits); k += 64) s.key[k] = -1;
       wave_sync();
-      for (int k = lane; k < du; k += 64) hash_insert(s.key, hbits, nu[k]);
+      for (int k = lane; k < du; k += 64) bhash_insert(s.key, hbits, nu[k]);
     } else {
       for (int k = lane; k < du; k += 64) s.key[k] = nu[k];
     }
@@ -271,10 +272,10 @@ __global__ __launch_bounds__(NT) void k_tc_block(const int64_t *__restrict__ row
     int hbits = 0;
     if (lds_member) {
       if (STRAT == STRAT_HASH) {
-        hbits = 31 - __clz(CAP << HSHIFT);  // the whole table, as in k_tc_wave
-        for (int k = threadIdx.x; k < (1 << hbits); k += NT) s.key[k] = -1;
+        hbits = 31 - __clz(CAP << HSHIFT) - 2;  // the whole table in buckets of 4, as in k_tc_wave
+        for (int k = threadIdx.x; k < (4 << hbits); k += NT) s.key[k] = -1;
         __syncthreads();
-        for (int k = threadIdx.x; k < du; k += NT) hash_insert(s.key, hbits, nu[k]);
+        for (int k = threadIdx.x; k < du; k += NT) bhash_insert(s.key, hbits, nu[k]);
       } else {
         for (int k = threadIdx.x; k < du; k += NT) s.key[k] = nu[k];
       }

@@ -222,6 +222,10 @@ struct WindowLane {
 template <int U, class Seg, class F>
 __device__ __forceinline__ void wave_walk(const Seg &t, const int32_t *__restrict__ colidx, int ns,
                                           int lo, int hi, int lane, WaveFlags &fl, F &&f) {
+  // the range and table size are the same on every lane (scalar registers)
+  lo = __builtin_amdgcn_readfirstlane(lo);
+  hi = __builtin_amdgcn_readfirstlane(hi);
+  ns = __builtin_amdgcn_readfirstlane(ns);
   if (lo >= hi) return;
   // first segment whose end exceeds lo (ends are starts of the next one)
   int jl;
@@ -231,7 +235,9 @@ __device__ __forceinline__ void wave_walk(const Seg &t, const int32_t *__restric
       const int mid = (a + b) >> 1;
       if (t.os[mid].y <= lo) a = mid; else b = mid;
     }
-    jl = a;
+    // wave-uniform: held in scalar registers, so the per-window branch below
+    // is a scalar branch instead of two exec-masked paths
+    jl = __builtin_amdgcn_readfirstlane(a);
   }
   for (int base = lo; base < hi; base += 64 * U) {
     WindowLane r[U];
@@ -245,12 +251,14 @@ __device__ __forceinline__ void wave_walk(const Seg &t, const int32_t *__restric
       pos[u] = 0;
       // jl is wave-uniform: a window inside one segment (the common case
       // for long lists) needs no lane -> segment mapping
+      // both reads issued together: one LDS round trip on the fast path
       const int2 cur = t.os[jl];
-      const int e0 = t.os[jl + 1].y;
+      const int e0 = __builtin_amdgcn_readfirstlane(t.os[jl + 1].y);
       if (e0 >= b + 64) {
+        const int cx = __builtin_amdgcn_readfirstlane(cur.x), cy = __builtin_amdgcn_readfirstlane(cur.y);
         r[u].j = jl;
         r[u].heads = 1ull;
-        if (r[u].valid) pos[u] = (int64_t)(uint32_t)cur.x + (i - cur.y);
+        if (r[u].valid) pos[u] = (int64_t)(uint32_t)cx + (i - cy);
         if (e0 == b + 64) ++jl;
         continue;
       }
