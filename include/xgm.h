@@ -126,8 +126,9 @@ void xgm_graph_free(xgm_dgraph *g);
 
 /* Degree-orient an uploaded undirected graph (sorted, symmetric, not a
  * batch) in HBM: the DAG, its reverse (in-neighbour) lists and its vertex
- * task order are built by kernels, with no host pass over the edges; the
- * host mirrors used for task preparation are copied back.  *out equals
+ * task order are built by kernels, with no host pass over the edges; only
+ * d+ and the task order (two ints per vertex) are copied back -- the CSR
+ * stays in HBM and the triangle items are built there too.  *out equals
  * xgm_graph_upload() of xgm_orient_dag() of the same CSR.  Replaces
  * Graph::orientation (codes/GraphMiner/src/common/graph.cc:233-279) and
  * Graph_V2::orientation (xgminer/src/graph_v2.cpp:243-289). */
@@ -140,6 +141,15 @@ int xgm_graph_download(xgm_ctx *ctx, const xgm_dgraph *g, xgm_hgraph **out);
  * decreasing sum over N+(u) of d+, ties by increasing id.  Writes
  * min(cap, len) ids to out; *len receives the full length. */
 int xgm_graph_task_order(const xgm_dgraph *g, int32_t *out, int64_t cap, int64_t *len);
+
+/* Triangle task items of a DAG device graph under opts (strategy AUTO /
+ * HASH / BSEARCH and its tier split): int4 (t, k0, k1, 0) per item, tiers
+ * concatenated (4 of them, items per tier in tier_counts, which may be
+ * NULL), each tier by decreasing work.  Built on the host for a graph
+ * uploaded from a host CSR and in HBM for one built by xgm_graph_orient;
+ * both give the same items.  Writes min(cap, len) ints; *len = 4 x items. */
+int xgm_graph_tc_items(xgm_ctx *ctx, const xgm_dgraph *g, const xgm_opts *opts, int32_t *out, int64_t cap,
+                       int64_t *len, int64_t *tier_counts);
 
 /* Count `pattern` on a device graph; *out receives the uint64 count.
  * opts may be NULL (auto).  Synchronous. */

@@ -93,3 +93,28 @@ def test_upload_rejects_asymmetric(xgm, ctx):
     ok = xgm.Graph.from_arrays(np.array([0, 2, 3, 4]), np.array([1, 2, 0, 0]), flags)
     assert ok.validate()
     assert ctx.count(ctx.upload(ok), xgm.DIAMOND_EDGE) == 0
+
+
+def test_device_tc_items_equal_host(xgm, ctx):
+    """Triangle task items built in HBM (a DAG from xgm_graph_orient, whose
+    CSR never comes back to the host) equal the host builder's items for the
+    same DAG uploaded from the host, item for item and tier for tier, under
+    every tier split; hubs above 2,048 out-neighbours and items cut at the
+    64K-probe slice bound included."""
+    graphs = dict(_graphs(xgm))
+    graphs["k2200"] = complete_graph(xgm, 2200)        # tier > 2048, many slices per hub
+    graphs["rmat16"] = xgm.gen_rmat(16, 16, 0.57, 0.19, 0.19, 1)
+    splits = [{}, dict(wave_max_small=32, wave_max_large=128), dict(wave_max_small=16, wave_max_large=64),
+              dict(wave_max_small=64, wave_max_large=256)]
+    for name, g in graphs.items():
+        _say("tc items", name)
+        host = ctx.upload(g.orient())
+        dev = ctx.orient(ctx.upload(g))
+        for opts in splits:
+            hi, ht = host.tc_items(**opts)
+            di, dt = dev.tc_items(**opts)
+            assert ht == dt, (name, opts)
+            assert np.array_equal(hi, di), (name, opts)
+            assert ctx.count(dev, xgm.TRIANGLE, **opts) == ctx.count(host, xgm.TRIANGLE, **opts)
+        dev.close()
+        host.close()

@@ -115,6 +115,7 @@ def _load():
         "xgm_graph_orient": (C.c_int, [VP, VP, P(VP)]),
         "xgm_graph_download": (C.c_int, [VP, VP, P(VP)]),
         "xgm_graph_task_order": (C.c_int, [VP, P(I32), I64, P(I64)]),
+        "xgm_graph_tc_items": (C.c_int, [VP, VP, P(_Opts), P(I32), I64, P(I64), P(I64)]),
         "xgm_count": (C.c_int, [VP, VP, C.c_int, P(_Opts), P(U64)]),
         "xgm_count_csr": (C.c_int, [VP, P(_Csr), C.c_int, P(_Opts), P(U64)]),
         "xgm_last_profile": (C.c_int, [VP, P(_Profile)]),
@@ -561,6 +562,17 @@ class DeviceGraph:
         out = np.zeros(n.value, np.int32)
         _check(lib.xgm_graph_task_order(self._h, out.ctypes.data_as(C.POINTER(C.c_int32)), n.value, C.byref(n)))
         return out
+
+    def tc_items(self, **opts):
+        """Triangle task items (n x 4: t, k0, k1, 0; tiers concatenated) and
+        the items per tier."""
+        o = _opts(**opts)
+        n, tc = C.c_int64(), (C.c_int64 * 4)()
+        _check(lib.xgm_graph_tc_items(self.ctx._h, self._h, C.byref(o), None, 0, C.byref(n), tc), self.ctx._h)
+        out = np.zeros(max(1, n.value), np.int32)
+        _check(lib.xgm_graph_tc_items(self.ctx._h, self._h, C.byref(o), out.ctypes.data_as(C.POINTER(C.c_int32)),
+                                      n.value, C.byref(n), tc), self.ctx._h)
+        return out[:n.value].reshape(-1, 4), list(tc)
 
     def close(self):
         if getattr(self, "_h", None) is not None:

@@ -137,6 +137,17 @@ __device__ __forceinline__ bool bucket_has(const int4 &q, int key) {
   return d == 0u;
 }
 
+// Rest of a lookup whose home bucket h was full and lacked the key.
+__device__ __noinline__ bool bhash_contains_after(const int *tab, int bbits, uint32_t h, int key) {
+  const uint32_t mask = (1u << bbits) - 1u;
+  for (;;) {
+    h = (h + 1) & mask;
+    const int4 q = reinterpret_cast<const int4 *>(tab)[h];
+    if (bucket_has(q, key)) return true;
+    if (q.w < 0) return false;
+  }
+}
+
 __device__ __forceinline__ bool bhash_contains(const int *tab, int bbits, int key) {
   const uint32_t mask = (1u << bbits) - 1u;
   uint32_t h = hash_slot(key, bbits);

@@ -17,6 +17,9 @@
 #include <hip/hip_runtime.h>
 #include <hipcub/hipcub.hpp>
 
+#include <algorithm>
+#include <vector>
+
 #include "device_common.h"
 #include "kernels.h"
 
@@ -162,6 +165,143 @@ __global__ __launch_bounds__(kPrepThreads) void k_prep_mirror(const int64_t *__r
   }
 }
 
+// ---- triangle task items (xgm_api.hip build_tc_items, on the device) ----
+// Every DAG edge is charged to the endpoint t with the longer out-list (ties
+// to the head): position k of N+(t) ++ N-(t) holds partner x, charged to t
+// iff d+(x) < d+(t) (out-partner) or d+(x) <= d+(t) (in-partner), with work
+// d+(x).  An item (t, k0, k1, 0) ends where its work reaches kTcSlice or
+// where it has cap positions (64 / 256 in the wave tiers); items of zero
+// work are dropped.  One wave per vertex resolves that greedy cut 64
+// positions at a time: an inclusive scan of the works, then one ballot per
+// cut.  Pass 1 counts the items of every vertex, pass 2 writes them at the
+// offsets of the tier-major, vertex-ordered layout; a stable descending sort
+// by work inside each tier then gives exactly the host builder's lists.
+constexpr int64_t kTcSlice = 1 << 16;
+
+__device__ __forceinline__ u64 wave_incl_scan_u64(u64 v, int lane) {
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const u64 t = __shfl_up(v, o, 64);
+    if (lane >= o) v += t;
+  }
+  return v;
+}
+
+__device__ __forceinline__ int tc_tier(int d, int max_small, int max_large) {
+  return d <= max_small ? 0 : d <= max_large ? 1 : d <= 2048 ? 2 : 3;
+}
+
+template <bool WRITE>
+__global__ __launch_bounds__(kPrepThreads) void k_tc_items(const int64_t *__restrict__ rowptr,
+                                                           const int32_t *__restrict__ colidx,
+                                                           const int64_t *__restrict__ rrowptr,
+                                                           const int32_t *__restrict__ rcolidx, int64_t n,
+                                                           int max_small, int max_large, int32_t *__restrict__ cnt,
+                                                           const int32_t *__restrict__ off, int4 *__restrict__ items,
+                                                           uint32_t *__restrict__ work) {
+  const int lane = threadIdx.x & 63;
+  const int64_t nw = (int64_t)gridDim.x * (kPrepThreads / 64);
+  for (int64_t t = (int64_t)blockIdx.x * (kPrepThreads / 64) + (threadIdx.x >> 6); t < n; t += nw) {
+    const int64_t ob = rowptr[t];
+    const int d = (int)(rowptr[t + 1] - ob);
+    int count = 0;
+    if (d >= 1) {
+      const int tier = tc_tier(d, max_small, max_large);
+      const int cap = tier == 0 ? 64 : tier == 1 ? 256 : INT32_MAX;
+      const int64_t ib = rrowptr[t];
+      const int P = d + (int)(rrowptr[t + 1] - ib);
+      int64_t o = WRITE ? off[t] : 0;
+      u64 A = 0;   // work of the open item before this chunk
+      int k0 = 0;  // its first position
+      for (int c0 = 0; c0 < P; c0 += 64) {
+        const int k = c0 + lane;
+        u64 w = 0;
+        if (k < P) {
+          const int x = k < d ? colidx[ob + k] : rcolidx[ib + (k - d)];
+          const int dx = (int)(rowptr[x + 1] - rowptr[x]);
+          if (k < d ? dx < d : dx <= d) w = (u64)dx;
+        }
+        const u64 S = wave_incl_scan_u64(w, lane);
+        const int nvalid = min(64, P - c0);
+        int ls = 0;
+        u64 Sbase = 0;  // inclusive prefix at lane ls - 1
+        for (;;) {
+          const bool cond = lane >= ls && lane < nvalid &&
+                            (A + (S - Sbase) >= (u64)kTcSlice || c0 + lane + 1 - k0 >= cap);
+          const u64 m = __ballot(cond);
+          if (!m) {
+            A += __shfl(S, nvalid - 1, 64) - Sbase;
+            break;
+          }
+          const int l = __builtin_ctzll(m);
+          const u64 Sl = __shfl(S, l, 64);
+          const u64 wk = A + (Sl - Sbase);
+          if (wk) {
+            if (WRITE && lane == 0) {
+              items[o] = make_int4((int)t, k0, c0 + l + 1, 0);
+              work[o] = (uint32_t)wk;
+            }
+            ++o;
+            ++count;
+          }
+          k0 = c0 + l + 1;
+          A = 0;
+          Sbase = Sl;
+          ls = l + 1;
+        }
+      }
+      if (A) {
+        if (WRITE && lane == 0) {
+          items[o] = make_int4((int)t, k0, P, 0);
+          work[o] = (uint32_t)A;
+        }
+        ++count;
+      }
+    }
+    if (!WRITE && lane == 0) cnt[t] = count;
+  }
+}
+
+// masked[t] = cnt[t] if t is in tier `tier`, else 0
+__global__ __launch_bounds__(kPrepThreads) void k_tc_tier_mask(const int64_t *__restrict__ rowptr,
+                                                               const int32_t *__restrict__ cnt, int64_t n,
+                                                               int max_small, int max_large, int tier,
+                                                               int32_t *__restrict__ masked) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t <= n; t += stride) {
+    if (t == n) {  // the scan's extra element (its output is the tier total)
+      masked[t] = 0;
+      continue;
+    }
+    const int d = (int)(rowptr[t + 1] - rowptr[t]);
+    masked[t] = (d >= 1 && tc_tier(d, max_small, max_large) == tier) ? cnt[t] : 0;
+  }
+}
+
+// off[t] = base + scan[t] for the vertices of tier `tier`
+__global__ __launch_bounds__(kPrepThreads) void k_tc_tier_off(const int64_t *__restrict__ rowptr,
+                                                              const int32_t *__restrict__ scan, int64_t n,
+                                                              int max_small, int max_large, int tier, int32_t base,
+                                                              int32_t *__restrict__ off) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n; t += stride) {
+    const int d = (int)(rowptr[t + 1] - rowptr[t]);
+    if (d >= 1 && tc_tier(d, max_small, max_large) == tier) off[t] = base + scan[t];
+  }
+}
+
+__global__ __launch_bounds__(kPrepThreads) void k_iota(uint32_t *__restrict__ v, int64_t n) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += stride) v[i] = (uint32_t)i;
+}
+
+__global__ __launch_bounds__(kPrepThreads) void k_gather_int4(const int4 *__restrict__ src,
+                                                              const uint32_t *__restrict__ idx, int64_t n,
+                                                              int4 *__restrict__ dst) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += stride) dst[i] = src[idx[i]];
+}
+
 int prep_grid(int64_t items, int per_block, int num_cu) {
   const int64_t want = (items + per_block - 1) / per_block;
   return (int)std::max<int64_t>(1, std::min<int64_t>(want, (int64_t)num_cu * 16));
@@ -208,6 +348,104 @@ hipError_t launch_prep_mirror(const int64_t *rowptr, const int32_t *colidx, int6
   hipLaunchKernelGGL(k_prep_mirror, dim3(prep_grid(nnz, kPrepThreads, num_cu)), dim3(kPrepThreads), 0, s, rowptr,
                      colidx, n, nnz, bad);
   return hipGetLastError();
+}
+
+// Device buffers released at scope exit.
+struct PrepBufs {
+  std::vector<void *> p;
+  ~PrepBufs() {
+    for (void *x : p) (void)hipFree(x);
+  }
+  template <class T>
+  hipError_t get(T **out, size_t count) {
+    void *x = nullptr;
+    hipError_t e = hipMalloc(&x, std::max<size_t>(count, 1) * sizeof(T));
+    if (e == hipSuccess) {
+      p.push_back(x);
+      *out = static_cast<T *>(x);
+    }
+    return e;
+  }
+};
+
+hipError_t build_tc_items_device(const DevCSR &g, int max_small, int max_large, int num_cu, hipStream_t s,
+                                 int32_t **out, int64_t counts[NUM_TIERS]) {
+  *out = nullptr;
+  const int64_t n = g.n;
+  PrepBufs B;
+  int32_t *cnt = nullptr, *masked = nullptr, *scan = nullptr, *off = nullptr;
+  hipError_t e;
+  if ((e = B.get(&cnt, n)) || (e = B.get(&masked, n + 1)) || (e = B.get(&scan, n + 1)) || (e = B.get(&off, n)))
+    return e;
+  const int wgrid = prep_grid(n, kPrepThreads / 64, num_cu), tgrid = prep_grid(n, kPrepThreads, num_cu);
+  if (n)
+    hipLaunchKernelGGL(k_tc_items<false>, dim3(wgrid), dim3(kPrepThreads), 0, s, g.rowptr, g.colidx, g.rrowptr,
+                       g.rcolidx, n, max_small, max_large, cnt, (const int32_t *)nullptr, (int4 *)nullptr,
+                       (uint32_t *)nullptr);
+  if ((e = hipGetLastError())) return e;
+  // tier-major offsets: one masked exclusive scan per tier
+  size_t scan_bytes = 0;
+  if ((e = hipcub::DeviceScan::ExclusiveSum(nullptr, scan_bytes, masked, scan, (int)(n + 1), s))) return e;
+  void *scan_tmp = nullptr;
+  if ((e = B.get((char **)&scan_tmp, scan_bytes))) return e;
+  int64_t total = 0;
+  for (int tier = 0; tier < NUM_TIERS; ++tier) {
+    hipLaunchKernelGGL(k_tc_tier_mask, dim3(prep_grid(n + 1, kPrepThreads, num_cu)), dim3(kPrepThreads), 0, s,
+                       g.rowptr, (const int32_t *)cnt, n, max_small, max_large, tier, masked);
+    size_t b = scan_bytes;
+    if ((e = hipcub::DeviceScan::ExclusiveSum(scan_tmp, b, masked, scan, (int)(n + 1), s))) return e;
+    int32_t tier_items = 0;
+    if ((e = hipMemcpyAsync(&tier_items, scan + n, sizeof(int32_t), hipMemcpyDeviceToHost, s))) return e;
+    if ((e = hipStreamSynchronize(s))) return e;
+    if (n)
+      hipLaunchKernelGGL(k_tc_tier_off, dim3(tgrid), dim3(kPrepThreads), 0, s, g.rowptr, (const int32_t *)scan, n,
+                         max_small, max_large, tier, (int32_t)total, off);
+    counts[tier] = tier_items;
+    total += tier_items;
+  }
+  if ((e = hipGetLastError())) return e;
+  int4 *items = nullptr, *sorted = nullptr;
+  uint32_t *work = nullptr, *work_out = nullptr, *idx = nullptr, *idx_out = nullptr;
+  if ((e = B.get(&items, total)) || (e = B.get(&work, total)) || (e = B.get(&work_out, total)) ||
+      (e = B.get(&idx, total)) || (e = B.get(&idx_out, total)))
+    return e;
+  if (n)
+    hipLaunchKernelGGL(k_tc_items<true>, dim3(wgrid), dim3(kPrepThreads), 0, s, g.rowptr, g.colidx, g.rrowptr,
+                       g.rcolidx, n, max_small, max_large, cnt, (const int32_t *)off, items, work);
+  if (total)
+    hipLaunchKernelGGL(k_iota, dim3(prep_grid(total, kPrepThreads, num_cu)), dim3(kPrepThreads), 0, s, idx, total);
+  if ((e = hipGetLastError())) return e;
+  // stable descending sort by work inside each tier (the host's stable_sort)
+  size_t sort_bytes = 0;
+  if ((e = hipcub::DeviceRadixSort::SortPairsDescending(nullptr, sort_bytes, work, work_out, idx, idx_out,
+                                                          (int)std::max<int64_t>(1, total), 0, 32, s)))
+    return e;
+  void *sort_tmp = nullptr;
+  if ((e = B.get((char **)&sort_tmp, sort_bytes))) return e;
+  int64_t base = 0;
+  for (int tier = 0; tier < NUM_TIERS; ++tier) {
+    if (counts[tier] > 1) {
+      size_t b = sort_bytes;
+      if ((e = hipcub::DeviceRadixSort::SortPairsDescending(sort_tmp, b, work + base, work_out + base, idx + base,
+                                                              idx_out + base, (int)counts[tier], 0, 32, s)))
+        return e;
+    } else if (counts[tier] == 1) {
+      if ((e = hipMemcpyAsync(idx_out + base, idx + base, sizeof(uint32_t), hipMemcpyDeviceToDevice, s))) return e;
+    }
+    base += counts[tier];
+  }
+  void *res = nullptr;
+  if ((e = hipMalloc(&res, std::max<int64_t>(1, total) * sizeof(int4)))) return e;
+  sorted = static_cast<int4 *>(res);
+  if (total)
+    hipLaunchKernelGGL(k_gather_int4, dim3(prep_grid(total, kPrepThreads, num_cu)), dim3(kPrepThreads), 0, s,
+                       (const int4 *)items, (const uint32_t *)idx_out, total, sorted);
+  if ((e = hipGetLastError()) || (e = hipStreamSynchronize(s))) {
+    (void)hipFree(res);
+    return e;
+  }
+  *out = reinterpret_cast<int32_t *>(sorted);
+  return hipSuccess;
 }
 
 // Temporary storage (bytes) for prep_scan / prep_sort; called with

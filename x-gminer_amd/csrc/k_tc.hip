@@ -43,6 +43,28 @@ struct Stage {
   Segs<PCAP, false> seg;
 };
 
+// Hash membership of a group of U windows: the U home buckets are read
+// together (one LDS round trip for the group), the rare full-bucket case
+// continues in bhash_contains_after.  Adds the hits of valid lanes.
+template <int U>
+__device__ __forceinline__ int group_hash_hits(const int *key, int hbits, const WindowLane (&r)[U]) {
+  uint32_t h[U];
+  int4 q[U];
+#pragma unroll
+  for (int u = 0; u < U; ++u) {
+    h[u] = hash_slot(r[u].w, hbits);
+    q[u] = reinterpret_cast<const int4 *>(key)[h[u]];
+  }
+  int c = 0;
+#pragma unroll
+  for (int u = 0; u < U; ++u) {
+    bool hit = bucket_has(q[u], r[u].w);
+    if (__builtin_expect(r[u].valid && !hit && q[u].w >= 0, 0)) hit = bhash_contains_after(key, hbits, h[u], r[u].w);
+    c += (r[u].valid && hit) ? 1 : 0;
+  }
+  return c;
+}
+
 // hbits: log2 of the hash table's buckets (4 slots each)
 template <int STRAT>
 __device__ __forceinline__ int member(const int *key, int hbits, const int32_t *nu_global, int du, bool lds, int w) {
@@ -106,8 +128,14 @@ __global__ __launch_bounds__(256) void k_tc_wave(const int64_t *__restrict__ row
     wave_segments_fn(s.seg, rowptr, it.y, it.z, TcVerts{nu, rcolidx + rrowptr[u], du}, TcKeep{du}, lane, ns, L);
     wave_sync();
     int cnt = 0;
-    wave_walk<UNROLL>(s.seg, colidx, ns, 0, L, lane, fl, [&](const WindowLane &r) {
-      if (r.valid) cnt += member<STRAT>(s.key, hbits, nu, du, true, r.w);
+    wave_walk_group<UNROLL>(s.seg, colidx, ns, 0, L, lane, fl, [&](const WindowLane (&r)[UNROLL]) {
+      if (STRAT == STRAT_HASH) {
+        cnt += group_hash_hits<UNROLL>(s.key, hbits, r);
+      } else {
+#pragma unroll
+        for (int q = 0; q < UNROLL; ++q)
+          if (r[q].valid) cnt += member<STRAT>(s.key, hbits, nu, du, true, r[q].w);
+      }
     });
     task_flush(acc, (u64)cnt, slots, u, out, lane);
     wave_sync();  // stage is rewritten by the next item
@@ -288,8 +316,14 @@ __global__ __launch_bounds__(NT) void k_tc_block(const int64_t *__restrict__ row
       block_segments_fn<NT>(s.seg, tmp, rowptr, k0, k1, vert, TcKeep{du}, &sh_ns, &sh_L);
       int lo, hi;
       wave_share(sh_L, wave, NW, lo, hi);
-      wave_walk<UNROLL>(s.seg, colidx, sh_ns, lo, hi, lane, fl, [&](const WindowLane &r) {
-        if (r.valid) cnt += member<STRAT>(s.key, hbits, nu, du, lds_member, r.w);
+      wave_walk_group<UNROLL>(s.seg, colidx, sh_ns, lo, hi, lane, fl, [&](const WindowLane (&r)[UNROLL]) {
+        if (STRAT == STRAT_HASH && lds_member) {
+          cnt += group_hash_hits<UNROLL>(s.key, hbits, r);
+        } else {
+#pragma unroll
+          for (int q = 0; q < UNROLL; ++q)
+            if (r[q].valid) cnt += member<STRAT>(s.key, hbits, nu, du, lds_member, r[q].w);
+        }
       });
     }
     task_flush(acc, (u64)cnt, slots, u, out, lane);

@@ -208,6 +208,12 @@ hipError_t prep_scan(void *temp, size_t &bytes, const int32_t *in, int32_t *out,
 hipError_t prep_sort(void *temp, size_t &bytes, const uint32_t *key_in, uint32_t *key_out, const int32_t *ids_in,
                      int32_t *ids_out, int64_t n, hipStream_t s);
 
+// Triangle task items built in HBM from the DAG and its reverse lists,
+// identical to the host builder (xgm_api.hip build_tc_items): *out (int4
+// items, tiers concatenated, owned by the caller) and the items per tier.
+hipError_t build_tc_items_device(const DevCSR &g, int max_small, int max_large, int num_cu, hipStream_t s,
+                                 int32_t **out, int64_t counts[NUM_TIERS]);
+
 hipError_t launch_zero(void *ptr, size_t n_u64, hipStream_t s);
 hipError_t launch_copy(void *dst, const void *src, size_t n16, int num_cu, hipStream_t s);
 

@@ -217,11 +217,12 @@ struct WindowLane {
 };
 
 // Walk flattened elements [lo, hi) with one wave, U windows per step.
-// f(const WindowLane&) is called by all 64 lanes for every window (invalid
-// lanes included, so f may use ballots).
+// f(const WindowLane (&)[U]) is called by all 64 lanes for every group of
+// U windows (invalid lanes included, so f may use ballots), after the U
+// global loads of the group were issued.
 template <int U, class Seg, class F>
-__device__ __forceinline__ void wave_walk(const Seg &t, const int32_t *__restrict__ colidx, int ns,
-                                          int lo, int hi, int lane, WaveFlags &fl, F &&f) {
+__device__ __forceinline__ void wave_walk_group(const Seg &t, const int32_t *__restrict__ colidx, int ns,
+                                                int lo, int hi, int lane, WaveFlags &fl, F &&f) {
   // the range and table size are the same on every lane (scalar registers)
   lo = __builtin_amdgcn_readfirstlane(lo);
   hi = __builtin_amdgcn_readfirstlane(hi);
@@ -279,9 +280,18 @@ __device__ __forceinline__ void wave_walk(const Seg &t, const int32_t *__restric
     }
 #pragma unroll
     for (int u = 0; u < U; ++u) r[u].w = r[u].valid ? colidx[pos[u]] : 0;
+    f(r);
+  }
+}
+
+// wave_walk_group with f(const WindowLane&) called for every window.
+template <int U, class Seg, class F>
+__device__ __forceinline__ void wave_walk(const Seg &t, const int32_t *__restrict__ colidx, int ns,
+                                          int lo, int hi, int lane, WaveFlags &fl, F &&f) {
+  wave_walk_group<U>(t, colidx, ns, lo, hi, lane, fl, [&](const WindowLane (&r)[U]) {
 #pragma unroll
     for (int u = 0; u < U; ++u) f(r[u]);
-  }
+  });
 }
 
 // Contiguous, 64-aligned share [lo, hi) of [0, L) for wave `wave` of `nw`.

@@ -191,6 +191,8 @@ struct xgm_dgraph {
   }
 };
 
+static int ensure_host_mirror(xgm_ctx *ctx, xgm_dgraph *g);
+
 namespace {
 
 #define HIP_TRY(ctx, call)                                  \
@@ -578,6 +580,18 @@ void build_tc_items(const xgm_dgraph *g, int max_small, int max_large, std::vect
 // unknown codes fall back to the edge LUT.
 int build_clique_sched(xgm_ctx *ctx, const xgm_dgraph *g, int pattern, const int8_t *codes, xgm_sched *sc) {
   const int k = pattern == XGM_CLIQUE4 ? 4 : 5;
+  // edge items (codes 2, 3 and LUT codes out of the LUT range) read the
+  // forward lists on the host; a DAG built in HBM copies them back only then
+  bool edges = false;
+  for (int32_t v : g->order) {
+    const int d = g->dplus[v];
+    if (d < k - 1) continue;
+    int c = codes ? codes[v] : 0;
+    if (c < 0 || c > 4) c = 2;
+    edges |= c == 2 || c == 3 || (d > kBinCap[3] && !((c == 1 || c == 4) && d <= kHubLutMax));
+  }
+  if (edges)
+    if (int rc = ensure_host_mirror(ctx, const_cast<xgm_dgraph *>(g))) return rc;
   sc->pattern = pattern;
   sc->k = k;
   sc->device = ctx->device;
@@ -745,24 +759,14 @@ int orient_device(xgm_ctx *ctx, const xgm_dgraph *u, std::unique_ptr<xgm_dgraph>
     size_t b = tmp_bytes;
     HIP_TRY(ctx, prep_sort(tmp.p, b, k_in, k_out, i_in, i_out, n, s));
   }
+  // only d+ and the task order come back (2 ints per vertex): the task
+  // lists are built from them or in HBM, so the CSR stays on the device
   std::vector<int32_t> sorted;
   try {
-    d->h_rowptr.resize(n + 1);
-    d->h_rrowptr.resize(n + 1);
-    d->h_colidx.resize(d->nnz);
-    d->h_rcolidx.resize(d->nnz);
     d->dplus.resize(n);
     sorted.resize(n);
   } catch (const std::bad_alloc &) {
     return XGM_ENOMEM;
-  }
-  HIP_TRY(ctx, hipMemcpyAsync(d->h_rowptr.data(), d->d_rowptr, (n + 1) * sizeof(int64_t), hipMemcpyDeviceToHost, s));
-  HIP_TRY(ctx, hipMemcpyAsync(d->h_rrowptr.data(), d->d_rrowptr, (n + 1) * sizeof(int64_t), hipMemcpyDeviceToHost,
-                              s));
-  if (d->nnz) {
-    HIP_TRY(ctx, hipMemcpyAsync(d->h_colidx.data(), d->d_colidx, d->nnz * sizeof(int32_t), hipMemcpyDeviceToHost, s));
-    HIP_TRY(ctx,
-            hipMemcpyAsync(d->h_rcolidx.data(), d->d_rcolidx, d->nnz * sizeof(int32_t), hipMemcpyDeviceToHost, s));
   }
   if (n) {
     HIP_TRY(ctx, hipMemcpyAsync(d->dplus.data(), dplus.p, n * sizeof(int32_t), hipMemcpyDeviceToHost, s));
@@ -927,6 +931,32 @@ int xgm_graph_download(xgm_ctx *ctx, const xgm_dgraph *g, xgm_hgraph **out) {
   return XGM_OK;
 }
 
+int xgm_graph_tc_items(xgm_ctx *ctx, const xgm_dgraph *g, const xgm_opts *opts, int32_t *out, int64_t cap,
+                       int64_t *len, int64_t *tier_counts) {
+  if (!ctx || !g || !len) return XGM_EINVAL;
+  if (!(g->flags & XGM_CSR_DAG)) return ctx->fail(XGM_EINVAL, "triangle items need a DAG");
+  if (opts && opts->strategy != XGM_STRAT_AUTO && opts->strategy != XGM_STRAT_HASH &&
+      opts->strategy != XGM_STRAT_BSEARCH)
+    return ctx->fail(XGM_EINVAL, "items of the LDS-staged triangle strategies only");
+  uint64_t c = 0;
+  int rc = xgm_count(ctx, g, XGM_TRIANGLE, opts, &c);  // builds and caches the items
+  if (rc) return rc;
+  const int ws = (opts && opts->wave_max_small > 0) ? std::min(opts->wave_max_small, kTcWaveCap[0]) : kTcWaveCap[0];
+  const int wl = (opts && opts->wave_max_large > 0) ? std::max(ws, std::min(opts->wave_max_large, kTcWaveCap[1]))
+                                                    : std::max(ws, kTcWaveLargeDefault);
+  auto it = g->task_cache.find("tc_items_" + std::to_string(ws) + "_" + std::to_string(wl));
+  if (it == g->task_cache.end()) return ctx->fail(XGM_EINVAL, "no triangle items cached");
+  int64_t items = 0;
+  for (size_t t = 0; t < it->second.second.size(); ++t) {
+    if (tier_counts && t < (size_t)NUM_TIERS) tier_counts[t] = it->second.second[t];
+    items += it->second.second[t];
+  }
+  *len = 4 * items;
+  if (out && cap > 0 && items)
+    HIP_TRY(ctx, hipMemcpy(out, it->second.first, (size_t)std::min(cap, *len) * sizeof(int32_t), hipMemcpyDeviceToHost));
+  return XGM_OK;
+}
+
 int xgm_graph_task_order(const xgm_dgraph *g, int32_t *out, int64_t cap, int64_t *len) {
   if (!g || !len || cap < 0 || (cap > 0 && !out)) return XGM_EINVAL;
   if (!(g->flags & XGM_CSR_DAG)) return XGM_EINVAL;
@@ -952,7 +982,8 @@ void xgm_graph_free(xgm_dgraph *g) {
 static DevCSR dev_view(const xgm_dgraph *g);
 
 // Host copy of a device graph's CSR (task preparation of the diamond,
-// 4-cycle and motif paths), copied from HBM on first use.
+// 4-cycle and motif paths, and of clique edge items), copied from HBM on
+// first use.
 static int ensure_host_mirror(xgm_ctx *ctx, xgm_dgraph *g) {
   if ((int64_t)g->h_rowptr.size() == g->n + 1) return XGM_OK;
   std::vector<int64_t> rp;
@@ -1068,7 +1099,17 @@ static int dispatch_pattern(xgm_ctx *ctx, const xgm_dgraph *g, int pattern, cons
                                                         : std::max(ws, kTcWaveLargeDefault);
       const std::string key = "tc_items_" + std::to_string(ws) + "_" + std::to_string(wl);
       std::vector<std::vector<int32_t>> parts;
-      if (!cached(gm, key)) build_tc_items(gm, ws, wl, parts);
+      if (!cached(gm, key)) {
+        if ((int64_t)gm->h_rrowptr.size() == g->n + 1) {
+          build_tc_items(gm, ws, wl, parts);  // host copy of the DAG at hand (host upload)
+        } else {
+          // DAG built in HBM (xgm_graph_orient): the same items, built there
+          int32_t *d_items = nullptr;
+          int64_t counts[NUM_TIERS] = {};
+          HIP_TRY(ctx, build_tc_items_device(dg, ws, wl, ctx->num_cu, ctx->stream, &d_items, counts));
+          gm->task_cache.emplace(key, std::make_pair(d_items, std::vector<int64_t>(counts, counts + NUM_TIERS)));
+        }
+      }
       if ((rc = cache_parts(ctx, gm, key, parts, 4, tl))) return rc;
       TaskList tiers[NUM_TIERS];
       for (int t = 0; t < NUM_TIERS; ++t) tiers[t] = tl[t];
