@@ -31,7 +31,7 @@ namespace {
 template <int HCAP, int PCAP>
 struct DiaStage {
   static_assert((HCAP & (HCAP - 1)) == 0, "power-of-two hash table");
-  int key[2 * HCAP];
+  alignas(16) int key[2 * HCAP];  // buckets of 4 slots
   Segs<PCAP, false> seg;
   int cnt[PCAP];
   int ns, L, task;
@@ -71,10 +71,10 @@ __global__ __launch_bounds__(NT) void k_diamond(const int64_t *__restrict__ rowp
     if (LDS_SET) {
       // the whole table (2 * HCAP slots, allocated anyway): the smaller the
       // owner, the lower the load and the shorter a missing probe's run
-      hb = 31 - __clz(2 * HCAP);
-      for (int q = tid; q < (1 << hb); q += NT) S.key[q] = -1;
+      hb = 31 - __clz(2 * HCAP) - 2;  // 2^hb buckets of 4 slots
+      for (int q = tid; q < (4 << hb); q += NT) S.key[q] = -1;
       __syncthreads();
-      for (int q = tid; q < dt; q += NT) hash_insert(S.key, hb, nt[q]);
+      for (int q = tid; q < dt; q += NT) bhash_insert(S.key, hb, nt[q]);
     }
     // partner p owned by v iff (deg p, p) < (deg v, v)
     auto keep = [dt, v](int p, int lp) { return lp < dt || (lp == dt && p < v); };
@@ -87,12 +87,33 @@ __global__ __launch_bounds__(NT) void k_diamond(const int64_t *__restrict__ rowp
       __syncthreads();
       int lo, hi;
       wave_share(L, wave, NT / 64, lo, hi);
-      wave_walk<4>(S.seg, colidx, ns, lo, hi, lane, fl, [&](const WindowLane &r) {
-        bool hit = false;
-        if (r.valid) hit = LDS_SET ? hash_contains(S.key, hb, r.w) : sorted_contains(nt, dt, r.w);
-        const u64 hits = __ballot(hit);
-        const int h = segment_run_hits(hits, r, lane);
-        if (h) atomicAdd(&S.cnt[r.j], h);
+      wave_walk_group<4>(S.seg, colidx, ns, lo, hi, lane, fl, [&](const WindowLane (&r)[4]) {
+        bool hit[4];
+        if (LDS_SET) {
+          // the four home buckets read together, then the rare full-bucket walks
+          uint32_t h[4];
+          int4 q[4];
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            h[u] = hash_slot(r[u].w, hb);
+            q[u] = reinterpret_cast<const int4 *>(S.key)[h[u]];
+          }
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            hit[u] = bucket_has(q[u], r[u].w);
+            if (__builtin_expect(r[u].valid && !hit[u] && q[u].w >= 0, 0))
+              hit[u] = bhash_contains_after(S.key, hb, h[u], r[u].w);
+            hit[u] = hit[u] && r[u].valid;
+          }
+        } else {
+#pragma unroll
+          for (int u = 0; u < 4; ++u) hit[u] = r[u].valid && sorted_contains(nt, dt, r[u].w);
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const int c = segment_run_hits(__ballot(hit[u]), r[u], lane);
+          if (c) atomicAdd(&S.cnt[r[u].j], c);
+        }
       });
       __syncthreads();
       for (int q = tid; q < ns; q += NT) {
