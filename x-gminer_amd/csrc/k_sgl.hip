@@ -266,7 +266,8 @@ __global__ __launch_bounds__(NT) void k_diamond_big(const int64_t *__restrict__ 
 template <int G>
 struct DiaPack {
   static constexpr int SLOTS = 128;  // 64 keys at load <= 1/2
-  int key[G * SLOTS];
+  static constexpr int BBITS = 5;     // the whole region: 32 buckets of 4 slots
+  alignas(16) int key[G * SLOTS];
   Segs<64 * G, true> seg;            // idx[] = item of the segment
   int cnt[64 * G];
   int u[G], k0[G], d[G], bits[G];
@@ -305,7 +306,6 @@ __global__ __launch_bounds__(128) void k_diamond_packed(const int64_t *__restric
       s.k0[lane] = it.y;
       s.d[lane] = dt;
       s.ob[lane] = b;
-      s.bits[lane] = hash_bits_for(dt);
 #pragma unroll
       for (int j = 0; j < 4; ++j) s.sum[lane][j] = 0;
     }
@@ -326,7 +326,7 @@ __global__ __launch_bounds__(128) void k_diamond_packed(const int64_t *__restric
     };
     for (int q = lane; q < D; q += 64) {
       const int g = item_of(q, s.doff);
-      hash_insert(s.key + g * PS::SLOTS, s.bits[g], colidx[s.ob[g] + (q - s.doff[g])]);
+      bhash_insert(s.key + g * PS::SLOTS, PS::BBITS, colidx[s.ob[g] + (q - s.doff[g])]);
     }
     int ns = 0, L = 0;
     wave_segments_fn(s.seg, rowptr, 0, P,
@@ -350,7 +350,7 @@ __global__ __launch_bounds__(128) void k_diamond_packed(const int64_t *__restric
       bool hit = false;
       if (r.valid) {
         const int g = s.seg.idx[r.j];
-        hit = hash_contains(s.key + g * PS::SLOTS, s.bits[g], r.w);
+        hit = bhash_contains(s.key + g * PS::SLOTS, PS::BBITS, r.w);
       }
       const int h = segment_run_hits(__ballot(hit), r, lane);
       if (h) atomicAdd(&s.cnt[r.j], h);

@@ -155,7 +155,8 @@ __global__ __launch_bounds__(256) void k_tc_wave(const int64_t *__restrict__ row
 template <int G, int HSHIFT>
 struct PackStage {
   static constexpr int SLOTS = 64 << HSHIFT;  // per item
-  int key[G * SLOTS];
+  static constexpr int BBITS = 4 + HSHIFT;     // the whole region in buckets of 4 slots
+  alignas(16) int key[G * SLOTS];
   Segs<64 * G, true> seg;                     // idx[] = item of the segment
   int u[G], k0[G], n[G], du[G], bits[G], cnt[G];
   int poff[G + 1], doff[G + 1];               // prefix sums of partners / staged lengths
@@ -197,7 +198,6 @@ __global__ __launch_bounds__(128) void k_tc_wave_packed(const int64_t *__restric
       s.du[lane] = du;
       s.ob[lane] = b;
       s.ib[lane] = rrowptr[it.x];
-      s.bits[lane] = hash_bits_for(du, HSHIFT);
       s.cnt[lane] = 0;
     }
     const int pin = wave_incl_scan(n, lane), din = wave_incl_scan(du, lane);
@@ -218,7 +218,7 @@ __global__ __launch_bounds__(128) void k_tc_wave_packed(const int64_t *__restric
     // staged lists N+(u_g) into hash region g
     for (int q = lane; q < D; q += 64) {
       const int g = item_of(q, s.doff);
-      hash_insert(s.key + g * PS::SLOTS, s.bits[g], colidx[s.ob[g] + (q - s.doff[g])]);
+      bhash_insert(s.key + g * PS::SLOTS, PS::BBITS, colidx[s.ob[g] + (q - s.doff[g])]);
     }
     // one segment table over the partners of all items
     int ns = 0, L = 0;
@@ -243,7 +243,7 @@ __global__ __launch_bounds__(128) void k_tc_wave_packed(const int64_t *__restric
       bool hit = false;
       if (r.valid) {
         const int g = s.seg.idx[r.j];
-        hit = hash_contains(s.key + g * PS::SLOTS, s.bits[g], r.w);
+        hit = bhash_contains(s.key + g * PS::SLOTS, PS::BBITS, r.w);
       }
       if (!slots) {
         cnt += hit;
