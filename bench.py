@@ -751,7 +751,12 @@ class Suite:
     def __init__(self, args, xgm):
         self.args, self.xgm = args, xgm
         self.gold = json.load(open(os.path.join(ROOT, "tests", "golden", "counts.json")))
-        self.cache = tempfile.mkdtemp(prefix="xgm_bench_")
+        self.keep = bool(args.keep_cache)
+        if self.keep:
+            os.makedirs(args.keep_cache, exist_ok=True)
+            self.cache = os.path.abspath(args.keep_cache)
+        else:
+            self.cache = tempfile.mkdtemp(prefix="xgm_bench_")
         self.rows, self.cpu_done = [], {}
         self.info = cpu_info()
         self.threads = cpu_threads(self.info, args.cpu_threads)
@@ -1028,7 +1033,7 @@ class Suite:
             model, kb = "clique_kernel_bytes", clique_kernel_bytes(ud, dag.sched_heuristic(2, st_, tuned[p]["opts"]
                                                                                           ["lut_max_degree"]), 4)
         else:
-            model, kb = "cycle4_bytes (pattern)", {dom: cycle4_bytes(uu)}
+            model, kb = None, {}
         cpu = None
         if not a.no_cpu:
             cpu = cpu_baseline_batch(specs, inst, self.threads)
@@ -1042,7 +1047,12 @@ class Suite:
                  default_plan_ms_per_step=round(pms_step, 4), tuned_vs_default=round(pms_step / ms_step, 4),
                  autotune={q: tuned[q] for q in BATCH_PATTERNS}, autotune_s=round(tune_s, 1),
                  kernels_ms_eager={k: round(v, 4) for k, v in kernels.items()}, dominant_kernel=dom,
-                 roofline=hbm_roof(kb.get(dom), kernels[dom], dom, model), cpu_baseline=cpu)
+                 roofline=hbm_roof(kb.get(dom), kernels[dom], dom, model) if model else None, cpu_baseline=cpu)
+        if p == "rectangle":
+            # the 4-cycle kernels split tasks by wedge work: bytes of all of
+            # them over their summed time
+            c4_ms = sum(v for k, v in kernels.items() if owner[k] == "rectangle")
+            r["roofline_pattern"] = hbm_roof(cycle4_bytes(uu), c4_ms, "all 4-cycle kernels", "cycle4_bytes")
         self.rows.append(r)
         log(f"row batch256: {ms_step:.2f} ms/step (default plan {pms_step:.2f}), dominant {dom}")
         # counter pass: the dominant pattern's eager count on the unions
@@ -1161,6 +1171,9 @@ class Suite:
 
     def close(self):
         self.ctx.close()
+        if self.keep:  # the counter pass can be replayed: bench.py --profile-pass DIR/rows.json
+            json.dump(self.pass_rows, open(os.path.join(self.cache, "rows.json"), "w"))
+            return
         import shutil
         shutil.rmtree(self.cache, ignore_errors=True)
 
@@ -1232,10 +1245,53 @@ def _pmc_segments(counter, argv, timeout=240):
     return calib, segs
 
 
+def trace_summary(trace_dir, spec):
+    """Per-row kernel durations of a `rocprofv3 --kernel-trace` run of the
+    counter pass (bench.py --profile-pass SPEC): the dispatches between the
+    marker copies of each row, matched to the row's launch names (SPEC's
+    .names.json), as a markdown table on stdout."""
+    import csv
+    import glob
+    rows = json.load(open(spec))
+    names = json.load(open(spec + ".names.json"))
+    disp = []
+    for f in glob.glob(os.path.join(trace_dir, "**", "*kernel_trace.csv"), recursive=True):
+        for r in csv.DictReader(open(f)):
+            disp.append((int(r.get("Dispatch_Id") or r.get("Correlation_Id")), r["Kernel_Name"],
+                         (int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) * 1e-6))
+    disp.sort()
+    segs, cur, groups, in_marker = [], [], 0, False
+    for _, name, ms in disp:
+        if "k_copy_u4" in name:
+            if not in_marker:
+                groups += 1
+                if groups > 1:
+                    segs.append(cur)
+                cur = []
+            in_marker = True
+            continue
+        in_marker = False
+        if "k_zero_u64" in name or name.startswith("__amd"):
+            continue
+        sym = re.search(r"k_\w+(<[^>]*>)?", name)
+        cur.append((sym.group(0) if sym else name, ms))
+    print("| row | launch | kernel symbol | ms |")
+    print("|---|---|---|---|")
+    for i, r in enumerate(rows):
+        seg = segs[2 * i + 1] if 2 * i + 1 < len(segs) else []
+        kn = names[i]
+        if len(seg) != len(kn):
+            print(f"| {r['row']} | (dispatches {len(seg)} != launches {len(kn)}) | | |")
+            continue
+        for nm, (sym, ms) in zip(kn, seg):
+            print(f"| {r['row']} | {nm} | `{sym}` | {ms:.3f} |")
+
+
 def profile_pass(spec):
     """Counter pass: the copy calibration (1 GiB), then every row of the
-    suite counted once, each followed by a marker copy; writes the kernel
-    launch names of every row (in launch order) beside the spec."""
+    suite: set-up and a warm-up count, a marker copy, the measured count and
+    another marker copy; writes the kernel launch names of every row (in
+    launch order) beside the spec."""
     from conftest import load_xgm
     xgm = load_xgm()
     rows = json.load(open(spec))
@@ -1256,6 +1312,7 @@ def profile_pass(spec):
                 ci = g.colidx[g.rowptr[lo]:g.rowptr[hi]] - np.int32(lo)
                 parts.append(xgm.Graph.from_arrays(rp, ci, r["flags"]))
             dg = ctx.upload_batch(parts)
+            ctx.count_instances(dg, r["pattern"], **r["opts"])  # warm-up (task lists, caches)
             ctx.copy_bandwidth(1 << 16, 1)
             ctx.count_instances(dg, r["pattern"], **r["opts"])
         else:
@@ -1265,6 +1322,10 @@ def profile_pass(spec):
                 graphs[r["graph"]] = ctx.upload(xgm.load_csr(r["graph"], r["flags"]))
             dg = graphs[r["graph"]]
             s = xgm.Schedule(dg, r["pattern"], np.load(r["codes"])) if r["kind"] == "sched" else None
+            if s is not None:  # warm-up (task lists, caches)
+                s.count()
+            else:
+                ctx.count(dg, r["pattern"], **r["opts"])
             ctx.copy_bandwidth(1 << 16, 1)
             if s is not None:
                 s.count()
@@ -1386,6 +1447,10 @@ def main():
     ap.add_argument("--copy-calibrate", action="store_true",
                     help="only run the plain copy kernel over 1 GiB (counter calibration)")
     ap.add_argument("--profile-pass", default="", help=argparse.SUPPRESS)
+    ap.add_argument("--keep-cache", default="",
+                    help="suite: keep the saved graphs and the counter-pass spec (DIR/rows.json) in DIR")
+    ap.add_argument("--trace-summary", nargs=2, metavar=("TRACE_DIR", "SPEC"),
+                    help="per-row kernel times of a rocprofv3 --kernel-trace run of --profile-pass SPEC")
     ap.add_argument("--cpu-threads", type=int, default=0)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -1393,6 +1458,8 @@ def main():
                           "n_gpus": args.gpus}))
         sys.exit(2)
 
+    if args.trace_summary:
+        return trace_summary(*args.trace_summary)
     from conftest import load_xgm
     xgm = load_xgm()
     if args.profile_pass:
