@@ -115,6 +115,18 @@ void xgm_ctx_destroy(xgm_ctx *ctx);
 /* Last error message recorded by ctx (static until the next call). */
 const char *xgm_ctx_last_error(const xgm_ctx *ctx);
 
+/* Where a count builds the task lists it has not cached yet: its work items,
+ * orders and relabelled copies.  XGM_PREP_AUTO (the default): on the host
+ * when a host copy of the graph's CSR is at hand (a DAG uploaded from the
+ * host), else in HBM with no copy of the CSR back (undirected uploads, DAGs
+ * from xgm_graph_orient, batches).  XGM_PREP_HOST: always on the host
+ * (copying the CSR back once when needed).  XGM_PREP_DEVICE: always in HBM.
+ * Both routes give the same lists.  Replaces the host task preparation of
+ * xgminer/src/graph_v2.cpp:307-358 and glumin_GM_solver.cu:326-335.
+ * Lists cached on a graph stay as built. */
+enum { XGM_PREP_AUTO = 0, XGM_PREP_HOST = 1, XGM_PREP_DEVICE = 2 };
+int xgm_ctx_set_prep(xgm_ctx *ctx, int route);
+
 /* Copy a CSR into HBM and build its task lists (degree bins).  The CSR must
  * satisfy xgm_csr_validate() for its flags; nnz must be below 2^31.  A device
  * graph is used only with the context that uploaded it; xgm_graph_free() may
@@ -150,6 +162,19 @@ int xgm_graph_task_order(const xgm_dgraph *g, int32_t *out, int64_t cap, int64_t
  * both give the same items.  Writes min(cap, len) ints; *len = 4 x items. */
 int xgm_graph_tc_items(xgm_ctx *ctx, const xgm_dgraph *g, const xgm_opts *opts, int32_t *out, int64_t cap,
                        int64_t *len, int64_t *tier_counts);
+
+/* Task lists cached on a device graph, for inspection (no counterpart in
+ * the reference): xgm_graph_task_list_names writes the names of the cached
+ * lists, each ended by '\n', into names (min(cap, len) bytes; *len = the full
+ * length); xgm_graph_task_list copies the list `name` (ints, tiers
+ * concatenated: *len ints in all, tier sizes in units of tasks into
+ * counts[0 .. min(counts_cap, *n_counts))).  Names: "tc_items_<s>_<l>"
+ * (triangles, int4 items), "diamond" / "motif_edges" (int4 owner items),
+ * "vmotif" / "vmotif_list" (vertex ids), "cycle4_wedge_<m>" (vertex ids),
+ * "cycle4_range_<m>" / "cycle4_hubs_<rows>_<m>" (int4 items). */
+int xgm_graph_task_list_names(const xgm_dgraph *g, char *names, int64_t cap, int64_t *len);
+int xgm_graph_task_list(xgm_ctx *ctx, const xgm_dgraph *g, const char *name, int32_t *out, int64_t cap,
+                        int64_t *len, int64_t *counts, int counts_cap, int *n_counts);
 
 /* Count `pattern` on a device graph; *out receives the uint64 count.
  * opts may be NULL (auto).  Synchronous. */

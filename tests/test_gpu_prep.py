@@ -118,3 +118,108 @@ def test_device_tc_items_equal_host(xgm, ctx):
             assert ctx.count(dev, xgm.TRIANGLE, **opts) == ctx.count(host, xgm.TRIANGLE, **opts)
         dev.close()
         host.close()
+
+
+def _lists(dg):
+    return {nm: dg.task_list(nm) for nm in dg.task_list_names()}
+
+
+def _same_lists(a, b, tag):
+    la, lb = _lists(a), _lists(b)
+    assert sorted(la) == sorted(lb), tag
+    for nm in la:
+        assert la[nm][1] == lb[nm][1], (tag, nm)
+        assert np.array_equal(la[nm][0], lb[nm][0]), (tag, nm)
+
+
+def _routes(xgm, ctx, make):
+    """(host-route graph, device-route graph) from make(), each uploaded and
+    prepared under its route by the caller's counts."""
+    ctx.set_prep(xgm.PREP_HOST)
+    h = make()
+    ctx.set_prep(xgm.PREP_DEVICE)
+    d = make()
+    return h, d
+
+
+def test_device_undirected_lists_equal_host(xgm, ctx):
+    """Task lists of undirected graphs built in HBM (diamond owners and their
+    hub slices, the 4-cycle relabel with its wedge, range and hub lists, the
+    vertex-induced motif orders, the motif-census owners) equal the host
+    builders' lists name for name and int for int, and give the same counts;
+    hubs above 1024 / 8192 neighbours, 4-cycle hub rounds and batches of
+    instances included."""
+    graphs = dict(_graphs(xgm))
+    leaves = np.arange(1, 9001, dtype=np.int32)
+    graphs["star9000"] = xgm.from_edges(9001, np.zeros(9000, np.int32), leaves)  # owner tier above 8192
+    graphs["k1100"] = complete_graph(xgm, 1100)   # owners above 1024, cut into 64K-work slices
+    cases = [(xgm.DIAMOND_EDGE, {}), (xgm.CYCLE4_EDGE, {}), (xgm.CYCLE4_EDGE, dict(wedge_range=1)),
+             (xgm.CYCLE4_EDGE, dict(wedge_range=2, hub_rows=3)), (xgm.STAR3_V, {}),
+             (xgm.TAILEDTRI_V, dict(strategy=xgm.STRAT_LIST))]
+    try:
+        for name, g in graphs.items():
+            for pid, opts in cases:
+                if name == "k1100" and pid in (xgm.STAR3_V, xgm.TAILEDTRI_V):
+                    continue
+                _say("undirected lists", name, pid, opts)
+                h, d = _routes(xgm, ctx, lambda: ctx.upload(g))
+                ctx.set_prep(xgm.PREP_HOST)
+                ch = ctx.count(h, pid, **opts)
+                ctx.set_prep(xgm.PREP_DEVICE)
+                assert ctx.count(d, pid, **opts) == ch, (name, pid, opts)
+                _same_lists(h, d, (name, pid, opts))
+                h.close()
+                d.close()
+            for k in (3, 4):
+                _say("motif census lists", name, k)
+                h, d = _routes(xgm, ctx, lambda: ctx.upload(g))
+                ctx.set_prep(xgm.PREP_HOST)
+                mh = ctx.motifs(h, k)
+                ctx.set_prep(xgm.PREP_DEVICE)
+                assert np.array_equal(ctx.motifs(d, k), mh), (name, k)
+                _same_lists(h, d, (name, k))
+                h.close()
+                d.close()
+        parts = [graphs["citeseer"], graphs["k9"], graphs["star300"], graphs["empty"], graphs["rmat12"],
+                 graphs["chunglu"]]
+        for pid, opts in cases[:4]:
+            _say("batch lists", pid, opts)
+            h, d = _routes(xgm, ctx, lambda: ctx.upload_batch(parts))
+            ctx.set_prep(xgm.PREP_HOST)
+            ch = ctx.count_instances(h, pid, **opts)
+            ctx.set_prep(xgm.PREP_DEVICE)
+            assert np.array_equal(ctx.count_instances(d, pid, **opts), ch), (pid, opts)
+            _same_lists(h, d, ("batch", pid, opts))
+            h.close()
+            d.close()
+        _say("batch motif census")
+        h, d = _routes(xgm, ctx, lambda: ctx.upload_batch(parts))
+        ctx.set_prep(xgm.PREP_HOST)
+        mh = ctx.motifs(h, 4)
+        ctx.set_prep(xgm.PREP_DEVICE)
+        assert np.array_equal(ctx.motifs(d, 4), mh)
+        h.close()
+        d.close()
+    finally:
+        ctx.set_prep(xgm.PREP_AUTO)
+
+
+def test_prep_route_host_for_device_dag(xgm, ctx):
+    """PREP_HOST on a DAG built in HBM copies it back and builds the triangle
+    items on the host: the same items as the device builder's."""
+    g = xgm.gen_rmat(14, 16, 0.57, 0.19, 0.19, 5)
+    try:
+        _say("tc items host route")
+        ctx.set_prep(xgm.PREP_HOST)
+        h = ctx.orient(ctx.upload(g))
+        hi, ht = h.tc_items()
+        ctx.set_prep(xgm.PREP_DEVICE)
+        d = ctx.orient(ctx.upload(g))
+        di, dt = d.tc_items()
+        assert ht == dt
+        assert np.array_equal(hi, di)
+        assert ctx.count(h, xgm.TRIANGLE) == ctx.count(d, xgm.TRIANGLE)
+        with pytest.raises(xgm.XGMError):
+            ctx.set_prep(7)
+    finally:
+        ctx.set_prep(xgm.PREP_AUTO)

@@ -44,6 +44,8 @@ SETOP_INTERSECT, SETOP_DIFF, SETOP_DIFF_EXCL = 1, 2, 3
 SETOP_MERGE, SETOP_HYBRID = 0x100, 0x200
 SETOP_MAX_ANCESTORS = 4
 NOALLOC_MAX_PAIRS = 1 << 20
+# where task lists are built (Context.set_prep)
+PREP_AUTO, PREP_HOST, PREP_DEVICE = 0, 1, 2
 
 
 class XGMError(RuntimeError):
@@ -116,6 +118,10 @@ def _load():
         "xgm_graph_download": (C.c_int, [VP, VP, P(VP)]),
         "xgm_graph_task_order": (C.c_int, [VP, P(I32), I64, P(I64)]),
         "xgm_graph_tc_items": (C.c_int, [VP, VP, P(_Opts), P(I32), I64, P(I64), P(I64)]),
+        "xgm_graph_task_list_names": (C.c_int, [VP, C.c_char_p, I64, P(I64)]),
+        "xgm_graph_task_list": (C.c_int, [VP, VP, C.c_char_p, P(I32), I64, P(I64), P(I64), C.c_int,
+                                          P(C.c_int)]),
+        "xgm_ctx_set_prep": (C.c_int, [VP, C.c_int]),
         "xgm_count": (C.c_int, [VP, VP, C.c_int, P(_Opts), P(U64)]),
         "xgm_count_csr": (C.c_int, [VP, P(_Csr), C.c_int, P(_Opts), P(U64)]),
         "xgm_last_profile": (C.c_int, [VP, P(_Profile)]),
@@ -313,6 +319,12 @@ class Context:
 
     def upload(self, g: Graph) -> "DeviceGraph":
         return DeviceGraph(self, g)
+
+    def set_prep(self, route: int):
+        """Build uncached task lists on the host (PREP_HOST), in HBM
+        (PREP_DEVICE), or in HBM unless a host copy of the CSR is at hand
+        (PREP_AUTO, the default)."""
+        _check(lib.xgm_ctx_set_prep(self._h, route), self._h)
 
     def orient(self, dg: "DeviceGraph") -> "DeviceGraph":
         """Degree-oriented DAG of an uploaded undirected graph, built in HBM
@@ -573,6 +585,26 @@ class DeviceGraph:
         _check(lib.xgm_graph_tc_items(self.ctx._h, self._h, C.byref(o), out.ctypes.data_as(C.POINTER(C.c_int32)),
                                       n.value, C.byref(n), tc), self.ctx._h)
         return out[:n.value].reshape(-1, 4), list(tc)
+
+    def task_list_names(self):
+        """Names of the task lists cached on this graph."""
+        n = C.c_int64()
+        _check(lib.xgm_graph_task_list_names(self._h, None, 0, C.byref(n)))
+        buf = C.create_string_buffer(max(1, n.value))
+        _check(lib.xgm_graph_task_list_names(self._h, buf, n.value, C.byref(n)))
+        return [x for x in buf.raw[:n.value].decode().split("\n") if x]
+
+    def task_list(self, name: str):
+        """A cached task list: (ints, tiers concatenated; tasks per tier)."""
+        n, nc = C.c_int64(), C.c_int()
+        cnt = (C.c_int64 * 16)()
+        key = name.encode()
+        _check(lib.xgm_graph_task_list(self.ctx._h, self._h, key, None, 0, C.byref(n), cnt, 16, C.byref(nc)),
+               self.ctx._h)
+        out = np.zeros(max(1, n.value), np.int32)
+        _check(lib.xgm_graph_task_list(self.ctx._h, self._h, key, out.ctypes.data_as(C.POINTER(C.c_int32)), n.value,
+                                       C.byref(n), cnt, 16, C.byref(nc)), self.ctx._h)
+        return out[:n.value], list(cnt)[:nc.value]
 
     def close(self):
         if getattr(self, "_h", None) is not None:

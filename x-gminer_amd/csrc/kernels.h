@@ -214,6 +214,33 @@ hipError_t prep_sort(void *temp, size_t &bytes, const uint32_t *key_in, uint32_t
 hipError_t build_tc_items_device(const DevCSR &g, int max_small, int max_large, int num_cu, hipStream_t s,
                                  int32_t **out, int64_t counts[NUM_TIERS]);
 
+// Task lists of an undirected graph built in HBM; each equals the host
+// builder's list of the same name in xgm_api.hip.
+// Diamond / motif owner items (owners of degree >= min_deg): *out
+// (hipMalloc'd, the caller frees) holds counts[0..3] int4 items, tier-major.
+hipError_t build_dia_items_device(const DevCSR &g, int min_deg, int num_cu, hipStream_t s, int32_t **out,
+                                  int64_t counts[4]);
+// Vertex-induced motif tasks: degree >= 3 by decreasing degree, lists of
+// degree <= 64 / 256 / 1024 / above (list_all: all in the last); *max_deg =
+// the largest degree in the last list (0 if empty).
+hipError_t build_deg3_order_device(const DevCSR &g, bool list_all, int num_cu, hipStream_t s, int32_t **out,
+                                   int64_t counts[4], int64_t *max_deg);
+// host_out[i] = sum of d(d-1) over the vertices of instance i (slot may be
+// null: one instance)
+hipError_t degree_pair_sums_device(const DevCSR &g, const int32_t *slot, int n_inst, hipStream_t s, u64 *host_out);
+// Relabel by ascending (degree, id) inside each instance: the relabelled CSR
+// in HBM (*rl_rowptr, *rl_colidx, hipMalloc'd), and on the host the vertices
+// of positive wedge work by decreasing work, the work and the lower-neighbour
+// count of every vertex.
+hipError_t relabel_by_degree_device(const DevCSR &g, const int32_t *slot, int num_cu, hipStream_t s,
+                                    int64_t **rl_rowptr, int32_t **rl_colidx, std::vector<int32_t> &order,
+                                    std::vector<int64_t> &work, std::vector<int32_t> &nlow);
+// cnt[hoff[h] + k] = |N(v_k) ∩ [0, x)| for hub x = hubs[h] of a relabelled
+// CSR and its k-th lower neighbour v_k
+hipError_t cycle4_hub_counts_device(const int64_t *rl_rowptr, const int32_t *rl_colidx,
+                                    const std::vector<int32_t> &hubs, const std::vector<int64_t> &hoff, int num_cu,
+                                    hipStream_t s, std::vector<int32_t> &cnt);
+
 hipError_t launch_zero(void *ptr, size_t n_u64, hipStream_t s);
 hipError_t launch_copy(void *dst, const void *src, size_t n16, int num_cu, hipStream_t s);
 

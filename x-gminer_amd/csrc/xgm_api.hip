@@ -34,6 +34,7 @@ struct xgm_ctx {
   u64 *d_out = nullptr;
   int *d_queues = nullptr;
   int32_t *d_noalloc = nullptr;  // this device's static staging array (xgm_setop_pairs_noalloc)
+  int prep = XGM_PREP_AUTO;      // where task lists are built (xgm_ctx_set_prep)
   std::vector<hipEvent_t> ev;  // 2 per profiled launch
   xgm_profile prof{};
   int n_marks = 0;
@@ -132,6 +133,7 @@ struct xgm_dgraph {
   int32_t *d_rl_colidx = nullptr;
   std::vector<int32_t> rl_order;   // relabelled task ids by decreasing wedge work
   std::vector<int64_t> rl_work;
+  std::vector<int32_t> rl_nlow;    // lower-neighbour counts (relabelled in HBM; else empty)
   std::vector<int64_t> rl_rowptr;  // host copy of the relabelled CSR
   std::vector<int32_t> rl_colidx;
   int *d_hub = nullptr;            // 4-cycle hub counter rows (hub_rows * stride ints)
@@ -192,6 +194,15 @@ struct xgm_dgraph {
 };
 
 static int ensure_host_mirror(xgm_ctx *ctx, xgm_dgraph *g);
+
+// Task lists of g from its host copy (true) or in HBM (false): the host
+// builders under XGM_PREP_HOST or, under XGM_PREP_AUTO, when a host copy of
+// the CSR is already at hand.
+static bool host_prep(const xgm_ctx *ctx, const xgm_dgraph *g) {
+  if (ctx->prep == XGM_PREP_HOST) return true;
+  if (ctx->prep == XGM_PREP_DEVICE) return false;
+  return (int64_t)g->h_rowptr.size() == g->n + 1;
+}
 
 namespace {
 
@@ -434,43 +445,6 @@ int tasks_split(xgm_ctx *ctx, xgm_dgraph *g, const std::string &key, const std::
     out.push_back(t);
     off += c;
   }
-  return XGM_OK;
-}
-
-// Edge items (v0, v1) for every DAG edge out of the vertices v0 in `verts`
-// with d+(v1) >= min_d1, split by min(d+(v0), d+(v1)) <= split; cached.
-int edge_items(xgm_ctx *ctx, xgm_dgraph *g, const std::string &key, const std::vector<int32_t> &verts, int min_d1,
-               int split, EdgeItems out[2]) {
-  auto it = g->task_cache.find(key);
-  if (it == g->task_cache.end()) {
-    std::vector<int32_t> small, big;
-    for (int32_t v0 : verts) {
-      for (int64_t e = g->h_rowptr[v0]; e < g->h_rowptr[v0 + 1]; ++e) {
-        int32_t v1 = g->h_colidx[e];
-        if (g->dplus[v1] < min_d1) continue;
-        auto &dst = std::min(g->dplus[v0], g->dplus[v1]) <= split ? small : big;
-        dst.push_back(v0);
-        dst.push_back(v1);
-      }
-    }
-    std::vector<int32_t> all(big);
-    all.insert(all.end(), small.begin(), small.end());
-    int32_t *d = nullptr;
-    HIP_TRY(ctx, hipMalloc(&d, std::max<size_t>(2, all.size()) * sizeof(int32_t)));
-    if (!all.empty()) {
-      hipError_t e = hipMemcpy(d, all.data(), all.size() * sizeof(int32_t), hipMemcpyHostToDevice);
-      if (e != hipSuccess) {
-        (void)hipFree(d);
-        return ctx->hip_fail(e, "H2D edge items");
-      }
-    }
-    std::vector<int64_t> counts{(int64_t)small.size() / 2, (int64_t)big.size() / 2};
-    it = g->task_cache.emplace(key, std::make_pair(d, counts)).first;
-  }
-  out[1].items = it->second.first;
-  out[1].count = it->second.second[1];
-  out[0].items = it->second.first + 2 * out[1].count;
-  out[0].count = it->second.second[0];
   return XGM_OK;
 }
 
@@ -823,6 +797,14 @@ void xgm_ctx_destroy(xgm_ctx *ctx) {
 
 const char *xgm_ctx_last_error(const xgm_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
 
+int xgm_ctx_set_prep(xgm_ctx *ctx, int route) {
+  if (!ctx) return XGM_EINVAL;
+  if (route != XGM_PREP_AUTO && route != XGM_PREP_HOST && route != XGM_PREP_DEVICE)
+    return ctx->fail(XGM_EINVAL, "prep route must be XGM_PREP_AUTO, _HOST or _DEVICE");
+  ctx->prep = route;
+  return XGM_OK;
+}
+
 int xgm_graph_upload(xgm_ctx *ctx, const xgm_csr *g, xgm_dgraph **out) {
   if (!ctx || !g || !out) return XGM_EINVAL;
   *out = nullptr;
@@ -957,6 +939,43 @@ int xgm_graph_tc_items(xgm_ctx *ctx, const xgm_dgraph *g, const xgm_opts *opts, 
   return XGM_OK;
 }
 
+// ints per task of a cached list
+static int list_stride(const std::string &name) {
+  for (const char *p : {"tc_items", "diamond", "motif_edges", "cycle4_range", "cycle4_hubs"})
+    if (name.rfind(p, 0) == 0) return 4;
+  return 1;
+}
+
+int xgm_graph_task_list_names(const xgm_dgraph *g, char *names, int64_t cap, int64_t *len) {
+  if (!g || !len || cap < 0 || (cap > 0 && !names)) return XGM_EINVAL;
+  std::string all;
+  for (const auto &kv : g->task_cache) all += kv.first + "\n";
+  *len = (int64_t)all.size();
+  std::copy(all.begin(), all.begin() + std::min<int64_t>(cap, *len), names);
+  return XGM_OK;
+}
+
+int xgm_graph_task_list(xgm_ctx *ctx, const xgm_dgraph *g, const char *name, int32_t *out, int64_t cap,
+                        int64_t *len, int64_t *counts, int counts_cap, int *n_counts) {
+  if (!ctx || !g || !name || !len || cap < 0 || (cap > 0 && !out)) return XGM_EINVAL;
+  auto it = g->task_cache.find(name);
+  if (it == g->task_cache.end()) return ctx->fail(XGM_EINVAL, std::string("no cached task list ") + name);
+  const auto &c = it->second.second;
+  int64_t tasks = 0;
+  for (size_t i = 0; i < c.size(); ++i) {
+    if (counts && (int)i < counts_cap) counts[i] = c[i];
+    tasks += c[i];
+  }
+  if (n_counts) *n_counts = (int)c.size();
+  *len = tasks * list_stride(name);
+  if (cap > 0 && *len > 0) {
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    HIP_TRY(ctx, hipMemcpy(out, it->second.first, (size_t)std::min(cap, *len) * sizeof(int32_t),
+                           hipMemcpyDeviceToHost));
+  }
+  return XGM_OK;
+}
+
 int xgm_graph_task_order(const xgm_dgraph *g, int32_t *out, int64_t cap, int64_t *len) {
   if (!g || !len || cap < 0 || (cap > 0 && !out)) return XGM_EINVAL;
   if (!(g->flags & XGM_CSR_DAG)) return XGM_EINVAL;
@@ -1006,10 +1025,18 @@ static int ensure_host_mirror(xgm_ctx *ctx, xgm_dgraph *g) {
 // larger), owners above 1024 split into slices of ~64K probed elements so
 // hubs spread over the whole GPU.  Cached as "diamond".
 // Owners of degree <= 2 hold no diamond (c <= 1) and are left out; the
-// motif sums (with_deg2) also need the degree-2 owners.
+// motif sums (with_deg2) also need the degree-2 owners.  Built on the host
+// from its copy of the CSR, or in HBM (build_dia_items_device: the same
+// items).
 static int diamond_lists(xgm_ctx *ctx, xgm_dgraph *gm, std::vector<TaskList> &tl, bool with_deg2 = false) {
   std::vector<std::vector<int32_t>> parts(4);
   const char *key = with_deg2 ? "motif_edges" : "diamond";
+  if (!cached(gm, key) && !host_prep(ctx, gm)) {
+    int32_t *d_items = nullptr;
+    int64_t counts[4] = {};
+    HIP_TRY(ctx, build_dia_items_device(dev_view(gm), with_deg2 ? 2 : 3, ctx->num_cu, ctx->stream, &d_items, counts));
+    gm->task_cache.emplace(key, std::make_pair(d_items, std::vector<int64_t>(counts, counts + 4)));
+  }
   if (!cached(gm, key)) {
     if (int rc = ensure_host_mirror(ctx, gm)) return rc;
     if ((int64_t)gm->dplus.size() != gm->n) {  // owner order, built on first use
@@ -1100,8 +1127,17 @@ static int dispatch_pattern(xgm_ctx *ctx, const xgm_dgraph *g, int pattern, cons
       const std::string key = "tc_items_" + std::to_string(ws) + "_" + std::to_string(wl);
       std::vector<std::vector<int32_t>> parts;
       if (!cached(gm, key)) {
-        if ((int64_t)gm->h_rrowptr.size() == g->n + 1) {
-          build_tc_items(gm, ws, wl, parts);  // host copy of the DAG at hand (host upload)
+        const bool have_host = (int64_t)gm->h_rrowptr.size() == g->n + 1;  // a host upload
+        if (ctx->prep == XGM_PREP_HOST || (ctx->prep == XGM_PREP_AUTO && have_host)) {
+          if (!have_host) {  // a DAG built in HBM: copy it back, reverse lists on the host
+            if ((rc = ensure_host_mirror(ctx, gm))) return rc;
+            try {
+              build_reverse(gm->h_rowptr, gm->h_colidx, g->n, gm->h_rrowptr, gm->h_rcolidx);
+            } catch (const std::bad_alloc &) {
+              return XGM_ENOMEM;
+            }
+          }
+          build_tc_items(gm, ws, wl, parts);
         } else {
           // DAG built in HBM (xgm_graph_orient): the same items, built there
           int32_t *d_items = nullptr;
@@ -1154,6 +1190,13 @@ static int dispatch_pattern(xgm_ctx *ctx, const xgm_dgraph *g, int pattern, cons
       const std::string key = list_all ? "vmotif_list" : "vmotif";
       std::vector<int32_t> ord;
       int md = 0;
+      if (!cached(gm, key) && !host_prep(ctx, gm)) {  // the same lists, built in HBM
+        int32_t *d_ids = nullptr;
+        int64_t counts[4] = {}, mx = 0;
+        HIP_TRY(ctx, build_deg3_order_device(dg, list_all, ctx->num_cu, ctx->stream, &d_ids, counts, &mx));
+        gm->task_cache.emplace(key, std::make_pair(d_ids, std::vector<int64_t>(counts, counts + 4)));
+        gm->list_md[key] = mx;
+      }
       if (!cached(gm, key)) {
         if ((rc = ensure_host_mirror(ctx, gm))) return rc;
         for (int64_t v = 0; v < g->n; ++v)
@@ -1214,6 +1257,13 @@ static int dispatch_pattern(xgm_ctx *ctx, const xgm_dgraph *g, int pattern, cons
       }
       if (strategy != XGM_STRAT_AUTO && strategy != XGM_STRAT_HASH)
         return ctx->fail(XGM_EUNSUPPORTED, "strategy not available for CYCLE4_EDGE");
+      if (!gm->relabelled && !host_prep(ctx, gm)) {
+        // the same relabelled CSR, order and wedge work, built in HBM; only
+        // the order, the work and the lower-neighbour counts come back
+        HIP_TRY(ctx, relabel_by_degree_device(dg, g->d_slot, ctx->num_cu, ctx->stream, &gm->d_rl_rowptr,
+                                              &gm->d_rl_colidx, gm->rl_order, gm->rl_work, gm->rl_nlow));
+        gm->relabelled = true;
+      }
       if (!gm->relabelled) {
         if ((rc = ensure_host_mirror(ctx, gm))) return rc;
         xgm_csr hv{g->n, g->nnz, gm->h_rowptr.data(), gm->h_colidx.data(), g->flags};
@@ -1255,6 +1305,12 @@ static int dispatch_pattern(xgm_ctx *ctx, const xgm_dgraph *g, int pattern, cons
       if (!cached(gm, wkey) || !cached(gm, hkey) || !cached(gm, rkey)) {
         const auto &rp = gm->rl_rowptr;
         const auto &ci = gm->rl_colidx;
+        // |N(x) ∩ [0, x)| in the relabelled ids
+        auto nlow_of = [&](int32_t x) -> int64_t {
+          if (!gm->rl_nlow.empty()) return gm->rl_nlow[x];
+          const int32_t *nx = ci.data() + rp[x];
+          return std::lower_bound(nx, nx + (rp[x + 1] - rp[x]), x) - nx;
+        };
         std::vector<int32_t> hubs;
         int64_t range_nlow = 0;
         for (int32_t x : gm->rl_order) {
@@ -1262,8 +1318,7 @@ static int dispatch_pattern(xgm_ctx *ctx, const xgm_dgraph *g, int pattern, cons
           const bool forced = rmode == 1 || rmode == 3;
           if (rmode != 2 && w > (forced ? 512 : 4096)) {
             const int64_t lo = gm->h_slot.empty() ? 0 : gm->inst_base[gm->h_slot[x]];
-            const int32_t *nx = ci.data() + rp[x];
-            const int64_t nlow = std::lower_bound(nx, nx + (rp[x + 1] - rp[x]), x) - nx;
+            const int64_t nlow = nlow_of(x);
             const bool half = rmode != 3 && nlow < 65536;
             const int64_t span = half ? 2 * (int64_t)kRangeCounters : kRangeCounters;
             const int64_t P = (x - lo + span - 1) / span;
@@ -1280,27 +1335,36 @@ static int dispatch_pattern(xgm_ctx *ctx, const xgm_dgraph *g, int pattern, cons
           if (w > kHubWork) {
             hubs.push_back(x);
           } else if (w <= 128) {
-            const int32_t *nx = ci.data() + rp[x];
-            const int64_t nlow = std::lower_bound(nx, nx + (rp[x + 1] - rp[x]), x) - nx;
-            parts[nlow <= 64 ? 3 : 0].push_back(x);
+            parts[nlow_of(x) <= 64 ? 3 : 0].push_back(x);
           } else {
             parts[w <= 512 ? 0 : w <= 4096 ? 1 : 2].push_back(x);
           }
         }
         const int64_t R = std::min<int64_t>((int64_t)hubs.size(), rows_max);
         if (R > 0) hubp.assign((hubs.size() + R - 1) / R, {});
+        // per hub x and lower neighbour v_k: |N(v_k) ∩ [0, x)|, from the host
+        // copy or, for a CSR relabelled in HBM, counted there
+        std::vector<int64_t> hoff(hubs.size() + 1, 0);
+        for (size_t h = 0; h < hubs.size(); ++h) hoff[h + 1] = hoff[h] + nlow_of(hubs[h]);
+        std::vector<int32_t> hcnt;
+        if (!hubs.empty() && rp.empty())
+          HIP_TRY(ctx, cycle4_hub_counts_device(gm->d_rl_rowptr, gm->d_rl_colidx, hubs, hoff, ctx->num_cu,
+                                                ctx->stream, hcnt));
         for (size_t h = 0; h < hubs.size(); ++h) {
           const int32_t x = hubs[h];
           auto &P = hubp[h / R];
           const int32_t slot = (int32_t)(h % R);
-          const int32_t *nx = ci.data() + rp[x];
-          const int nlow = (int)(std::lower_bound(nx, nx + (rp[x + 1] - rp[x]), x) - nx);
+          const int nlow = (int)(hoff[h + 1] - hoff[h]);
           int64_t acc = 0;
           int k0 = 0;
           for (int k = 0; k < nlow; ++k) {
-            const int32_t v = nx[k];
-            const int32_t *nv = ci.data() + rp[v];
-            acc += std::lower_bound(nv, nv + (rp[v + 1] - rp[v]), x) - nv;
+            if (rp.empty()) {
+              acc += hcnt[hoff[h] + k];
+            } else {
+              const int32_t v = ci[rp[x] + k];
+              const int32_t *nv = ci.data() + rp[v];
+              acc += std::lower_bound(nv, nv + (rp[v + 1] - rp[v]), x) - nv;
+            }
             if (acc >= kSlice || k + 1 - k0 >= 4096) {
               P.insert(P.end(), {x, k0, k + 1, slot});
               k0 = k + 1;
@@ -1625,14 +1689,11 @@ int xgm_motifs(xgm_ctx *ctx, const xgm_dgraph *g, int k, uint64_t *out) {
   int rc = ensure_dag(ctx, gm);
   if (rc) return rc;
   std::vector<uint64_t> tri(ni), k4(ni), c4(ni);
-  if ((rc = ensure_host_mirror(ctx, gm))) return rc;
   if (k == 3) {
     // wedges = sum_v d(d-1) / 2 - 3 * triangles (src/motif/omp_formula.cc:38-39)
     std::vector<u64> dd(ni, 0);
-    for (int64_t v = 0; v < g->n; ++v) {
-      const u64 d = (u64)(g->h_rowptr[v + 1] - g->h_rowptr[v]);
-      dd[g->d_slot ? g->h_slot[v] : 0] += d * (d - 1);
-    }
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    HIP_TRY(ctx, degree_pair_sums_device(dev_view(g), g->d_slot, ni, ctx->stream, dd.data()));
     if ((rc = count_into(ctx, gm->dag.get(), XGM_TRIANGLE, nullptr, tri.data(), true))) return rc;
     for (int i = 0; i < ni; ++i) {
       out[2 * i] = dd[i] / 2 - 3 * tri[i];
