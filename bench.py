@@ -366,6 +366,49 @@ def bench_prep(args, xgm, g, m, desc):
     ctx.close()
 
 
+def bench_prep_undirected(args, xgm, g, m, desc):
+    """First counts of the undirected patterns (diamonds, 4-cycles, the
+    4-motif census) from a fresh upload, with the task lists built on the
+    host (XGM_PREP_HOST: the CSR copied back, the host builders) and in HBM
+    (XGM_PREP_DEVICE), beside the next count, which finds them cached.  Wall
+    time per leg, median over --steps after --warmup; both routes must give
+    the same counts."""
+    ctx = xgm.Context(0)
+    pats = [("diamond", lambda dg: ctx.count(dg, PATTERN_IDS["diamond"])),
+            ("rectangle", lambda dg: ctx.count(dg, PATTERN_IDS["rectangle"])),
+            ("motif4", lambda dg: tuple(ctx.motifs(dg, 4)[0].tolist()))]
+    legs = {}
+    for i in range(args.warmup + args.steps):
+        for name, f in pats:
+            res = {}
+            for route, rname in ((xgm.PREP_HOST, "host"), (xgm.PREP_DEVICE, "device")):
+                ctx.set_prep(route)
+                dg = ctx.upload(g)
+                t0 = time.perf_counter()
+                res[rname] = f(dg)
+                t1 = time.perf_counter()
+                f(dg)
+                t2 = time.perf_counter()
+                dg.close()
+                log(f"step {i} {name} {rname} route: first {1e3 * (t1 - t0):.1f} ms, cached {1e3 * (t2 - t1):.1f} ms")
+                if i >= args.warmup:
+                    legs.setdefault(f"{name}_{rname}_route_first", []).append((t1 - t0) * 1e3)
+                    legs.setdefault(f"{name}_cached", []).append((t2 - t1) * 1e3)
+            if res["host"] != res["device"]:
+                raise SystemExit(f"{name}: the routes differ: {res['host']} vs {res['device']}")
+    ctx.set_prep(xgm.PREP_AUTO)
+    med = {k: round(float(np.median(v)), 3) for k, v in legs.items()}
+    print(json.dumps({
+        "metric": f"first count of the undirected patterns on {args.workload}, ms (task lists built in HBM)",
+        "value": med["rectangle_device_route_first"], "unit": "ms", "higher_is_better": False, "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup,
+        "config": {"workload": desc, "n": g.n, "undirected_edges": m},
+        "legs_ms": med,
+        "host_threads": int(os.environ.get("XGM_NUM_THREADS", 0)) or (os.cpu_count() or 1),
+    }), flush=True)
+    ctx.close()
+
+
 def bench_motif(args, xgm, g, m, desc, gold_counts):
     """Vertex-induced 3- or 4-motif census of the undirected workload graph
     (xgm_motifs: the reference's formula path).  Wall time per census over
@@ -1506,6 +1549,8 @@ def bench_single(args, xgm):
     if args.sched == "compare":
         return bench_sched(args, xgm, d, m, desc, golden)
     if args.prep == "compare":
+        if on != "dag":
+            return bench_prep_undirected(args, xgm, g, m, desc)
         return bench_prep(args, xgm, g, m, desc)
     if args.pattern.startswith("motif"):
         return bench_motif(args, xgm, g, m, desc, gold.get(gname, {}).get("counts", {}) if golden_hash_ok else {})

@@ -55,11 +55,14 @@ enum xgm_pattern {
    *   XGM_DIAMOND_V    <- GLUMIN --patternID 18 (local id 2, P2; P2-GM.yml.cuh).  GLUMIN's P2
    *                       walks both directions of every edge and reports twice this count.
    *   XGM_TAILEDTRI_V  <- GLUMIN --patternID 19 (local id 3, P3; P3-GM.yml.cuh)
-   * Strategies: AUTO / LUT -- LDS LUT over N(v0) with AND-NOT row popcounts for
+   * Strategies: LUT -- LDS LUT over N(v0) with AND-NOT row popcounts for
    * degree <= 1024 (GM_LUT_kernels/P1.yml.cuh, codegen_bitmap.cuh:392-407), set
    * difference and hybrid merge / binary-search intersections above; LIST --
    * the set-difference path for every vertex.  The enumeration is cubic in the
-   * degree: xgm_motifs (the formula path) is the census for graphs with hubs. */
+   * degree, so AUTO is LUT while every vertex has at most 1024 neighbours and
+   * otherwise the census identities of xgm_motifs (src/motif/omp_formula.cc:
+   * 40-45: per-edge sums of the motif edge kernel and the 4-cliques of the
+   * degree-oriented copy).  All give the same counts. */
   XGM_STAR3_V = 6,
   XGM_DIAMOND_V = 7,
   XGM_TAILEDTRI_V = 8

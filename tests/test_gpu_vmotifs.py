@@ -17,7 +17,9 @@ pytestmark = pytest.mark.gpu
 
 # census order of oracle.motif(k=4): 3-stars, 4-paths, tailed, 4-cycles, diamonds, 4-cliques
 VM = {"star": (6, 0), "diamond": (7, 4), "tailed": (8, 2)}
-STRATS = {"auto": 0, "list": 4}
+# auto: LUT kernels, or the census identities when a task lies above 1024
+# neighbours; lut: LUT kernels with the set-difference list path above 1024
+STRATS = {"auto": 0, "lut": 3, "list": 4}
 
 
 def _say(*a):
@@ -115,3 +117,23 @@ def test_vmotif_rejects_dag_and_solver_entry(xgm, ctx):
         ctx.count(d, 6)
     g = xgm.load_csr(CITESEER)
     assert [xgm.PatternSolver(g, k) for k in (1, 2, 3, 4)] == [222630, 2200, 22900, 255]
+
+
+def test_vmotif_auto_route(xgm, orc, ctx):
+    """XGM_STRAT_AUTO counts graphs whose tasks fit the LUT tiers with the LUT
+    kernels, and graphs with a task above 1024 neighbours by the census
+    identities; both equal the kernels' counts and the oracle's census."""
+    _say("vmotif auto route")
+    small = ctx.upload(xgm.load_csr(CITESEER))  # largest degree 99
+    for name, (pid, _) in VM.items():
+        ctx.count(small, pid)
+        assert any(k.name.startswith("vm_") for k in ctx.last_profile()[1]), name
+    small.close()
+    g = _hub_graph(xgm, 75, [50, 300, 1500, 2500])
+    census = orc.motif(g.rowptr, g.colidx, 4)
+    dg = ctx.upload(g)
+    for name, (pid, ci) in VM.items():
+        auto = ctx.count(dg, pid)
+        assert not any(k.name.startswith("vm_") for k in ctx.last_profile()[1]), name
+        assert auto == ctx.count(dg, pid, STRATS["lut"]) == census[ci], name
+    dg.close()

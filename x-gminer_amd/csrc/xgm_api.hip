@@ -1079,6 +1079,44 @@ static int diamond_lists(xgm_ctx *ctx, xgm_dgraph *gm, std::vector<TaskList> &tl
   return cache_parts(ctx, gm, key, parts, 4, tl);
 }
 
+// Vertex-induced motif tasks (degree >= 3, by decreasing degree): LUT
+// tiers by degree (<= 64, <= 256, <= 1024) and the set-difference list path
+// above 1024, or every task in the list path (list_all); cached as
+// "vmotif" / "vmotif_list", with the largest list-task degree in list_md.
+static int vmotif_lists(xgm_ctx *ctx, xgm_dgraph *gm, bool list_all, std::vector<TaskList> &tl) {
+  const std::string key = list_all ? "vmotif_list" : "vmotif";
+  int rc = XGM_OK;
+  std::vector<int32_t> ord;
+  if (!cached(gm, key) && !host_prep(ctx, gm)) {  // the same lists, built in HBM
+    int32_t *d_ids = nullptr;
+    int64_t counts[4] = {}, mx = 0;
+    HIP_TRY(ctx, build_deg3_order_device(dev_view(gm), list_all, ctx->num_cu, ctx->stream, &d_ids, counts, &mx));
+    gm->task_cache.emplace(key, std::make_pair(d_ids, std::vector<int64_t>(counts, counts + 4)));
+    gm->list_md[key] = mx;
+  }
+  if (!cached(gm, key)) {
+    if ((rc = ensure_host_mirror(ctx, gm))) return rc;
+    for (int64_t v = 0; v < gm->n; ++v)
+      if (gm->h_rowptr[v + 1] - gm->h_rowptr[v] >= 3) ord.push_back((int32_t)v);
+    std::stable_sort(ord.begin(), ord.end(), [&](int32_t a, int32_t b) {
+      return gm->h_rowptr[a + 1] - gm->h_rowptr[a] > gm->h_rowptr[b + 1] - gm->h_rowptr[b];
+    });
+  }
+  const int64_t *hrp = gm->h_rowptr.data();
+  auto deg_of = [hrp](int32_t v) { return hrp[v + 1] - hrp[v]; };
+  const std::vector<int64_t> bounds = list_all ? std::vector<int64_t>{-1, -1, -1, INT64_MAX}
+                                               : std::vector<int64_t>{64, 256, 1024, INT64_MAX};
+  if ((rc = tasks_split(ctx, gm, key, ord, deg_of, bounds, tl))) return rc;
+  if (!ord.empty()) {
+    // largest degree of a list task bounds its difference / intersection slots
+    int64_t m = 0;
+    for (int32_t v : ord)
+      if (deg_of(v) > (list_all ? 2 : 1024)) m = std::max<int64_t>(m, deg_of(v));
+    gm->list_md[key] = m;
+  }
+  return XGM_OK;
+}
+
 // Enqueue the kernels of one pattern count on ctx->stream (no host sync,
 // no D2H): task lists, scratch and relabelled copies are built and cached
 // on first use, so a second call enqueues kernels only (capturable).
@@ -1188,36 +1226,9 @@ static int dispatch_pattern(xgm_ctx *ctx, const xgm_dgraph *g, int pattern, cons
       // above 1024 (every task under XGM_STRAT_LIST)
       const bool list_all = strategy == XGM_STRAT_LIST;
       const std::string key = list_all ? "vmotif_list" : "vmotif";
-      std::vector<int32_t> ord;
-      int md = 0;
-      if (!cached(gm, key) && !host_prep(ctx, gm)) {  // the same lists, built in HBM
-        int32_t *d_ids = nullptr;
-        int64_t counts[4] = {}, mx = 0;
-        HIP_TRY(ctx, build_deg3_order_device(dg, list_all, ctx->num_cu, ctx->stream, &d_ids, counts, &mx));
-        gm->task_cache.emplace(key, std::make_pair(d_ids, std::vector<int64_t>(counts, counts + 4)));
-        gm->list_md[key] = mx;
-      }
-      if (!cached(gm, key)) {
-        if ((rc = ensure_host_mirror(ctx, gm))) return rc;
-        for (int64_t v = 0; v < g->n; ++v)
-          if (gm->h_rowptr[v + 1] - gm->h_rowptr[v] >= 3) ord.push_back((int32_t)v);
-        std::stable_sort(ord.begin(), ord.end(), [&](int32_t a, int32_t b) {
-          return gm->h_rowptr[a + 1] - gm->h_rowptr[a] > gm->h_rowptr[b + 1] - gm->h_rowptr[b];
-        });
-      }
-      const int64_t *hrp = gm->h_rowptr.data();
-      auto deg_of = [hrp](int32_t v) { return hrp[v + 1] - hrp[v]; };
-      const std::vector<int64_t> bounds = list_all ? std::vector<int64_t>{-1, -1, -1, INT64_MAX}
-                                                   : std::vector<int64_t>{64, 256, 1024, INT64_MAX};
-      if ((rc = tasks_split(ctx, gm, key, ord, deg_of, bounds, tl))) return rc;
+      if ((rc = vmotif_lists(ctx, gm, list_all, tl))) return rc;
       TaskList t4[4] = {tl[0], tl[1], tl[2], tl[3]};
-      if (!ord.empty()) {
-        // largest degree of a list task bounds its difference / intersection slots
-        int64_t m = 0;
-        for (int32_t v : ord)
-          if (deg_of(v) > (list_all ? 2 : 1024)) m = std::max<int64_t>(m, deg_of(v));
-        gm->list_md[key] = m;
-      }
+      int md = 0;
       if (t4[3].count) {
         md = (int)gm->list_md[key];
         const size_t need = (size_t)vmotif_list_grid(ctx->num_cu) * 8 * (size_t)std::max(1, md);
@@ -1421,6 +1432,7 @@ static int dispatch_pattern(xgm_ctx *ctx, const xgm_dgraph *g, int pattern, cons
 
 static int count_into(xgm_ctx *ctx, const xgm_dgraph *g, int pattern, const xgm_opts *opts, uint64_t *out,
                       bool per_inst);
+static int vmotif_formula(xgm_ctx *ctx, xgm_dgraph *gm, int pattern, u64 *out);
 
 int xgm_count(xgm_ctx *ctx, const xgm_dgraph *g, int pattern, const xgm_opts *opts, uint64_t *out) {
   return count_into(ctx, g, pattern, opts, out, false);
@@ -1540,11 +1552,24 @@ static int count_into(xgm_ctx *ctx, const xgm_dgraph *g, int pattern, const xgm_
   if (g->ctx != ctx) return ctx->fail(XGM_EINVAL, "graph belongs to another context");
   if (opts && opts->flags != 0) return ctx->fail(XGM_EINVAL, "xgm_opts.flags must be 0");
   HIP_TRY(ctx, hipSetDevice(ctx->device));
+  const int ni = g->d_slot ? g->n_inst : 1;
+  const bool vmotif = pattern == XGM_STAR3_V || pattern == XGM_DIAMOND_V || pattern == XGM_TAILEDTRI_V;
+  if (vmotif && (!opts || opts->strategy == XGM_STRAT_AUTO) && (g->flags & XGM_CSR_SYMMETRIC)) {
+    xgm_dgraph *gm = const_cast<xgm_dgraph *>(g);
+    std::vector<TaskList> tl;
+    if (int rc = vmotif_lists(ctx, gm, false, tl)) return rc;
+    if (tl[3].count) {  // tasks above the LUT tiers: the census identities
+      std::vector<u64> h(ni);
+      if (int rc = vmotif_formula(ctx, gm, pattern, h.data())) return rc;
+      if (per_inst) std::memcpy(out, h.data(), ni * sizeof(u64));
+      else out[0] = std::accumulate(h.begin(), h.end(), (u64)0);
+      return XGM_OK;
+    }
+  }
   ctx->prof = xgm_profile{};
   HIP_TRY(ctx, launch_zero(ctx->d_out, kScratchU64, ctx->stream));
   HIP_TRY(ctx, launch_zero(ctx->d_queues, kScratchQueues / 2, ctx->stream));
   Launch l = make_launch(ctx, 0);
-  const int ni = g->d_slot ? g->n_inst : 1;
   if (g->d_slot) {
     HIP_TRY(ctx, launch_zero(g->d_inst_out, ni, ctx->stream));
     l.d_out = g->d_inst_out;
@@ -1679,6 +1704,52 @@ static int ensure_dag(xgm_ctx *ctx, xgm_dgraph *g) {
   return XGM_OK;
 }
 
+// Per instance: the sums S0..S3 over the edges of the motif edge kernel
+// (S[4 i .. 4 i + 3]) and the 4-cliques of the degree-oriented copy.
+static int motif4_sums(xgm_ctx *ctx, xgm_dgraph *gm, std::vector<u64> &S, std::vector<uint64_t> &k4) {
+  const int ni = gm->d_slot ? gm->n_inst : 1;
+  int rc = ensure_dag(ctx, gm);
+  if (rc) return rc;
+  // per-edge sums on the GPU (motif mode of the diamond kernels)
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  if (!gm->d_motif) HIP_TRY(ctx, hipMalloc(&gm->d_motif, 4 * (size_t)ni * sizeof(u64)));
+  std::vector<TaskList> tl;
+  if ((rc = diamond_lists(ctx, gm, tl, true))) return rc;
+  ctx->prof = xgm_profile{};
+  HIP_TRY(ctx, launch_zero(ctx->d_out, kScratchU64, ctx->stream));
+  HIP_TRY(ctx, launch_zero(ctx->d_queues, kScratchQueues / 2, ctx->stream));
+  HIP_TRY(ctx, launch_zero(gm->d_motif, 4 * (size_t)ni, ctx->stream));
+  Launch l = make_launch(ctx, 0);
+  l.d_out = gm->d_motif;
+  l.slots = gm->d_slot;
+  TaskList t4[4] = {tl[0], tl[1], tl[2], tl[3]};
+  HIP_TRY(ctx, launch_motif_edges(dev_view(gm), t4, l, l.d_queue));
+  S.assign(4 * (size_t)ni, 0);
+  k4.assign(ni, 0);
+  HIP_TRY(ctx, hipMemcpyAsync(S.data(), gm->d_motif, S.size() * sizeof(u64), hipMemcpyDeviceToHost, ctx->stream));
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  return count_into(ctx, gm->dag.get(), XGM_CLIQUE4, nullptr, k4.data(), true);
+}
+
+// Vertex-induced 3-stars, diamonds or tailed triangles per instance by the
+// census identities (src/motif/omp_formula.cc:40-45): induced diamonds =
+// S3 / 2 - 6 K4, tailed triangles = S2 / 2 - 2 diamonds, 3-stars = S0 / 6 -
+// tailed / 3.  XGM_STRAT_AUTO takes it when a task lies above the LUT
+// tiers (degree > 1024), where the set-difference list path grows with the
+// cube of the degree.
+static int vmotif_formula(xgm_ctx *ctx, xgm_dgraph *gm, int pattern, u64 *out) {
+  std::vector<u64> S;
+  std::vector<uint64_t> k4;
+  if (int rc = motif4_sums(ctx, gm, S, k4)) return rc;
+  for (size_t i = 0; i < k4.size(); ++i) {
+    const u64 *s4 = &S[4 * i];
+    const u64 dia = s4[3] / 2 - 6 * k4[i];
+    const u64 tail = s4[2] / 2 - 2 * dia;
+    out[i] = pattern == XGM_DIAMOND_V ? dia : pattern == XGM_TAILEDTRI_V ? tail : s4[0] / 6 - tail / 3;
+  }
+  return XGM_OK;
+}
+
 int xgm_motifs(xgm_ctx *ctx, const xgm_dgraph *g, int k, uint64_t *out) {
   if (!ctx || !g || !out) return XGM_EINVAL;
   if (g->ctx != ctx) return ctx->fail(XGM_EINVAL, "graph belongs to another context");
@@ -1701,24 +1772,8 @@ int xgm_motifs(xgm_ctx *ctx, const xgm_dgraph *g, int k, uint64_t *out) {
     }
     return XGM_OK;
   }
-  // per-edge sums S0..S3 on the GPU (motif mode of the diamond kernels)
-  HIP_TRY(ctx, hipSetDevice(ctx->device));
-  if (!gm->d_motif) HIP_TRY(ctx, hipMalloc(&gm->d_motif, 4 * (size_t)ni * sizeof(u64)));
-  std::vector<TaskList> tl;
-  if ((rc = diamond_lists(ctx, gm, tl, true))) return rc;
-  ctx->prof = xgm_profile{};
-  HIP_TRY(ctx, launch_zero(ctx->d_out, kScratchU64, ctx->stream));
-  HIP_TRY(ctx, launch_zero(ctx->d_queues, kScratchQueues / 2, ctx->stream));
-  HIP_TRY(ctx, launch_zero(gm->d_motif, 4 * (size_t)ni, ctx->stream));
-  Launch l = make_launch(ctx, 0);
-  l.d_out = gm->d_motif;
-  l.slots = g->d_slot;
-  TaskList t4[4] = {tl[0], tl[1], tl[2], tl[3]};
-  HIP_TRY(ctx, launch_motif_edges(dev_view(g), t4, l, l.d_queue));
-  std::vector<u64> S(4 * (size_t)ni);
-  HIP_TRY(ctx, hipMemcpyAsync(S.data(), gm->d_motif, S.size() * sizeof(u64), hipMemcpyDeviceToHost, ctx->stream));
-  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
-  if ((rc = count_into(ctx, gm->dag.get(), XGM_CLIQUE4, nullptr, k4.data(), true))) return rc;
+  std::vector<u64> S;
+  if ((rc = motif4_sums(ctx, gm, S, k4))) return rc;
   if ((rc = count_into(ctx, gm, XGM_CYCLE4_EDGE, nullptr, c4.data(), true))) return rc;
   // solve for the vertex-induced counts (src/motif/omp_formula.cc:40-45); the
   // induced 4-cycles come from the edge-induced count:
