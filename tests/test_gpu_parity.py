@@ -203,6 +203,33 @@ def test_sgl_hub_star_plus_clique(xgm, ctx, orc):
         assert ctx.count(g, pid) == want, name
 
 
+def test_sgl_binned_hub_sets(xgm, ctx, orc):
+    """Owners above 8192 neighbours search N(t) as a binned LDS set (2^14 bins
+    of 2^6 ids for 2^20 vertices): hubs of 50,000 random neighbours, of 20,000
+    neighbours packed into 59,000 ids (about 22 per bin), and of 60,000 (above
+    the binned capacity: the sample index over HBM), with a community among
+    the hub neighbours so partners share neighbours with the hub; diamonds,
+    4-cycles and the 4-motif census (the same kernel in motif mode) against
+    the oracle."""
+    import numpy as np
+    _say("sgl binned hubs")
+    rng = np.random.default_rng(11)
+    n = 1 << 20
+    h0 = rng.choice(np.arange(3, n), 50000, replace=False)
+    h1 = rng.choice(np.arange(1000, 60000), 20000, replace=False)
+    h2 = rng.choice(np.arange(3, n), 60000, replace=False)
+    comm = np.concatenate([h0[:2000], h1[:2000], h2[:1000]])
+    src = np.concatenate([rng.integers(0, n, 300000), np.zeros(50000, np.int64), np.ones(20000, np.int64),
+                          np.full(60000, 2), rng.choice(comm, 40000)])
+    dst = np.concatenate([rng.integers(0, n, 300000), h0, h1, h2, rng.choice(comm, 40000)])
+    g = xgm.from_edges(n, src, dst)
+    dg = ctx.upload(g)
+    for name, pid in SGL.items():
+        assert ctx.count(dg, pid) == orc.count(name, g.rowptr, g.colidx)[0], name
+    assert ctx.motifs(dg, 4)[0].tolist() == list(orc.motif(g.rowptr, g.colidx, 4))
+    dg.close()
+
+
 @pytest.mark.parametrize("seed", [6, 7])
 def test_sgl_random_vs_oracle(xgm, orc, ctx, seed):
     _say("sgl random", seed)

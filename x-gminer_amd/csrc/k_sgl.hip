@@ -158,26 +158,38 @@ __global__ __launch_bounds__(NT) void k_diamond(const int64_t *__restrict__ rowp
   }
 }
 
-// Owners of degree > 8192 (hub slices): N(t) is staged sorted in LDS when
-// it fits (d <= LCAP) and binary-searched there; above that an LDS index of
-// 1024 samples narrows each search in HBM to one d/1024-element stretch.
+// Owners of degree > 8192 (hub slices).  N(t) is staged in LDS as a binned
+// sorted set when ids fit 16 + kDiaBinBits bits and d <= kDiaBinCap: the id
+// space is cut into 2^kDiaBinBits bins of 2^shift ids, first[b] is the
+// position of the first element of bin b, and an element keeps only its low
+// shift bits (uint16), so a lookup reads first[b], first[b + 1] and searches
+// the few elements of its bin.  Otherwise N(t) is staged sorted (d <= LCAP)
+// and binary-searched, or, above that, an LDS index of 1024 samples narrows
+// each search in HBM to one d/1024-element stretch.
+constexpr int kDiaBinBits = 14, kDiaBins = 1 << kDiaBinBits, kDiaBinCap = 56000;
 template <int LCAP, int PCAP, int NT>
 struct DiaBigStage {
   union {
     int list[LCAP];
     int sample[1024];
+    struct {
+      uint16_t first[kDiaBins + 2];
+      uint16_t low[kDiaBinCap];
+    } bin;
   };
   Segs<PCAP, false> seg;
   int cnt[PCAP];
   int ns, L, task;
 };
+static_assert(sizeof(((DiaBigStage<36352, 1000, 1024> *)nullptr)->bin) <= 36352 * sizeof(int),
+              "the binned set shares the sorted list's LDS");
 
 template <int LCAP, int PCAP, int NT, bool MOTIF>
 __global__ __launch_bounds__(NT) void k_diamond_big(const int64_t *__restrict__ rowptr,
                                                     const int32_t *__restrict__ colidx,
                                                     const int4 *__restrict__ items, int ntasks,
                                                     int *__restrict__ queue, u64 *__restrict__ out,
-                                                    const int32_t *__restrict__ slots) {
+                                                    const int32_t *__restrict__ slots, int shift) {
   __shared__ DiaBigStage<LCAP, PCAP, NT> S;
   __shared__ int flagrow[NT / 64][64];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -195,13 +207,44 @@ __global__ __launch_bounds__(NT) void k_diamond_big(const int64_t *__restrict__ 
     const int64_t bt = rowptr[v];
     const int dt = (int)(rowptr[v + 1] - bt);
     const int32_t *nt = colidx + bt;
-    const bool in_lds = dt <= LCAP;
-    if (in_lds) {
+    const bool binned = shift <= 16 && dt <= kDiaBinCap;
+    const bool in_lds = !binned && dt <= LCAP;
+    const int lowmask = (1 << shift) - 1;
+    if (binned) {
+      // bins before the first element start at 0, bins after the last at dt
+      const int b0 = dt ? nt[0] >> shift : kDiaBins;
+      const int bl = dt ? nt[dt - 1] >> shift : -1;
+      for (int b = tid; b <= b0; b += NT) S.bin.first[b] = 0;
+      for (int b = bl + 1 + tid; b <= kDiaBins; b += NT) S.bin.first[b] = (uint16_t)dt;
+      for (int q = tid; q < dt; q += NT) {
+        const int w = nt[q];
+        S.bin.low[q] = (uint16_t)(w & lowmask);
+        if (q > 0)  // element q opens the bins (bin of q-1, bin of q]
+          for (int b = (nt[q - 1] >> shift) + 1; b <= (w >> shift); ++b) S.bin.first[b] = (uint16_t)q;
+      }
+    } else if (in_lds) {
       for (int q = tid; q < dt; q += NT) S.list[q] = nt[q];
     } else {
       for (int q = tid; q < 1024; q += NT) S.sample[q] = nt[(int64_t)q * dt / 1024];
     }
     auto contains = [&](int w) {
+      if (binned) {
+        const int b = w >> shift;
+        int lo = S.bin.first[b];
+        const int hi = S.bin.first[b + 1];
+        const int key = w & lowmask;
+        int n = hi - lo;  // lower bound of key in low[lo, hi)
+        while (n > 0) {
+          const int h = n >> 1;
+          if ((int)S.bin.low[lo + h] < key) {
+            lo += h + 1;
+            n -= h + 1;
+          } else {
+            n = h;
+          }
+        }
+        return lo < hi && (int)S.bin.low[lo] == key;
+      }
       if (in_lds) return sorted_contains(S.list, dt, w);
       // last sample <= w, then its stretch [j d / 1024, (j + 1) d / 1024) in HBM
       const int j = upper_bound_dev(S.sample, 1024, w) - 1;
@@ -932,12 +975,15 @@ hipError_t run_wedge_lds(const char *name, const DevCSR &g, const TaskList &tl, 
 template <bool MOTIF>
 hipError_t launch_diamond_mode(const DevCSR &g, const TaskList tl[4], Launch &l, int *d_queues) {
   hipError_t e;
-  if (tl[3].count) {  // owners above 8192: sorted set in LDS (36,352 ints), else a sample index
+  if (tl[3].count) {  // owners above 8192: binned set in LDS, sorted set (36,352 ints), or a sample index
     auto k = k_diamond_big<36352, 1000, 1024, MOTIF>;
     const int grid = grid_of(k, 1024, l.num_cu, tl[3].count);
     l.begin(MOTIF ? "motif_edges_hbm_set" : "diamond_hbm_set", tl[3].count);
+    // bins of 2^shift ids: ids < 2^idbits in 2^kDiaBinBits bins
+    const int idbits = g.n > 1 ? 64 - __builtin_clzll((unsigned long long)(g.n - 1)) : 1;
+    const int shift = std::max(0, idbits - kDiaBinBits);
     k<<<grid, 1024, 0, l.stream>>>(g.rowptr, g.colidx, (const int4 *)tl[3].tasks, (int)tl[3].count, d_queues + 3,
-                                   l.d_out, l.slots);
+                                   l.d_out, l.slots, shift);
     l.end();
     if ((e = hipGetLastError())) return e;
   }
