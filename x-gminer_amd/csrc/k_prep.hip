@@ -555,6 +555,42 @@ __global__ __launch_bounds__(kPrepThreads) void k_rl_work(const int64_t *__restr
   if (lane == 0 && pos) atomicAdd(npos, pos);
 }
 
+// cut[e] = |N(v) ∩ [0, x)| for every entry e of row x with v = ci[e] < x
+// (0 for the entries above x), nlow[x] = the number of entries below x.
+// One wave per row (spread order); the counting kernels read these instead
+// of binary-searching every lower list at every count.
+__global__ __launch_bounds__(kPrepThreads) void k_rl_cuts(const int64_t *__restrict__ rp,
+                                                          const int32_t *__restrict__ ci, int64_t n,
+                                                          int32_t *__restrict__ cut, int32_t *__restrict__ nlow,
+                                                          int64_t spread_p, u64 spread_m) {
+  const int lane = threadIdx.x & 63;
+  const int64_t nw = (int64_t)gridDim.x * (kPrepThreads / 64);
+  const int64_t i0 = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * (kPrepThreads / 64) + (threadIdx.x >> 6)));
+  for (int64_t i = i0; i < spread_p; i += nw) {
+    const int64_t x = spread_vertex(i, spread_p, spread_m);
+    if (x >= n) continue;
+    const int64_t b = rp[x];
+    const int d = (int)(rp[x + 1] - b);
+    int lo = 0;
+    for (int k0 = 0; k0 < d; k0 += 64) {
+      const int k = k0 + lane;
+      int c = 0;
+      bool below = false;
+      if (k < d) {
+        const int v = ci[b + k];
+        below = v < x;
+        if (below) {
+          const int64_t bv = rp[v];
+          c = lower_bound_dev(ci + bv, (int)(rp[v + 1] - bv), (int)x);
+        }
+        cut[b + k] = c;
+      }
+      lo += __popcll(__ballot(below));
+    }
+    if (lane == 0) nlow[x] = lo;
+  }
+}
+
 // cnt[hoff[h] + k] = |N(v_k) ∩ [0, x)| for the lower neighbours v_k of hub
 // x = hubs[h] (k < hoff[h + 1] - hoff[h]); one block per hub
 __global__ __launch_bounds__(kPrepThreads) void k_rl_hub_counts(const int64_t *__restrict__ rp,
@@ -977,6 +1013,32 @@ hipError_t relabel_by_degree_device(const DevCSR &g, const int32_t *slot, int nu
     return fail(e);
   *rl_rowptr = rp;
   *rl_colidx = ci;
+  return hipSuccess;
+}
+
+hipError_t build_rl_cuts_device(const int64_t *rl_rowptr, const int32_t *rl_colidx, int64_t n, int64_t nnz,
+                                int num_cu, hipStream_t s, int32_t **cut, int32_t **nlow) {
+  *cut = nullptr;
+  *nlow = nullptr;
+  void *c = nullptr, *l = nullptr;
+  hipError_t e;
+  if ((e = hipMalloc(&c, std::max<int64_t>(1, nnz) * sizeof(int32_t)))) return e;
+  if ((e = hipMalloc(&l, std::max<int64_t>(1, n) * sizeof(int32_t)))) {
+    (void)hipFree(c);
+    return e;
+  }
+  const int64_t sp = spread_prime(n);
+  const u64 sm = 2654435761ull % (u64)sp;
+  if (n)
+    hipLaunchKernelGGL(k_rl_cuts, dim3(prep_grid(n, kPrepThreads / 64, num_cu)), dim3(kPrepThreads), 0, s, rl_rowptr,
+                       rl_colidx, n, static_cast<int32_t *>(c), static_cast<int32_t *>(l), sp, sm);
+  if ((e = hipGetLastError()) || (e = hipStreamSynchronize(s))) {
+    (void)hipFree(c);
+    (void)hipFree(l);
+    return e;
+  }
+  *cut = static_cast<int32_t *>(c);
+  *nlow = static_cast<int32_t *>(l);
   return hipSuccess;
 }
 

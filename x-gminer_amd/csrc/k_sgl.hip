@@ -477,6 +477,8 @@ __device__ __forceinline__ int hash_bump(int *key, int *val, int bits, int w) {
 template <int HSLOTS, int PCAP, int NT, bool DENSE>
 __global__ __launch_bounds__(NT) void k_cycle4_wedge(const int64_t *__restrict__ rowptr,
                                                      const int32_t *__restrict__ colidx,
+                                                     const int32_t *__restrict__ cut,
+                                                     const int32_t *__restrict__ nlowv,
                                                      const int32_t *__restrict__ tasks, int ntasks,
                                                      int *__restrict__ dense, int64_t stride,
                                                      const int64_t *__restrict__ inst_lo,
@@ -498,7 +500,7 @@ __global__ __launch_bounds__(NT) void k_cycle4_wedge(const int64_t *__restrict__
     u64 part = 0;
     const int64_t bx = rowptr[x];
     const int32_t *nx = colidx + bx;
-    const int nlow = lower_bound_dev(nx, (int)(rowptr[x + 1] - bx), x);  // |N(x) ∩ [0, x)|
+    const int nlow = nlowv[x];  // |N(x) ∩ [0, x)|
     int *cnt = DENSE ? row - (inst_lo ? inst_lo[slots[x]] : 0) : nullptr;
     int hb = 0;
     if (!DENSE) {
@@ -509,7 +511,7 @@ __global__ __launch_bounds__(NT) void k_cycle4_wedge(const int64_t *__restrict__
     for (int k0 = 0; k0 < nlow; k0 += PCAP) {
       const int k1 = min(nlow, k0 + PCAP);
       __syncthreads();
-      block_segments<NT>(S.seg, S.tmp, rowptr, colidx, nx, k0, k1, x, KeepAll(), &S.ns, &S.L);
+      block_segments_cut<NT>(S.seg, S.tmp, rowptr, nx, cut + bx, k0, k1, &S.ns, &S.L);
       const int ns = S.ns, L = S.L;
       int lo, hi;
       wave_share(L, wave, NT / 64, lo, hi);
@@ -526,7 +528,7 @@ __global__ __launch_bounds__(NT) void k_cycle4_wedge(const int64_t *__restrict__
       for (int k0 = 0; k0 < nlow; k0 += PCAP) {
         const int k1 = min(nlow, k0 + PCAP);
         __syncthreads();
-        block_segments<NT>(S.seg, S.tmp, rowptr, colidx, nx, k0, k1, x, KeepAll(), &S.ns, &S.L);
+        block_segments_cut<NT>(S.seg, S.tmp, rowptr, nx, cut + bx, k0, k1, &S.ns, &S.L);
         const int ns = S.ns, L = S.L;
         int lo, hi;
         wave_share(L, wave, NT / 64, lo, hi);
@@ -561,6 +563,8 @@ struct WedgePack {
 template <int G>
 __global__ __launch_bounds__(128) void k_cycle4_wedge_packed(const int64_t *__restrict__ rowptr,
                                                              const int32_t *__restrict__ colidx,
+                                                             const int32_t *__restrict__ cut,
+                                                             const int32_t *__restrict__ nlowv,
                                                              const int32_t *__restrict__ tasks, int ntasks,
                                                              int *__restrict__ queue, u64 *__restrict__ out,
                                                              const int32_t *__restrict__ slots) {
@@ -582,7 +586,7 @@ __global__ __launch_bounds__(128) void k_cycle4_wedge_packed(const int64_t *__re
     if (lane < ng) {
       const int x = tasks[base + lane];
       const int64_t b = rowptr[x];
-      nl = lower_bound_dev(colidx + b, (int)(rowptr[x + 1] - b), x);  // |N(x) ∩ [0, x)| <= 64
+      nl = nlowv[x];  // |N(x) ∩ [0, x)| <= 64
       S.x[lane] = x;
       S.bx[lane] = b;
       S.nl[lane] = nl;
@@ -611,9 +615,9 @@ __global__ __launch_bounds__(128) void k_cycle4_wedge_packed(const int64_t *__re
       int64_t vb = 0;
       if (q < P) {
         g = item_of(q);
-        const int v = colidx[S.bx[g] + (q - S.off[g])];
-        vb = rowptr[v];
-        len = lower_bound_dev(colidx + vb, (int)(rowptr[v + 1] - vb), S.x[g]);
+        const int64_t e = S.bx[g] + (q - S.off[g]);
+        vb = rowptr[colidx[e]];
+        len = cut[e];  // |N(v) ∩ [0, x_g)|
       }
       const int incl = wave_incl_scan(len, lane) + L;
       const u64 m = __ballot(len > 0);
@@ -698,6 +702,7 @@ __device__ __forceinline__ int gallop_lower_bound(const int32_t *__restrict__ a,
 template <int R, int PCAP, int NT>
 __global__ __launch_bounds__(NT) void k_cycle4_range(const int64_t *__restrict__ rowptr,
                                                      const int32_t *__restrict__ colidx,
+                                                     const int32_t *__restrict__ nlowv,
                                                      const int4 *__restrict__ items, int nitems,
                                                      int4 *__restrict__ cursor, int64_t cursor_stride,
                                                      int *__restrict__ queue, u64 *__restrict__ out,
@@ -720,7 +725,7 @@ __global__ __launch_bounds__(NT) void k_cycle4_range(const int64_t *__restrict__
     const int span = half ? 2 * R : R;
     const int64_t bx = rowptr[x];
     const int32_t *nx = colidx + bx;
-    const int nlow = lower_bound_dev(nx, (int)(rowptr[x + 1] - bx), x);
+    const int nlow = nlowv[x];
     u64 part = 0;
     for (int s0 = r0, s1 = r0; s0 < r1; s0 = s1) {  // s1 never passes r1: no int overflow near 2^31
       s1 = r1 - s0 > span ? s0 + span : r1;
@@ -825,6 +830,7 @@ __global__ __launch_bounds__(NT) void k_cycle4_range(const int64_t *__restrict__
 template <int PCAP, int NT>
 __global__ __launch_bounds__(NT) void k_cycle4_hub(const int64_t *__restrict__ rowptr,
                                                    const int32_t *__restrict__ colidx,
+                                                   const int32_t *__restrict__ cut,
                                                    const int4 *__restrict__ items, int nitems,
                                                    int *__restrict__ hub, int64_t stride,
                                                    const int64_t *__restrict__ inst_lo,
@@ -845,15 +851,20 @@ __global__ __launch_bounds__(NT) void k_cycle4_hub(const int64_t *__restrict__ r
     const int4 it = items[t];
     const int x = it.x;
     u64 part = 0;
-    const int32_t *nx = colidx + rowptr[x];
+    const int64_t bx = rowptr[x];
+    const int32_t *nx = colidx + bx;
     int *cnt = hub + (size_t)it.w * (size_t)stride - (inst_lo ? inst_lo[slots[x]] : 0);
     for (int k0 = it.y; k0 < it.z; k0 += PCAP) {
       const int k1 = min(it.z, k0 + PCAP);
       __syncthreads();
       if (wave == 0) {
-        int ns, L;
-        wave_segments(seg, rowptr, colidx, nx, k0, k1, x, KeepAll(), lane, ns, L);
-        if (lane == 0) { sh_ns = ns; sh_L = L; }
+        int ns = 0, L = 0;
+        wave_segments_range_cut(seg, rowptr, nx, cut + bx, k0, k1, lane, ns, L);
+        if (lane == 0) {
+          seg.os[ns] = make_int2(0, L);
+          sh_ns = ns;
+          sh_L = L;
+        }
       }
       __syncthreads();
       int lo, hi;
@@ -964,8 +975,8 @@ hipError_t run_wedge_lds(const char *name, const DevCSR &g, const TaskList &tl, 
   auto k = k_cycle4_wedge<HSLOTS, PCAP, NT, false>;
   int grid = grid_of(k, NT, l.num_cu, tl.count);
   l.begin(name, tl.count);
-  k<<<grid, NT, 0, l.stream>>>(g.rowptr, g.colidx, tl.tasks, (int)tl.count, nullptr, 0, nullptr, queue, l.d_out,
-                               l.slots);
+  k<<<grid, NT, 0, l.stream>>>(g.rowptr, g.colidx, g.cut, g.nlow, tl.tasks, (int)tl.count, nullptr, 0, nullptr,
+                               queue, l.d_out, l.slots);
   l.end();
   return hipGetLastError();
 }
@@ -1027,6 +1038,7 @@ int cycle4_range_grid(int num_cu, int64_t items) {
 hipError_t launch_cycle4_wedge(const DevCSR &g, const TaskList tl[4], int *dense, const TaskList *hub_rounds,
                                int n_rounds, int *hub, size_t hub_round_u64, const TaskList &range_items,
                                const CounterRows &rows, Launch &l, int *d_queues) {
+  if (!g.cut || !g.nlow) return hipErrorInvalidValue;  // the relabelled graph's cut positions
   hipError_t e;
   if (range_items.count) {
     // 2048-neighbour pieces: 128 KB of counters + 24 KB of segment tables
@@ -1034,7 +1046,7 @@ hipError_t launch_cycle4_wedge(const DevCSR &g, const TaskList tl[4], int *dense
     auto k = k_cycle4_range<kRangeCounters, 2048, 1024>;
     const int grid = cycle4_range_grid(l.num_cu, range_items.count);
     l.begin("cycle4_wedge_range", range_items.count);
-    k<<<grid, 1024, 0, l.stream>>>(g.rowptr, g.colidx, (const int4 *)range_items.tasks, (int)range_items.count,
+    k<<<grid, 1024, 0, l.stream>>>(g.rowptr, g.colidx, g.nlow, (const int4 *)range_items.tasks, (int)range_items.count,
                                    rows.cursor, rows.cursor_stride, d_queues + 6, l.d_out, l.slots);
     l.end();
     if ((e = hipGetLastError())) return e;
@@ -1050,7 +1062,7 @@ hipError_t launch_cycle4_wedge(const DevCSR &g, const TaskList tl[4], int *dense
       if ((e = launch_zero(hub, hub_round_u64, l.stream))) return e;
       if ((e = launch_zero(d_queues + 4, 1, l.stream))) return e;
       int grid = grid_of(k, 256, l.num_cu, it.count);
-      k<<<grid, 256, 0, l.stream>>>(g.rowptr, g.colidx, (const int4 *)it.tasks, (int)it.count, hub, rows.stride,
+      k<<<grid, 256, 0, l.stream>>>(g.rowptr, g.colidx, g.cut, (const int4 *)it.tasks, (int)it.count, hub, rows.stride,
                                     rows.inst_lo, d_queues + 4, l.d_out, l.slots);
       if ((e = hipGetLastError())) return e;
     }
@@ -1059,7 +1071,7 @@ hipError_t launch_cycle4_wedge(const DevCSR &g, const TaskList tl[4], int *dense
   if (tl[2].count) {
     int grid = cycle4_dense_grid(l.num_cu, tl[2].count);
     l.begin("cycle4_wedge_hbm", tl[2].count);
-    k_cycle4_wedge<1, 1024, 256, true><<<grid, 256, 0, l.stream>>>(g.rowptr, g.colidx, tl[2].tasks,
+    k_cycle4_wedge<1, 1024, 256, true><<<grid, 256, 0, l.stream>>>(g.rowptr, g.colidx, g.cut, g.nlow, tl[2].tasks,
                                                                     (int)tl[2].count, dense, rows.stride, rows.inst_lo,
                                                                     d_queues + 2, l.d_out, l.slots);
     l.end();
@@ -1074,8 +1086,8 @@ hipError_t launch_cycle4_wedge(const DevCSR &g, const TaskList tl[4], int *dense
     auto k = k_cycle4_wedge_packed<kPack>;
     const int grid = grid_of(k, 128, l.num_cu, (tl[3].count + 2 * kPack - 1) / (2 * kPack));
     l.begin("cycle4_wedge_packed", tl[3].count);
-    k<<<grid, 128, 0, l.stream>>>(g.rowptr, g.colidx, tl[3].tasks, (int)tl[3].count, d_queues + 7, l.d_out,
-                                  l.slots);
+    k<<<grid, 128, 0, l.stream>>>(g.rowptr, g.colidx, g.cut, g.nlow, tl[3].tasks, (int)tl[3].count, d_queues + 7,
+                                  l.d_out, l.slots);
     l.end();
     if ((e = hipGetLastError())) return e;
   }

@@ -23,6 +23,11 @@ struct DevCSR {
   // reverse (in-neighbour) lists of a DAG; null for undirected graphs
   const int64_t *rrowptr = nullptr;
   const int32_t *rcolidx = nullptr;
+  // (degree, id)-relabelled undirected graphs (4-cycle wedges): for entry e
+  // of row x with v = colidx[e] < x, cut[e] = |N(v) ∩ [0, x)|; nlow[x] =
+  // |N(x) ∩ [0, x)|; null elsewhere
+  const int32_t *cut = nullptr;
+  const int32_t *nlow = nullptr;
 };
 
 // A list of vertex tasks (int32 ids) resident in HBM, ordered by
@@ -235,6 +240,10 @@ hipError_t degree_pair_sums_device(const DevCSR &g, const int32_t *slot, int n_i
 hipError_t relabel_by_degree_device(const DevCSR &g, const int32_t *slot, int num_cu, hipStream_t s,
                                     int64_t **rl_rowptr, int32_t **rl_colidx, std::vector<int32_t> &order,
                                     std::vector<int64_t> &work, std::vector<int32_t> &nlow);
+// cut[e] and nlow[x] of a relabelled CSR (DevCSR::cut / nlow), hipMalloc'd
+// (the caller frees)
+hipError_t build_rl_cuts_device(const int64_t *rl_rowptr, const int32_t *rl_colidx, int64_t n, int64_t nnz,
+                                int num_cu, hipStream_t s, int32_t **cut, int32_t **nlow);
 // cnt[hoff[h] + k] = |N(v_k) ∩ [0, x)| for hub x = hubs[h] of a relabelled
 // CSR and its k-th lower neighbour v_k
 hipError_t cycle4_hub_counts_device(const int64_t *rl_rowptr, const int32_t *rl_colidx,

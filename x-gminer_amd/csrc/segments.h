@@ -56,6 +56,31 @@ __device__ __forceinline__ void wave_segments_range(Seg &t, const int64_t *__res
   }
 }
 
+// wave_segments_range with every list verts[k] cut to its first cut[k]
+// elements (precomputed cut positions).
+template <class Seg>
+__device__ __forceinline__ void wave_segments_range_cut(Seg &t, const int64_t *__restrict__ rowptr, const int *verts,
+                                                        const int *cut, int k0, int k1, int lane, int &ns, int &L) {
+  for (int k = k0; k < k1; k += 64) {
+    const int kk = k + lane;
+    int len = 0;
+    int64_t vb = 0;
+    if (kk < k1) {
+      vb = rowptr[verts[kk]];
+      len = cut[kk];
+    }
+    const int incl = wave_incl_scan(len, lane) + L;
+    const u64 m = __ballot(len > 0);
+    if (len > 0) {
+      const int pos = ns + __popcll(m & lanemask_lt(lane));
+      t.os[pos] = make_int2((int)vb, incl - len);
+      if constexpr (Seg::kIdx) t.idx[pos] = kk;
+    }
+    ns += __popcll(m);
+    L = __shfl(incl, 63, 64);
+  }
+}
+
 // Same as wave_segments_range over a virtual vertex list: vert(k) gives
 // the k-th vertex and keep(k, v, len) selects it.
 template <class Seg, class VertAt, class KeepK>
@@ -98,25 +123,13 @@ __device__ __forceinline__ void wave_segments(Seg &t, const int64_t *__restrict_
   L_out = L;
 }
 
-// Block-cooperative build: every thread loads the (offset, length) of its
-// share of verts[k0, k1) (k1 - k0 <= PCAP), then wave 0 compacts the
-// non-empty lists in LDS.  tmp holds PCAP ints.  All threads call; the
-// result is visible to all threads on return.
-template <int NT, class Seg, class Keep>
-__device__ __forceinline__ void block_segments(Seg &t, int *tmp, const int64_t *__restrict__ rowptr,
-                                               const int32_t *__restrict__ colidx, const int *verts, int k0,
-                                               int k1, int bound, Keep keep, int *sh_ns, int *sh_L) {
+// Second half of the block-cooperative builds below: with t.os[q].x and
+// tmp[q] holding the offset and length of list q < n, wave 0 compacts the
+// non-empty lists in place and writes the flattened starts.
+template <class Seg>
+__device__ __forceinline__ void block_segments_compact(Seg &t, const int *tmp, int n, int k0, int *sh_ns,
+                                                       int *sh_L) {
   const int tid = threadIdx.x, lane = tid & 63;
-  const int n = k1 - k0;
-  for (int q = tid; q < n; q += NT) {
-    const int v = verts[k0 + q];
-    const int64_t vb = rowptr[v];
-    int len = (int)(rowptr[v + 1] - vb);
-    if (!keep(v, len)) len = 0;
-    else if (bound >= 0) len = lower_bound_dev(colidx + vb, len, bound);
-    t.os[q].x = (int)vb;
-    tmp[q] = len;
-  }
   __syncthreads();
   if (tid < 64) {
     int ns = 0, L = 0;
@@ -146,13 +159,47 @@ __device__ __forceinline__ void block_segments(Seg &t, int *tmp, const int64_t *
   __syncthreads();
 }
 
+// Block-cooperative build: every thread loads the (offset, length) of its
+// share of verts[k0, k1) (k1 - k0 <= PCAP), then wave 0 compacts the
+// non-empty lists in LDS.  tmp holds PCAP ints.  All threads call; the
+// result is visible to all threads on return.
+template <int NT, class Seg, class Keep>
+__device__ __forceinline__ void block_segments(Seg &t, int *tmp, const int64_t *__restrict__ rowptr,
+                                               const int32_t *__restrict__ colidx, const int *verts, int k0,
+                                               int k1, int bound, Keep keep, int *sh_ns, int *sh_L) {
+  const int n = k1 - k0;
+  for (int q = threadIdx.x; q < n; q += NT) {
+    const int v = verts[k0 + q];
+    const int64_t vb = rowptr[v];
+    int len = (int)(rowptr[v + 1] - vb);
+    if (!keep(v, len)) len = 0;
+    else if (bound >= 0) len = lower_bound_dev(colidx + vb, len, bound);
+    t.os[q].x = (int)vb;
+    tmp[q] = len;
+  }
+  block_segments_compact(t, tmp, n, k0, sh_ns, sh_L);
+}
+
+// As block_segments with every list verts[k] cut to its first cut[k]
+// elements (precomputed cut positions instead of binary searches).
+template <int NT, class Seg>
+__device__ __forceinline__ void block_segments_cut(Seg &t, int *tmp, const int64_t *__restrict__ rowptr,
+                                                   const int *verts, const int *cut, int k0, int k1, int *sh_ns,
+                                                   int *sh_L) {
+  const int n = k1 - k0;
+  for (int q = threadIdx.x; q < n; q += NT) {
+    t.os[q].x = (int)rowptr[verts[k0 + q]];
+    tmp[q] = cut[k0 + q];
+  }
+  block_segments_compact(t, tmp, n, k0, sh_ns, sh_L);
+}
+
 // Block-cooperative build over a virtual vertex list (see wave_segments_fn).
 template <int NT, class Seg, class VertAt, class KeepK>
 __device__ __forceinline__ void block_segments_fn(Seg &t, int *tmp, const int64_t *__restrict__ rowptr, int k0,
                                                   int k1, VertAt vert, KeepK keep, int *sh_ns, int *sh_L) {
-  const int tid = threadIdx.x, lane = tid & 63;
   const int n = k1 - k0;
-  for (int q = tid; q < n; q += NT) {
+  for (int q = threadIdx.x; q < n; q += NT) {
     const int v = vert(k0 + q);
     const int64_t vb = rowptr[v];
     int len = (int)(rowptr[v + 1] - vb);
@@ -160,33 +207,7 @@ __device__ __forceinline__ void block_segments_fn(Seg &t, int *tmp, const int64_
     t.os[q].x = (int)vb;
     tmp[q] = len;
   }
-  __syncthreads();
-  if (tid < 64) {
-    int ns = 0, L = 0;
-    for (int k = 0; k < n; k += 64) {
-      const int kk = k + lane;
-      int len = 0, vb = 0;
-      if (kk < n) {
-        len = tmp[kk];
-        vb = t.os[kk].x;
-      }
-      const int incl = wave_incl_scan(len, lane) + L;
-      const u64 m = __ballot(len > 0);
-      if (len > 0) {
-        const int pos = ns + __popcll(m & lanemask_lt(lane));
-        t.os[pos] = make_int2(vb, incl - len);
-        if constexpr (Seg::kIdx) t.idx[pos] = k0 + kk;
-      }
-      ns += __popcll(m);
-      L = __shfl(incl, 63, 64);
-    }
-    if (lane == 0) {
-      t.os[ns] = make_int2(0, L);
-      *sh_ns = ns;
-      *sh_L = L;
-    }
-  }
-  __syncthreads();
+  block_segments_compact(t, tmp, n, k0, sh_ns, sh_L);
 }
 
 struct KeepAll {

@@ -131,6 +131,8 @@ struct xgm_dgraph {
   size_t dense_ints = 0;
   int64_t *d_rl_rowptr = nullptr;
   int32_t *d_rl_colidx = nullptr;
+  int32_t *d_rl_cut = nullptr;     // per entry: |N(v) ∩ [0, x)| (DevCSR::cut)
+  int32_t *d_rl_nlow = nullptr;    // per vertex: |N(x) ∩ [0, x)|
   std::vector<int32_t> rl_order;   // relabelled task ids by decreasing wedge work
   std::vector<int64_t> rl_work;
   std::vector<int32_t> rl_nlow;    // lower-neighbour counts (relabelled in HBM; else empty)
@@ -183,6 +185,8 @@ struct xgm_dgraph {
     (void)hipFree(d_motif);
     (void)hipFree(d_rl_rowptr);
     (void)hipFree(d_rl_colidx);
+    (void)hipFree(d_rl_cut);
+    (void)hipFree(d_rl_nlow);
     for (auto &kv : task_cache) (void)hipFree(kv.second.first);
     (void)hipFree(d_rowptr);
     (void)hipFree(d_colidx);
@@ -1411,9 +1415,14 @@ static int dispatch_pattern(xgm_ctx *ctx, const xgm_dgraph *g, int pattern, cons
         if ((rc = grow_scratch(ctx, gm, (void **)&gm->d_dense, gm->dense_ints, need, sizeof(int), &grown))) return rc;
         if (grown) HIP_TRY(ctx, launch_zero(gm->d_dense, need / 2, ctx->stream));
       }
+      if (!gm->d_rl_cut)  // cut positions of every lower list, once per graph
+        HIP_TRY(ctx, build_rl_cuts_device(gm->d_rl_rowptr, gm->d_rl_colidx, g->n, g->nnz, ctx->num_cu, ctx->stream,
+                                          &gm->d_rl_cut, &gm->d_rl_nlow));
       DevCSR rg = dg;
       rg.rowptr = gm->d_rl_rowptr;
       rg.colidx = gm->d_rl_colidx;
+      rg.cut = gm->d_rl_cut;
+      rg.nlow = gm->d_rl_nlow;
       CounterRows rows;
       rows.stride = stride;
       rows.inst_lo = l.slots ? gm->d_inst_lo : nullptr;
