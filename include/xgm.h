@@ -81,9 +81,11 @@ enum xgm_strategy {
                            4-cycle: bounded pairwise intersections in the reference loop order */
   XGM_STRAT_ELUT = 5,   /* cliques: edge tasks, bitmap LUT over N+(v0) ∩ N+(v1) */
   XGM_STRAT_LUT_RANK = 6,/* cliques: vertex LUT with rows in DAG order (upper-triangular) */
-  XGM_STRAT_BUCKET = 7, /* triangles: X-GMiner bucket-bitmap prefilter -- lists split into 64
-                           buckets by id & 63, only buckets non-empty on both sides intersected
-                           (xgminer/include/xgminer_bitmap.cuh:161-320, 661-822) */
+  XGM_STRAT_BUCKET = 7, /* triangles and vertex-induced diamonds: X-GMiner bucket-bitmap
+                           prefilter -- lists split into 64 buckets by id & 63, only buckets
+                           non-empty on both sides intersected (xgminer/include/xgminer_bitmap.cuh:
+                           161-320, 661-822; the reference's P2 kernel,
+                           xgminer/src/gpu_kernels/xgminer/P2_kernel.cuh:110-230); degrees < 65536 */
   XGM_STRAT_MERGE = 8   /* triangles: merge-based intersection -- one lane merges N+(u) with N+(v)
                            per DAG edge, the wave64 merge path when both lists exceed 1024
                            (intersect_num_merge / intersect_num_hybrid,

@@ -137,3 +137,32 @@ def test_vmotif_auto_route(xgm, orc, ctx):
         assert not any(k.name.startswith("vm_") for k in ctx.last_profile()[1]), name
         assert auto == ctx.count(dg, pid, STRATS["lut"]) == census[ci], name
     dg.close()
+
+
+def test_vmotif_diamond_bucket_prefilter(xgm, orc, ctx):
+    """Induced diamonds with X-GMiner's bucket prefilter (XGM_STRAT_BUCKET,
+    the reference's P2 kernel): citeseer's published 2,200, hub graphs and
+    batches against the oracle's census; other motifs and degrees >= 65536
+    are refused."""
+    _say("vmotif bucket")
+    bucket = 7
+    dg = ctx.upload(xgm.load_csr(CITESEER))
+    assert ctx.count(dg, 7, bucket) == 2200
+    with pytest.raises(xgm.XGMError):
+        ctx.count(dg, 6, bucket)
+    dg.close()
+    for seed in (3, 4):
+        g = _hub_graph(xgm, 60 + seed, [40, 200, 900, 1300, 2500])
+        dg = ctx.upload(g)
+        assert ctx.count(dg, 7, bucket) == orc.motif(g.rowptr, g.colidx, 4)[4], seed
+        dg.close()
+    inst = [random_graph(xgm, 300 + 50 * i, 2500 + 300 * i, 120 + i) for i in range(4)]
+    inst.append(_hub_graph(xgm, 91, [70, 1100]))
+    dg = ctx.upload_batch(inst)
+    assert ctx.count_instances(dg, 7, strategy=bucket).tolist() == [orc.motif(g.rowptr, g.colidx, 4)[4] for g in inst]
+    dg.close()
+    leaves = np.arange(1, 70001, dtype=np.int32)
+    dg = ctx.upload(xgm.from_edges(70001, np.zeros(70000, np.int32), leaves))
+    with pytest.raises(xgm.XGMError):
+        ctx.count(dg, 7, bucket)
+    dg.close()

@@ -142,7 +142,126 @@ __global__ __launch_bounds__(kBucketThreads) void k_tc_bucket(const int64_t *__r
   if (lane == 0 && acc) atomicAdd(out, acc);
 }
 
+// Vertex-induced diamonds (GLUMIN P2) with X-GMiner's bucket prefilter
+// (xgminer/src/gpu_kernels/xgminer/P2_kernel.cuh:110-230 builds the
+// diagonal's common neighbourhood with XGMiner_BITMAP::intersection).  For
+// the diagonal (v0, v1), v1 < v0, W = N(v0) ∩ N(v1) is gathered from the
+// bucket pairs set in mask(v0) & mask(v1): lane b merges bucket b of both
+// lists, once to count and once to write its hits at its scanned offset.
+// The diamonds on the diagonal are the non-adjacent pairs of W: the sum over
+// q of q - |W[0, q) ∩ N(W[q])|, every membership test first checking the
+// element's bucket bit in mask(W[q]) and then searching that bucket only.
+// W is in bucket order, which counts every pair once like any other order.
+// A block per task v0, its waves take the v1 in turn; W lives in this
+// wave's HBM slot of md ints (md >= every degree).
+template <int NT>
+__global__ __launch_bounds__(NT) void k_vmotif_diamond_bucket(const int64_t *__restrict__ rowptr,
+                                                              const int32_t *__restrict__ bcol,
+                                                              const uint16_t *__restrict__ boff,
+                                                              const u64 *__restrict__ bmap,
+                                                              const int32_t *__restrict__ tasks, int ntasks,
+                                                              int32_t *__restrict__ scratch, int md,
+                                                              int *__restrict__ queue, u64 *__restrict__ out,
+                                                              const int32_t *__restrict__ slots) {
+  __shared__ int sh_task, sh_next;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  int32_t *W = scratch + ((size_t)blockIdx.x * (NT / 64) + wave) * (size_t)md;
+  u64 acc = 0;
+  for (;;) {
+    if (tid == 0) {
+      sh_task = atomicAdd(queue, 1);
+      sh_next = 0;
+    }
+    __syncthreads();
+    const int t = sh_task;
+    if (t >= ntasks) break;
+    const int v0 = tasks[t];
+    const int64_t b0 = rowptr[v0];
+    const int d0 = (int)(rowptr[v0 + 1] - b0);
+    const int32_t *x0 = bcol + b0;
+    const uint16_t *o0 = boff + (size_t)v0 * 64;
+    const u64 m0 = bmap[v0];
+    u64 part = 0;
+    for (;;) {
+      int i = 0;
+      if (lane == 0) i = atomicAdd(&sh_next, 1);
+      i = __shfl(i, 0, 64);
+      if (i >= d0) break;
+      const int v1 = x0[i];
+      if (v1 >= v0) continue;
+      const int64_t b1 = rowptr[v1];
+      const int d1 = (int)(rowptr[v1 + 1] - b1);
+      const int32_t *x1 = bcol + b1;
+      const uint16_t *o1 = boff + (size_t)v1 * 64;
+      const bool on = ((m0 & bmap[v1]) >> lane) & 1;
+      int s0 = 0, e0 = 0, s1 = 0, e1 = 0, c = 0;
+      if (on) {
+        s0 = o0[lane];
+        e0 = lane < 63 ? (int)o0[lane + 1] : d0;
+        s1 = o1[lane];
+        e1 = lane < 63 ? (int)o1[lane + 1] : d1;
+        for (int p = s0, r = s1; p < e0 && r < e1;) {
+          const int x = x0[p], y = x1[r];
+          c += x == y;
+          p += x <= y;
+          r += y <= x;
+        }
+      }
+      const int incl = wave_incl_scan(c, lane);
+      const int n0 = __shfl(incl, 63, 64);
+      if (c) {
+        int o = incl - c;
+        for (int p = s0, r = s1; p < e0 && r < e1;) {
+          const int x = x0[p], y = x1[r];
+          if (x == y) W[o++] = x;
+          p += x <= y;
+          r += y <= x;
+        }
+      }
+      __threadfence_block();  // W written by other lanes of this wave
+      wave_sync();
+      uint32_t cnt = 0;
+      for (int q = 1; q < n0; ++q) {
+        const int v2 = W[q];
+        const int64_t b2 = rowptr[v2];
+        const int d2 = (int)(rowptr[v2 + 1] - b2);
+        const uint16_t *o2 = boff + (size_t)v2 * 64;
+        const u64 m2 = bmap[v2];
+        uint32_t hit = 0;
+        for (int k = lane; k < q; k += 64) {
+          const int w = W[k];
+          const int bb = w & 63;
+          if ((m2 >> bb) & 1) {
+            const int s = o2[bb], e = bb < 63 ? (int)o2[bb + 1] : d2;
+            hit += sorted_contains(bcol + b2 + s, e - s, w) ? 1u : 0u;
+          }
+        }
+        cnt += (uint32_t)q - (uint32_t)wave_sum_i32((int)hit);
+      }
+      if (lane == 0) part += cnt;
+      wave_sync();  // W reads done before the next v1 overwrites it
+    }
+    task_flush(acc, part, slots, v0, out, lane);
+    __syncthreads();
+  }
+  acc = wave_sum_u64(acc);
+  if (lane == 0 && acc) atomicAdd(out, acc);
+}
+
 }  // namespace
+
+int vmotif_bucket_grid(int num_cu) { return num_cu * 2; }
+
+hipError_t launch_vmotif_diamond_bucket(const DevCSR &g, const int32_t *bcol, const uint16_t *boff, const u64 *bmap,
+                                        const TaskList &tasks, int32_t *scratch, int md, Launch &l, int *queue) {
+  if (!tasks.count) return hipSuccess;
+  const int grid = (int)std::min<int64_t>(tasks.count, vmotif_bucket_grid(l.num_cu));
+  l.begin("vm_diamond_bucket", tasks.count);
+  k_vmotif_diamond_bucket<256><<<grid, 256, 0, l.stream>>>(g.rowptr, bcol, boff, bmap, tasks.tasks,
+                                                           (int)tasks.count, scratch, md, queue, l.d_out, l.slots);
+  l.end();
+  return hipGetLastError();
+}
 
 hipError_t launch_bucket_build(const int64_t *rowptr, const int32_t *colidx, int64_t n, int32_t *bcol, uint16_t *boff,
                                u64 *bmap, int num_cu, hipStream_t s) {

@@ -78,6 +78,12 @@ hipError_t launch_tc_merge(const DevCSR &g, const TaskList &tasks, Launch &l, in
 // only the buckets set in both masks.  Adds into l.d_out.
 hipError_t launch_bucket_build(const int64_t *rowptr, const int32_t *colidx, int64_t n, int32_t *bcol, uint16_t *boff,
                                u64 *bmap, int num_cu, hipStream_t s);
+// Vertex-induced diamonds with the bucket prefilter over the tasks of
+// degree >= 3; scratch holds vmotif_bucket_grid(num_cu) * 4 * md ints, md
+// at least the largest degree.
+int vmotif_bucket_grid(int num_cu);
+hipError_t launch_vmotif_diamond_bucket(const DevCSR &g, const int32_t *bcol, const uint16_t *boff, const u64 *bmap,
+                                        const TaskList &tasks, int32_t *scratch, int md, Launch &l, int *queue);
 hipError_t launch_tc_bucket(const DevCSR &g, const int32_t *bcol, const uint16_t *boff, const u64 *bmap,
                             const TaskList &tasks, Launch &l, int *queue);
 

@@ -1223,6 +1223,26 @@ static int dispatch_pattern(xgm_ctx *ctx, const xgm_dgraph *g, int pattern, cons
     case XGM_DIAMOND_V:
     case XGM_TAILEDTRI_V: {
       if (!(g->flags & XGM_CSR_SYMMETRIC)) return ctx->fail(XGM_EINVAL, "vertex-induced motifs need an undirected CSR");
+      if (strategy == XGM_STRAT_BUCKET) {
+        // X-GMiner's bucket-bitmap P2 (diamonds only, as in the reference)
+        if (pattern != XGM_DIAMOND_V)
+          return ctx->fail(XGM_EUNSUPPORTED, "the bucket prefilter counts vertex-induced diamonds only");
+        if ((rc = vmotif_lists(ctx, gm, true, tl))) return rc;  // every vertex of degree >= 3
+        const int64_t md = gm->list_md["vmotif_list"];
+        if (md >= 65536) return ctx->fail(XGM_EUNSUPPORTED, "bucket strategy needs degrees < 65536");
+        if (!gm->d_bcol) {
+          HIP_TRY(ctx, hipMalloc(&gm->d_bcol, std::max<int64_t>(1, g->nnz) * sizeof(int32_t)));
+          HIP_TRY(ctx, hipMalloc(&gm->d_boff, std::max<int64_t>(1, g->n) * 64 * sizeof(uint16_t)));
+          HIP_TRY(ctx, hipMalloc(&gm->d_bmap, std::max<int64_t>(1, g->n) * sizeof(u64)));
+          HIP_TRY(ctx, launch_bucket_build(gm->d_rowptr, gm->d_colidx, g->n, gm->d_bcol, gm->d_boff, gm->d_bmap,
+                                           ctx->num_cu, l.stream));
+        }
+        const size_t need = (size_t)vmotif_bucket_grid(ctx->num_cu) * 4 * (size_t)std::max<int64_t>(1, md);
+        if ((rc = ensure_scratch(ctx, gm, need))) return rc;
+        e = launch_vmotif_diamond_bucket(dg, gm->d_bcol, gm->d_boff, gm->d_bmap, tl[3], gm->d_scratch,
+                                         (int)std::max<int64_t>(1, md), l, l.d_queue);
+        break;
+      }
       if (strategy != XGM_STRAT_AUTO && strategy != XGM_STRAT_LUT && strategy != XGM_STRAT_LIST)
         return ctx->fail(XGM_EUNSUPPORTED, "strategy not available for vertex-induced motifs");
       // vertex tasks of degree >= 3 by decreasing degree: LUT tiers by
