@@ -59,6 +59,10 @@ def test_cli_citeseer(xgm, tmp_path):
     assert _counts(xgm, "-g", d, "--patternID", "17") == {"P1": 222630}
     assert _counts(xgm, "-g", d, "--patternID", "18") == {"P2": 2200}
     assert _counts(xgm, "-g", d, "--patternID", "19") == {"P3": 22900}
+    # the reference's X-GMiner path: its bucket-bitmap P2 (main.cpp:48-52)
+    assert _counts(xgm, "-g", d, "--patternID", "18", "--run-xgminer", "1") == {"P2": 2200}
+    for st in ("lut", "list"):
+        assert _counts(xgm, "-g", d, "--patternID", "17", "--strategy", st) == {"P1": 222630}, st
     m = _counts(xgm, "-g", d, "--pattern-name", "motif4")
     assert m == {"3-star": 222630, "4-path": 111153, "tailed-triangle": 22900, "4-cycle": 3094,
                  "diamond": 2200, "4-clique": 255}

@@ -41,9 +41,10 @@ const char *kUsage =
     "  --strategy NAME         auto | hash | bsearch | lut | list | elut | rank | bucket | merge\n"
     "                          (overrides --algorithm)\n"
     "  --device D              HIP device (default 0)\n"
+    "  --run-xgminer 1         X-GMiner's bucket-bitmap P2 (--patternID 18; as --strategy bucket)\n"
     "  -h, --help              this text\n"
     "Accepted and ignored (no effect on counts): --dataname --do-validation --use-graphpi-sched\n"
-    "  --run-graphpi --run-our-baseline --run-xgminer --tunek --setk --prof-workload --use-vert-para\n"
+    "  --run-graphpi --run-our-baseline --tunek --setk --prof-workload --use-vert-para\n"
     "  --prof-edgecheck --vert-induced 0\n";
 
 int fail(const std::string &msg) {
@@ -96,7 +97,6 @@ int load_graph(const std::string &path, xgm_hgraph **g, std::string &how) {
 struct Pattern {
   int id = 0;          // xgm_pattern, or 0 for a motif census
   int motif_k = 0;     // 3 or 4 for a census
-  int motif_pick = -1; // one entry of the 4-motif census (GLUMIN P1-P3), or -1 for all
   std::string label;   // as printed
 };
 
@@ -132,10 +132,9 @@ bool pattern_from_matrix(int size, const std::string &m, Pattern &p) {
 int main(int argc, char **argv) {
   std::map<std::string, std::string> opt;
   const char *ignored[] = {"dataname", "do-validation", "use-graphpi-sched", "run-graphpi", "run-our-baseline",
-                           "run-xgminer", "tunek", "setk", "prof-workload", "use-vert-para", "prof-edgecheck",
-                           "vert-induced"};
+                           "tunek", "setk", "prof-workload", "use-vert-para", "prof-edgecheck", "vert-induced"};
   const char *known[] = {"graph", "algorithm", "patternID", "pattern-name", "pattern-size", "pattern-adj-mat",
-                         "repeat", "strategy", "device"};
+                         "repeat", "strategy", "device", "run-xgminer"};
   for (int i = 1; i < argc; ++i) {
     std::string a = argv[i];
     if (a == "-h" || a == "--help") {
@@ -193,16 +192,15 @@ int main(int argc, char **argv) {
     // GLUMIN numbering: local id = patternID - 16 (xgminer/include/glumin.h:36)
     const int pid = opt.count("patternID") ? std::atoi(opt["patternID"].c_str()) : 0;
     const int local = pid - 16;
-    // P1-P3 are the vertex-induced 3-star, diamond and tailed triangle, read
-    // from the 4-motif census in unique-embedding (GraphMiner) semantics;
-    // GLUMIN's P2 kernel walks both directions of every edge and reports
-    // twice the induced-diamond count (glumin_GM_solver.cu:267).
+    // P1-P3 are the vertex-induced 3-star, diamond and tailed triangle in
+    // unique-embedding (GraphMiner) semantics; GLUMIN's P2 kernel walks both
+    // directions of every edge and reports twice the induced-diamond count
+    // (glumin_GM_solver.cu:267).
     if (local == 4) pat.id = XGM_CLIQUE4;
     else if (local == 5) pat.id = XGM_CLIQUE5;
-    else if (local >= 1 && local <= 3) {
-      pat.motif_k = 4;
-      pat.motif_pick = local == 1 ? 0 : local == 2 ? 4 : 2;
-    } else
+    else if (local >= 1 && local <= 3)
+      pat.id = local == 1 ? XGM_STAR3_V : local == 2 ? XGM_DIAMOND_V : XGM_TAILEDTRI_V;
+    else
       return fail("unsupported --patternID " + std::to_string(pid) +
                   " (17-19: vertex-induced 3-star / diamond / tailed triangle, 20: 4-clique, 21: 5-clique)");
     pat.label = "P" + std::to_string(local);
@@ -220,6 +218,12 @@ int main(int argc, char **argv) {
     return fail("this build has no CPU path; use a GPU algorithm");
   } else if (algo != "graphminer") {
     return fail("unsupported algorithm '" + algo + "'");
+  }
+  // the reference's X-GMiner path (main.cpp:48-52): its bucket-bitmap P2
+  // kernel for vertex-induced diamonds (xgminer_solver.cu:158)
+  if (opt.count("run-xgminer") && std::atoi(opt["run-xgminer"].c_str()) != 0) {
+    if (pat.id != XGM_DIAMOND_V) return fail("--run-xgminer counts the vertex-induced diamond (--patternID 18)");
+    strategy = XGM_STRAT_BUCKET;
   }
   if (opt.count("strategy")) {
     const std::map<std::string, int> s = {{"auto", XGM_STRAT_AUTO}, {"hash", XGM_STRAT_HASH},
@@ -277,11 +281,8 @@ int main(int argc, char **argv) {
       const char *n3[] = {"wedge", "triangle"};
       const char *n4[] = {"3-star", "4-path", "tailed-triangle", "4-cycle", "diamond", "4-clique"};
       const int cnt = pat.motif_k == 3 ? 2 : 6;
-      if (pat.motif_pick >= 0)
-        std::printf("Pattern %s count: %llu\n", pat.label.c_str(), (unsigned long long)m[pat.motif_pick]);
-      else
-        for (int i = 0; i < cnt; ++i)
-          std::printf("Pattern %s count: %llu\n", pat.motif_k == 3 ? n3[i] : n4[i], (unsigned long long)m[i]);
+      for (int i = 0; i < cnt; ++i)
+        std::printf("Pattern %s count: %llu\n", pat.motif_k == 3 ? n3[i] : n4[i], (unsigned long long)m[i]);
     } else {
       uint64_t c = 0;
       if ((rc = xgm_count(ctx, needs_dag ? dag : und, pat.id, &o, &c))) {
