@@ -241,6 +241,14 @@ struct WindowLane {
 // f(const WindowLane (&)[U]) is called by all 64 lanes for every group of
 // U windows (invalid lanes included, so f may use ballots), after the U
 // global loads of the group were issued.
+//
+// The segment holding the next window's first element (index jl, its
+// offset delta x - y and its end) is kept in scalar registers, so a window
+// that lies inside it (the common case for long lists) maps its lanes
+// without touching LDS.  A window that crosses ends reads the next 64 table
+// entries with one LDS access, marks the lanes that open a segment in the
+// flag row, takes each lane's offset from the entry's lane by a cross-lane
+// permute and the next window's segment from the same entries (readlane).
 template <int U, class Seg, class F>
 __device__ __forceinline__ void wave_walk_group(const Seg &t, const int32_t *__restrict__ colidx, int ns,
                                                 int lo, int hi, int lane, WaveFlags &fl, F &&f) {
@@ -249,10 +257,10 @@ __device__ __forceinline__ void wave_walk_group(const Seg &t, const int32_t *__r
   hi = __builtin_amdgcn_readfirstlane(hi);
   ns = __builtin_amdgcn_readfirstlane(ns);
   if (lo >= hi) return;
-  // first segment whose end exceeds lo (ends are starts of the next one)
+  // segment holding lo: the last j with os[j].y <= lo
   int jl;
   {
-    int a = 0, b = ns;  // find last j with os[j].y <= lo
+    int a = 0, b = ns;
     while (b - a > 1) {
       const int mid = (a + b) >> 1;
       if (t.os[mid].y <= lo) a = mid; else b = mid;
@@ -261,9 +269,17 @@ __device__ __forceinline__ void wave_walk_group(const Seg &t, const int32_t *__r
     // is a scalar branch instead of two exec-masked paths
     jl = __builtin_amdgcn_readfirstlane(a);
   }
+  // delta (x - y: element i of the segment is colidx[delta + i]) and end of
+  // segment jl; jl < ns here, so jl + 1 <= ns is a table entry
+  int cur_d, cur_e;
+  {
+    const int2 c = t.os[jl];
+    cur_d = __builtin_amdgcn_readfirstlane(c.x - c.y);
+    cur_e = __builtin_amdgcn_readfirstlane(t.os[jl + 1].y);
+  }
   for (int base = lo; base < hi; base += 64 * U) {
     WindowLane r[U];
-    int64_t pos[U];
+    int pos[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       const int b = base + 64 * u;
@@ -271,36 +287,53 @@ __device__ __forceinline__ void wave_walk_group(const Seg &t, const int32_t *__r
       r[u].valid = i < hi;
       r[u].next_valid = i + 1 < hi;
       pos[u] = 0;
-      // jl is wave-uniform: a window inside one segment (the common case
-      // for long lists) needs no lane -> segment mapping
-      // both reads issued together: one LDS round trip on the fast path
-      const int2 cur = t.os[jl];
-      const int e0 = __builtin_amdgcn_readfirstlane(t.os[jl + 1].y);
-      if (e0 >= b + 64) {
-        const int cx = __builtin_amdgcn_readfirstlane(cur.x), cy = __builtin_amdgcn_readfirstlane(cur.y);
+      // invariant: os[jl].y <= b < cur_e = os[jl + 1].y (no empty segments)
+      if (cur_e >= b + 64) {
         r[u].j = jl;
         r[u].heads = 1ull;
-        if (r[u].valid) pos[u] = (int64_t)(uint32_t)cx + (i - cy);
-        if (e0 == b + 64) ++jl;
+        if (r[u].valid) pos[u] = cur_d + i;
+        if (cur_e == b + 64 && b + 64 < hi) {
+          // the next window opens segment jl + 1, which is not the end
+          // sentinel (its start b + 64 < hi <= L), so jl + 2 <= ns
+          ++jl;
+          const int2 c = t.os[jl];
+          cur_d = __builtin_amdgcn_readfirstlane(c.x - c.y);
+          cur_e = __builtin_amdgcn_readfirstlane(t.os[jl + 1].y);
+        }
         continue;
       }
-      const int e = (jl + lane < ns) ? t.os[jl + lane + 1].y : INT_MAX;
-      const int p = e - b;  // >= 1 inside the range
+      // entries jl + 1 + lane: the segments that start after b (the sentinel
+      // os[ns] = (0, L) included; lanes past it get a start beyond any window)
+      const int k = jl + 1 + lane;
+      const int2 o = (k <= ns) ? t.os[k] : make_int2(0, INT_MAX);
+      const int p = o.y - b;  // >= 1: segment jl holds b
+      const int dk = o.x - o.y;
       ++fl.stamp;
       if (p > 0 && p < 64) fl.row[p] = fl.stamp;
       wave_sync();
       const u64 heads = __ballot(fl.row[lane] == fl.stamp);
-      const int j = jl + __popcll(heads & lanemask_le(lane));
-      r[u].j = j;
+      const int nh = __popcll(heads & lanemask_le(lane));  // segments opened at or before this lane
+      const int dsh = __shfl(dk, nh > 0 ? nh - 1 : 0, 64);
+      r[u].j = jl + nh;
       r[u].heads = heads | 1ull;
-      if (r[u].valid) {
-        const int2 s = t.os[j];
-        pos[u] = (int64_t)(uint32_t)s.x + (i - s.y);
+      if (r[u].valid) pos[u] = (nh ? dsh : cur_d) + i;
+      // next window: segment jl + cnt, cnt = segments starting in (b, b + 64]
+      // (>= 1: os[jl + 1].y = cur_e < b + 64)
+      const int cnt = __popcll(__ballot(p <= 64));
+      if (b + 64 < hi) {
+        jl += cnt;
+        if (cnt < 64) {
+          cur_d = __builtin_amdgcn_readlane(dk, cnt - 1);
+          cur_e = __builtin_amdgcn_readlane(o.y, cnt);
+        } else {  // 64 one-element segments: the next entries were not read
+          const int2 c = t.os[jl];
+          cur_d = __builtin_amdgcn_readfirstlane(c.x - c.y);
+          cur_e = __builtin_amdgcn_readfirstlane(t.os[jl + 1].y);
+        }
       }
-      jl += __popcll(__ballot(p <= 64));
     }
 #pragma unroll
-    for (int u = 0; u < U; ++u) r[u].w = r[u].valid ? colidx[pos[u]] : 0;
+    for (int u = 0; u < U; ++u) r[u].w = r[u].valid ? colidx[(uint32_t)pos[u]] : 0;
     f(r);
   }
 }
