@@ -270,7 +270,7 @@ __global__ __launch_bounds__(128) void k_tc_wave_packed(const int64_t *__restric
 // membership test binary-searches N+(t) in HBM.  Partners are taken in
 // pieces of PCAP.
 template <int CAP, int PCAP, int STRAT, int NT, int HSHIFT>
-__global__ __launch_bounds__(NT) void k_tc_block(const int64_t *__restrict__ rowptr,
+__global__ __launch_bounds__(NT, 8) void k_tc_block(const int64_t *__restrict__ rowptr,
                                                  const int32_t *__restrict__ colidx,
                                                  const int64_t *__restrict__ rrowptr,
                                                  const int32_t *__restrict__ rcolidx,
@@ -316,7 +316,7 @@ __global__ __launch_bounds__(NT) void k_tc_block(const int64_t *__restrict__ row
       block_segments_fn<NT>(s.seg, tmp, rowptr, k0, k1, vert, TcKeep{du}, &sh_ns, &sh_L);
       int lo, hi;
       wave_share(sh_L, wave, NW, lo, hi);
-      wave_walk_group<UNROLL>(s.seg, colidx, sh_ns, lo, hi, lane, fl, [&](const WindowLane (&r)[UNROLL]) {
+      wave_walk_group<UNROLL, true>(s.seg, colidx, sh_ns, lo, hi, lane, fl, [&](const WindowLane (&r)[UNROLL]) {
         if (STRAT == STRAT_HASH && lds_member) {
           cnt += group_hash_hits<UNROLL>(s.key, hbits, r);
         } else {

@@ -249,7 +249,7 @@ struct WindowLane {
 // entries with one LDS access, marks the lanes that open a segment in the
 // flag row, takes each lane's offset from the entry's lane by a cross-lane
 // permute and the next window's segment from the same entries (readlane).
-template <int U, class Seg, class F>
+template <int U, bool PIPE = false, class Seg, class F>
 __device__ __forceinline__ void wave_walk_group(const Seg &t, const int32_t *__restrict__ colidx, int ns,
                                                 int lo, int hi, int lane, WaveFlags &fl, F &&f) {
   // the range and table size are the same on every lane (scalar registers)
@@ -277,8 +277,12 @@ __device__ __forceinline__ void wave_walk_group(const Seg &t, const int32_t *__r
     cur_d = __builtin_amdgcn_readfirstlane(c.x - c.y);
     cur_e = __builtin_amdgcn_readfirstlane(t.os[jl + 1].y);
   }
-  for (int base = lo; base < hi; base += 64 * U) {
-    WindowLane r[U];
+  // map the U windows of a group at base and issue their loads.  PIPE
+  // loads every lane (invalid lanes read colidx[0], which exists because the
+  // range is not empty) so the loads of the next group can be in flight
+  // while f handles the current one: the same number of loads is
+  // outstanding on every path, and f waits for the older group only.
+  auto map_group = [&](int base, WindowLane (&r)[U]) {
     int pos[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
@@ -333,8 +337,34 @@ __device__ __forceinline__ void wave_walk_group(const Seg &t, const int32_t *__r
       }
     }
 #pragma unroll
-    for (int u = 0; u < U; ++u) r[u].w = r[u].valid ? colidx[(uint32_t)pos[u]] : 0;
-    f(r);
+    for (int u = 0; u < U; ++u) {
+      if (PIPE) {
+        const int w = colidx[(uint32_t)pos[u]];
+        r[u].w = r[u].valid ? w : 0;
+      } else {
+        r[u].w = r[u].valid ? colidx[(uint32_t)pos[u]] : 0;
+      }
+    }
+  };
+  if constexpr (!PIPE) {
+    for (int base = lo; base < hi; base += 64 * U) {
+      WindowLane r[U];
+      map_group(base, r);
+      f(r);
+    }
+  } else {
+    WindowLane r[U];
+    map_group(lo, r);
+    for (int base = lo;; base += 64 * U) {
+      // the group past the range maps lanes that are all invalid; its state
+      // updates are guarded by b + 64 < hi
+      WindowLane nx[U];
+      map_group(base + 64 * U, nx);
+      f(r);
+      if (base + 64 * U >= hi) break;
+#pragma unroll
+      for (int u = 0; u < U; ++u) r[u] = nx[u];
+    }
   }
 }
 
