@@ -254,7 +254,7 @@ __device__ u64 block_sum_intersections(const int64_t *__restrict__ rowptr, const
     __syncthreads();
     int lo, hi;
     wave_share(S.L, wave, NT / 64, lo, hi);
-    wave_walk<4>(S.seg, colidx, S.ns, lo, hi, lane, fl, [&](const WindowLane &r) {
+    wave_walk<4, true>(S.seg, colidx, S.ns, lo, hi, lane, fl, [&](const WindowLane &r) {
       if (r.valid) acc += m.contains(r.w);
     });
   }
@@ -279,7 +279,7 @@ __device__ void block_lut_build(const int64_t *__restrict__ rowptr, const int32_
   block_segments<NT>(S.seg, S.x2, rowptr, colidx, S.x, 0, s, -1, KeepAll(), &S.ns, &S.L);
   int lo, hi;
   wave_share(S.L, wave, NT / 64, lo, hi);
-  wave_walk<4>(S.seg, colidx, S.ns, lo, hi, lane, fl, [&](const WindowLane &r) {
+  wave_walk<4, true>(S.seg, colidx, S.ns, lo, hi, lane, fl, [&](const WindowLane &r) {
     if (r.valid) {
       const int pos = lower_bound_dev(S.x, s, r.w);
       if (pos < s && S.x[pos] == r.w) atomicOr(&S.rows[lut_word<W>(S.seg.idx[r.j], pos >> 6)], 1ull << (pos & 63));
@@ -444,7 +444,7 @@ __global__ __launch_bounds__(NT) void k_clique_vlut_rank(const int64_t *__restri
       block_segments<NT>(S.seg, S.tmp, rowptr, colidx, S.x, 0, s, -1, KeepAll(), &S.ns, &S.L);
       int lo, hi;
       wave_share(S.L, wave, NT / 64, lo, hi);
-      wave_walk<4>(S.seg, colidx, S.ns, lo, hi, lane, fl, [&](const WindowLane &r) {
+      wave_walk<4, true>(S.seg, colidx, S.ns, lo, hi, lane, fl, [&](const WindowLane &r) {
         if (r.valid) {
           const int pos = lower_bound_dev(S.x, s, r.w);
           if (pos < s && S.x[pos] == r.w) {
@@ -561,7 +561,7 @@ __global__ __launch_bounds__(NT) void k_clique_vlut_rank_tri(const int64_t *__re
       __syncthreads();
       int lo, hi;
       wave_share(S.L, wave, NT / 64, lo, hi);
-      wave_walk<4>(S.seg, colidx, S.ns, lo, hi, lane, fl, [&](const WindowLane &r) {
+      wave_walk<4, true>(S.seg, colidx, S.ns, lo, hi, lane, fl, [&](const WindowLane &r) {
         if (r.valid) {
           const int pos = lower_bound_dev(S.x, s, r.w);
           if (pos < s && S.x[pos] == r.w) {
@@ -692,7 +692,7 @@ __global__ __launch_bounds__(NT) void k_clique_vlut_rank_hbm(const int64_t *__re
       __syncthreads();
       int lo, hi;
       wave_share(S.L, wave, NT / 64, lo, hi);
-      wave_walk<4>(S.seg, colidx, S.ns, lo, hi, lane, fl, [&](const WindowLane &r) {
+      wave_walk<4, true>(S.seg, colidx, S.ns, lo, hi, lane, fl, [&](const WindowLane &r) {
         if (r.valid) {
           const int pos = lower_bound_dev(S.x, s, r.w);
           if (pos < s && S.x[pos] == r.w) {

@@ -87,7 +87,7 @@ __global__ __launch_bounds__(NT) void k_diamond(const int64_t *__restrict__ rowp
       __syncthreads();
       int lo, hi;
       wave_share(L, wave, NT / 64, lo, hi);
-      wave_walk_group<4>(S.seg, colidx, ns, lo, hi, lane, fl, [&](const WindowLane (&r)[4]) {
+      wave_walk_group<4, true>(S.seg, colidx, ns, lo, hi, lane, fl, [&](const WindowLane (&r)[4]) {
         bool hit[4];
         if (LDS_SET) {
           // the four home buckets read together, then the rare full-bucket walks
@@ -263,7 +263,7 @@ __global__ __launch_bounds__(NT) void k_diamond_big(const int64_t *__restrict__ 
       __syncthreads();
       int lo, hi;
       wave_share(L, wave, NT / 64, lo, hi);
-      wave_walk<4>(S.seg, colidx, ns, lo, hi, lane, fl, [&](const WindowLane &r) {
+      wave_walk<4, true>(S.seg, colidx, ns, lo, hi, lane, fl, [&](const WindowLane &r) {
         const bool hit = r.valid && contains(r.w);
         const int h = segment_run_hits(__ballot(hit), r, lane);
         if (h) atomicAdd(&S.cnt[r.j], h);

@@ -369,10 +369,10 @@ __device__ __forceinline__ void wave_walk_group(const Seg &t, const int32_t *__r
 }
 
 // wave_walk_group with f(const WindowLane&) called for every window.
-template <int U, class Seg, class F>
+template <int U, bool PIPE = false, class Seg, class F>
 __device__ __forceinline__ void wave_walk(const Seg &t, const int32_t *__restrict__ colidx, int ns,
                                           int lo, int hi, int lane, WaveFlags &fl, F &&f) {
-  wave_walk_group<U>(t, colidx, ns, lo, hi, lane, fl, [&](const WindowLane (&r)[U]) {
+  wave_walk_group<U, PIPE>(t, colidx, ns, lo, hi, lane, fl, [&](const WindowLane (&r)[U]) {
 #pragma unroll
     for (int u = 0; u < U; ++u) f(r[u]);
   });
