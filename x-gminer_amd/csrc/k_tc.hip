@@ -55,13 +55,23 @@ __device__ __forceinline__ int group_hash_hits(const int *key, int hbits, const 
     h[u] = hash_slot(r[u].w, hbits);
     q[u] = reinterpret_cast<const int4 *>(key)[h[u]];
   }
-  int c = 0;
+  bool hit[U], more[U];
+  u64 any = 0;
 #pragma unroll
   for (int u = 0; u < U; ++u) {
-    bool hit = bucket_has(q[u], r[u].w);
-    if (__builtin_expect(r[u].valid && !hit && q[u].w >= 0, 0)) hit = bhash_contains_after(key, hbits, h[u], r[u].w);
-    c += (r[u].valid && hit) ? 1 : 0;
+    hit[u] = bucket_has(q[u], r[u].w);
+    more[u] = r[u].valid && !hit[u] && q[u].w >= 0;
+    any |= __ballot(more[u]);
   }
+  // one wave-uniform branch per group for the rare full home buckets
+  if (__builtin_expect(any != 0, 0)) {
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+      if (more[u]) hit[u] = bhash_contains_after(key, hbits, h[u], r[u].w);
+  }
+  int c = 0;
+#pragma unroll
+  for (int u = 0; u < U; ++u) c += (r[u].valid && hit[u]) ? 1 : 0;
   return c;
 }
 
