@@ -1212,6 +1212,66 @@ class Suite:
         # rows sharing a graph and pattern with a measured row (same kernels)
         return {c: out[c] for c in ("FETCH_SIZE", "WRITE_SIZE")}
 
+    ISSUE_COUNTERS = ("SQ_ACTIVE_INST_VALU", "SQ_INSTS_VALU", "SQ_THREAD_CYCLES_VALU", "SQ_INSTS_SALU",
+                      "SQ_ACTIVE_INST_LDS", "SQ_LDS_BANK_CONFLICT", "SQ_LDS_IDX_ACTIVE", "GRBM_GUI_ACTIVE")
+
+    def issue(self):
+        """One rocprofv3 --pmc run of the counter pass with the issue
+        counters: for the dominant kernel of every row, the fraction of SIMD
+        cycles issuing VALU instructions (SQ_ACTIVE_INST_VALU counts one per
+        4-cycle wave64 issue: busy = 4 ACTIVE / (SIMDs x cycles per XCC)),
+        the active-lane fraction of those instructions, SALU per VALU
+        instruction, LDS active cycles per CU and the LDS bank-conflict share
+        of them.  The LUT kernels are bound by this issue rate, not by HBM."""
+        spec = os.path.join(self.cache, "rows.json")
+        json.dump(self.pass_rows, open(spec, "w"))
+        log("issue counters: rocprofv3 --pmc " + " ".join(self.ISSUE_COUNTERS))
+        res = _pmc_run(self.ISSUE_COUNTERS, ["--profile-pass", spec], timeout=240)
+        if res is None:
+            return None
+        data, agent = res
+        names = json.load(open(spec + ".names.json"))
+        simd, xcc, cu = agent.get("simd", 1024), agent.get("xcc", 8), agent.get("cu", 256)
+        per_row = {}
+        for i, pr in enumerate(self.pass_rows):
+            kn = names[i]
+            vals = {}
+            for c in self.ISSUE_COUNTERS:
+                segs = data[c][1]
+                seg = segs[2 * i + 1] if 2 * i + 1 < len(segs) else []
+                if len(seg) != len(kn):
+                    vals = None
+                    break
+                for nm, (sym, v) in zip(kn, seg):
+                    vals.setdefault(nm, {}).setdefault(c, 0.0)
+                    vals[nm][c] += v
+                    vals[nm]["symbol"] = sym
+            if vals is not None:
+                per_row[pr["row"]] = vals
+        for r in self.rows:
+            dom, vals = r.get("dominant_kernel"), per_row.get(r["row"])
+            if not dom or not vals or dom not in vals:
+                continue
+            v = vals[dom]
+            cyc = v["GRBM_GUI_ACTIVE"] / xcc
+            if cyc <= 0 or v["SQ_ACTIVE_INST_VALU"] <= 0:
+                continue
+            busy = 4.0 * v["SQ_ACTIVE_INST_VALU"] / (simd * cyc)
+            r["issue"] = dict(
+                kernel=dom, symbol=v["symbol"], valu_busy=round(busy, 4),
+                valu_lane_util=round(v["SQ_THREAD_CYCLES_VALU"] / (64.0 * v["SQ_ACTIVE_INST_VALU"]), 4),
+                salu_per_valu=round(v["SQ_INSTS_SALU"] / max(1.0, v["SQ_INSTS_VALU"]), 3),
+                lds_active=round(v["SQ_LDS_IDX_ACTIVE"] / (cu * cyc), 4),
+                lds_bank_conflict_share=round(v["SQ_LDS_BANK_CONFLICT"] / max(1.0, v["SQ_LDS_IDX_ACTIVE"]), 4),
+                cycles_per_xcc=int(cyc), counters={c: v[c] for c in self.ISSUE_COUNTERS})
+            if r.get("bound") == "valu" or r["row"].startswith(("k4_", "k5_")):
+                # the LUT rows: LDS/VALU-bound by design, so their roofline is
+                # the VALU issue rate (one wave64 instruction per SIMD per 4 cycles)
+                r["roofline_valu"] = dict(bound="valu", kernel=dom, achieved=round(busy, 4), peak=1.0,
+                                          unit="fraction of SIMD cycles issuing a VALU instruction",
+                                          frac=round(busy, 4))
+        return dict(simd=simd, xcc=xcc, cu=cu)
+
     def close(self):
         self.ctx.close()
         if self.keep:  # the counter pass can be replayed: bench.py --profile-pass DIR/rows.json
@@ -1245,47 +1305,66 @@ def _pmc_segments(counter, argv, timeout=240):
     segment lists (kernel symbol, counter KiB) of the dispatches between two
     marker copies, in dispatch order, without the zero-fill kernel and the
     runtime's blit kernels; or None."""
+    res = _pmc_run([counter], argv, timeout)
+    return None if res is None else res[0][counter]
+
+
+def _pmc_run(counters, argv, timeout=240):
+    """One rocprofv3 --pmc run of `python3 bench.py argv` collecting every
+    counter of `counters` (within one pass's hardware limits).  Returns
+    ({counter: (calibration values, [segment])} as in _pmc_segments,
+    agent info of the GPU: SIMD and XCC counts), or None."""
     import csv
     import glob
     import shutil
     exe = shutil.which("rocprofv3")
     if not exe:
         return None
+    what = "rocprofv3 --pmc " + " ".join(counters)
     with tempfile.TemporaryDirectory(prefix="xgm_pmc_") as d:
-        cmd = [exe, "--pmc", counter, "--kernel-trace", "--output-format", "csv", "-d", d, "-o", "run", "--",
-               sys.executable, os.path.abspath(__file__)] + argv
-        if not _run_logged(cmd, timeout, f"rocprofv3 --pmc {counter}"):
+        cmd = [exe, "--pmc"] + list(counters) + ["--kernel-trace", "--output-format", "csv", "-d", d, "-o", "run",
+                                                 "--", sys.executable, os.path.abspath(__file__)] + argv
+        if not _run_logged(cmd, timeout, what):
             return None
-        disp = {}
+        disp = {c: {} for c in counters}
         for f in glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True):
             for row in csv.DictReader(open(f)):
-                if row.get("Counter_Name") != counter:
+                c = row.get("Counter_Name")
+                if c not in disp:
                     continue
-                key = int(row.get("Dispatch_Id") or row.get("Correlation_Id") or len(disp))
+                key = int(row.get("Dispatch_Id") or row.get("Correlation_Id") or len(disp[c]))
                 name = row.get("Kernel_Name", "")
-                v = disp.get(key, (name, 0.0))[1] + float(row["Counter_Value"])
-                disp[key] = (name, v)
-    if not disp:
-        return None
-    calib, segs, cur, groups, in_marker = [], [], [], 0, False
-    for key in sorted(disp):
-        name, v = disp[key]
-        if "k_copy_u4" in name:
-            if not in_marker:
-                groups += 1
-                if groups > 1:
-                    segs.append(cur)
-                cur = []
-            in_marker = True
-            if groups == 1:
-                calib.append(v)
-            continue
-        in_marker = False
-        if "k_zero_u64" in name or name.startswith("__amd"):
-            continue
-        sym = re.search(r"k_\w+(<[^>]*>)?", name)
-        cur.append((sym.group(0) if sym else name, v))
-    return calib, segs
+                disp[c][key] = (name, disp[c].get(key, (name, 0.0))[1] + float(row["Counter_Value"]))
+        agent = {}
+        for f in glob.glob(os.path.join(d, "**", "*agent_info.csv"), recursive=True):
+            for row in csv.DictReader(open(f)):
+                if int(row.get("Simd_Count") or 0) > 0:
+                    agent = dict(simd=int(row["Simd_Count"]), xcc=int(row.get("Num_Xcc") or 1),
+                                 cu=int(row.get("Cu_Count") or 0))
+    out = {}
+    for c in counters:
+        if not disp[c]:
+            return None
+        calib, segs, cur, groups, in_marker = [], [], [], 0, False
+        for key in sorted(disp[c]):
+            name, v = disp[c][key]
+            if "k_copy_u4" in name:
+                if not in_marker:
+                    groups += 1
+                    if groups > 1:
+                        segs.append(cur)
+                    cur = []
+                in_marker = True
+                if groups == 1:
+                    calib.append(v)
+                continue
+            in_marker = False
+            if "k_zero_u64" in name or name.startswith("__amd"):
+                continue
+            sym = re.search(r"k_\w+(<[^>]*>)?", name)
+            cur.append((sym.group(0) if sym else name, v))
+        out[c] = (calib, segs)
+    return out, agent
 
 
 def trace_summary(trace_dir, spec):
@@ -1397,6 +1476,7 @@ def run_suite(args, xgm):
     S.batch()
     S.rmat23()
     traffic = None if args.no_traffic else S.traffic()
+    issue = None if args.no_traffic else S.issue()
     head = S.rows[0]
     assert head["row"] == "tc_rmat22"
     dom = head["dominant_kernel"]
@@ -1437,6 +1517,7 @@ def run_suite(args, xgm):
         "count_matches_golden": head["count_matches_golden"],
         "matches_per_s": head["matches_per_s"],
         "roofline": roof,
+        "issue": head.get("issue"),
         "cpu_baseline": head["cpu_baseline"],
         "learned_vs_heuristic": {
             "rmat23_k5_speedup_vs_rank_heuristic": next(r for r in S.rows if r["row"] == "k5_rmat23_learned")
@@ -1452,6 +1533,7 @@ def run_suite(args, xgm):
         "host_cpu": dict(S.info, threads_used=S.threads, omp_proc_bind=os.environ.get("OMP_PROC_BIND"),
                          omp_places=os.environ.get("OMP_PLACES")),
         "traffic_calibration": traffic,
+        "issue_counters": issue,
         "wall_s": round(time.time() - t_start, 1),
     }
     for k in ("kernels_ms", "kernel_total_ms"):

@@ -98,21 +98,37 @@ __global__ __launch_bounds__(NT) void k_diamond(const int64_t *__restrict__ rowp
             h[u] = hash_slot(r[u].w, hb);
             q[u] = reinterpret_cast<const int4 *>(S.key)[h[u]];
           }
+          bool more[4];
+          u64 any = 0;
 #pragma unroll
           for (int u = 0; u < 4; ++u) {
             hit[u] = bucket_has(q[u], r[u].w);
-            if (__builtin_expect(r[u].valid && !hit[u] && q[u].w >= 0, 0))
-              hit[u] = bhash_contains_after(S.key, hb, h[u], r[u].w);
-            hit[u] = hit[u] && r[u].valid;
+            more[u] = r[u].valid && !hit[u] && q[u].w >= 0;
+            any |= __ballot(more[u]);
           }
+          // one wave-uniform branch per group for the rare full home buckets
+          if (__builtin_expect(any != 0, 0)) {
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+              if (more[u]) hit[u] = bhash_contains_after(S.key, hb, h[u], r[u].w);
+          }
+#pragma unroll
+          for (int u = 0; u < 4; ++u) hit[u] = hit[u] && r[u].valid;
         } else {
 #pragma unroll
           for (int u = 0; u < 4; ++u) hit[u] = r[u].valid && sorted_contains(nt, dt, r[u].w);
         }
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
-          const int c = segment_run_hits(__ballot(hit[u]), r[u], lane);
-          if (c) atomicAdd(&S.cnt[r[u].j], c);
+          const u64 hm = __ballot(hit[u]);
+          if (r[u].heads == 1ull) {
+            // the window lies in one list (wave-uniform): its hits are one
+            // scalar popcount, added by one lane
+            if (lane == 0 && hm) atomicAdd(&S.cnt[r[u].j], __popcll(hm));
+          } else {
+            const int c = segment_run_hits(hm, r[u], lane);
+            if (c) atomicAdd(&S.cnt[r[u].j], c);
+          }
         }
       });
       __syncthreads();
@@ -265,8 +281,13 @@ __global__ __launch_bounds__(NT) void k_diamond_big(const int64_t *__restrict__ 
       wave_share(L, wave, NT / 64, lo, hi);
       wave_walk<4, true>(S.seg, colidx, ns, lo, hi, lane, fl, [&](const WindowLane &r) {
         const bool hit = r.valid && contains(r.w);
-        const int h = segment_run_hits(__ballot(hit), r, lane);
-        if (h) atomicAdd(&S.cnt[r.j], h);
+        const u64 hm = __ballot(hit);
+        if (r.heads == 1ull) {  // one list (wave-uniform): one lane adds the popcount
+          if (lane == 0 && hm) atomicAdd(&S.cnt[r.j], __popcll(hm));
+        } else {
+          const int h = segment_run_hits(hm, r, lane);
+          if (h) atomicAdd(&S.cnt[r.j], h);
+        }
       });
       __syncthreads();
       for (int q = tid; q < ns; q += NT) {
