@@ -36,6 +36,7 @@ sys.path.insert(0, os.path.join(ROOT, "tests"))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+VALU_PEAK_CLOCK_GHZ = 2.4  # MI355X peak engine clock
 COPY_BYTES = 1 << 30  # copy kernel size for bandwidth and counter calibration
 PATTERN_IDS = {"triangle": 1, "clique4": 2, "clique5": 3, "diamond": 4, "rectangle": 5}
 # workload name -> (generator, params, golden key, graph the patterns run on)
@@ -1222,7 +1223,8 @@ class Suite:
         4-cycle wave64 issue: busy = 4 ACTIVE / (SIMDs x cycles per XCC)),
         the active-lane fraction of those instructions, SALU per VALU
         instruction, LDS active cycles per CU and the LDS bank-conflict share
-        of them.  The LUT kernels are bound by this issue rate, not by HBM."""
+        of them; and the VALU issue roofline (roofline_valu) of that kernel.  The
+        LUT kernels are bound by this issue rate, not by HBM."""
         spec = os.path.join(self.cache, "rows.json")
         json.dump(self.pass_rows, open(spec, "w"))
         log("issue counters: rocprofv3 --pmc " + " ".join(self.ISSUE_COUNTERS))
@@ -1264,12 +1266,16 @@ class Suite:
                 lds_active=round(v["SQ_LDS_IDX_ACTIVE"] / (cu * cyc), 4),
                 lds_bank_conflict_share=round(v["SQ_LDS_BANK_CONFLICT"] / max(1.0, v["SQ_LDS_IDX_ACTIVE"]), 4),
                 cycles_per_xcc=int(cyc), counters={c: v[c] for c in self.ISSUE_COUNTERS})
-            if r.get("bound") == "valu" or r["row"].startswith(("k4_", "k5_")):
-                # the LUT rows: LDS/VALU-bound by design, so their roofline is
-                # the VALU issue rate (one wave64 instruction per SIMD per 4 cycles)
-                r["roofline_valu"] = dict(bound="valu", kernel=dom, achieved=round(busy, 4), peak=1.0,
-                                          unit="fraction of SIMD cycles issuing a VALU instruction",
-                                          frac=round(busy, 4))
+            ms = (r.get("kernels_ms") or r.get("kernels_ms_eager") or {}).get(dom)
+            if ms:
+                # VALU issue roofline: one wave64 VALU instruction per SIMD per
+                # 4 cycles at the 2.4 GHz peak engine clock; achieved = issue
+                # slots used per second over the kernel's HIP-event time
+                ach = v["SQ_ACTIVE_INST_VALU"] / (ms * 1e-3) / 1e9
+                peak = simd * VALU_PEAK_CLOCK_GHZ / 4.0
+                r["roofline_valu"] = dict(bound="valu", kernel=dom, achieved=round(ach, 1), peak=round(peak, 1),
+                                          unit="G wave64 VALU issues/s", frac=round(ach / peak, 4),
+                                          avg_launch_ms=round(ms, 4))
         return dict(simd=simd, xcc=xcc, cu=cu)
 
     def close(self):
@@ -1518,6 +1524,7 @@ def run_suite(args, xgm):
         "matches_per_s": head["matches_per_s"],
         "roofline": roof,
         "issue": head.get("issue"),
+        "roofline_valu": head.get("roofline_valu"),
         "cpu_baseline": head["cpu_baseline"],
         "learned_vs_heuristic": {
             "rmat23_k5_speedup_vs_rank_heuristic": next(r for r in S.rows if r["row"] == "k5_rmat23_learned")
