@@ -1500,7 +1500,7 @@ def run_suite(args, xgm):
                 traffic=kr.get("traffic"), kernel=dom, alg_bytes_per_launch=kr["bytes"], avg_launch_ms=round(ms, 4),
                 alg_bytes_model="tc_kernel_bytes: per charging vertex t 32 + 4 d+(t) (N+(t) staged once), per "
                                 "charged edge 20 + 4 d+(partner)",
-                traffic_frac=kr.get("traffic_frac"),
+                traffic_frac=kr.get("traffic_frac"), fetch_bytes=kr.get("fetch_bytes"), write_bytes=kr.get("write_bytes"),
                 reference_op_stream=dict(bytes=ra["alg_bytes_per_launch"], definition=ra["definition"],
                                          achieved_gbs=round(ref_ach, 1), frac=round(ref_ach / HBM_PEAK_GBS, 4)))
     out = {
