@@ -108,14 +108,17 @@ struct HbmRows {
 // row-a words are the same address across the wave.  TRI: rows hold only
 // later ranks (rank-ordered LUT), so words below (j+1)/64 are skipped.
 // jq: 128 ints of LDS for this wave; rowq: the block's row counter, zero
-// on entry.  All lanes of every wave call.
+// on entry.  Only rows a0, a0 + astep, a0 + 2 astep, ... are counted (one
+// part of a task split over several blocks; a0 = 0, astep = 1 counts all).
+// All lanes of every wave call.
 template <int W, bool TRI, class Rows>
-__device__ __forceinline__ u64 lut_count_wq(Rows row, int s, int weff, int depth, int *jq, int *rowq, int lane) {
+__device__ __forceinline__ u64 lut_count_wq(Rows row, int s, int weff, int depth, int *jq, int *rowq, int lane,
+                                            int a0 = 0, int astep = 1) {
   u64 acc = 0;
   for (;;) {
     int a = 0;
     if (lane == 0) a = atomicAdd(rowq, 1);
-    a = __shfl(a, 0, 64);
+    a = a0 + astep * __shfl(a, 0, 64);
     if (a >= s) break;
     int pending = 0;
     for (int q = TRI ? (a >> 6) : 0; q <= weff; ++q) {
@@ -179,12 +182,13 @@ __device__ __forceinline__ u64 lut_count_wq(Rows row, int s, int weff, int depth
 // past the task's last one (wave-uniform branch per task); other widths take
 // the WMAX-wide loop with per-word predicates.
 template <int WMIN, int WMAX, bool TRI, class Rows>
-__device__ __forceinline__ u64 lut_count_wq_fit(Rows row, int s, int weff, int depth, int *jq, int *rowq, int lane) {
+__device__ __forceinline__ u64 lut_count_wq_fit(Rows row, int s, int weff, int depth, int *jq, int *rowq, int lane,
+                                                int a0 = 0, int astep = 1) {
   if constexpr (WMIN < WMAX) {
-    if (weff == WMIN) return lut_count_wq<WMIN, TRI>(row, s, WMIN, depth, jq, rowq, lane);
-    return lut_count_wq_fit<WMIN + 1, WMAX, TRI>(row, s, weff, depth, jq, rowq, lane);
+    if (weff == WMIN) return lut_count_wq<WMIN, TRI>(row, s, WMIN, depth, jq, rowq, lane, a0, astep);
+    return lut_count_wq_fit<WMIN + 1, WMAX, TRI>(row, s, weff, depth, jq, rowq, lane, a0, astep);
   } else {
-    return lut_count_wq<WMAX, TRI>(row, s, weff, depth, jq, rowq, lane);
+    return lut_count_wq<WMAX, TRI>(row, s, weff, depth, jq, rowq, lane, a0, astep);
   }
 }
 
@@ -523,7 +527,7 @@ __global__ __launch_bounds__(NT) void k_clique_vlut_rank_tri(const int64_t *__re
                                                              const int32_t *__restrict__ colidx,
                                                              const int64_t *__restrict__ rrowptr,
                                                              const int32_t *__restrict__ tasks, int ntasks, int min_s,
-                                                             int max_s, int depth, int *__restrict__ queue,
+                                                             int max_s, int depth, int nparts, int *__restrict__ queue,
                                                              u64 *__restrict__ out, u64 *__restrict__ err,
                                                              u64 *__restrict__ tlog, const int32_t *__restrict__ slots) {
   static_assert(W * 64 >= SMAX, "stage bounds");
@@ -532,11 +536,16 @@ __global__ __launch_bounds__(NT) void k_clique_vlut_rank_tri(const int64_t *__re
   WaveFlags fl;
   fl.init(S.flagrow[wave], lane);
   u64 acc = 0;
+  // item = task * nparts + piece: every piece builds the task's whole LUT
+  // and counts the rows a = piece (mod nparts), so a few large tasks spread
+  // over many blocks instead of one block each
+  const int nitems = ntasks * nparts;
   for (;;) {
     if (tid == 0) S.item = atomicAdd(queue, 1);
     __syncthreads();
-    const int t = S.item;
-    if (t >= ntasks) break;
+    const int item = S.item;
+    if (item >= nitems) break;
+    const int t = item / nparts, piece = item - t * nparts;
     const u64 t_start = tlog ? (u64)wall_clock64() : 0;
     u64 part = 0;
     const int v0 = tasks[t];
@@ -577,7 +586,8 @@ __global__ __launch_bounds__(NT) void k_clique_vlut_rank_tri(const int64_t *__re
       __syncthreads();
       static_assert(sizeof(S.key) >= (NT / 64) * 128 * sizeof(int), "queue space");
       part += lut_count_wq_fit<(W > 16 ? 17 : W / 4 + 1), W, true>(
-          TriRows{S.rows, weff}, s, weff, depth, reinterpret_cast<int *>(S.key) + wave * 128, &S.rowq, lane);
+          TriRows{S.rows, weff}, s, weff, depth, reinterpret_cast<int *>(S.key) + wave * 128, &S.rowq, lane, piece,
+          nparts);
     }
     task_flush(acc, part, slots, v0, out, tid & 63);
     __syncthreads();
@@ -1162,11 +1172,13 @@ hipError_t run_vlut_rank_tri(const char *name, const DevCSR &g, const TaskList &
   auto kern = k_clique_vlut_rank_tri<SMAX, W, NT>;
   int grid = grid_for(kern, NT, 0, l.num_cu, tl.count);
   l.begin(name, tl.count);
-  kern<<<grid, NT, 0, l.stream>>>(g.rowptr, g.colidx, g.rrowptr, tl.tasks, (int)tl.count, k - 1, SMAX, k - 1, queue,
-                                  l.d_out, l.d_err, tlog, l.slots);
+  kern<<<grid, NT, 0, l.stream>>>(g.rowptr, g.colidx, g.rrowptr, tl.tasks, (int)tl.count, k - 1, SMAX, k - 1, 1,
+                                  queue, l.d_out, l.d_err, tlog, l.slots);
   l.end();
   return hipGetLastError();
 }
+
+constexpr int kHubPieces = 64;  // most pieces one hub task of the triangular LDS kernel is split into
 
 // Hub tasks (1024 < d+ <= kHubLutMax): those up to kTriLutMax with packed
 // triangular rows in LDS, the rest with rows in HBM.  Both kernels walk the
@@ -1175,10 +1187,16 @@ hipError_t run_vlut_rank_hubs(const char *name_tri, const char *name_hbm, const 
                               u64 *slots, int *queue, u64 *tlog, Launch &l) {
   if (!tl.count) return hipSuccess;
   auto tri = k_clique_vlut_rank_tri<kTriLutMax, kTriLutMax / 64, 1024>;
-  int grid = grid_for(tri, 1024, 0, l.num_cu, tl.count);
+  // a hub task per block would leave most CUs idle behind the few largest
+  // tasks: split each into pieces (rows a = piece mod nparts) so that the
+  // list fills about two waves of resident blocks; per-task timing logs keep
+  // one block per task
+  const int resident = grid_for(tri, 1024, 0, l.num_cu, 1 << 30);
+  const int nparts = tlog ? 1 : (int)std::min<int64_t>(kHubPieces, std::max<int64_t>(1, (2 * resident + tl.count - 1) / tl.count));
+  int grid = grid_for(tri, 1024, 0, l.num_cu, tl.count * nparts);
   l.begin(name_tri, tl.count);
   tri<<<grid, 1024, 0, l.stream>>>(g.rowptr, g.colidx, g.rrowptr, tl.tasks, (int)tl.count, 1024 + 1,
-                                   kTriLutMax, k - 1, queue, l.d_out, l.d_err, tlog, l.slots);
+                                   kTriLutMax, k - 1, nparts, queue, l.d_out, l.d_err, tlog, l.slots);
   l.end();
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
