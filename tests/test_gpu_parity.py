@@ -368,6 +368,37 @@ def test_clique_hubs_hbm_lut(xgm, orc, ctx, nb, nl):
     dg.close()
 
 
+def test_clique_edge_hub_pairs_merge_path(xgm, orc, ctx):
+    """Edge tasks between two vertices of out-degree > 1024 take the merge
+    path for W1 = N+(v0) ∩ N+(v1) (the hybrid rule): the L vertices of the
+    hub graph, all pointing to the 1100 B vertices, get a sparse ring among
+    themselves, so DAG edges L_i -> L_j join two lists of 1100+ ids.  The
+    edge-task strategies and schedule codes must match the oracle."""
+    import numpy as np
+    _say("clique edge hub pairs merge path")
+    nb, nl = 1100, 1200
+    g = _hub_graph(xgm, nb=nb, nl=nl)
+    L = 1 + nb + np.arange(nl)
+    src = np.concatenate([np.repeat(np.arange(g.n), np.diff(g.rowptr)), L, L[::5]])
+    dst = np.concatenate([g.colidx, np.roll(L, 1), np.roll(L, 7)[::5]])
+    d = xgm.from_edges(g.n, src, dst).orient()
+    dout = np.diff(d.rowptr)
+    both = (dout[np.repeat(np.arange(d.n), dout)] > 1024) & (dout[d.colidx] > 1024)
+    assert both.sum() > 1000  # hub x hub DAG edges exist
+    want4 = orc.count("clique4", d.rowptr, d.colidx)[0]
+    want5 = orc.count("clique5", d.rowptr, d.colidx)[0]
+    dg = ctx.upload(d)
+    for strat in (xgm.STRAT_LIST, xgm.STRAT_ELUT):
+        assert ctx.count(dg, 2, strat) == want4, strat
+        assert ctx.count(dg, 3, strat) == want5, strat
+    task = dout >= 4
+    for c in (2, 3):
+        s = xgm.Schedule(dg, 3, np.where(task, c, -1))
+        assert s.count() == want5, c
+        s.close()
+    dg.close()
+
+
 @pytest.mark.parametrize("n,spread", [(40000, 1), (20000, 1), (20000, 97)])
 def test_sgl_sampled_hub_owner(xgm, ctx, orc, n, spread):
     """An owner of degree n - 1: above the 36,352-entry LDS set the probes go

@@ -874,7 +874,23 @@ __global__ __launch_bounds__(128) void k_clique_vlut_packed(const int64_t *__res
 // encoded as a bitmap sub-DAG and K4 adds its edges, K5 its triangles
 // (GLUMIN's edge-subgraph scheme); without LUT (or |W1| > SMAX) the nested
 // list intersections of automine_4clique / automine_5clique run instead.
+// W1 follows the reference's hybrid rule (intersect_num_hybrid,
+// xgminer/include/glumin/operations.cuh:208-300): when both lists exceed
+// kHybridMin, wave 0 merges them along the wave64 merge path; otherwise
+// every thread binary-searches one element of N+(v1) in N+(v0).
 // scratch: per block 2*md ints in HBM for sets larger than SMAX.
+template <bool ON>
+struct EdgeMergeLds {  // wave 0's merge-path tiles (the 1024-row edge kernels only)
+  int As[kMergeTile];
+  int Bs[kMergeTile + 1];
+  int split[64];
+  int n;
+};
+template <>
+struct EdgeMergeLds<false> {
+  int n;
+};
+
 template <int SMAX, int W, int NT>
 __global__ __launch_bounds__(NT) void k_clique_edge(const int64_t *__restrict__ rowptr,
                                                     const int32_t *__restrict__ colidx,
@@ -883,6 +899,7 @@ __global__ __launch_bounds__(NT) void k_clique_edge(const int64_t *__restrict__ 
                                                     int *__restrict__ queue, u64 *__restrict__ out,
                                                     u64 *__restrict__ tlog, const int32_t *__restrict__ slots) {
   __shared__ CStage<SMAX, W, NT> S;
+  __shared__ EdgeMergeLds<(SMAX >= 1024)> mg;
   const int tid = threadIdx.x;
   WaveFlags fl;
   fl.init(S.flagrow[tid >> 6], tid & 63);
@@ -902,7 +919,21 @@ __global__ __launch_bounds__(NT) void k_clique_edge(const int64_t *__restrict__ 
     const int d0 = (int)(rowptr[it.x + 1] - b0), d1 = (int)(rowptr[it.y + 1] - b1);
     if (d0 >= k - 2 && d1 >= k - 2) {
       int *w1 = (min(d0, d1) <= SMAX) ? S.x : g_w1;
-      const int n1 = block_intersect_ordered<NT>(colidx + b1, d1, Member{colidx + b0, d0}, w1, S);
+      int n1 = 0;
+      bool merged = false;
+      if constexpr (SMAX >= 1024) {
+        if (d0 > kHybridMin && d1 > kHybridMin) {
+          if (tid < 64) {
+            const uint32_t c = merge_walk(colidx + b1, d1, colidx + b0, d0, mg.As, mg.Bs, mg.split, tid,
+                                          [](int, bool found) { return found; }, w1);
+            if (tid == 0) mg.n = (int)c;
+          }
+          __syncthreads();  // W1 written by wave 0
+          n1 = mg.n;
+          merged = true;
+        }
+      }
+      if (!merged) n1 = block_intersect_ordered<NT>(colidx + b1, d1, Member{colidx + b0, d0}, w1, S);
       if (n1 >= k - 2) {
         bool done = false;
         if constexpr (W > 0) {
