@@ -282,6 +282,17 @@ int xgm_count_batch(xgm_ctx *ctx, const xgm_csr *gs, const int *patterns, int n_
 typedef struct xgm_plan xgm_plan;
 int xgm_plan_create(xgm_ctx *ctx, const xgm_dgraph *dag, const xgm_dgraph *und, const int *patterns, int n_pat,
                     const xgm_opts *opts, xgm_plan **out);
+/* As xgm_plan_create, with an explicit clique schedule per pattern: scheds
+ * (NULL, or one entry per pattern, NULL = the pattern's opts) gives a
+ * schedule made by xgm_sched_create on `dag` for that pattern -- for
+ * instance the learned codes of xgm_sched_learned -- so the learned
+ * schedule replays inside the captured batch.  The plan keeps its own copy
+ * of each schedule's task lists: a schedule may be freed after this call.
+ * No counterpart in the reference (its GLUMIN clique path has one fixed
+ * edge-parallel schedule, glumin_GM_solver.cu:1136-1145). */
+typedef struct xgm_sched xgm_sched;
+int xgm_plan_create_sched(xgm_ctx *ctx, const xgm_dgraph *dag, const xgm_dgraph *und, const int *patterns,
+                          int n_pat, const xgm_opts *opts, const xgm_sched *const *scheds, xgm_plan **out);
 /* out: n_pat x n_inst counts (pattern-major); *ms = replay time (HIP events). */
 int xgm_plan_run(xgm_ctx *ctx, xgm_plan *plan, uint64_t *out, float *ms);
 void xgm_plan_free(xgm_plan *plan);
@@ -350,8 +361,7 @@ int xgm_mlp_grads(xgm_mlp *m, const float *X, const float *T, const float *M, in
  * 1024 < d+ <= 4096 the rank-ordered codes (1, 4) run the hub LUT (rows
  * packed upper-triangular in LDS up to 1344, in HBM above); other LUT codes
  * above 1024 run as edge LUT.  Every schedule gives the same
- * count. */
-typedef struct xgm_sched xgm_sched;
+ * count.  (xgm_sched is declared with the plans above.) */
 /* Degree-threshold codes of a strategy (XGM_STRAT_*), no model: AUTO and
  * LUT_RANK = rank-ordered LUT up to lut_max_degree and for hubs (1024 < d+ <=
  * 4096), edge LUT between lut_max_degree and 1024 and above 4096; LUT =

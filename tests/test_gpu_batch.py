@@ -101,6 +101,35 @@ def test_batch_per_instance_counts(xgm, ctx, mixed):
     und.close()
 
 
+def test_batch_plan_learned_schedule(xgm, ctx, mixed):
+    """Learned + batched: the cost model's codes over the whole batch DAG
+    (one schedule for all instances), captured with the other patterns in one
+    plan and replayed; every per-instance count equals the oracle's, and the
+    plan keeps replaying after the schedule object is freed."""
+    from xgminer_amd import scheduler
+    inst, want = mixed
+    _say("plan with learned clique schedules")
+    dag = ctx.upload_batch([g.orient() for g in inst])
+    und = ctx.upload_batch(inst)
+    model = scheduler.load()
+    s5 = xgm.Schedule(dag, xgm.CLIQUE5, scheduler.schedule(dag, model))
+    s4 = xgm.Schedule(dag, xgm.CLIQUE4, dag.sched_heuristic(xgm.CLIQUE4, xgm.STRAT_ELUT))  # edge-LUT items
+    pats = [1, 2, 3, 4]
+    plan = ctx.plan(pats, dag=dag, und=und, scheds=[None, s4, s5, None])
+    s4.close()
+    s5.close()
+    expect = np.stack([_col(want, p) for p in ("triangle", "clique4", "clique5", "diamond")])
+    for rep in range(3):
+        _say("learned plan replay", rep)
+        got, ms = plan.run()
+        assert np.array_equal(got, expect), rep
+        assert ms > 0
+        assert ctx.count(dag, 3) == int(expect[2].sum())  # eager work between replays
+    plan.close()
+    dag.close()
+    und.close()
+
+
 def test_batch_plan_replays(xgm, ctx, mixed):
     inst, want = mixed
     _say("plan capture")

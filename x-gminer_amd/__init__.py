@@ -129,6 +129,7 @@ def _load():
         "xgm_graph_instances": (C.c_int, [VP]),
         "xgm_count_instances": (C.c_int, [VP, VP, C.c_int, P(_Opts), P(U64)]),
         "xgm_plan_create": (C.c_int, [VP, VP, VP, P(C.c_int), C.c_int, P(_Opts), P(VP)]),
+        "xgm_plan_create_sched": (C.c_int, [VP, VP, VP, P(C.c_int), C.c_int, P(_Opts), P(VP), P(VP)]),
         "xgm_plan_run": (C.c_int, [VP, VP, P(U64), P(C.c_float)]),
         "xgm_plan_free": (None, [VP]),
         "xgm_autotune": (C.c_int, [VP, VP, C.c_int, C.c_int, P(_Opts), P(C.c_float)]),
@@ -428,8 +429,10 @@ class Context:
                                            len(a), counts.ctypes.data_as(C.POINTER(C.c_uint32))), self._h)
         return counts
 
-    def plan(self, patterns, dag=None, und=None, opts=None) -> "Plan":
-        return Plan(self, patterns, dag, und, opts)
+    def plan(self, patterns, dag=None, und=None, opts=None, scheds=None) -> "Plan":
+        """scheds: None, or one Schedule (or None) per pattern: clique
+        patterns then replay that schedule (e.g. the learned one)."""
+        return Plan(self, patterns, dag, und, opts, scheds)
 
     def count(self, g, pattern: int, strategy: int = STRAT_AUTO, lut_max_degree: int = 0, **kw) -> int:
         opts = _opts(strategy, lut_max_degree, **kw)
@@ -481,14 +484,21 @@ class Plan:
     """Counts of several patterns over resident batches captured as one
     hipGraph (xgm_plan_create); run() replays it."""
 
-    def __init__(self, ctx: Context, patterns, dag=None, und=None, opts=None):
+    def __init__(self, ctx: Context, patterns, dag=None, und=None, opts=None, scheds=None):
         pats = (C.c_int * len(patterns))(*patterns)
         oa = None
         if opts is not None:
             oa = (_Opts * len(patterns))(*[_opts(**o) for o in opts])
         h = C.c_void_p()
-        _check(lib.xgm_plan_create(ctx._h, dag._h if dag else None, und._h if und else None, pats, len(patterns),
-                                   oa, C.byref(h)), ctx._h)
+        if scheds is None:
+            _check(lib.xgm_plan_create(ctx._h, dag._h if dag else None, und._h if und else None, pats,
+                                       len(patterns), oa, C.byref(h)), ctx._h)
+        else:
+            if len(scheds) != len(patterns):
+                raise ValueError("one schedule (or None) per pattern")
+            sa = (C.c_void_p * len(patterns))(*[s._h if s is not None else None for s in scheds])
+            _check(lib.xgm_plan_create_sched(ctx._h, dag._h if dag else None, und._h if und else None, pats,
+                                             len(patterns), oa, sa, C.byref(h)), ctx._h)
         self._h, self.ctx, self.patterns = h, ctx, list(patterns)
         # the captured kernels read these graphs' buffers: keep them alive
         # as long as the plan (the library also defers their release)

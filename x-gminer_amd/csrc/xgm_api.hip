@@ -1003,6 +1003,7 @@ void xgm_graph_free(xgm_dgraph *g) {
 }
 
 static DevCSR dev_view(const xgm_dgraph *g);
+static int clique_k(int pattern);
 
 // Host copy of a device graph's CSR (task preparation of the diamond,
 // 4-cycle and motif paths, and of clique edge items), copied from HBM on
@@ -1647,10 +1648,14 @@ struct xgm_plan {
   hipEvent_t ev[2] = {nullptr, nullptr};
   // graphs the captured kernels read (each holds one plan reference)
   xgm_dgraph *uses[2] = {nullptr, nullptr};
+  // copies of the explicit clique schedules (their task lists are read by
+  // the captured kernels), one slot per pattern
+  std::vector<std::unique_ptr<xgm_sched>> scheds;
   ~xgm_plan() {
     (void)hipSetDevice(device);
     if (exec) (void)hipGraphExecDestroy(exec);
     if (graph) (void)hipGraphDestroy(graph);
+    scheds.clear();
     for (auto e : ev)
       if (e) (void)hipEventDestroy(e);
     (void)hipFree(d_out);
@@ -1966,10 +1971,17 @@ int xgm_setop_pairs_noalloc(xgm_ctx *ctx, const xgm_dgraph *g, int op, const int
   return XGM_OK;
 }
 
-int xgm_plan_create(xgm_ctx *ctx, const xgm_dgraph *dag, const xgm_dgraph *und, const int *patterns, int n_pat,
-                    const xgm_opts *opts, xgm_plan **out) {
+static int plan_create_impl(xgm_ctx *ctx, const xgm_dgraph *dag, const xgm_dgraph *und, const int *patterns,
+                            int n_pat, const xgm_opts *opts, const xgm_sched *const *scheds, xgm_plan **out) {
   if (!ctx || !patterns || n_pat < 1 || n_pat > 16 || !out) return XGM_EINVAL;
   *out = nullptr;
+  for (int i = 0; scheds && i < n_pat; ++i) {
+    const xgm_sched *s = scheds[i];
+    if (!s) continue;
+    if (!dag || s->pattern != patterns[i] || !clique_k(patterns[i]))
+      return ctx->fail(XGM_EINVAL, "plan schedule " + std::to_string(i) + " is not a clique schedule of its pattern");
+    if (s->device != ctx->device) return ctx->fail(XGM_EINVAL, "plan schedule belongs to another device");
+  }
   int n_inst = -1;
   for (int i = 0; i < n_pat; ++i) {
     const xgm_dgraph *g = dag_pattern(patterns[i]) ? dag : und;
@@ -1985,12 +1997,30 @@ int xgm_plan_create(xgm_ctx *ctx, const xgm_dgraph *dag, const xgm_dgraph *und, 
   std::vector<uint64_t> tmp(n_inst);
   for (int i = 0; i < n_pat; ++i) {
     const xgm_dgraph *g = dag_pattern(patterns[i]) ? dag : und;
-    int rc = count_into(ctx, g, patterns[i], opts ? &opts[i] : nullptr, tmp.data(), true);
+    int rc = (scheds && scheds[i]) ? xgm_count_sched(ctx, g, scheds[i], nullptr, tmp.data())
+                                   : count_into(ctx, g, patterns[i], opts ? &opts[i] : nullptr, tmp.data(), true);
     if (rc) return rc;
   }
   std::unique_ptr<xgm_plan> p(new (std::nothrow) xgm_plan);
   if (!p) return XGM_ENOMEM;
   p->device = ctx->device;
+  p->scheds.resize(n_pat);
+  for (int i = 0; scheds && i < n_pat; ++i) {
+    const xgm_sched *s = scheds[i];
+    if (!s) continue;
+    std::unique_ptr<xgm_sched> c(new (std::nothrow) xgm_sched);
+    if (!c) return XGM_ENOMEM;
+    c->pattern = s->pattern;
+    c->k = s->k;
+    c->device = s->device;
+    std::copy(s->off, s->off + kClqLists, c->off);
+    std::copy(s->cnt, s->cnt + kClqLists, c->cnt);
+    // the lists are laid out back to back; the last (hub tasks) is 1 int per item
+    const size_t ints = std::max<size_t>(2, (size_t)(s->off[kClqLists - 1] + s->cnt[kClqLists - 1]));
+    HIP_TRY(ctx, hipMalloc(&c->d_buf, ints * sizeof(int32_t)));
+    HIP_TRY(ctx, hipMemcpy(c->d_buf, s->d_buf, ints * sizeof(int32_t), hipMemcpyDeviceToDevice));
+    p->scheds[i] = std::move(c);
+  }
   p->n_inst = n_inst;
   p->patterns.assign(patterns, patterns + n_pat);
   p->out_words = (((size_t)n_pat * n_inst + 1) + 1) & ~(size_t)1;
@@ -2013,7 +2043,10 @@ int xgm_plan_create(xgm_ctx *ctx, const xgm_dgraph *dag, const xgm_dgraph *und, 
     l.d_err = p->d_out + (size_t)n_pat * n_inst;
     l.d_queue = p->d_queues + (size_t)i * kScratchQueues;
     l.slots = g->d_slot;
-    rc = dispatch_pattern(ctx, g, patterns[i], opts ? &opts[i] : nullptr, l);
+    if (p->scheds[i])
+      rc = run_clique_sched(ctx, const_cast<xgm_dgraph *>(g), p->scheds[i].get(), dev_view(g), l, nullptr);
+    else
+      rc = dispatch_pattern(ctx, g, patterns[i], opts ? &opts[i] : nullptr, l);
   }
   hipGraph_t graph = nullptr;
   e = hipStreamEndCapture(ctx->stream, &graph);
@@ -2036,6 +2069,16 @@ int xgm_plan_create(xgm_ctx *ctx, const xgm_dgraph *dag, const xgm_dgraph *und, 
   }
   *out = p.release();
   return XGM_OK;
+}
+
+int xgm_plan_create(xgm_ctx *ctx, const xgm_dgraph *dag, const xgm_dgraph *und, const int *patterns, int n_pat,
+                    const xgm_opts *opts, xgm_plan **out) {
+  return plan_create_impl(ctx, dag, und, patterns, n_pat, opts, nullptr, out);
+}
+
+int xgm_plan_create_sched(xgm_ctx *ctx, const xgm_dgraph *dag, const xgm_dgraph *und, const int *patterns,
+                          int n_pat, const xgm_opts *opts, const xgm_sched *const *scheds, xgm_plan **out) {
+  return plan_create_impl(ctx, dag, und, patterns, n_pat, opts, scheds, out);
 }
 
 int xgm_plan_run(xgm_ctx *ctx, xgm_plan *plan, uint64_t *out, float *ms) {
