@@ -1139,13 +1139,27 @@ class Suite:
             res[name] = t
         if res["learned"]["count"] != res["rank"]["count"]:
             raise SystemExit("RMAT-23 schedules disagree")
+        # learned + captured: the learned schedule replayed as a hipGraph plan
+        s = self.xgm.Schedule(dg, 3, lc)
+        plan = self.ctx.plan([3], dag=dg, scheds=[s])
+        s.close()
+        replay = []
+        for i in range(min(a.steps, 5) + 1):
+            got, ms = plan.run()
+            if int(got[0, 0]) != res["learned"]["count"]:
+                raise SystemExit("RMAT-23 learned plan replay disagrees")
+            replay.append(ms)
+        plan.close()
+        plan_ms = float(np.median(replay[1:]))
+        log(f"rmat23 learned schedule as a plan replay: {plan_ms:.2f} ms")
         np.save(os.path.join(self.cache, "rmat23_k5_learned.npy"), lc)
         for name, codes in (("learned", lc), ("rank", rank)):
             extra = dict(code_hist=np.bincount(codes[codes >= 0], minlength=5).tolist())
             if name == "learned":
                 extra.update(schedule_build_ms=round(sched_ms, 2),
                              speedup_vs_rank_heuristic=round(res["rank"]["ms_per_step"] /
-                                                             res["learned"]["ms_per_step"], 4))
+                                                             res["learned"]["ms_per_step"], 4),
+                             learned_plan_replay_ms=round(plan_ms, 3))
             self.row(f"k5_rmat23_{name}", desc + (", learned schedule" if name == "learned" else
                                                   ", rank-ordered LUT degree thresholds (model-free heuristic)"),
                      "clique5", m, res[name], want, clique_kernel_bytes(d, codes, 5), "clique_kernel_bytes", cpu,

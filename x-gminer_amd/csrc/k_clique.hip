@@ -388,11 +388,53 @@ __device__ __forceinline__ void block_bitonic_sort(u64 *key, int *val, int n) {
   }
 }
 
+// Member ranks of a rank-LUT task in an LDS hash: 2^bbits buckets of two
+// (id, rank) slots in 16 bytes, empty id -1, at least two slots per member.
+// A lookup reads one bucket and compares two ids; only a full bucket without
+// the id continues to the next one.
+__device__ __forceinline__ void rank_hash_insert(int2 *slot, int bbits, int key, int rank) {
+  const uint32_t mask = (1u << bbits) - 1u;
+  uint32_t h = hash_slot(key, bbits);
+  for (;;) {
+    int2 *b = slot + 2 * h;
+    if (atomicCAS(&b[0].x, -1, key) == -1) {
+      b[0].y = rank;
+      return;
+    }
+    if (atomicCAS(&b[1].x, -1, key) == -1) {
+      b[1].y = rank;
+      return;
+    }
+    h = (h + 1) & mask;
+  }
+}
+
+// rank of key, or -1 if it is not a member
+__device__ __forceinline__ int rank_hash_find(const int2 *slot, int bbits, int key) {
+  const uint32_t mask = (1u << bbits) - 1u;
+  uint32_t h = hash_slot(key, bbits);
+  int4 q = reinterpret_cast<const int4 *>(slot)[h];
+  int rank = q.x == key ? q.y : (q.z == key ? q.w : -1);
+  bool more = rank < 0 && q.z >= 0;
+  while (__builtin_expect(more, 0)) {
+    h = (h + 1) & mask;
+    q = reinterpret_cast<const int4 *>(slot)[h];
+    rank = q.x == key ? q.y : (q.z == key ? q.w : -1);
+    more = rank < 0 && q.z >= 0;
+  }
+  return rank;
+}
+
 // Rank-ordered vertex LUT (strategy LUT_RANK): X = N+(v0) stays id-sorted
 // for lookup; perm[i] = position of X[i] in (undirected degree, id) order,
 // which is the DAG orientation order, so the LUT is upper-triangular.
+// Stages up to 256 members find a probed id's rank in a rank hash (hslot,
+// SMAX buckets); larger ones binary-search X.
 template <int SMAX, int W, int NT>
 struct RStage {
+  static constexpr bool kHash = SMAX <= 256;
+  static constexpr int kHashBits = kHash ? 31 - __builtin_clz(SMAX) : 0;  // SMAX is a power of two here
+  alignas(16) int2 hslot[kHash ? 2 * SMAX : 2];
   int x[SMAX];
   int perm[SMAX];
   u64 rows[lut_words<W>(SMAX)];
@@ -409,7 +451,7 @@ struct RStage {
 };
 
 template <int SMAX, int W, int NT>
-__global__ __launch_bounds__(NT) void k_clique_vlut_rank(const int64_t *__restrict__ rowptr,
+__global__ __launch_bounds__(NT, (SMAX <= 256 ? 7 : 1)) void k_clique_vlut_rank(const int64_t *__restrict__ rowptr,
                                                          const int32_t *__restrict__ colidx,
                                                          const int64_t *__restrict__ rrowptr,
                                                          const int32_t *__restrict__ tasks, int ntasks, int min_s,
@@ -441,21 +483,38 @@ __global__ __launch_bounds__(NT) void k_clique_vlut_rank(const int64_t *__restri
         S.val[q] = q;
       }
       for (int q = tid; q < lut_words<W>(s); q += NT) S.rows[q] = 0ull;
+      using RS = RStage<SMAX, W, NT>;
+      if constexpr (RS::kHash)
+        for (int q = tid; q < 2 * SMAX; q += NT) S.hslot[q] = make_int2(-1, 0);
       __syncthreads();
       block_bitonic_sort<SMAX, NT>(S.key, S.val, s);
       for (int r = tid; r < s; r += NT) S.perm[S.val[r]] = r;
       __syncthreads();  // keys are overwritten by the segment table
+      if constexpr (RS::kHash)
+        for (int q = tid; q < s; q += NT) rank_hash_insert(S.hslot, RS::kHashBits, S.x[q], S.perm[q]);
       block_segments<NT>(S.seg, S.tmp, rowptr, colidx, S.x, 0, s, -1, KeepAll(), &S.ns, &S.L);
+      if constexpr (RS::kHash) {
+        // segment j's member index becomes its rank (the LUT row)
+        for (int j = tid; j < S.ns; j += NT) S.seg.idx[j] = S.perm[S.seg.idx[j]];
+        __syncthreads();
+      }
       int lo, hi;
       wave_share(S.L, wave, NT / 64, lo, hi);
       wave_walk<4, true>(S.seg, colidx, S.ns, lo, hi, lane, fl, [&](const WindowLane &r) {
         if (r.valid) {
-          const int pos = lower_bound_dev(S.x, s, r.w);
-          if (pos < s && S.x[pos] == r.w) {
-            const int col = S.perm[pos], row = S.perm[S.seg.idx[r.j]];
-            if (col <= row) atomicOr(err, 1ull);  // DAG not in (degree, id) orientation
-            atomicOr(&S.rows[lut_word<W>(row, col >> 6)], 1ull << (col & 63));
+          int col, row;
+          if constexpr (RS::kHash) {
+            col = rank_hash_find(S.hslot, RS::kHashBits, r.w);
+            if (col < 0) return;
+            row = S.seg.idx[r.j];
+          } else {
+            const int pos = lower_bound_dev(S.x, s, r.w);
+            if (!(pos < s && S.x[pos] == r.w)) return;
+            col = S.perm[pos];
+            row = S.perm[S.seg.idx[r.j]];
           }
+          if (col <= row) atomicOr(err, 1ull);  // DAG not in (degree, id) orientation
+          atomicOr(&S.rows[lut_word<W>(row, col >> 6)], 1ull << (col & 63));
         }
       });
       if (tid == 0) S.rowq = 0;
