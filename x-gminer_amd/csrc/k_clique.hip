@@ -1506,6 +1506,8 @@ __global__ void k_clique_features(const int64_t *__restrict__ rowptr, const int3
   f[15] = 1.f;
 }
 
+constexpr float kSchedMargin = 0.2f;
+
 __global__ void k_sched_argmin(const int64_t *__restrict__ rowptr, int64_t n, const float *__restrict__ F,
                                const float *__restrict__ Y, int8_t *__restrict__ codes) {
   const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1516,13 +1518,18 @@ __global__ void k_sched_argmin(const int64_t *__restrict__ rowptr, int64_t n, co
   }
   const int d = (int)(rowptr[v + 1] - rowptr[v]);
   const float *y = Y + v * 5;
-  int best = 2;
-  float bv = y[2];
+  // the model-free code of this task (rank LUT; edge LUT above the hub LUT)
+  const int h = d <= kHubLutMax ? 1 : 2;
+  int best = h;
+  float bv = y[h];
   for (int c = 0; c < 5; ++c) {
     // id-ordered LUT up to 1024 rows; rank LUT up to kHubLutMax (hub kernels above 1024)
     const bool ok = (c == 2 || c == 3) || (c == 0 ? d <= 1024 : d <= kHubLutMax);
     if (ok && y[c] < bv) { bv = y[c]; best = c; }
   }
+  // leave the model-free code only for a predicted saving above the model's
+  // error (kSchedMargin in log2(1 + 1000 us) units, about 13% of the cost)
+  if (best != h && !(bv < y[h] - kSchedMargin)) best = h;
   codes[v] = (int8_t)best;
 }
 

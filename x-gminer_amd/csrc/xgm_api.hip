@@ -2246,7 +2246,11 @@ static void critical_path_guard(xgm_ctx *ctx, const xgm_dgraph *g, const std::ve
     lists[li].sum += sh;
     lists[li].walls.push_back({sh * clique_list_capacity(ctx->num_cu, li), (int32_t)v});
   }
-  for (auto &L : lists) {
+  for (int li = 0; li < kClqLists; ++li) {
+    // hub tasks (list 12) are split into row pieces over many blocks by
+    // their kernel, so they are not one block's tail
+    if (li == 12) continue;
+    auto &L = lists[li];
     std::sort(L.walls.begin(), L.walls.end(), [](const auto &x, const auto &y) { return x.first > y.first; });
     for (size_t i = 0; i < L.walls.size(); ++i) {
       const double top = L.walls[i].first;
