@@ -903,7 +903,7 @@ __global__ __launch_bounds__(128) void k_clique_vlut_packed(const int64_t *__res
       S.seg.idx[j] = g * 64 + (q - S.off[g]);  // LUT row of the segment's member
     }
     wave_sync();
-    wave_walk<4>(S.seg, colidx, ns, 0, L, lane, fl, [&](const WindowLane &r) {
+    wave_walk<4, true>(S.seg, colidx, ns, 0, L, lane, fl, [&](const WindowLane &r) {
       if (r.valid) {
         const int row = S.seg.idx[r.j], g = row >> 6;
         const int *xg = S.x + g * 64;

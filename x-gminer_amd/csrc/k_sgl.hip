@@ -410,7 +410,7 @@ __global__ __launch_bounds__(128) void k_diamond_packed(const int64_t *__restric
       s.cnt[j] = 0;
     }
     wave_sync();
-    wave_walk<4>(s.seg, colidx, ns, 0, L, lane, fl, [&](const WindowLane &r) {
+    wave_walk<4, true>(s.seg, colidx, ns, 0, L, lane, fl, [&](const WindowLane &r) {
       bool hit = false;
       if (r.valid) {
         const int g = s.seg.idx[r.j];
@@ -536,7 +536,7 @@ __global__ __launch_bounds__(NT) void k_cycle4_wedge(const int64_t *__restrict__
       const int ns = S.ns, L = S.L;
       int lo, hi;
       wave_share(L, wave, NT / 64, lo, hi);
-      wave_walk<4>(S.seg, colidx, ns, lo, hi, lane, fl, [&](const WindowLane &r) {
+      wave_walk<4, DENSE>(S.seg, colidx, ns, lo, hi, lane, fl, [&](const WindowLane &r) {
         if (r.valid) {
           const int old = DENSE ? atomicAdd(&cnt[r.w], 1) : hash_bump(S.key, S.val, hb, r.w);
           part += (u64)old;
@@ -653,7 +653,7 @@ __global__ __launch_bounds__(128) void k_cycle4_wedge_packed(const int64_t *__re
     if (lane == 0) S.seg.os[ns] = make_int2(0, L);
     wave_sync();
     u64 part = 0;
-    wave_walk<4>(S.seg, colidx, ns, 0, L, lane, fl, [&](const WindowLane &r) {
+    wave_walk<4, true>(S.seg, colidx, ns, 0, L, lane, fl, [&](const WindowLane &r) {
       if (r.valid) {
         const int g = S.seg.idx[r.j];
         const int old = hash_bump(S.key + g * PS::SLOTS, S.val + g * PS::SLOTS, 8, r.w);

@@ -249,7 +249,7 @@ __global__ __launch_bounds__(128) void k_tc_wave_packed(const int64_t *__restric
     for (int j = lane; j < ns; j += 64) s.seg.idx[j] = item_of(s.seg.idx[j], s.poff);
     wave_sync();
     int cnt = 0;
-    wave_walk<UNROLL>(s.seg, colidx, ns, 0, L, lane, fl, [&](const WindowLane &r) {
+    wave_walk<UNROLL, true>(s.seg, colidx, ns, 0, L, lane, fl, [&](const WindowLane &r) {
       bool hit = false;
       if (r.valid) {
         const int g = s.seg.idx[r.j];
