@@ -368,6 +368,36 @@ def test_clique_hubs_hbm_lut(xgm, orc, ctx, nb, nl):
     dg.close()
 
 
+def test_clique_hub_pieces(xgm, orc, ctx):
+    """Two hub tasks (out-degree 1100) and nothing else in the triangular LDS
+    LUT bin: each is split into 64 row pieces over as many blocks, and the
+    pieces' counts must add up to the oracle's."""
+    import numpy as np
+    _say("clique hub pieces")
+    rng = np.random.default_rng(5)
+    nb, nleaf = 1100, 1100
+    B = 2 + np.arange(nb)                      # two hubs 0, 1 point to every B vertex
+    leaves = 2 + nb + np.arange(nb * nleaf)    # each B vertex gets its own leaves, so deg(B) > deg(hub)
+    src = np.concatenate([np.zeros(nb, np.int64), np.ones(nb, np.int64), np.repeat(B, nleaf), np.repeat(B, 40)])
+    dst = np.concatenate([B, B, leaves, rng.choice(B, nb * 40)])
+    d = xgm.from_edges(2 + nb + nb * nleaf, src, dst).orient()
+    dout = np.diff(d.rowptr)
+    assert int(((dout > 1024) & (dout <= 1344)).sum()) == 2
+    want4 = orc.count("clique4", d.rowptr, d.colidx)[0]
+    want5 = orc.count("clique5", d.rowptr, d.colidx)[0]
+    assert want5 > 0
+    dg = ctx.upload(d)
+    for strat in (0, 6):
+        assert ctx.count(dg, 2, strat) == want4, strat
+        assert ctx.count(dg, 3, strat) == want5, strat
+    task = dout >= 4
+    for c in (1, 4):
+        s = xgm.Schedule(dg, 3, np.where(task, c, -1))
+        assert s.count() == want5, c
+        s.close()
+    dg.close()
+
+
 def test_clique_edge_hub_pairs_merge_path(xgm, orc, ctx):
     """Edge tasks between two vertices of out-degree > 1024 take the merge
     path for W1 = N+(v0) ∩ N+(v1) (the hybrid rule): the L vertices of the
