@@ -129,13 +129,19 @@ __device__ __forceinline__ void bhash_insert(int *tab, int bbits, int key) {
   }
 }
 
-// key in bucket q: the minimum of the four XORs is zero (straight-line
-// integer ops, no per-slot branches)
-__device__ __forceinline__ bool bucket_has(const int4 &q, int key) {
+// Minimum of the four XORs of key with bucket q: zero iff the bucket holds
+// key (straight-line integer ops, no per-slot branches or lane masks).
+__device__ __forceinline__ uint32_t bucket_dist(const int4 &q, int key) {
   const uint32_t k = (uint32_t)key;
-  const uint32_t d = min(min((uint32_t)q.x ^ k, (uint32_t)q.y ^ k), min((uint32_t)q.z ^ k, (uint32_t)q.w ^ k));
-  return d == 0u;
+  return min(min((uint32_t)q.x ^ k, (uint32_t)q.y ^ k), min((uint32_t)q.z ^ k, (uint32_t)q.w ^ k));
 }
+
+__device__ __forceinline__ bool bucket_has(const int4 &q, int key) { return bucket_dist(q, key) == 0u; }
+
+// Probe key of a walker's invalid lanes where the probed set is a bucketed
+// hash: ids are >= 0 and an empty slot is -1, so no bucket holds it and a
+// match implies a valid lane.
+constexpr int kNoKey = -2;
 
 // Rest of a lookup whose home bucket h was full and lacked the key.
 __device__ __noinline__ bool bhash_contains_after(const int *tab, int bbits, uint32_t h, int key) {

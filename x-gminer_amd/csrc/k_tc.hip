@@ -43,35 +43,35 @@ struct Stage {
   Segs<PCAP, false> seg;
 };
 
-// Hash membership of a group of U windows: the U home buckets are read
-// together (one LDS round trip for the group), the rare full-bucket case
-// continues in bhash_contains_after.  Adds the hits of valid lanes.
+// Hash hits of a group of U windows whose invalid lanes carry kNoKey: the U
+// home buckets are read together (one LDS round trip for the group), and
+// hits and the full-bucket misses are found with vector integer ops only --
+// no lane mask is formed per window, so the scalar unit waits on no vector
+// compare until the one wave-uniform branch per group into the rare
+// continuation (bhash_contains_after).  Returns this lane's hits.
 template <int U>
 __device__ __forceinline__ int group_hash_hits(const int *key, int hbits, const WindowLane (&r)[U]) {
-  uint32_t h[U];
   int4 q[U];
 #pragma unroll
-  for (int u = 0; u < U; ++u) {
-    h[u] = hash_slot(r[u].w, hbits);
-    q[u] = reinterpret_cast<const int4 *>(key)[h[u]];
-  }
-  bool hit[U], more[U];
-  u64 any = 0;
-#pragma unroll
-  for (int u = 0; u < U; ++u) {
-    hit[u] = bucket_has(q[u], r[u].w);
-    more[u] = r[u].valid && !hit[u] && q[u].w >= 0;
-    any |= __ballot(more[u]);
-  }
-  // one wave-uniform branch per group for the rare full home buckets
-  if (__builtin_expect(any != 0, 0)) {
-#pragma unroll
-    for (int u = 0; u < U; ++u)
-      if (more[u]) hit[u] = bhash_contains_after(key, hbits, h[u], r[u].w);
-  }
+  for (int u = 0; u < U; ++u) q[u] = reinterpret_cast<const int4 *>(key)[hash_slot(r[u].w, hbits)];
   int c = 0;
+  uint32_t open = 0;
 #pragma unroll
-  for (int u = 0; u < U; ++u) c += (r[u].valid && hit[u]) ? 1 : 0;
+  for (int u = 0; u < U; ++u) {
+    const uint32_t d = bucket_dist(q[u], r[u].w);
+    c += d == 0u ? 1 : 0;
+    open |= q[u].w >= 0 ? d : 0u;  // nonzero: a miss in a full home bucket
+  }
+  if (__builtin_expect(__ballot(open != 0u) != 0, 0)) {
+    // home buckets read again rather than kept in registers across the loop
+    __asm__ volatile("" ::: "memory");
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const uint32_t h = hash_slot(r[u].w, hbits);
+      const int4 q0 = reinterpret_cast<const int4 *>(key)[h];
+      if (q0.w >= 0 && !bucket_has(q0, r[u].w)) c += bhash_contains_after(key, hbits, h, r[u].w) ? 1 : 0;
+    }
+  }
   return c;
 }
 
@@ -138,7 +138,7 @@ __global__ __launch_bounds__(256) void k_tc_wave(const int64_t *__restrict__ row
     wave_segments_fn(s.seg, rowptr, it.y, it.z, TcVerts{nu, rcolidx + rrowptr[u], du}, TcKeep{du}, lane, ns, L);
     wave_sync();
     int cnt = 0;
-    wave_walk_group<UNROLL>(s.seg, colidx, ns, 0, L, lane, fl, [&](const WindowLane (&r)[UNROLL]) {
+    wave_walk_group<UNROLL, false, kNoKey>(s.seg, colidx, ns, 0, L, lane, fl, [&](const WindowLane (&r)[UNROLL]) {
       if (STRAT == STRAT_HASH) {
         cnt += group_hash_hits<UNROLL>(s.key, hbits, r);
       } else {
@@ -299,7 +299,7 @@ __global__ __launch_bounds__(NT, 8) void k_tc_block(const int64_t *__restrict__ 
   for (;;) {
     if (threadIdx.x == 0) sh_task = atomicAdd(queue, 1);
     __syncthreads();
-    const int t = sh_task;
+    const int t = __builtin_amdgcn_readfirstlane(sh_task);  // uniform: the item is read with scalar loads
     if (t >= nitems) break;
     const int4 it = items[t];
     const int u = it.x;
@@ -326,7 +326,7 @@ __global__ __launch_bounds__(NT, 8) void k_tc_block(const int64_t *__restrict__ 
       block_segments_fn<NT>(s.seg, tmp, rowptr, k0, k1, vert, TcKeep{du}, &sh_ns, &sh_L);
       int lo, hi;
       wave_share(sh_L, wave, NW, lo, hi);
-      wave_walk_group<UNROLL, true>(s.seg, colidx, sh_ns, lo, hi, lane, fl, [&](const WindowLane (&r)[UNROLL]) {
+      wave_walk_group<UNROLL, true, kNoKey>(s.seg, colidx, sh_ns, lo, hi, lane, fl, [&](const WindowLane (&r)[UNROLL]) {
         if (STRAT == STRAT_HASH && lds_member) {
           cnt += group_hash_hits<UNROLL>(s.key, hbits, r);
         } else {

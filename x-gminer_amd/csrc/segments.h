@@ -239,8 +239,8 @@ struct WindowLane {
 
 // Walk flattened elements [lo, hi) with one wave, U windows per step.
 // f(const WindowLane (&)[U]) is called by all 64 lanes for every group of
-// U windows (invalid lanes included, so f may use ballots), after the U
-// global loads of the group were issued.
+// U windows (invalid lanes included, so f may use ballots; their w is INV),
+// after the U global loads of the group were issued.
 //
 // The segment holding the next window's first element (index jl, its
 // offset delta x - y and its end) is kept in scalar registers, so a window
@@ -249,7 +249,7 @@ struct WindowLane {
 // entries with one LDS access, marks the lanes that open a segment in the
 // flag row, takes each lane's offset from the entry's lane by a cross-lane
 // permute and the next window's segment from the same entries (readlane).
-template <int U, bool PIPE = false, class Seg, class F>
+template <int U, bool PIPE = false, int INV = 0, class Seg, class F>
 __device__ __forceinline__ void wave_walk_group(const Seg &t, const int32_t *__restrict__ colidx, int ns,
                                                 int lo, int hi, int lane, WaveFlags &fl, F &&f) {
   // the range and table size are the same on every lane (scalar registers)
@@ -340,9 +340,9 @@ __device__ __forceinline__ void wave_walk_group(const Seg &t, const int32_t *__r
     for (int u = 0; u < U; ++u) {
       if (PIPE) {
         const int w = colidx[(uint32_t)pos[u]];
-        r[u].w = r[u].valid ? w : 0;
+        r[u].w = r[u].valid ? w : INV;
       } else {
-        r[u].w = r[u].valid ? colidx[(uint32_t)pos[u]] : 0;
+        r[u].w = r[u].valid ? colidx[(uint32_t)pos[u]] : INV;
       }
     }
   };
