@@ -388,67 +388,113 @@ __device__ __forceinline__ void block_bitonic_sort(u64 *key, int *val, int n) {
   }
 }
 
-// Member ranks of a rank-LUT task in an LDS hash: 2^bbits buckets of two
-// (id, rank) slots in 16 bytes, empty id -1, at least two slots per member.
-// A lookup reads one bucket and compares two ids; only a full bucket without
-// the id continues to the next one.
-__device__ __forceinline__ void rank_hash_insert(int2 *slot, int bbits, int key, int rank) {
+// Member ranks of a rank-LUT task in an LDS hash: 2^bbits buckets of four
+// member ids (16 bytes, empty -1, filled in order, so a bucket whose last
+// slot is empty ends a lookup) and the rank of the id in slot i at rank[i].
+// A probe's home bucket settles it except when that bucket is full, so a
+// lookup is one 16-byte read plus one rank read on a hit, where a binary
+// search over the sorted members took log2(s) dependent reads.
+__device__ __forceinline__ void rank_hash_insert(int *ids, uint16_t *rank, int bbits, int key, int r) {
   const uint32_t mask = (1u << bbits) - 1u;
   uint32_t h = hash_slot(key, bbits);
   for (;;) {
-    int2 *b = slot + 2 * h;
-    if (atomicCAS(&b[0].x, -1, key) == -1) {
-      b[0].y = rank;
-      return;
-    }
-    if (atomicCAS(&b[1].x, -1, key) == -1) {
-      b[1].y = rank;
-      return;
-    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+      if (atomicCAS(&ids[4 * h + q], -1, key) == -1) {
+        rank[4 * h + q] = (uint16_t)r;
+        return;
+      }
     h = (h + 1) & mask;
   }
 }
 
-// rank of key, or -1 if it is not a member
-__device__ __forceinline__ int rank_hash_find(const int2 *slot, int bbits, int key) {
+// Slot of key in bucket q (0..3), or -1.
+__device__ __forceinline__ int bucket_slot(const int4 &q, int key) {
+  int si = q.w == key ? 3 : -1;
+  si = q.z == key ? 2 : si;
+  si = q.y == key ? 1 : si;
+  return q.x == key ? 0 : si;
+}
+
+// Rest of a lookup whose home bucket h was full and lacked key: the slot
+// index 4 * bucket + i of key, or -1.
+__device__ __noinline__ int rank_hash_after(const int *ids, int bbits, uint32_t h, int key) {
   const uint32_t mask = (1u << bbits) - 1u;
-  uint32_t h = hash_slot(key, bbits);
-  int4 q = reinterpret_cast<const int4 *>(slot)[h];
-  int rank = q.x == key ? q.y : (q.z == key ? q.w : -1);
-  bool more = rank < 0 && q.z >= 0;
-  while (__builtin_expect(more, 0)) {
+  for (;;) {
     h = (h + 1) & mask;
-    q = reinterpret_cast<const int4 *>(slot)[h];
-    rank = q.x == key ? q.y : (q.z == key ? q.w : -1);
-    more = rank < 0 && q.z >= 0;
+    const int4 q = reinterpret_cast<const int4 *>(ids)[h];
+    const int si = bucket_slot(q, key);
+    if (si >= 0) return 4 * (int)h + si;
+    if (q.w < 0) return -1;
   }
-  return rank;
 }
 
-// Rank-ordered vertex LUT (strategy LUT_RANK): X = N+(v0) stays id-sorted
-// for lookup; perm[i] = position of X[i] in (undirected degree, id) order,
-// which is the DAG orientation order, so the LUT is upper-triangular.
-// Stages up to 256 members find a probed id's rank in a rank hash (hslot,
-// SMAX buckets); larger ones binary-search X.
+// LUT build of a rank-LUT task: walks the flattened lists of the members
+// (segment j = the list of the member of rank seg.idx[j]) and calls
+// set_bit(slot, j) for every probed id the rank hash holds.  Invalid lanes
+// probe kNoKey, which no bucket holds; the four home buckets of a window
+// group are read together, and one wave-uniform branch per group covers
+// the misses in full buckets.
+template <class Seg, class SetBit>
+__device__ __forceinline__ void rank_hash_walk(const Seg &seg, const int32_t *__restrict__ colidx, int ns, int lo,
+                                               int hi, int lane, WaveFlags &fl, const int *ids, int bbits,
+                                               SetBit &&set_bit) {
+  wave_walk_group<4, true, kNoKey>(seg, colidx, ns, lo, hi, lane, fl, [&](const WindowLane (&r)[4]) {
+    int4 q[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) q[u] = reinterpret_cast<const int4 *>(ids)[hash_slot(r[u].w, bbits)];
+    uint32_t open = 0;
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int si = bucket_slot(q[u], r[u].w);
+      if (si >= 0) set_bit(4 * (int)hash_slot(r[u].w, bbits) + si, r[u].j);
+      open |= (si < 0 && q[u].w >= 0) ? 1u : 0u;
+    }
+    if (__builtin_expect(__ballot(open != 0u) != 0, 0)) {
+      __asm__ volatile("" ::: "memory");  // home buckets read again, not kept live
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const uint32_t h = hash_slot(r[u].w, bbits);
+        const int4 q0 = reinterpret_cast<const int4 *>(ids)[h];
+        if (q0.w >= 0 && bucket_slot(q0, r[u].w) < 0) {
+          const int at = rank_hash_after(ids, bbits, h, r[u].w);
+          if (at >= 0) set_bit(at, r[u].j);
+        }
+      }
+    }
+  });
+}
+
+// Rank-ordered vertex LUT (strategy LUT_RANK): the members of X = N+(v0)
+// are ranked in (undirected degree, id) order, which is the DAG
+// orientation order, so the LUT is upper-triangular.  A probed id's rank
+// comes from the rank hash: 4 SMAX id slots (load <= 1/4) up to 256
+// members, 2 SMAX above.  perm (rank of member q) and the segment-build
+// scratch live in the rows' space until the rows are zeroed.
 template <int SMAX, int W, int NT>
 struct RStage {
-  static constexpr bool kHash = SMAX <= 256;
-  static constexpr int kHashBits = kHash ? 31 - __builtin_clz(SMAX) : 0;  // SMAX is a power of two here
-  alignas(16) int2 hslot[kHash ? 2 * SMAX : 2];
-  int x[SMAX];
-  int perm[SMAX];
-  u64 rows[lut_words<W>(SMAX)];
-  union {  // the sort keys are dead once perm is built, before the segments
+  static constexpr int kSlots = SMAX <= 256 ? 4 * SMAX : 2 * SMAX;
+  static constexpr int kHashBits = 31 - __builtin_clz(kSlots / 4);  // SMAX is a power of two here
+  alignas(16) int hid[kSlots];
+  uint16_t hrank[kSlots];
+  union {
+    u64 rows[lut_words<W>(SMAX)];
+    struct {
+      int tmp[SMAX];  // scratch of the block-cooperative segment build
+      uint16_t perm[SMAX];
+    } pre;
+  };
+  union {  // the sort keys are dead once the hash is built, before the segments
     struct {
       u64 key[SMAX];
       int val[SMAX];
     };
     Segs<SMAX> seg;
   };
-  int tmp[SMAX];  // scratch of the block-cooperative segment build
   int flagrow[NT / 64][64];
   int ns, L, item, rowq;
 };
+static_assert(sizeof(RStage<64, 1, 64>::rows) >= sizeof(RStage<64, 1, 64>::pre), "pre fits the rows");
 
 template <int SMAX, int W, int NT>
 __global__ __launch_bounds__(NT, (SMAX <= 256 ? 7 : 1)) void k_clique_vlut_rank(const int64_t *__restrict__ rowptr,
@@ -475,47 +521,35 @@ __global__ __launch_bounds__(NT, (SMAX <= 256 ? 7 : 1)) void k_clique_vlut_rank(
     const int s = (int)(rowptr[v0 + 1] - b);
     if (s >= min_s) {
       const int weff = (s + 63) >> 6;
+      const int32_t *nv = colidx + b;  // the members, id-sorted
       for (int q = tid; q < s; q += NT) {
-        const int x = colidx[b + q];
-        S.x[q] = x;
+        const int x = nv[q];
         const u64 deg = (u64)((rowptr[x + 1] - rowptr[x]) + (rrowptr[x + 1] - rrowptr[x]));
         S.key[q] = (deg << 32) | (u64)(uint32_t)x;  // (undirected degree, id): the orientation order
         S.val[q] = q;
       }
-      for (int q = tid; q < lut_words<W>(s); q += NT) S.rows[q] = 0ull;
       using RS = RStage<SMAX, W, NT>;
-      if constexpr (RS::kHash)
-        for (int q = tid; q < 2 * SMAX; q += NT) S.hslot[q] = make_int2(-1, 0);
+      for (int q = tid; q < RS::kSlots; q += NT) S.hid[q] = -1;
       __syncthreads();
       block_bitonic_sort<SMAX, NT>(S.key, S.val, s);
-      for (int r = tid; r < s; r += NT) S.perm[S.val[r]] = r;
-      __syncthreads();  // keys are overwritten by the segment table
-      if constexpr (RS::kHash)
-        for (int q = tid; q < s; q += NT) rank_hash_insert(S.hslot, RS::kHashBits, S.x[q], S.perm[q]);
-      block_segments<NT>(S.seg, S.tmp, rowptr, colidx, S.x, 0, s, -1, KeepAll(), &S.ns, &S.L);
-      if constexpr (RS::kHash) {
-        // segment j's member index becomes its rank (the LUT row)
-        for (int j = tid; j < S.ns; j += NT) S.seg.idx[j] = S.perm[S.seg.idx[j]];
-        __syncthreads();
+      for (int r = tid; r < s; r += NT) {
+        const int q = S.val[r];
+        S.pre.perm[q] = (uint16_t)r;
+        rank_hash_insert(S.hid, S.hrank, RS::kHashBits, nv[q], r);
       }
+      __syncthreads();  // keys are overwritten by the segment table
+      block_segments<NT>(S.seg, S.pre.tmp, rowptr, colidx, nv, 0, s, -1, KeepAll(), &S.ns, &S.L);
+      // segment j's member index becomes its rank (the LUT row)
+      for (int j = tid; j < S.ns; j += NT) S.seg.idx[j] = S.pre.perm[S.seg.idx[j]];
+      __syncthreads();  // perm and tmp are dead: the rows take their space
+      for (int q = tid; q < lut_words<W>(s); q += NT) S.rows[q] = 0ull;
+      __syncthreads();
       int lo, hi;
       wave_share(S.L, wave, NT / 64, lo, hi);
-      wave_walk<4, true>(S.seg, colidx, S.ns, lo, hi, lane, fl, [&](const WindowLane &r) {
-        if (r.valid) {
-          int col, row;
-          if constexpr (RS::kHash) {
-            col = rank_hash_find(S.hslot, RS::kHashBits, r.w);
-            if (col < 0) return;
-            row = S.seg.idx[r.j];
-          } else {
-            const int pos = lower_bound_dev(S.x, s, r.w);
-            if (!(pos < s && S.x[pos] == r.w)) return;
-            col = S.perm[pos];
-            row = S.perm[S.seg.idx[r.j]];
-          }
-          if (col <= row) atomicOr(err, 1ull);  // DAG not in (degree, id) orientation
-          atomicOr(&S.rows[lut_word<W>(row, col >> 6)], 1ull << (col & 63));
-        }
+      rank_hash_walk(S.seg, colidx, S.ns, lo, hi, lane, fl, S.hid, RS::kHashBits, [&](int at, int j) {
+        const int col = S.hrank[at], row = S.seg.idx[j];
+        if (col <= row) atomicOr(err, 1ull);  // DAG not in (degree, id) orientation
+        atomicOr(&S.rows[lut_word<W>(row, col >> 6)], 1ull << (col & 63));
       });
       if (tid == 0) S.rowq = 0;
       __syncthreads();
@@ -564,11 +598,21 @@ constexpr int pow2_ceil(int n) { return n <= 1 ? 1 : 2 * pow2_ceil((n + 1) / 2);
 template <int SMAX, int W, int NT>
 struct TStage {
   static constexpr int KP = pow2_ceil(SMAX);  // the bitonic sort pads to a power of two
-  int x[SMAX];
-  int perm[SMAX];
+  // up to 512 members: ranks from a rank hash of 2 KP id slots (load <= 1/2);
+  // the hub bin (1344 members, whose rows fill the LDS) binary-searches x
+  static constexpr bool kHash = SMAX <= 512;
+  static constexpr int kSlots = kHash ? 2 * KP : 4;
+  static constexpr int kHashBits = 31 - __builtin_clz(kSlots / 4);
+  alignas(16) int hid[kSlots];
+  uint16_t hrank[kSlots];
+  int x[kHash ? 1 : SMAX];
+  int perm[kHash ? 1 : SMAX];
   union {
     u64 rows[tri_words(SMAX, W)];
-    int tmp[SMAX];
+    struct {
+      int tmp[SMAX];  // scratch of the block-cooperative segment build
+      uint16_t perm[kHash ? SMAX : 1];  // rank of member q (hash builds)
+    } pre;
   };
   union {
     struct {
@@ -612,35 +656,57 @@ __global__ __launch_bounds__(NT) void k_clique_vlut_rank_tri(const int64_t *__re
     const int s = (int)(rowptr[v0 + 1] - b);
     const bool mine = s >= min_s && s <= max_s;
     if (mine) {
+      using TS = TStage<SMAX, W, NT>;
       const int weff = (s + 63) >> 6;
+      const int32_t *nv = colidx + b;  // the members, id-sorted
       for (int q = tid; q < s; q += NT) {
-        const int x = colidx[b + q];
-        S.x[q] = x;
+        const int x = nv[q];
+        if constexpr (!TS::kHash) S.x[q] = x;
         const u64 deg = (u64)((rowptr[x + 1] - rowptr[x]) + (rrowptr[x + 1] - rrowptr[x]));
         S.key[q] = (deg << 32) | (u64)(uint32_t)x;  // (undirected degree, id): the orientation order
         S.val[q] = q;
       }
+      if constexpr (TS::kHash)
+        for (int q = tid; q < TS::kSlots; q += NT) S.hid[q] = -1;
       __syncthreads();
-      block_bitonic_sort<TStage<SMAX, W, NT>::KP, NT>(S.key, S.val, s);
-      for (int r = tid; r < s; r += NT) S.perm[S.val[r]] = r;
+      block_bitonic_sort<TS::KP, NT>(S.key, S.val, s);
+      for (int r = tid; r < s; r += NT) {
+        if constexpr (TS::kHash) {
+          const int q = S.val[r];
+          S.pre.perm[q] = (uint16_t)r;
+          rank_hash_insert(S.hid, S.hrank, TS::kHashBits, nv[q], r);
+        } else {
+          S.perm[S.val[r]] = r;
+        }
+      }
       __syncthreads();  // keys are overwritten by the segment table
-      block_segments<NT>(S.seg, S.tmp, rowptr, colidx, S.x, 0, s, -1, KeepAll(), &S.ns, &S.L);
+      block_segments<NT>(S.seg, S.pre.tmp, rowptr, colidx, nv, 0, s, -1, KeepAll(), &S.ns, &S.L);
+      if constexpr (TS::kHash) {
+        // segment j's member index becomes its rank (the LUT row)
+        for (int j = tid; j < S.ns; j += NT) S.seg.idx[j] = S.pre.perm[S.seg.idx[j]];
+        __syncthreads();  // perm is dead: the rows take its space
+      }
       for (int q = tid; q < tri_words(s, weff); q += NT) S.rows[q] = 0ull;  // tmp is dead
       __syncthreads();
       int lo, hi;
       wave_share(S.L, wave, NT / 64, lo, hi);
-      wave_walk<4, true>(S.seg, colidx, S.ns, lo, hi, lane, fl, [&](const WindowLane &r) {
-        if (r.valid) {
-          const int pos = lower_bound_dev(S.x, s, r.w);
-          if (pos < s && S.x[pos] == r.w) {
-            const int col = S.perm[pos], row = S.perm[S.seg.idx[r.j]];
-            if (col <= row)
-              atomicOr(err, 1ull);  // DAG not in (degree, id) orientation
-            else
-              atomicOr(&S.rows[tri_base(row, weff) + (col >> 6)], 1ull << (col & 63));
+      auto set_bit = [&](int col, int row) {
+        if (col <= row)
+          atomicOr(err, 1ull);  // DAG not in (degree, id) orientation
+        else
+          atomicOr(&S.rows[tri_base(row, weff) + (col >> 6)], 1ull << (col & 63));
+      };
+      if constexpr (TS::kHash) {
+        rank_hash_walk(S.seg, colidx, S.ns, lo, hi, lane, fl, S.hid, TS::kHashBits,
+                       [&](int at, int j) { set_bit(S.hrank[at], S.seg.idx[j]); });
+      } else {
+        wave_walk<4, true>(S.seg, colidx, S.ns, lo, hi, lane, fl, [&](const WindowLane &r) {
+          if (r.valid) {
+            const int pos = lower_bound_dev(S.x, s, r.w);
+            if (pos < s && S.x[pos] == r.w) set_bit(S.perm[pos], S.perm[S.seg.idx[r.j]]);
           }
-        }
-      });
+        });
+      }
       if (tid == 0) S.rowq = 0;
       __syncthreads();
       static_assert(sizeof(S.key) >= (NT / 64) * 128 * sizeof(int), "queue space");
