@@ -899,15 +899,36 @@ __global__ __launch_bounds__(NT) void k_clique_vlut(const int64_t *__restrict__ 
 // a wave claims G tasks with one atomic; the G sets N+(v0) are staged in
 // one flattened pass and the lists of all their members form one segment
 // table whose index is the LUT row (g * 64 + i), so the dependent loads of
-// G tasks are in flight together.  Counting runs lut_count per task.
+// G tasks are in flight together.  Task g's members are found through its
+// own id hash (kPackBuckets buckets of four slots, load <= 1/2; hpos = the
+// member's position in N+(v0), its LUT column).  Counting runs lut_count
+// per task.
+constexpr int kPackBucketBits = 5, kPackSlots = 4 << kPackBucketBits;
 template <int G>
 struct LutPack {
-  int x[G * 64];
+  alignas(16) int hid[G * kPackSlots];
+  uint8_t hpos[G * kPackSlots];
   u64 rows[G * 64];
   Segs<64 * G, true> seg;
   int v0[G], s[G], off[G + 1];
   int64_t b[G];
 };
+
+// Insert key with value v into a hash of 4-slot id buckets (as
+// rank_hash_insert, byte values).
+__device__ __forceinline__ void id_hash_insert(int *ids, uint8_t *val, int bbits, int key, int v) {
+  const uint32_t mask = (1u << bbits) - 1u;
+  uint32_t h = hash_slot(key, bbits);
+  for (;;) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+      if (atomicCAS(&ids[4 * h + q], -1, key) == -1) {
+        val[4 * h + q] = (uint8_t)v;
+        return;
+      }
+    h = (h + 1) & mask;
+  }
+}
 
 template <int G>
 __global__ __launch_bounds__(128) void k_clique_vlut_packed(const int64_t *__restrict__ rowptr,
@@ -942,6 +963,7 @@ __global__ __launch_bounds__(128) void k_clique_vlut_packed(const int64_t *__res
     if (lane < ng) S.off[lane + 1] = inc;
     if (lane == 0) S.off[0] = 0;
     for (int q = lane; q < G * 64; q += 64) S.rows[q] = 0ull;
+    for (int q = lane; q < ng * kPackSlots; q += 64) S.hid[q] = -1;
     wave_sync();
     const int tot = __shfl(inc, ng - 1, 64);
     auto item_of = [&](int q) {
@@ -951,15 +973,15 @@ __global__ __launch_bounds__(128) void k_clique_vlut_packed(const int64_t *__res
       return g;
     };
     for (int q = lane; q < tot; q += 64) {
-      const int g = item_of(q);
-      S.x[g * 64 + (q - S.off[g])] = colidx[S.b[g] + (q - S.off[g])];
+      const int g = item_of(q), i = q - S.off[g];
+      id_hash_insert(S.hid + g * kPackSlots, S.hpos + g * kPackSlots, kPackBucketBits, colidx[S.b[g] + i], i);
     }
     wave_sync();
     int ns = 0, L = 0;
     wave_segments_fn(S.seg, rowptr, 0, tot,
                      [&](int q) {
                        const int g = item_of(q);
-                       return S.x[g * 64 + (q - S.off[g])];
+                       return colidx[S.b[g] + (q - S.off[g])];
                      },
                      [](int, int, int) { return true; }, lane, ns, L);
     if (lane == 0) S.seg.os[ns] = make_int2(0, L);
@@ -969,12 +991,39 @@ __global__ __launch_bounds__(128) void k_clique_vlut_packed(const int64_t *__res
       S.seg.idx[j] = g * 64 + (q - S.off[g]);  // LUT row of the segment's member
     }
     wave_sync();
-    wave_walk<4, true>(S.seg, colidx, ns, 0, L, lane, fl, [&](const WindowLane &r) {
-      if (r.valid) {
-        const int row = S.seg.idx[r.j], g = row >> 6;
-        const int *xg = S.x + g * 64;
-        const int pos = lower_bound_dev(xg, S.s[g], r.w);
-        if (pos < S.s[g] && xg[pos] == r.w) atomicOr(&S.rows[row], 1ull << pos);
+    // invalid lanes probe kNoKey in table 0, which holds no such id; the
+    // four home buckets of a window group are read together, and one
+    // wave-uniform branch per group covers the misses in full buckets
+    wave_walk_group<4, true, kNoKey>(S.seg, colidx, ns, 0, L, lane, fl, [&](const WindowLane (&r)[4]) {
+      int row[4];
+      int4 q[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        row[u] = r[u].valid ? S.seg.idx[r[u].j] : 0;
+        q[u] = reinterpret_cast<const int4 *>(S.hid + (row[u] >> 6) * kPackSlots)[hash_slot(r[u].w, kPackBucketBits)];
+      }
+      uint32_t open = 0;
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int si = bucket_slot(q[u], r[u].w);
+        if (si >= 0) {
+          const int at = (row[u] >> 6) * kPackSlots + 4 * (int)hash_slot(r[u].w, kPackBucketBits) + si;
+          atomicOr(&S.rows[row[u]], 1ull << S.hpos[at]);
+        }
+        open |= (si < 0 && q[u].w >= 0) ? 1u : 0u;
+      }
+      if (__builtin_expect(__ballot(open != 0u) != 0, 0)) {
+        __asm__ volatile("" ::: "memory");  // home buckets read again, not kept live
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const int *tab = S.hid + (row[u] >> 6) * kPackSlots;
+          const uint32_t h = hash_slot(r[u].w, kPackBucketBits);
+          const int4 q0 = reinterpret_cast<const int4 *>(tab)[h];
+          if (q0.w >= 0 && bucket_slot(q0, r[u].w) < 0) {
+            const int at = rank_hash_after(tab, kPackBucketBits, h, r[u].w);
+            if (at >= 0) atomicOr(&S.rows[row[u]], 1ull << S.hpos[(row[u] >> 6) * kPackSlots + at]);
+          }
+        }
       }
     });
     wave_sync();
@@ -1584,8 +1633,9 @@ __global__ void k_sched_argmin(const int64_t *__restrict__ rowptr, int64_t n, co
   }
   const int d = (int)(rowptr[v + 1] - rowptr[v]);
   const float *y = Y + v * 5;
-  // the model-free code of this task (rank LUT; edge LUT above the hub LUT)
-  const int h = d <= kHubLutMax ? 1 : 2;
+  // the model-free code of this task: the id LUT up to 64 rows (its packed
+  // kernel), the rank LUT up to the hub LUT, the edge LUT above
+  const int h = d <= 64 ? 0 : (d <= kHubLutMax ? 1 : 2);
   int best = h;
   float bv = y[h];
   for (int c = 0; c < 5; ++c) {
@@ -1596,6 +1646,10 @@ __global__ void k_sched_argmin(const int64_t *__restrict__ rowptr, int64_t n, co
   // leave the model-free code only for a predicted saving above the model's
   // error (kSchedMargin in log2(1 + 1000 us) units, about 13% of the cost)
   if (best != h && !(bv < y[h] - kSchedMargin)) best = h;
+  // up to 64 rows the packed id-LUT kernel (8 tasks per wave) runs a task
+  // faster than a rank-LUT block, in this bin or one bin up; the per-task
+  // logs the model learned from time one task per kernel and do not see it
+  if ((best == 1 || best == 4) && d <= 64) best = 0;
   codes[v] = (int8_t)best;
 }
 

@@ -619,9 +619,11 @@ int build_clique_sched(xgm_ctx *ctx, const xgm_dgraph *g, int pattern, const int
 // Degree-threshold codes of a strategy (no model).  AUTO and LUT_RANK: the
 // rank-ordered LUT for every task up to lut_max out-degree and for hubs
 // (1024 < d+ <= kHubLutMax, the hub rank LUT), the edge LUT between lut_max
-// and 1024 and above kHubLutMax.  LUT: the id-ordered LUT up to lut_max, the
-// edge LUT above (the reference's fixed-threshold scheme, GLUMIN's LUT tasks
-// up to 1024 out-neighbours, xgminer/include/common.h:61).
+// and 1024 and above kHubLutMax; AUTO takes the id-ordered LUT for tasks of
+// at most 64 out-neighbours, whose packed kernel (8 tasks per wave) runs
+// them faster than one rank-LUT block per task.  LUT: the id-ordered LUT up
+// to lut_max, the edge LUT above (the reference's fixed-threshold scheme,
+// GLUMIN's LUT tasks up to 1024 out-neighbours, xgminer/include/common.h:61).
 void heuristic_codes(const xgm_dgraph *g, int k, int strategy, int lut_max, int8_t *codes) {
   for (int64_t v = 0; v < g->n; ++v) {
     const int d = g->dplus[v];
@@ -631,6 +633,12 @@ void heuristic_codes(const xgm_dgraph *g, int k, int strategy, int lut_max, int8
         case XGM_STRAT_LIST: c = 3; break;
         case XGM_STRAT_ELUT: c = 2; break;
         case XGM_STRAT_LUT: c = d <= lut_max ? 0 : 2; break;
+        case XGM_STRAT_AUTO:
+          if (d <= kBinCap[0] && d <= lut_max) {
+            c = 0;
+            break;
+          }
+          [[fallthrough]];
         default: c = (d <= lut_max || d > 1024) ? 1 : 2; break;  // rank LUT; hubs: hub rank LUT
       }
     }
@@ -1203,9 +1211,10 @@ static int dispatch_pattern(xgm_ctx *ctx, const xgm_dgraph *g, int pattern, cons
         return ctx->fail(XGM_EUNSUPPORTED, "strategy not available for CLIQUE");
       const int k = (pattern == XGM_CLIQUE4) ? 4 : 5;
       const int lut_max = (opts && opts->lut_max_degree > 0) ? std::min(opts->lut_max_degree, 1024) : 1024;
-      // AUTO: the rank-ordered LUT, or the id-ordered one on a DAG that a
-      // rank-LUT count found outside (degree, id) orientation
-      const int strat = (strategy == XGM_STRAT_AUTO) ? (gm->rank_ok ? XGM_STRAT_LUT_RANK : XGM_STRAT_LUT) : strategy;
+      // AUTO: the rank-ordered LUT (the id-ordered one for the smallest
+      // tasks), or the id-ordered one everywhere on a DAG that a rank-LUT
+      // count found outside (degree, id) orientation
+      const int strat = (strategy == XGM_STRAT_AUTO && !gm->rank_ok) ? XGM_STRAT_LUT : strategy;
       const std::string key = "csched" + std::to_string(k) + "_" + std::to_string(strat) + "_" +
                               std::to_string(lut_max);
       auto it = gm->scheds.find(key);
