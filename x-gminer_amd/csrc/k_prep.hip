@@ -210,7 +210,8 @@ __device__ __forceinline__ int64_t spread_vertex(int64_t i, int64_t p, u64 m) {
 // calls emit(k0, k1, work) with the same wave-uniform arguments, once per
 // item in position order, items of zero work included.
 template <class WorkAt, class Emit>
-__device__ __forceinline__ void wave_greedy_cut(int P, int cap, int lane, WorkAt work_at, Emit emit) {
+__device__ __forceinline__ void wave_greedy_cut(int P, int cap, int lane, WorkAt work_at, Emit emit,
+                                                u64 slice = (u64)kTcSlice) {
   u64 A = 0;   // work of the open item before this chunk
   int k0 = 0;  // its first position
   for (int c0 = 0; c0 < P; c0 += 64) {
@@ -222,7 +223,7 @@ __device__ __forceinline__ void wave_greedy_cut(int P, int cap, int lane, WorkAt
     u64 Sbase = 0;  // inclusive prefix at lane ls - 1
     for (;;) {
       const bool cond = lane >= ls && lane < nvalid &&
-                        (A + (S - Sbase) >= (u64)kTcSlice || c0 + lane + 1 - k0 >= cap);
+                        (A + (S - Sbase) >= slice || c0 + lane + 1 - k0 >= cap);
       const u64 m = __ballot(cond);
       if (!m) {
         A += __shfl(S, nvalid - 1, 64) - Sbase;
@@ -335,8 +336,9 @@ __device__ __forceinline__ int dia_tier(int d) { return d <= 64 ? 0 : d <= 1024 
 // Pass 1 writes key[t] = work + 2 for degree >= 3, 1 for an owner of degree
 // 2, else 0 (a stable descending sort then gives the owner order: degree >=
 // 3 by decreasing work, then degree 2 by id), and cnt[t] = the items of t:
-// one for owners of degree <= 1024, the greedy slices above (work 64K or
-// 4096 positions).  Pass 2 writes the items of owner t from off[t] on.
+// one for owners of degree <= 1024, the greedy slices above (work
+// kDiaSlice or kDiaSlicePositions positions).  Pass 2 writes the items of
+// owner t from off[t] on.
 template <bool WRITE>
 __global__ __launch_bounds__(kPrepThreads) void k_dia_items(const int64_t *__restrict__ rowptr,
                                                             const int32_t *__restrict__ colidx, int64_t n,
@@ -361,10 +363,10 @@ __global__ __launch_bounds__(kPrepThreads) void k_dia_items(const int64_t *__res
       u64 w = 0;
       int count = 0;
       if (sliced) {
-        wave_greedy_cut(d, 4096, lane, work_at, [&](int, int, u64 wk) {
+        wave_greedy_cut(d, kDiaSlicePositions, lane, work_at, [&](int, int, u64 wk) {
           w += wk;
           ++count;
-        });
+        }, (u64)kDiaSlice);
       } else {
         for (int k = lane; k < d; k += 64) w += work_at(k);
         w = wave_sum_u64(w);
@@ -379,10 +381,10 @@ __global__ __launch_bounds__(kPrepThreads) void k_dia_items(const int64_t *__res
       if (!sliced) {
         if (lane == 0) items[o] = make_int4((int)t, 0, d, 0);
       } else {
-        wave_greedy_cut(d, 4096, lane, work_at, [&](int k0, int k1, u64) {
+        wave_greedy_cut(d, kDiaSlicePositions, lane, work_at, [&](int k0, int k1, u64) {
           if (lane == 0) items[o] = make_int4((int)t, k0, k1, 0);
           ++o;
-        });
+        }, (u64)kDiaSlice);
       }
     }
   }

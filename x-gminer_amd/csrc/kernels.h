@@ -106,6 +106,12 @@ struct CliqueTasks {
   u64 *log_lut[4] = {}, *log_rlut[4] = {}, *log_elut[2] = {}, *log_list[2] = {}, *log_rlut_hbm = nullptr;
 };
 constexpr int kHubLutMax = 4096;
+// Diamond / motif owners of more than 1024 neighbours are cut into items of
+// about kDiaSlice probes or kDiaSlicePositions partners: every item stages
+// N(t) again (an LDS hash, or the binned set of a hub), so slices 16 times
+// the triangle items' length keep that cost small next to the probes.
+constexpr int64_t kDiaSlice = 1 << 20;
+constexpr int kDiaSlicePositions = 1 << 16;
 // Hubs up to kTriLutMax (21 row words) keep the rank LUT in LDS as packed
 // upper-triangular rows; larger ones up to kHubLutMax use rows in HBM.
 constexpr int kTriLutMax = 1344;

@@ -1061,7 +1061,6 @@ static int diamond_lists(xgm_ctx *ctx, xgm_dgraph *gm, std::vector<TaskList> &tl
       }
     }
     const auto &deg = gm->dplus;
-    const int64_t kChunk = 1 << 16;
     std::vector<int32_t> owners = gm->order;  // degree >= 3, by decreasing work
     if (with_deg2)
       for (int64_t u = 0; u < gm->n; ++u)
@@ -1080,7 +1079,7 @@ static int diamond_lists(xgm_ctx *ctx, xgm_dgraph *gm, std::vector<TaskList> &tl
       for (int k = 0; k < d; ++k) {
         const int32_t p = nt[k];
         if (deg[p] < d || (deg[p] == d && p < t)) acc += deg[p];
-        if (acc >= kChunk || k + 1 - k0 >= 4096) {
+        if (acc >= kDiaSlice || k + 1 - k0 >= kDiaSlicePositions) {
           P.insert(P.end(), {t, k0, k + 1, 0});
           k0 = k + 1;
           acc = 0;
