@@ -183,7 +183,6 @@ __global__ __launch_bounds__(kPrepThreads) void k_prep_mirror(const int64_t *__r
 // every t of one residue mod its wave count, a power of two, and R-MAT ids
 // of one low-bit pattern share their degree class: on R-MAT scale 22 one
 // wave drew 65 times the mean work.
-constexpr int64_t kTcSlice = 1 << 16;
 
 __device__ __forceinline__ u64 wave_incl_scan_u64(u64 v, int lane) {
 #pragma unroll

@@ -106,6 +106,10 @@ struct CliqueTasks {
   u64 *log_lut[4] = {}, *log_rlut[4] = {}, *log_elut[2] = {}, *log_list[2] = {}, *log_rlut_hbm = nullptr;
 };
 constexpr int kHubLutMax = 4096;
+// Triangle items end where their probes reach kTcSlice (or at a tier's
+// position cap): long enough that staging N+(t) once per item costs little
+// next to its probes (2^16 -> 2^18: tc_block_2048 on R-MAT-22 12.0 -> 11.5 ms).
+constexpr int64_t kTcSlice = 1 << 18;
 // Diamond / motif owners of more than 1024 neighbours are cut into items of
 // about kDiaSlice probes or kDiaSlicePositions partners: every item stages
 // N(t) again (an LDS hash, or the binned set of a hub), so slices 16 times

@@ -517,7 +517,7 @@ void build_reverse(const std::vector<int64_t> &rp, const std::vector<int32_t> &c
 void build_tc_items(const xgm_dgraph *g, int max_small, int max_large, std::vector<std::vector<int32_t>> &parts) {
   const int64_t n = g->n;
   const auto &dp = g->dplus;
-  const int64_t kSlice = 1 << 16;
+  const int64_t kSlice = kTcSlice;
   struct It { int64_t work; int32_t t, k0, k1; };
   std::vector<std::vector<It>> tiers(NUM_TIERS);
   for (int64_t t = 0; t < n; ++t) {
