@@ -702,7 +702,7 @@ template <int R, int PCAP, int NT>
 struct RangeStage {
   uint32_t cnt[R];
   Segs<PCAP, false> seg;
-  int tmp[PCAP];
+  int wcnt[NT / 64], wlen[NT / 64];  // per-wave totals of the segment compaction
   int flagrow[NT / 64][64];
   int ns, L, task;
 };
@@ -795,30 +795,39 @@ __global__ __launch_bounds__(NT) void k_cycle4_range(const int64_t *__restrict__
             const int p = c[h].x + len;
             if (!last) cur[k0 + q] = make_int4(p, c[h].y, p < c[h].y ? colidx[p] : INT32_MAX, 0);
           }
-          S.seg.os[q].x = c[h].x;
-          S.tmp[q] = len;
+          c[h].w = len;  // the piece's length in this span (0: none)
+        }
+        // compact the non-empty pieces with a block-wide scan of (pieces,
+        // elements): every thread writes its own at the scanned positions
+        int pc = 0, pl = 0;
+#pragma unroll
+        for (int h = 0; h < PCAP / NT; ++h) {
+          const bool here = tid + h * NT < n && c[h].w > 0;
+          pc += here ? 1 : 0;
+          pl += here ? c[h].w : 0;
+        }
+        const int ic = wave_incl_scan(pc, lane), il = wave_incl_scan(pl, lane);
+        if (lane == 63) {
+          S.wcnt[wave] = ic;
+          S.wlen[wave] = il;
         }
         __syncthreads();
-        if (wave == 0) {  // compact the non-empty pieces in place
-          int ns = 0, L = 0;
-          for (int k = 0; k < n; k += 64) {
-            const int kk = k + lane;
-            int len = 0, vb = 0;
-            if (kk < n) {
-              len = S.tmp[kk];
-              vb = S.seg.os[kk].x;
-            }
-            const int incl = wave_incl_scan(len, lane) + L;
-            const u64 m = __ballot(len > 0);
-            if (len > 0) S.seg.os[ns + __popcll(m & lanemask_lt(lane))] = make_int2(vb, incl - len);
-            ns += __popcll(m);
-            L = __shfl(incl, 63, 64);
+        int pos = ic - pc, start = il - pl;
+        for (int w = 0; w < wave; ++w) {
+          pos += S.wcnt[w];
+          start += S.wlen[w];
+        }
+#pragma unroll
+        for (int h = 0; h < PCAP / NT; ++h) {
+          if (tid + h * NT < n && c[h].w > 0) {
+            S.seg.os[pos++] = make_int2(c[h].x, start);
+            start += c[h].w;
           }
-          if (lane == 0) {
-            S.seg.os[ns] = make_int2(0, L);
-            S.ns = ns;
-            S.L = L;
-          }
+        }
+        if (tid == NT - 1) {  // pos and start are the totals here
+          S.seg.os[pos] = make_int2(0, start);
+          S.ns = pos;
+          S.L = start;
         }
         __syncthreads();
         int lo, hi;
