@@ -911,6 +911,7 @@ struct LutPack {
   u64 rows[G * 64];
   Segs<64 * G, true> seg;
   int v0[G], s[G], off[G + 1];
+  uint32_t tcnt[G];  // per-task counts (instances)
   int64_t b[G];
 };
 
@@ -958,6 +959,7 @@ __global__ __launch_bounds__(128) void k_clique_vlut_packed(const int64_t *__res
       S.v0[lane] = v;
       S.b[lane] = b;
       S.s[lane] = sz;
+      S.tcnt[lane] = 0;
     }
     const int inc = wave_incl_scan(sz, lane);
     if (lane < ng) S.off[lane + 1] = inc;
@@ -1027,15 +1029,43 @@ __global__ __launch_bounds__(128) void k_clique_vlut_packed(const int64_t *__res
       }
     });
     wave_sync();
-    for (int g = 0; g < ng; ++g) {
-      const int sg = S.s[g];
-      u64 c = sg ? lut_count<1>(S.rows + g * 64, sg, 1, depth, lane, 64) : 0;
+    // count over the rows of all ng tasks at once, one row per lane (row i
+    // of task g at flattened position off[g] + i), so a round of 64 rows is
+    // not spent on one small task; as lut_count with one word per row
+    for (int t = lane; t < tot; t += 64) {
+      const int g = item_of(t);
+      const u64 *rg = S.rows + g * 64;
+      const u64 ri = rg[t - S.off[g]];
+      uint32_t c = 0;  // below C(64, 4)
+      if (depth == 2) {
+        c = (uint32_t)__popcll(ri);
+      } else {
+        u64 bits = ri;
+        while (bits) {
+          const int j = __builtin_ctzll(bits);
+          bits &= bits - 1;
+          const u64 m = ri & rg[j];
+          if (depth == 3) {
+            c += (uint32_t)__popcll(m);
+          } else {
+            u64 tb = m;
+            while (tb) {
+              const int k = __builtin_ctzll(tb);
+              tb &= tb - 1;
+              c += (uint32_t)__popcll(m & rg[k]);
+            }
+          }
+        }
+      }
       if (!slots) {
         acc += c;
-      } else {
-        c = wave_sum_u64(c);
-        if (lane == 0 && c) atomicAdd(out + slots[S.v0[g]], c);
+      } else if (c) {
+        atomicAdd(&S.tcnt[g], c);
       }
+    }
+    if (slots) {
+      wave_sync();
+      if (lane < ng && S.tcnt[lane]) atomicAdd(out + slots[S.v0[lane]], (u64)S.tcnt[lane]);
     }
     wave_sync();  // stage is rewritten by the next claim
   }
