@@ -315,16 +315,20 @@ __device__ u64 block_lut_count(const int64_t *__restrict__ rowptr, const int32_t
 
 // Triangular variant of lut_count: rows are indexed in DAG (degree, id)
 // order, so row a only holds bits b > a and every AND below starts at the
-// word holding bit b+1 (LDS words below it are never read).
-template <int W>
-__device__ __forceinline__ u64 lut_count_tri(const u64 *rows, int s, int weff, int depth, int tid, int nthreads) {
+// word holding bit b+1 (words below it are never read, so the packed
+// upper-triangular rows serve as well).  One thread per row word; only
+// rows a0, a0 + astep, ... are counted.
+template <int W, class Rows>
+__device__ __forceinline__ u64 lut_count_tri_rows(Rows row, int s, int weff, int depth, int tid, int nthreads,
+                                                   int a0 = 0, int astep = 1) {
   u64 acc = 0;
   // items (a, q) with q >= a / 64 only; enumerate the triangle row by row
-  const int items = s * weff;
+  const int na = s > a0 ? (s - a0 + astep - 1) / astep : 0;
+  const int items = na * weff;
   for (int t = tid; t < items; t += nthreads) {
-    const int a = t / weff, q = t - a * weff;
+    const int i = t / weff, q = t - i * weff, a = a0 + astep * i;
     if (q < (a >> 6)) continue;
-    u64 bits = rows[lut_word<W>(a, q)];
+    u64 bits = row(a, q);
     if (depth == 2) {
       acc += __popcll(bits);
       continue;
@@ -336,11 +340,11 @@ __device__ __forceinline__ u64 lut_count_tri(const u64 *rows, int s, int weff, i
       if (depth == 3) {
 #pragma unroll
         for (int w = 0; w < W; ++w)
-          if (w >= ws && w < weff) acc += __popcll(rows[lut_word<W>(a, w)] & rows[lut_word<W>(j, w)]);
+          if (w >= ws && w < weff) acc += __popcll(row(a, w) & row(j, w));
       } else {
         u64 tw[W];
 #pragma unroll
-        for (int w = 0; w < W; ++w) tw[w] = (w >= ws && w < weff) ? (rows[lut_word<W>(a, w)] & rows[lut_word<W>(j, w)]) : 0ull;
+        for (int w = 0; w < W; ++w) tw[w] = (w >= ws && w < weff) ? (row(a, w) & row(j, w)) : 0ull;
 #pragma unroll
         for (int w2 = 0; w2 < W; ++w2) {
           u64 tb = tw[w2];
@@ -350,13 +354,18 @@ __device__ __forceinline__ u64 lut_count_tri(const u64 *rows, int s, int weff, i
             const int ks = (k + 1) >> 6;
 #pragma unroll
             for (int w = 0; w < W; ++w)
-              if (w >= ks && w < weff) acc += __popcll(tw[w] & rows[lut_word<W>(k, w)]);
+              if (w >= ks && w < weff) acc += __popcll(tw[w] & row(k, w));
           }
         }
       }
     }
   }
   return acc;
+}
+
+template <int W>
+__device__ __forceinline__ u64 lut_count_tri(const u64 *rows, int s, int weff, int depth, int tid, int nthreads) {
+  return lut_count_tri_rows<W>(LdsRows<W>{rows}, s, weff, depth, tid, nthreads);
 }
 
 // In-LDS bitonic sort of n <= SMAX 64-bit keys with an int payload (padded
@@ -707,12 +716,19 @@ __global__ __launch_bounds__(NT) void k_clique_vlut_rank_tri(const int64_t *__re
           }
         });
       }
-      if (tid == 0) S.rowq = 0;
-      __syncthreads();
-      static_assert(sizeof(S.key) >= (NT / 64) * 128 * sizeof(int), "queue space");
-      part += lut_count_wq_fit<(W > 16 ? 17 : W / 4 + 1), W, true>(
-          TriRows{S.rows, weff}, s, weff, depth, reinterpret_cast<int *>(S.key) + wave * 128, &S.rowq, lane, piece,
-          nparts);
+      if (depth == 3 && W < 16) {
+        // K4 on rows of <= 8 words: one thread per row word (the rows'
+        // few bits leave most lanes of the pair queues idle)
+        __syncthreads();
+        part += lut_count_tri_rows<W>(TriRows{S.rows, weff}, s, weff, depth, tid, NT, piece, nparts);
+      } else {
+        if (tid == 0) S.rowq = 0;
+        __syncthreads();
+        static_assert(sizeof(S.key) >= (NT / 64) * 128 * sizeof(int), "queue space");
+        part += lut_count_wq_fit<(W > 16 ? 17 : W / 4 + 1), W, true>(
+            TriRows{S.rows, weff}, s, weff, depth, reinterpret_cast<int *>(S.key) + wave * 128, &S.rowq, lane, piece,
+            nparts);
+      }
     }
     task_flush(acc, part, slots, v0, out, tid & 63);
     __syncthreads();
