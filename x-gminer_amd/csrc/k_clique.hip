@@ -484,8 +484,8 @@ template <int SMAX, int W, int NT>
 struct RStage {
   static constexpr int kSlots = SMAX <= 256 ? 4 * SMAX : 2 * SMAX;
   static constexpr int kHashBits = 31 - __builtin_clz(kSlots / 4);  // SMAX is a power of two here
-  alignas(16) int hid[kSlots];
-  uint16_t hrank[kSlots];
+  // the rows first: at LDS offset 0 a row word's address is the row base
+  // plus an immediate offset, so the count loop adds nothing per read
   union {
     u64 rows[lut_words<W>(SMAX)];
     struct {
@@ -493,6 +493,8 @@ struct RStage {
       uint16_t perm[SMAX];
     } pre;
   };
+  alignas(16) int hid[kSlots];
+  uint16_t hrank[kSlots];
   union {  // the sort keys are dead once the hash is built, before the segments
     struct {
       u64 key[SMAX];
@@ -612,17 +614,17 @@ struct TStage {
   static constexpr bool kHash = SMAX <= 512;
   static constexpr int kSlots = kHash ? 2 * KP : 4;
   static constexpr int kHashBits = 31 - __builtin_clz(kSlots / 4);
-  alignas(16) int hid[kSlots];
-  uint16_t hrank[kSlots];
-  int x[kHash ? 1 : SMAX];
-  int perm[kHash ? 1 : SMAX];
-  union {
+  union {  // first: row words at immediate offsets from the row base
     u64 rows[tri_words(SMAX, W)];
     struct {
       int tmp[SMAX];  // scratch of the block-cooperative segment build
       uint16_t perm[kHash ? SMAX : 1];  // rank of member q (hash builds)
     } pre;
   };
+  alignas(16) int hid[kSlots];
+  uint16_t hrank[kSlots];
+  int x[kHash ? 1 : SMAX];
+  int perm[kHash ? 1 : SMAX];
   union {
     struct {
       u64 key[KP];
