@@ -450,3 +450,39 @@ def test_sgl_sampled_hub_owner(xgm, ctx, orc, n, spread):
     dg = ctx.upload(g)
     assert ctx.motifs(dg, 4)[0].tolist() == orc.motif(g.rowptr, g.colidx, 4)
     dg.close()
+
+
+def test_hash_bucket_overflow(xgm, orc, ctx):
+    """Neighbourhoods crowded into one LDS hash bucket: 120 of the 300 graph
+    vertices share the home bucket of every bucketed hash here (the same top
+    11 bits of id * 0x9E3779B1; the tables use 5 to 11 of them), so staged
+    sets overflow their home bucket by many buckets, and probes continue past
+    full buckets (the rare branch of the triangle, diamond, rank-LUT and
+    packed-LUT probes) for hits and misses alike."""
+    import numpy as np
+    _say("hash bucket overflow")
+    n = 300_000
+    ids = np.arange(n, dtype=np.uint64)
+    home = ((ids * np.uint64(0x9E3779B1)) & np.uint64(0xFFFFFFFF)) >> np.uint64(21)
+    counts = np.bincount(home.astype(np.int64), minlength=2048)
+    crowd = np.nonzero(home == np.uint64(int(np.argmax(counts))))[0][:120]
+    assert len(crowd) == 120
+    rng = np.random.default_rng(11)
+    rest = rng.choice(np.setdiff1d(np.arange(n), crowd), 180, replace=False)
+    verts = np.concatenate([crowd, rest])
+    iu, ju = np.triu_indices(len(verts), 1)
+    keep = rng.random(len(iu)) < 0.35
+    g = xgm.from_edges(n, verts[iu[keep]], verts[ju[keep]])
+    dg = ctx.upload(g)
+    for name, pid in SGL.items():
+        assert ctx.count(dg, pid) == orc.count(name, g.rowptr, g.colidx)[0], name
+    dg.close()
+    d = g.orient()
+    assert int(np.diff(d.rowptr).max()) > 64  # block-tier triangle items, 256-row LUT bin
+    dd = ctx.upload(d)
+    assert ctx.count(dd, xgm.TRIANGLE) == orc.count("triangle", d.rowptr, d.colidx)[0]
+    for name, (pid, _) in CLQ.items():
+        want = orc.count(name, d.rowptr, d.colidx)[0]
+        for strat in (0, xgm.STRAT_LUT, xgm.STRAT_LUT_RANK):
+            assert ctx.count(dd, pid, strat) == want, (name, strat)
+    dd.close()
