@@ -590,7 +590,14 @@ __global__ __launch_bounds__(NT, (SMAX <= 256 ? 7 : 1)) void k_clique_vlut_rank(
 __host__ __device__ constexpr int tri_words(int s, int weff) {
   return 64 * ((s >> 6) * weff - (s >> 6) * ((s >> 6) - 1) / 2) + (s & 63) * (weff - (s >> 6));
 }
-__device__ __forceinline__ int tri_base(int a, int weff) { return tri_words(a, weff) - (a >> 6); }
+// tri_words(a, weff) - a / 64 with 24-bit multiplies (every factor is below
+// 2^11), which issue at the full rate where 32-bit ones take four slots:
+// the count loops evaluate it for every row they touch
+__device__ __forceinline__ int tri_base(int a, int weff) {
+  const uint32_t q = (uint32_t)a >> 6, r = (uint32_t)a & 63u, e = (uint32_t)weff;
+  const uint32_t full = __umul24(q, e) - (__umul24(q, q - 1u) >> 1);  // words of the q full 64-row blocks / 64
+  return (int)((full << 6) + __umul24(r, e - q) - q);
+}
 
 struct TriRows {
   const u64 *rows;
@@ -693,8 +700,13 @@ __global__ __launch_bounds__(NT) void k_clique_vlut_rank_tri(const int64_t *__re
       __syncthreads();  // keys are overwritten by the segment table
       block_segments<NT>(S.seg, S.pre.tmp, rowptr, colidx, nv, 0, s, -1, KeepAll(), &S.ns, &S.L);
       if constexpr (TS::kHash) {
-        // segment j's member index becomes its rank (the LUT row)
-        for (int j = tid; j < S.ns; j += NT) S.seg.idx[j] = S.pre.perm[S.seg.idx[j]];
+        // segment j's member index becomes its rank (the LUT row) and the
+        // row's base word: rank | tri_base(rank) << 10
+        static_assert(SMAX <= 1024, "ranks fit 10 bits");
+        for (int j = tid; j < S.ns; j += NT) {
+          const int r = S.pre.perm[S.seg.idx[j]];
+          S.seg.idx[j] = r | (tri_base(r, weff) << 10);
+        }
         __syncthreads();  // perm is dead: the rows take its space
       }
       for (int q = tid; q < tri_words(s, weff); q += NT) S.rows[q] = 0ull;  // tmp is dead
@@ -708,8 +720,13 @@ __global__ __launch_bounds__(NT) void k_clique_vlut_rank_tri(const int64_t *__re
           atomicOr(&S.rows[tri_base(row, weff) + (col >> 6)], 1ull << (col & 63));
       };
       if constexpr (TS::kHash) {
-        rank_hash_walk(S.seg, colidx, S.ns, lo, hi, lane, fl, S.hid, TS::kHashBits,
-                       [&](int at, int j) { set_bit(S.hrank[at], S.seg.idx[j]); });
+        rank_hash_walk(S.seg, colidx, S.ns, lo, hi, lane, fl, S.hid, TS::kHashBits, [&](int at, int j) {
+          const int v = S.seg.idx[j], row = v & 1023, col = S.hrank[at];
+          if (col <= row)
+            atomicOr(err, 1ull);  // DAG not in (degree, id) orientation
+          else
+            atomicOr(&S.rows[(v >> 10) + (col >> 6)], 1ull << (col & 63));
+        });
       } else {
         wave_walk<4, true>(S.seg, colidx, S.ns, lo, hi, lane, fl, [&](const WindowLane &r) {
           if (r.valid) {
