@@ -557,11 +557,16 @@ __global__ __launch_bounds__(NT, (SMAX <= 256 ? 7 : 1)) void k_clique_vlut_rank(
       __syncthreads();
       int lo, hi;
       wave_share(S.L, wave, NT / 64, lo, hi);
+      // a bit at or below the diagonal means the DAG is not in (degree, id)
+      // orientation: flagged once per wave after the walk (the count is
+      // then discarded), so the bit writes need no branch
+      bool bad = false;
       rank_hash_walk(S.seg, colidx, S.ns, lo, hi, lane, fl, S.hid, RS::kHashBits, [&](int at, int j) {
         const int col = S.hrank[at], row = S.seg.idx[j];
-        if (col <= row) atomicOr(err, 1ull);  // DAG not in (degree, id) orientation
+        bad |= col <= row;
         atomicOr(&S.rows[lut_word<W>(row, col >> 6)], 1ull << (col & 63));
       });
+      if (__ballot(bad) && lane == 0) atomicOr(err, 1ull);
       if (tid == 0) S.rowq = 0;
       __syncthreads();
       if (W >= 4 && (depth == 4 || (depth == 3 && W >= 16))) {
@@ -720,13 +725,17 @@ __global__ __launch_bounds__(NT) void k_clique_vlut_rank_tri(const int64_t *__re
           atomicOr(&S.rows[tri_base(row, weff) + (col >> 6)], 1ull << (col & 63));
       };
       if constexpr (TS::kHash) {
+        // as in k_clique_vlut_rank: a bit at or below the diagonal is
+        // flagged once per wave; its write stays inside the rows' space
+        // (tri_base(row) + col / 64 with col <= row is this row's first
+        // stored word or a word before it)
+        bool bad = false;
         rank_hash_walk(S.seg, colidx, S.ns, lo, hi, lane, fl, S.hid, TS::kHashBits, [&](int at, int j) {
           const int v = S.seg.idx[j], row = v & 1023, col = S.hrank[at];
-          if (col <= row)
-            atomicOr(err, 1ull);  // DAG not in (degree, id) orientation
-          else
-            atomicOr(&S.rows[(v >> 10) + (col >> 6)], 1ull << (col & 63));
+          bad |= col <= row;
+          atomicOr(&S.rows[(v >> 10) + (col >> 6)], 1ull << (col & 63));
         });
+        if (__ballot(bad) && lane == 0) atomicOr(err, 1ull);
       } else {
         wave_walk<4, true>(S.seg, colidx, S.ns, lo, hi, lane, fl, [&](const WindowLane &r) {
           if (r.valid) {
