@@ -72,8 +72,9 @@ enum xgm_pattern {
  * identical counts; they differ only in speed. */
 enum xgm_strategy {
   XGM_STRAT_AUTO = 0,   /* library default: triangles -- LDS hash; cliques -- rank-ordered LUT
-                           (as XGM_STRAT_LUT_RANK; XGM_STRAT_LUT on a DAG not in (degree, id)
-                           orientation); diamonds / 4-cycles -- the tiered kernels */
+                           (as XGM_STRAT_LUT_RANK, but tasks of d+ <= 64 on the packed id-ordered
+                           LUT; XGM_STRAT_LUT on a DAG not in (degree, id) orientation);
+                           diamonds / 4-cycles -- the tiered kernels */
   XGM_STRAT_HASH = 1,   /* vertex task, LDS hash of the staged list, flattened probes */
   XGM_STRAT_BSEARCH = 2,/* vertex task, binary search in the staged sorted list */
   XGM_STRAT_LUT = 3,    /* cliques: vertex tasks with a bitmap LUT in LDS, edge LUT above lut_max */
@@ -364,8 +365,10 @@ int xgm_mlp_grads(xgm_mlp *m, const float *X, const float *T, const float *M, in
  * count.  (xgm_sched is declared with the plans above.) */
 /* Degree-threshold codes of a strategy (XGM_STRAT_*), no model: AUTO and
  * LUT_RANK = rank-ordered LUT up to lut_max_degree and for hubs (1024 < d+ <=
- * 4096), edge LUT between lut_max_degree and 1024 and above 4096; LUT =
- * id-ordered LUT up to lut_max_degree, edge LUT above. */
+ * 4096), edge LUT between lut_max_degree and 1024 and above 4096, where AUTO
+ * gives tasks of d+ <= min(64, lut_max_degree) the id-ordered LUT (code 0,
+ * 8 tasks per wave); LUT = id-ordered LUT up to lut_max_degree, edge LUT
+ * above. */
 int xgm_sched_heuristic(const xgm_dgraph *g, int pattern, int strategy, int lut_max_degree, int8_t *codes);
 /* Per-vertex scheduler features (n x XGM_MLP_IN floats, computed on the GPU):
  * log out-degree, log sum / max of 2-hop out-degrees, sampled neighbour
@@ -373,8 +376,10 @@ int xgm_sched_heuristic(const xgm_dgraph *g, int pattern, int strategy, int lut_
  * 4-cliques, capacity bin, row words, log in-degree, bias. */
 int xgm_sched_features(xgm_ctx *ctx, const xgm_dgraph *g, int pattern, float *feats);
 /* Learned codes: features, model inference and argmin on the GPU (the
- * model predicts each task's share of its kernel's time under every code),
- * then a critical-path guard on the host: the lists run as consecutive
+ * model predicts each task's share of its kernel's time under every code;
+ * a task leaves the AUTO code only for a predicted saving above the model's
+ * error, and rank-LUT codes on tasks of d+ <= 64 become code 0, the packed
+ * id-ordered LUT), then a critical-path guard on the host: the lists run as consecutive
  * kernels of about max(sum of shares, longest task) each, so a vertex task
  * predicted to be its kernel's tail moves to the edge LUT (single-edge
  * items) when that lowers the estimate. */
