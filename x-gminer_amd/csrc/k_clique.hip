@@ -197,9 +197,9 @@ __device__ __forceinline__ u64 lut_count_wq_fit(Rows row, int s, int weff, int d
 // the segment table and block-reduction slots.
 template <int SMAX, int W, int NT>
 struct CStage {
+  u64 rows[W ? lut_words<W>(SMAX) : 1];  // first: row words at immediate offsets from the row base
   int x[SMAX];
   int x2[SMAX];
-  u64 rows[W ? lut_words<W>(SMAX) : 1];
   Segs<SMAX> seg;
   int cnt[NT / 64];
   int flagrow[NT / 64][64];
